@@ -1,0 +1,978 @@
+// The GLST engine behind the C ABI of include/glst_hip.h: one handle = one
+// system on one GPU with one HIP stream.  Host setup follows
+// src/glst_force.cu:178-327; the force step follows :431-1614.
+#include "glst_hip.h"
+#include "kernels.hpp"
+#include "numerics.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace glst {
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct StateError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct ArgError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct AllocError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define GLST_HIP(call)                                                        \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      throw HipError(std::string("HIP error: ") + hipGetErrorString(e_) +    \
+                     " at " + __FILE__ + ":" + std::to_string(__LINE__));     \
+  } while (0)
+
+static void check_launch(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    throw HipError(std::string("HIP launch error in ") + what + ": " +
+                   hipGetErrorString(e));
+}
+
+static std::mutex g_mutex;
+static std::string g_last_error;
+static std::string g_tdesign_dir;
+static bool g_tdesign_init = false;
+
+static std::string tdesign_dir() {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  if (!g_tdesign_init) {
+    const char *env = std::getenv("GLST_TDESIGN_DIR");
+    g_tdesign_dir = env ? env : "";
+    g_tdesign_init = true;
+  }
+  return g_tdesign_dir;
+}
+
+class DeviceBuffer {
+ public:
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer &) = delete;
+  DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+  ~DeviceBuffer() { release(); }
+  void resize(size_t bytes) {
+    if (bytes == bytes_) return;
+    release();
+    if (bytes == 0) return;
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw AllocError("hipMalloc of " + std::to_string(bytes) +
+                       " bytes failed: " + hipGetErrorString(e));
+    }
+    ptr_ = p;
+    bytes_ = bytes;
+  }
+  void release() {
+    if (ptr_) (void)hipFree(ptr_);
+    ptr_ = nullptr;
+    bytes_ = 0;
+  }
+  template <typename T>
+  T *as() const {
+    return static_cast<T *>(ptr_);
+  }
+  size_t bytes() const { return bytes_; }
+
+ private:
+  void *ptr_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+class KernelTimer {
+ public:
+  ~KernelTimer() {
+    for (auto &kv : stats_)
+      for (auto &pr : kv.second.pending) {
+        (void)hipEventDestroy(pr.first);
+        (void)hipEventDestroy(pr.second);
+      }
+    for (auto e : pool_) (void)hipEventDestroy(e);
+  }
+  bool enabled = false;
+  hipEvent_t take() {
+    if (!pool_.empty()) {
+      hipEvent_t e = pool_.back();
+      pool_.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    GLST_HIP(hipEventCreate(&e));
+    return e;
+  }
+  void begin(const char *name, hipStream_t s) {
+    if (!enabled) return;
+    cur_name_ = name;
+    cur_start_ = take();
+    GLST_HIP(hipEventRecord(cur_start_, s));
+  }
+  // Called from a destructor: never throws; a failed record drops the
+  // sample.
+  void end(hipStream_t s) noexcept {
+    if (!enabled) return;
+    try {
+      hipEvent_t stop = take();
+      if (hipEventRecord(stop, s) == hipSuccess)
+        stats_[cur_name_].pending.emplace_back(cur_start_, stop);
+    } catch (...) {
+    }
+  }
+  void collect() {
+    for (auto &kv : stats_) {
+      for (auto &pr : kv.second.pending) {
+        GLST_HIP(hipEventSynchronize(pr.second));
+        float ms = 0.f;
+        GLST_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+        kv.second.ms += ms;
+        kv.second.n += 1;
+        pool_.push_back(pr.first);
+        pool_.push_back(pr.second);
+      }
+      kv.second.pending.clear();
+    }
+  }
+  void get(const std::string &name, double &ms, uint32_t &n) {
+    collect();
+    auto it = stats_.find(name);
+    ms = it == stats_.end() ? 0.0 : it->second.ms;
+    n = it == stats_.end() ? 0 : it->second.n;
+  }
+  void reset() {
+    collect();
+    stats_.clear();
+  }
+
+ private:
+  struct Stat {
+    double ms = 0.0;
+    uint32_t n = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  };
+  std::map<std::string, Stat> stats_;
+  std::vector<hipEvent_t> pool_;
+  std::string cur_name_;
+  hipEvent_t cur_start_{};
+};
+
+struct Scope {
+  KernelTimer &t;
+  hipStream_t s;
+  Scope(KernelTimer &timer, const char *name, hipStream_t stream)
+      : t(timer), s(stream) {
+    t.begin(name, s);
+  }
+  ~Scope() { t.end(s); }
+};
+
+}  // namespace glst
+
+using namespace glst;
+
+struct glst_engine {
+  std::string last_error;
+  int ct = GLST_CT_DOUBLE;
+  bool mixed() const { return ct == GLST_CT_FLOAT; }
+  uint32_t natom = 0;
+  double tol = 0.0;
+  double box[3] = {0, 0, 0};
+  dev::GridShape grid{};
+  double inv_cell = 0.0;  // CT(1)/CT(rcut), widened
+  AlphaGroups groups;
+  NodeTable nodes;
+  uint32_t K = 0, Kp = 0;
+  bool has_lr = false;
+  int spherical_rule = 0;
+  int device = 0;
+  size_t lds_limit = 65536;
+
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  KernelTimer timer;
+
+  // atoms
+  DeviceBuffer atom_cell, rank, hist, cell_count, cell_start, maxc;
+  DeviceBuffer sorted_idx, cell_sorted, atoms;
+  DeviceBuffer fx, fy, fz, en;
+  // nodes
+  DeviceBuffer k64, krev, group, r_in, r_out;
+  // fields
+  DeviceBuffer S, R, partial;
+  uint32_t nsplit = 1;
+  bool assigned = false;
+  bool sf_done = false;
+  bool rmt_done = false;
+
+  ~glst_engine() {
+    if (own_stream && stream) (void)hipStreamDestroy(stream);
+  }
+
+  size_t device_bytes() const {
+    const DeviceBuffer *bufs[] = {&atom_cell, &rank,  &hist,       &cell_count,
+                                  &cell_start, &maxc, &sorted_idx, &cell_sorted,
+                                  &atoms,     &fx,    &fy,         &fz,
+                                  &en,        &k64,   &krev,       &group,
+                                  &r_in,      &r_out, &S,          &R,
+                                  &partial};
+    size_t s = 0;
+    for (auto b : bufs) s += b->bytes();
+    return s;
+  }
+
+  void init(int ct_, uint32_t natom_, double tol_, double bx, double by,
+            double bz, double rcut, void *stream_) {
+    if (ct_ != GLST_CT_DOUBLE && ct_ != GLST_CT_FLOAT)
+      throw ArgError("ct must be GLST_CT_DOUBLE or GLST_CT_FLOAT");
+    if (!(bx > 0 && by > 0 && bz > 0 && rcut > 0))
+      throw ArgError("box dimensions and rcut must be positive");
+    if (!(tol_ > 0)) throw ArgError("tol must be positive");
+    ct = ct_;
+    natom = natom_;
+    tol = tol_;
+    box[0] = bx;
+    box[1] = by;
+    box[2] = bz;
+    GLST_HIP(hipGetDevice(&device));
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                              device) == hipSuccess &&
+        lds > 0)
+      lds_limit = static_cast<size_t>(lds);
+    if (stream_) {
+      stream = static_cast<hipStream_t>(stream_);
+      own_stream = false;
+    } else {
+      GLST_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+      own_stream = true;
+    }
+
+    // Grid in CT precision (src/glst_force.cu:223-241).
+    double cell;
+    if (mixed()) {
+      grid_dims<float>(bx, by, bz, rcut, grid.nx, grid.ny, grid.nz);
+      const float cf = static_cast<float>(rcut);
+      cell = static_cast<double>(cf);
+      inv_cell = static_cast<double>(1.0f / cf);
+    } else {
+      grid_dims<double>(bx, by, bz, rcut, grid.nx, grid.ny, grid.nz);
+      cell = rcut;
+      inv_cell = 1.0 / rcut;
+    }
+    grid.h = cell;
+    const uint64_t nc = static_cast<uint64_t>(grid.nx) * grid.ny * grid.nz;
+    if (nc == 0 || nc > 0x7FFFFFFFull)
+      throw ArgError("cell grid of " + std::to_string(nc) +
+                     " cells is out of range");
+    grid.ncell = static_cast<uint32_t>(nc);
+    has_lr = grid.nx > 2 && grid.ny > 2 && grid.nz > 2;
+
+    // Alpha groups and cubature (src/glst_force.cu:1642-1724,
+    // src/cubature.cu:99-159).
+    groups = alpha_groups(grid.nx, grid.ny, grid.nz, cell, cell, cell, tol);
+    const std::string dir = tdesign_dir();
+    spherical_rule = dir.empty() ? 0 : 1;
+    nodes = build_nodes(tol, groups, dir);
+    K = static_cast<uint32_t>(nodes.kx.size());
+    Kp = (K + 255u) / 256u * 256u;
+
+    allocate();
+    upload_nodes();
+    dev::fill_f64(fx.as<double>(), 0.0, natom, stream);
+    dev::fill_f64(fy.as<double>(), 0.0, natom, stream);
+    dev::fill_f64(fz.as<double>(), 0.0, natom, stream);
+    dev::fill_f64(en.as<double>(), 0.0, natom, stream);
+    check_launch("init");
+    GLST_HIP(hipStreamSynchronize(stream));
+  }
+
+  void allocate() {
+    const size_t n = natom;
+    const uint32_t nchunk = (natom + dev::kBinChunk - 1) / dev::kBinChunk;
+    atom_cell.resize(n * 4);
+    rank.resize(n * 4);
+    hist.resize(static_cast<size_t>(nchunk) * grid.ncell * 4);
+    cell_count.resize(static_cast<size_t>(grid.ncell) * 4);
+    cell_start.resize((static_cast<size_t>(grid.ncell) + 1) * 4);
+    maxc.resize(4);
+    sorted_idx.resize(n * 4);
+    cell_sorted.resize(n * 4);
+    atoms.resize(n * (mixed() ? sizeof(float4) : sizeof(double4)));
+    fx.resize(n * 8);
+    fy.resize(n * 8);
+    fz.resize(n * 8);
+    en.resize(n * 8);
+    if (Kp > 0) {
+      k64.resize(static_cast<size_t>(Kp) * sizeof(double4));
+      krev.resize(static_cast<size_t>(Kp) * sizeof(float4));
+      group.resize(static_cast<size_t>(Kp) * 4);
+      r_in.resize(groups.r_in.size() * 4);
+      r_out.resize(groups.r_out.size() * 4);
+      if (has_lr) {
+        const size_t field = static_cast<size_t>(grid.ncell) * Kp;
+        S.resize(field * sizeof(double2));
+        R.resize(field * (mixed() ? sizeof(float2) : sizeof(double2)));
+        const uint32_t ntile = (natom + 255) / 256;
+        const uint32_t groups64 = Kp / 64;
+        nsplit = ntile > 0 ? (4096 + ntile - 1) / ntile : 1;
+        if (nsplit > groups64) nsplit = groups64;
+        if (nsplit < 1) nsplit = 1;
+        partial.resize(static_cast<size_t>(nsplit) * n * sizeof(double4));
+      }
+    }
+  }
+
+  void upload_nodes() {
+    if (Kp == 0) return;
+    std::vector<double4> h64(Kp, make_double4(0, 0, 0, 0));
+    std::vector<float4> hrev(Kp, make_float4(0, 0, 0, 0));
+    std::vector<uint32_t> hg(Kp, 0);
+    const double inv2pi = 1.0 / 6.283185307179586476925286766559;
+    for (uint32_t k = 0; k < K; ++k) {
+      h64[k] = make_double4(nodes.kx[k], nodes.ky[k], nodes.kz[k], nodes.w[k]);
+      hrev[k] = make_float4(static_cast<float>(nodes.kx[k] * inv2pi),
+                            static_cast<float>(nodes.ky[k] * inv2pi),
+                            static_cast<float>(nodes.kz[k] * inv2pi), 0.f);
+      hg[k] = nodes.group[k];
+    }
+    GLST_HIP(hipMemcpy(k64.as<void>(), h64.data(), Kp * sizeof(double4),
+                       hipMemcpyHostToDevice));
+    GLST_HIP(hipMemcpy(krev.as<void>(), hrev.data(), Kp * sizeof(float4),
+                       hipMemcpyHostToDevice));
+    GLST_HIP(hipMemcpy(group.as<void>(), hg.data(), Kp * 4,
+                       hipMemcpyHostToDevice));
+    GLST_HIP(hipMemcpy(r_in.as<void>(), groups.r_in.data(),
+                       groups.r_in.size() * 4, hipMemcpyHostToDevice));
+    GLST_HIP(hipMemcpy(r_out.as<void>(), groups.r_out.data(),
+                       groups.r_out.size() * 4, hipMemcpyHostToDevice));
+  }
+
+  // ---- the step ------------------------------------------------------------
+
+  void assign_atoms(const double *rx, const double *ry, const double *rz,
+                    const double *qc) {
+    if (natom > 0 && (!rx || !ry || !rz || !qc))
+      throw ArgError("assign_atoms: null coordinate or charge pointer");
+    const uint32_t nchunk = (natom + dev::kBinChunk - 1) / dev::kBinChunk;
+    {
+      Scope sc(timer, "bin", stream);
+      dev::fill_u32(hist.as<uint32_t>(), 0u,
+                    static_cast<size_t>(nchunk) * grid.ncell, stream);
+      dev::bin_chunks(mixed(), rx, ry, rz, natom, inv_cell, grid,
+                      atom_cell.as<uint32_t>(), rank.as<uint32_t>(),
+                      hist.as<uint32_t>(), stream);
+      check_launch("bin_chunks");
+    }
+    {
+      Scope sc(timer, "sort", stream);
+      dev::hist_offsets(hist.as<uint32_t>(), nchunk, grid.ncell,
+                        cell_count.as<uint32_t>(), stream);
+      dev::cell_scan(cell_count.as<uint32_t>(), grid.ncell,
+                     cell_start.as<uint32_t>(), maxc.as<uint32_t>(), stream);
+      dev::bin_scatter(mixed(), rx, ry, rz, qc, natom,
+                       atom_cell.as<uint32_t>(), rank.as<uint32_t>(),
+                       hist.as<uint32_t>(), cell_start.as<uint32_t>(), grid,
+                       sorted_idx.as<uint32_t>(), cell_sorted.as<uint32_t>(),
+                       atoms.as<void>(), stream);
+      check_launch("sort");
+    }
+    assigned = true;
+    sf_done = rmt_done = false;
+  }
+
+  void calc_sf() {
+    if (!has_lr || Kp == 0) return;  // src/glst_force.cu:667-670
+    if (!assigned) throw StateError("calc_sf called before assign_atoms");
+    Scope sc(timer, "sf", stream);
+    dev::sf(mixed(), atoms.as<void>(), cell_start.as<uint32_t>(),
+            k64.as<double4>(), krev.as<float4>(), Kp, grid, S.as<double2>(),
+            stream);
+    check_launch("sf");
+    sf_done = true;
+    rmt_done = false;
+  }
+
+  void sum_rmt_sf() {
+    if (!has_lr || Kp == 0) return;
+    if (!sf_done) throw StateError("sum_rmt_sf called before calc_sf");
+    Scope sc(timer, "rmt", stream);
+    if (!dev::rmt_lds(mixed(), S.as<double2>(), k64.as<double4>(),
+                      group.as<uint32_t>(), r_in.as<uint32_t>(),
+                      r_out.as<uint32_t>(), Kp, grid, lds_limit,
+                      R.as<void>(), stream)) {
+      dev::rmt_global(mixed(), S.as<double2>(), k64.as<double4>(),
+                      group.as<uint32_t>(), r_in.as<uint32_t>(),
+                      r_out.as<uint32_t>(), Kp, grid, R.as<void>(), stream);
+      sf_done = false;  // S now holds prefix sums
+    }
+    check_launch("rmt");
+    rmt_done = true;
+  }
+
+  void calc_lr_ef() {
+    if (!has_lr || Kp == 0 || natom == 0) return;
+    if (!rmt_done) throw StateError("calc_lr_ef called before sum_rmt_sf");
+    {
+      Scope sc(timer, "lr", stream);
+      dev::lr(mixed(), atoms.as<void>(), cell_sorted.as<uint32_t>(),
+              k64.as<double4>(), krev.as<float4>(), R.as<void>(), Kp, natom,
+              nsplit, partial.as<double4>(), stream);
+      check_launch("lr");
+    }
+    {
+      Scope sc(timer, "lr_reduce", stream);
+      dev::lr_reduce(partial.as<double4>(), nsplit, natom, fx.as<double>(),
+                     fy.as<double>(), fz.as<double>(), en.as<double>(),
+                     stream);
+      check_launch("lr_reduce");
+    }
+  }
+
+  void calc_sr_ef() {
+    if (natom == 0) return;
+    if (!assigned) throw StateError("calc_sr_ef called before assign_atoms");
+    Scope sc(timer, "sr", stream);
+    dev::sr(mixed(), atoms.as<void>(), cell_start.as<uint32_t>(), grid,
+            fx.as<double>(), fy.as<double>(), fz.as<double>(), en.as<double>(),
+            stream);
+    check_launch("sr");
+  }
+
+  void comm_ef() { GLST_HIP(hipStreamSynchronize(stream)); }
+
+  void step_async(const double *rx, const double *ry, const double *rz,
+                  const double *qc) {
+    assign_atoms(rx, ry, rz, qc);
+    calc_sf();
+    sum_rmt_sf();
+    calc_lr_ef();
+    calc_sr_ef();
+  }
+
+  void get_ef(double *ofx, double *ofy, double *ofz, double *oen) {
+    if (natom > 0 && (!ofx || !ofy || !ofz || !oen))
+      throw ArgError("get_ef: null output pointer");
+    if (natom > 0 && !assigned)
+      throw StateError("get_ef called before assign_atoms");
+    {
+      Scope sc(timer, "unpermute", stream);
+      dev::unpermute(sorted_idx.as<uint32_t>(), natom, fx.as<double>(),
+                     fy.as<double>(), fz.as<double>(), en.as<double>(), ofx,
+                     ofy, ofz, oen, stream);
+      check_launch("unpermute");
+    }
+    GLST_HIP(hipStreamSynchronize(stream));
+  }
+
+  void zero_ef() {
+    dev::fill_f64(fx.as<double>(), 0.0, natom, stream);
+    dev::fill_f64(fy.as<double>(), 0.0, natom, stream);
+    dev::fill_f64(fz.as<double>(), 0.0, natom, stream);
+    dev::fill_f64(en.as<double>(), 0.0, natom, stream);
+    check_launch("zero_ef");
+  }
+
+  uint32_t max_atoms_cell() {
+    if (!assigned) return 0;
+    uint32_t m = 0;
+    GLST_HIP(hipMemcpyAsync(&m, maxc.as<void>(), 4, hipMemcpyDeviceToHost,
+                            stream));
+    GLST_HIP(hipStreamSynchronize(stream));
+    return m;
+  }
+
+  template <typename T>
+  void d2h(T *dst, const DeviceBuffer &src, size_t count) {
+    if (!dst || count == 0) return;
+    GLST_HIP(hipMemcpyAsync(dst, src.as<void>(), count * sizeof(T),
+                            hipMemcpyDeviceToHost, stream));
+  }
+
+  void field_to_host(const DeviceBuffer &buf, bool single, double *re,
+                     double *im) {
+    const size_t n = static_cast<size_t>(grid.ncell) * Kp;
+    std::vector<double> tmp(2 * n);
+    if (single) {
+      std::vector<float> f(2 * n);
+      GLST_HIP(hipMemcpyAsync(f.data(), buf.as<void>(), f.size() * 4,
+                              hipMemcpyDeviceToHost, stream));
+      GLST_HIP(hipStreamSynchronize(stream));
+      for (size_t i = 0; i < 2 * n; ++i) tmp[i] = f[i];
+    } else {
+      GLST_HIP(hipMemcpyAsync(tmp.data(), buf.as<void>(), tmp.size() * 8,
+                              hipMemcpyDeviceToHost, stream));
+      GLST_HIP(hipStreamSynchronize(stream));
+    }
+    for (uint32_t c = 0; c < grid.ncell; ++c)
+      for (uint32_t k = 0; k < K; ++k) {
+        const size_t s = static_cast<size_t>(c) * Kp + k;
+        const size_t d = static_cast<size_t>(c) * K + k;
+        if (re) re[d] = tmp[2 * s];
+        if (im) im[d] = tmp[2 * s + 1];
+      }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+namespace {
+
+void set_global_error(const std::string &m) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  g_last_error = m;
+}
+
+void record_error(glst_engine *h, const std::string &m) {
+  if (h)
+    h->last_error = m;
+  else
+    set_global_error(m);
+}
+
+// Runs f, mapping exceptions to status codes and the handle's (or the
+// process-wide) last-error string.
+template <class F>
+glst_status_t guarded(glst_engine *h, F &&f) {
+  try {
+    f();
+    if (h) h->last_error.clear();
+    return GLST_OK;
+  } catch (const ArgError &e) {
+    record_error(h, e.what());
+    return GLST_ERR_INVALID;
+  } catch (const StateError &e) {
+    record_error(h, e.what());
+    return GLST_ERR_STATE;
+  } catch (const AllocError &e) {
+    record_error(h, e.what());
+    return GLST_ERR_ALLOC;
+  } catch (const HipError &e) {
+    record_error(h, e.what());
+    return GLST_ERR_HIP;
+  } catch (const std::exception &e) {
+    const std::string m = e.what();
+    record_error(h, m);
+    if (m.find("open") != std::string::npos ||
+        m.find("Malformed") != std::string::npos ||
+        m.find("directory") != std::string::npos)
+      return GLST_ERR_IO;
+    return GLST_ERR_NUMERICS;
+  }
+}
+
+#define NEED_HANDLE(h)                                                        \
+  if (!(h)) {                                                                 \
+    set_global_error("null handle");                                          \
+    return GLST_ERR_INVALID;                                                  \
+  }
+
+}  // namespace
+
+extern "C" {
+
+glst_status_t glst_create(int ct, uint32_t natom, double tol, double box_x,
+                          double box_y, double box_z, double rcut,
+                          void *stream, glst_handle_t *out) {
+  if (!out) {
+    set_global_error("glst_create: null output handle");
+    return GLST_ERR_INVALID;
+  }
+  *out = nullptr;
+  glst_engine *h = new (std::nothrow) glst_engine();
+  if (!h) return GLST_ERR_ALLOC;
+  glst_status_t st = guarded(nullptr, [&] {
+    if (!(tol > 0)) throw ArgError("tol must be positive");
+    h->init(ct, natom, tol, box_x, box_y, box_z, rcut, stream);
+  });
+  if (st != GLST_OK) {
+    delete h;
+    return st;
+  }
+  *out = h;
+  return GLST_OK;
+}
+
+glst_status_t glst_create_ncell(int ct, uint32_t natom, double tol,
+                                double box_x, double box_y, double box_z,
+                                uint32_t ncell_x, uint32_t ncell_y,
+                                uint32_t ncell_z, void *stream,
+                                glst_handle_t *out) {
+  if (ncell_x == 0 || ncell_y == 0 || ncell_z == 0) {
+    set_global_error("glst_create_ncell: cell counts must be positive");
+    if (out) *out = nullptr;
+    return GLST_ERR_INVALID;
+  }
+  // src/glst_force.cu:87-94
+  const double rx = box_x / static_cast<double>(ncell_x);
+  const double ry = box_y / static_cast<double>(ncell_y);
+  const double rz = box_z / static_cast<double>(ncell_z);
+  double rcut = rx;
+  rcut = (ry < rcut) ? ry : rcut;
+  rcut = (rz < rcut) ? rz : rcut;
+  return glst_create(ct, natom, tol, box_x, box_y, box_z, rcut, stream, out);
+}
+
+glst_status_t glst_destroy(glst_handle_t h) {
+  if (!h) return GLST_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  delete h;
+  return GLST_OK;
+}
+
+glst_status_t glst_set_stream(glst_handle_t h, void *stream) {
+  NEED_HANDLE(h);
+  return guarded(h, [&] {
+    GLST_HIP(hipStreamSynchronize(h->stream));
+    if (h->own_stream) GLST_HIP(hipStreamDestroy(h->stream));
+    if (stream) {
+      h->stream = static_cast<hipStream_t>(stream);
+      h->own_stream = false;
+    } else {
+      GLST_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+      h->own_stream = true;
+    }
+  });
+}
+
+glst_status_t glst_assign_atoms(glst_handle_t h, const double *d_rx,
+                                const double *d_ry, const double *d_rz,
+                                const double *d_qc) {
+  NEED_HANDLE(h);
+  return guarded(h, [&] { h->assign_atoms(d_rx, d_ry, d_rz, d_qc); });
+}
+
+glst_status_t glst_calc_sf(glst_handle_t h) {
+  NEED_HANDLE(h);
+  return guarded(h, [&] { h->calc_sf(); });
+}
+
+glst_status_t glst_sum_rmt_sf(glst_handle_t h) {
+  NEED_HANDLE(h);
+  return guarded(h, [&] { h->sum_rmt_sf(); });
+}
+
+glst_status_t glst_calc_lr_ef(glst_handle_t h) {
+  NEED_HANDLE(h);
+  return guarded(h, [&] { h->calc_lr_ef(); });
+}
+
+glst_status_t glst_calc_sr_ef(glst_handle_t h) {
+  NEED_HANDLE(h);
+  return guarded(h, [&] { h->calc_sr_ef(); });
+}
+
+glst_status_t glst_comm_ef(glst_handle_t h) {
+  NEED_HANDLE(h);
+  return guarded(h, [&] { h->comm_ef(); });
+}
+
+glst_status_t glst_calc_ener_force(glst_handle_t h, const double *d_rx,
+                                   const double *d_ry, const double *d_rz,
+                                   const double *d_qc) {
+  NEED_HANDLE(h);
+  return guarded(h, [&] {
+    h->step_async(d_rx, d_ry, d_rz, d_qc);
+    h->comm_ef();
+  });
+}
+
+glst_status_t glst_calc_ener_force_async(glst_handle_t h, const double *d_rx,
+                                         const double *d_ry,
+                                         const double *d_rz,
+                                         const double *d_qc) {
+  NEED_HANDLE(h);
+  return guarded(h, [&] { h->step_async(d_rx, d_ry, d_rz, d_qc); });
+}
+
+glst_status_t glst_get_ef(glst_handle_t h, double *d_fx, double *d_fy,
+                          double *d_fz, double *d_en) {
+  NEED_HANDLE(h);
+  return guarded(h, [&] { h->get_ef(d_fx, d_fy, d_fz, d_en); });
+}
+
+glst_status_t glst_sorted_ef_ptrs(glst_handle_t h, double **fx, double **fy,
+                                  double **fz, double **en) {
+  NEED_HANDLE(h);
+  if (fx) *fx = h->fx.as<double>();
+  if (fy) *fy = h->fy.as<double>();
+  if (fz) *fz = h->fz.as<double>();
+  if (en) *en = h->en.as<double>();
+  return GLST_OK;
+}
+
+glst_status_t glst_zero_ef(glst_handle_t h) {
+  NEED_HANDLE(h);
+  return guarded(h, [&] { h->zero_ef(); });
+}
+
+glst_status_t glst_sorted_index_ptr(glst_handle_t h, const uint32_t **idx) {
+  NEED_HANDLE(h);
+  if (!idx) return GLST_ERR_INVALID;
+  *idx = h->sorted_idx.as<uint32_t>();
+  return GLST_OK;
+}
+
+glst_status_t glst_get_info(glst_handle_t h, glst_info_t *info) {
+  NEED_HANDLE(h);
+  if (!info) return GLST_ERR_INVALID;
+  return guarded(h, [&] {
+    std::memset(info, 0, sizeof(*info));
+    info->natom = h->natom;
+    info->ncell[0] = h->grid.nx;
+    info->ncell[1] = h->grid.ny;
+    info->ncell[2] = h->grid.nz;
+    info->ncell_total = h->grid.ncell;
+    info->cell_dim[0] = info->cell_dim[1] = info->cell_dim[2] = h->grid.h;
+    info->ngroup = static_cast<uint32_t>(h->groups.width.size());
+    info->num_nodes = h->K;
+    info->has_long_range = h->has_lr ? 1u : 0u;
+    info->max_atoms_cell = h->max_atoms_cell();
+    info->ct = h->ct;
+    info->spherical_rule = h->spherical_rule;
+    info->device_bytes = h->device_bytes();
+  });
+}
+
+glst_status_t glst_get_group_info(glst_handle_t h, uint32_t g,
+                                  glst_group_info_t *info) {
+  NEED_HANDLE(h);
+  if (!info || g >= h->groups.width.size()) {
+    h->last_error = "glst_get_group_info: group index out of range";
+    return GLST_ERR_INVALID;
+  }
+  info->width = h->groups.width[g];
+  info->r_in = h->groups.r_in[g];
+  info->r_out = h->groups.r_out[g];
+  info->rmin = h->groups.rmin[g];
+  info->rmax = h->groups.rmax[g];
+  info->alpha = h->groups.alpha[g];
+  info->zcut = h->groups.zcut[g];
+  info->gl_order = h->nodes.gl[g].order;
+  info->sph_degree = 2 * h->nodes.sph_m[g] + 1;
+  info->num_dirs = h->nodes.sph_count[g];
+  info->first_node = h->nodes.first[g];
+  info->num_nodes = h->nodes.count[g];
+  return GLST_OK;
+}
+
+glst_status_t glst_get_nodes(glst_handle_t h, double *kx, double *ky,
+                             double *kz, double *w, uint32_t *group) {
+  NEED_HANDLE(h);
+  const size_t K = h->K;
+  if (kx) std::memcpy(kx, h->nodes.kx.data(), K * 8);
+  if (ky) std::memcpy(ky, h->nodes.ky.data(), K * 8);
+  if (kz) std::memcpy(kz, h->nodes.kz.data(), K * 8);
+  if (w) std::memcpy(w, h->nodes.w.data(), K * 8);
+  if (group) std::memcpy(group, h->nodes.group.data(), K * 4);
+  return GLST_OK;
+}
+
+glst_status_t glst_get_cells(glst_handle_t h, uint32_t *atom_cell,
+                             uint32_t *sorted_idx, uint32_t *cell_point,
+                             uint32_t *cell_count) {
+  NEED_HANDLE(h);
+  return guarded(h, [&] {
+    if (!h->assigned) throw StateError("glst_get_cells before assign_atoms");
+    h->d2h(atom_cell, h->atom_cell, h->natom);
+    h->d2h(sorted_idx, h->sorted_idx, h->natom);
+    h->d2h(cell_point, h->cell_start, h->grid.ncell);
+    h->d2h(cell_count, h->cell_count, h->grid.ncell);
+    GLST_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+glst_status_t glst_get_sf(glst_handle_t h, double *re, double *im) {
+  NEED_HANDLE(h);
+  return guarded(h, [&] {
+    if (!h->has_lr || h->K == 0) return;
+    if (!h->sf_done) throw StateError("glst_get_sf: no structure factors");
+    h->field_to_host(h->S, false, re, im);
+  });
+}
+
+glst_status_t glst_get_rmt(glst_handle_t h, double *re, double *im) {
+  NEED_HANDLE(h);
+  return guarded(h, [&] {
+    if (!h->has_lr || h->K == 0) return;
+    if (!h->rmt_done) throw StateError("glst_get_rmt: no remote sums");
+    const size_t n = static_cast<size_t>(h->grid.ncell) * h->K;
+    std::vector<double> r(n), i(n);
+    h->field_to_host(h->R, h->mixed(), r.data(), i.data());
+    if (h->mixed()) {
+      // undo the stored exp(+i k.c) factor in FP64
+      const auto &g = h->grid;
+      for (uint32_t c = 0; c < g.ncell; ++c) {
+        const uint32_t x = c / (g.ny * g.nz), y = (c / g.nz) % g.ny,
+                       z = c % g.nz;
+        const double cx = (x + 0.5) * g.h, cy = (y + 0.5) * g.h,
+                     cz = (z + 0.5) * g.h;
+        for (uint32_t k = 0; k < h->K; ++k) {
+          const double th =
+              h->nodes.kx[k] * cx + h->nodes.ky[k] * cy + h->nodes.kz[k] * cz;
+          const double cs = std::cos(th), sn = std::sin(th);
+          const size_t o = static_cast<size_t>(c) * h->K + k;
+          const double a = r[o], b = i[o];
+          r[o] = a * cs + b * sn;
+          i[o] = b * cs - a * sn;
+        }
+      }
+    }
+    if (re) std::memcpy(re, r.data(), n * 8);
+    if (im) std::memcpy(im, i.data(), n * 8);
+  });
+}
+
+glst_status_t glst_set_timing(glst_handle_t h, int enable) {
+  NEED_HANDLE(h);
+  h->timer.enabled = enable != 0;
+  return GLST_OK;
+}
+
+glst_status_t glst_kernel_time(glst_handle_t h, const char *name,
+                               double *total_ms, uint32_t *launches) {
+  NEED_HANDLE(h);
+  if (!name) return GLST_ERR_INVALID;
+  return guarded(h, [&] {
+    double ms;
+    uint32_t n;
+    h->timer.get(name, ms, n);
+    if (total_ms) *total_ms = ms;
+    if (launches) *launches = n;
+  });
+}
+
+glst_status_t glst_reset_timing(glst_handle_t h) {
+  NEED_HANDLE(h);
+  return guarded(h, [&] { h->timer.reset(); });
+}
+
+const char *glst_last_error(glst_handle_t h) {
+  if (!h) return glst_global_last_error();
+  return h->last_error.c_str();
+}
+
+const char *glst_global_last_error(void) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return g_last_error.c_str();
+}
+
+const char *glst_version(void) { return "glst-mi355x 0.1.0 (gfx950)"; }
+
+glst_status_t glst_coulomb_direct(const double *d_rx, const double *d_ry,
+                                  const double *d_rz, const double *d_qc,
+                                  uint32_t natom, double *d_fx, double *d_fy,
+                                  double *d_fz, double *d_en, void *stream) {
+  return guarded(nullptr, [&] {
+    if (natom > 0 && (!d_rx || !d_ry || !d_rz || !d_qc || !d_fx || !d_fy ||
+                      !d_fz || !d_en))
+      throw ArgError("glst_coulomb_direct: null pointer");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    dev::coulomb_direct(d_rx, d_ry, d_rz, d_qc, natom, d_fx, d_fy, d_fz, d_en,
+                        s);
+    check_launch("coulomb_direct");
+    GLST_HIP(hipStreamSynchronize(s));
+  });
+}
+
+glst_status_t glst_erfc_inv(double y, double tol, double *x) {
+  if (!x) return GLST_ERR_INVALID;
+  return guarded(nullptr, [&] { *x = erfc_inv(y, tol); });
+}
+
+glst_status_t glst_alpha_groups(uint32_t nx, uint32_t ny, uint32_t nz,
+                                double cdx, double cdy, double cdz, double tol,
+                                uint32_t cap, uint32_t *ngroup,
+                                uint32_t *width, double *rmax, double *alpha,
+                                double *zcut, uint32_t *r_in,
+                                uint32_t *r_out) {
+  if (!ngroup) return GLST_ERR_INVALID;
+  return guarded(nullptr, [&] {
+    AlphaGroups g = alpha_groups(nx, ny, nz, cdx, cdy, cdz, tol);
+    *ngroup = static_cast<uint32_t>(g.width.size());
+    const size_t n = std::min<size_t>(cap, g.width.size());
+    for (size_t i = 0; i < n; ++i) {
+      if (width) width[i] = g.width[i];
+      if (rmax) rmax[i] = g.rmax[i];
+      if (alpha) alpha[i] = g.alpha[i];
+      if (zcut) zcut[i] = g.zcut[i];
+      if (r_in) r_in[i] = g.r_in[i];
+      if (r_out) r_out[i] = g.r_out[i];
+    }
+  });
+}
+
+glst_status_t glst_gauss_legendre(uint32_t n, double *x, double *w,
+                                  uint32_t *count) {
+  return guarded(nullptr, [&] {
+    if (n == 0) throw ArgError("glst_gauss_legendre: n must be positive");
+    std::vector<double> xx, ww;
+    gauss_legendre_nonneg(n, xx, ww);
+    if (count) *count = static_cast<uint32_t>(xx.size());
+    if (x) std::memcpy(x, xx.data(), xx.size() * 8);
+    if (w) std::memcpy(w, ww.data(), ww.size() * 8);
+  });
+}
+
+glst_status_t glst_select_gl_order(double tol, double dist, double alpha,
+                                   double zcut, uint32_t *order) {
+  if (!order) return GLST_ERR_INVALID;
+  return guarded(nullptr, [&] {
+    *order = select_gauss_legendre(tol, dist, alpha, zcut).order;
+  });
+}
+
+glst_status_t glst_spherical_rule(uint32_t m, double *xyz, double *w,
+                                  uint32_t *count) {
+  return guarded(nullptr, [&] {
+    SphRule r = product_rule(m);
+    if (count) *count = static_cast<uint32_t>(r.s.size());
+    for (size_t i = 0; i < r.s.size(); ++i) {
+      if (xyz) {
+        xyz[3 * i] = r.s[i][0];
+        xyz[3 * i + 1] = r.s[i][1];
+        xyz[3 * i + 2] = r.s[i][2];
+      }
+      if (w) w[i] = r.w[i];
+    }
+  });
+}
+
+glst_status_t glst_set_tdesign_dir(const char *dir) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  g_tdesign_dir = dir ? dir : "";
+  g_tdesign_init = true;
+  return GLST_OK;
+}
+
+glst_status_t glst_grid_dims(int ct, double box_x, double box_y, double box_z,
+                             double rcut, uint32_t *nx, uint32_t *ny,
+                             uint32_t *nz) {
+  if (!nx || !ny || !nz) return GLST_ERR_INVALID;
+  return guarded(nullptr, [&] {
+    if (!(box_x > 0 && box_y > 0 && box_z > 0 && rcut > 0))
+      throw ArgError("box dimensions and rcut must be positive");
+    if (ct == GLST_CT_FLOAT)
+      grid_dims<float>(box_x, box_y, box_z, rcut, *nx, *ny, *nz);
+    else
+      grid_dims<double>(box_x, box_y, box_z, rcut, *nx, *ny, *nz);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,958 @@
+// HIP kernels of the GLST force step, written for gfx950 (CDNA4, wave64).
+//
+// Mapping choices (one MI355X holds the whole system):
+//  * assign_atoms: a deterministic, atomics-free stable counting sort.  Each
+//    1024-atom chunk is bitonic-sorted by (cell, index) in LDS, chunk
+//    histograms are scanned per cell, and atoms scatter straight to their
+//    final slot.  The order inside a cell is by original index, which is what
+//    a stable radix sort keyed by cell produces.
+//  * calc_sf: lane = cubature node, the wave walks a z-column of cells; the
+//    atom records are wave-uniform (scalar loads), so the inner loop is pure
+//    VALU + transcendental work with no LDS traffic or barriers.
+//  * sum_rmt_sf: a workgroup holds the whole cell grid of one or two nodes in
+//    LDS and does the 3-D summed-volume table and the shell gather there, so
+//    the C x K field is read once and written once.
+//  * calc_lr_ef: lane = atom, 64 consecutive cell-sorted atoms per wave (no
+//    lanes idle at cell boundaries); node records are wave-uniform scalar
+//    loads, remote sums are per-lane loads that coalesce to the 1-2 cells a
+//    wave spans.
+//  * calc_sr_ef: one wave per cell, neighbour atoms as wave-uniform loads.
+#include "kernels.hpp"
+
+#include <cfloat>
+#include <climits>
+#include <cmath>
+
+namespace glst {
+namespace dev {
+namespace {
+
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+// Blocks are dealt to the 8 XCDs round-robin (linear block b runs on XCD
+// b % 8).  Renumber so that each XCD processes a contiguous range of logical
+// tiles, which keeps neighbouring tiles in the same L2.
+__device__ inline uint32_t xcd_remap(uint32_t b, uint32_t nb) {
+  const uint32_t q = nb >> 3, r = nb & 7u;
+  const uint32_t x = b & 7u, i = b >> 3;
+  return x * q + (x < r ? x : r) + i;
+}
+
+// Tuning knobs (compile-time): FP32 partial-sum lengths before the FP64 flush.
+#ifndef GLST_SF_FLUSH
+#define GLST_SF_FLUSH 32
+#endif
+#ifndef GLST_LR_FLUSH
+#define GLST_LR_FLUSH 64
+#endif
+#ifndef GLST_ACCURATE_SINCOS
+#define GLST_ACCURATE_SINCOS 0
+#endif
+
+// sin(2 pi t), cos(2 pi t) for t in revolutions.  The hardware V_SIN_F32 /
+// V_COS_F32 take revolutions directly.
+#if GLST_ACCURATE_SINCOS
+__device__ inline float sin_rev(float t) { return sinpif(2.0f * t); }
+__device__ inline float cos_rev(float t) { return cospif(2.0f * t); }
+#else
+__device__ inline float sin_rev(float t) { return __builtin_amdgcn_sinf(t); }
+__device__ inline float cos_rev(float t) { return __builtin_amdgcn_cosf(t); }
+#endif
+__device__ inline float fract_rev(float t) { return __builtin_amdgcn_fractf(t); }
+
+// ---------------------------------------------------------------------------
+// fills
+
+__global__ void fill_u32_kernel(uint32_t *p, uint32_t v, size_t n) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void fill_f64_kernel(double *p, double v, size_t n) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// binning
+
+// c_d = (int)(CT(r_d) * inv), clamped to [0, n-1] (src/glst_force.cu:369-377).
+// Written as a compare chain so no out-of-range float->int conversion is ever
+// executed; the result equals truncate-then-clamp for every input (NaN -> 0).
+template <typename CT>
+__device__ inline uint32_t cell_coord(double r, CT inv, uint32_t n) {
+  const CT v = static_cast<CT>(r) * inv;
+  if (!(v >= static_cast<CT>(0))) return 0u;
+  if (v >= static_cast<CT>(n)) return n - 1u;
+  const int c = static_cast<int>(v);
+  return static_cast<uint32_t>(c) < n ? static_cast<uint32_t>(c) : n - 1u;
+}
+
+template <typename CT>
+__global__ __launch_bounds__(1024) void bin_chunk_kernel(
+    const double *__restrict__ rx, const double *__restrict__ ry,
+    const double *__restrict__ rz, uint32_t natom, CT inv, uint32_t nx,
+    uint32_t ny, uint32_t nz, uint32_t ncell, uint32_t *__restrict__ atom_cell,
+    uint32_t *__restrict__ rank, uint32_t *__restrict__ hist) {
+  __shared__ unsigned long long keys[kBinChunk];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t chunk = blockIdx.x;
+  const uint32_t i = chunk * kBinChunk + tid;
+  uint32_t cell = 0xFFFFFFFFu;  // sorts after every real cell
+  if (i < natom) {
+    const uint32_t cx = cell_coord<CT>(rx[i], inv, nx);
+    const uint32_t cy = cell_coord<CT>(ry[i], inv, ny);
+    const uint32_t cz = cell_coord<CT>(rz[i], inv, nz);
+    cell = (cx * ny + cy) * nz + cz;
+    atom_cell[i] = cell;
+  }
+  keys[tid] = (static_cast<unsigned long long>(cell) << 32) | tid;
+  __syncthreads();
+  for (uint32_t k = 2; k <= kBinChunk; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      const uint32_t p = tid ^ j;
+      if (p > tid) {
+        const unsigned long long a = keys[tid], b = keys[p];
+        const bool ascending = (tid & k) == 0;
+        if ((a > b) == ascending) {
+          keys[tid] = b;
+          keys[p] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const unsigned long long key = keys[tid];
+  const uint32_t c = static_cast<uint32_t>(key >> 32);
+  if (c != 0xFFFFFFFFu) {
+    // first sorted position holding this cell
+    uint32_t lo = 0, hi = tid;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (static_cast<uint32_t>(keys[mid] >> 32) < c)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    const uint32_t local = static_cast<uint32_t>(key & 0xFFFFFFFFull);
+    rank[chunk * kBinChunk + local] = tid - lo;
+    const bool last = (tid == kBinChunk - 1) ||
+                      (static_cast<uint32_t>(keys[tid + 1] >> 32) != c);
+    if (last) hist[static_cast<size_t>(chunk) * ncell + c] = tid - lo + 1;
+  }
+}
+
+__global__ void hist_offsets_kernel(uint32_t *__restrict__ hist,
+                                    uint32_t nchunk, uint32_t ncell,
+                                    uint32_t *__restrict__ count) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  uint32_t run = 0;
+  for (uint32_t ch = 0; ch < nchunk; ++ch) {
+    const size_t o = static_cast<size_t>(ch) * ncell + c;
+    const uint32_t v = hist[o];
+    hist[o] = run;
+    run += v;
+  }
+  count[c] = run;
+}
+
+__global__ __launch_bounds__(1024) void cell_scan_kernel(
+    const uint32_t *__restrict__ count, uint32_t ncell,
+    uint32_t *__restrict__ start, uint32_t *__restrict__ maxc) {
+  __shared__ uint32_t sums[1024];
+  __shared__ uint32_t maxs[1024];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t seg = (ncell + 1023u) / 1024u;
+  const uint32_t b = tid * seg;
+  const uint32_t e = (b + seg < ncell) ? b + seg : ncell;
+  uint32_t s = 0, m = 0;
+  for (uint32_t c = b; c < e; ++c) {
+    s += count[c];
+    m = count[c] > m ? count[c] : m;
+  }
+  sums[tid] = s;
+  maxs[tid] = m;
+  __syncthreads();
+  for (uint32_t off = 1; off < 1024; off <<= 1) {
+    const uint32_t add = tid >= off ? sums[tid - off] : 0u;
+    const uint32_t mx = tid >= off ? maxs[tid - off] : 0u;
+    __syncthreads();
+    sums[tid] += add;
+    maxs[tid] = mx > maxs[tid] ? mx : maxs[tid];
+    __syncthreads();
+  }
+  uint32_t run = sums[tid] - s;  // exclusive prefix of this segment
+  for (uint32_t c = b; c < e; ++c) {
+    start[c] = run;
+    run += count[c];
+  }
+  if (tid == 1023) {
+    start[ncell] = sums[1023];
+    *maxc = maxs[1023];
+  }
+}
+
+template <bool MIXED>
+__global__ void bin_scatter_kernel(
+    const double *__restrict__ rx, const double *__restrict__ ry,
+    const double *__restrict__ rz, const double *__restrict__ qc,
+    uint32_t natom, const uint32_t *__restrict__ atom_cell,
+    const uint32_t *__restrict__ rank, const uint32_t *__restrict__ hist,
+    const uint32_t *__restrict__ start, uint32_t ncell, uint32_t ny,
+    uint32_t nz, double h, uint32_t *__restrict__ sorted_idx,
+    uint32_t *__restrict__ cell_sorted, void *__restrict__ atoms) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= natom) return;
+  const uint32_t c = atom_cell[i];
+  const uint32_t ch = i / kBinChunk;
+  const uint32_t dst =
+      start[c] + hist[static_cast<size_t>(ch) * ncell + c] + rank[i];
+  sorted_idx[dst] = i;
+  cell_sorted[dst] = c;
+  if (MIXED) {
+    // Coordinates relative to the cell centre, rounded once from FP64: the
+    // large part of every phase k.r is carried by the FP64 cell centre.
+    const uint32_t ix = c / (ny * nz), iy = (c / nz) % ny, iz = c % nz;
+    const double cx = (static_cast<double>(ix) + 0.5) * h;
+    const double cy = (static_cast<double>(iy) + 0.5) * h;
+    const double cz = (static_cast<double>(iz) + 0.5) * h;
+    static_cast<float4 *>(atoms)[dst] =
+        make_float4(static_cast<float>(rx[i] - cx), static_cast<float>(ry[i] - cy),
+                    static_cast<float>(rz[i] - cz), static_cast<float>(qc[i]));
+  } else {
+    static_cast<double4 *>(atoms)[dst] = make_double4(rx[i], ry[i], rz[i], qc[i]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// structure factors
+
+constexpr uint32_t kSfBlock = 256;
+
+// FP64: S_c(k) = sum_j q_j (cos(k.r_j) - i sin(k.r_j)), absolute coordinates
+// (src/glst_force.cu:601-664).  blockIdx.x = z-column, blockIdx.y = node tile.
+__global__ __launch_bounds__(kSfBlock) void sf_f64_kernel(
+    const double4 *__restrict__ atoms, const uint32_t *__restrict__ start,
+    const double4 *__restrict__ k64, uint32_t Kp, uint32_t nz,
+    double2 *__restrict__ S) {
+  const uint32_t col = blockIdx.x;
+  const uint32_t node = blockIdx.y * kSfBlock + threadIdx.x;
+  const double4 k = k64[node];
+  for (uint32_t z = 0; z < nz; ++z) {
+    const uint32_t cell = col * nz + z;
+    const uint32_t a0 = start[cell], a1 = start[cell + 1];
+    double re0 = 0.0, im0 = 0.0, re1 = 0.0, im1 = 0.0;
+    uint32_t j = a0;
+    for (; j + 2 <= a1; j += 2) {
+      const double4 a = atoms[j];
+      const double4 b = atoms[j + 1];
+      double sa, ca, sb, cb;
+      sincos(k.x * a.x + k.y * a.y + k.z * a.z, &sa, &ca);
+      sincos(k.x * b.x + k.y * b.y + k.z * b.z, &sb, &cb);
+      re0 += a.w * ca;
+      im0 -= a.w * sa;
+      re1 += b.w * cb;
+      im1 -= b.w * sb;
+    }
+    if (j < a1) {
+      const double4 a = atoms[j];
+      double sa, ca;
+      sincos(k.x * a.x + k.y * a.y + k.z * a.z, &sa, &ca);
+      re0 += a.w * ca;
+      im0 -= a.w * sa;
+    }
+    S[static_cast<size_t>(cell) * Kp + node] = make_double2(re0 + re1, im0 + im1);
+  }
+}
+
+// Mixed: the phase splits as k.r_j = k.c + k.d_j with c the cell centre.
+// k.d_j (|k.d_j| < a few revolutions) runs in FP32 on the hardware sin/cos,
+// which take revolutions; FP32 partial sums are flushed to FP64 every 32 atoms;
+// the cell phase exp(-i k.c) is applied in FP64 and advanced along z by an
+// FP64 rotation.
+__global__ __launch_bounds__(kSfBlock) void sf_mixed_kernel(
+    const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
+    const double4 *__restrict__ k64, const float4 *__restrict__ krev,
+    uint32_t Kp, uint32_t ny, uint32_t nz, double h,
+    double2 *__restrict__ S) {
+  const uint32_t col = blockIdx.x;
+  const uint32_t node = blockIdx.y * kSfBlock + threadIdx.x;
+  const float4 kr = krev[node];
+  const double4 kd = k64[node];
+  const uint32_t ix = col / ny, iy = col % ny;
+  const double cx = (static_cast<double>(ix) + 0.5) * h;
+  const double cy = (static_cast<double>(iy) + 0.5) * h;
+  double ps, pc, ss, sc;
+  sincos(kd.x * cx + kd.y * cy + kd.z * (0.5 * h), &ps, &pc);
+  sincos(kd.z * h, &ss, &sc);
+  for (uint32_t z = 0; z < nz; ++z) {
+    const uint32_t cell = col * nz + z;
+    const uint32_t a0 = start[cell], a1 = start[cell + 1];
+    double RE = 0.0, IM = 0.0;
+    for (uint32_t jb = a0; jb < a1; jb += GLST_SF_FLUSH) {
+      const uint32_t je = (jb + GLST_SF_FLUSH < a1) ? jb + GLST_SF_FLUSH : a1;
+      float re0 = 0.f, im0 = 0.f, re1 = 0.f, im1 = 0.f;
+      uint32_t j = jb;
+      for (; j + 2 <= je; j += 2) {
+        const float4 a = atoms[j];
+        const float4 b = atoms[j + 1];
+        const float ta = fract_rev(kr.x * a.x + kr.y * a.y + kr.z * a.z);
+        const float tb = fract_rev(kr.x * b.x + kr.y * b.y + kr.z * b.z);
+        re0 += a.w * cos_rev(ta);
+        im0 -= a.w * sin_rev(ta);
+        re1 += b.w * cos_rev(tb);
+        im1 -= b.w * sin_rev(tb);
+      }
+      if (j < je) {
+        const float4 a = atoms[j];
+        const float ta = fract_rev(kr.x * a.x + kr.y * a.y + kr.z * a.z);
+        re0 += a.w * cos_rev(ta);
+        im0 -= a.w * sin_rev(ta);
+      }
+      RE += static_cast<double>(re0) + static_cast<double>(re1);
+      IM += static_cast<double>(im0) + static_cast<double>(im1);
+    }
+    // (RE + i IM) * (pc - i ps)
+    S[static_cast<size_t>(cell) * Kp + node] =
+        make_double2(RE * pc + IM * ps, IM * pc - RE * ps);
+    const double npc = pc * sc - ps * ss;
+    const double nps = ps * sc + pc * ss;
+    pc = npc;
+    ps = nps;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// shell sums
+
+struct LdsGrid {
+  const double2 *p;
+  __device__ double2 at(uint32_t c) const { return p[c]; }
+};
+
+struct GlobalGrid {
+  const double2 *p;
+  uint32_t ld, node;
+  __device__ double2 at(uint32_t c) const {
+    return p[static_cast<size_t>(c) * ld + node];
+  }
+};
+
+// Inclusion-exclusion on the summed-volume table P over the inclusive box
+// [x0,x1]x[y0,y1]x[z0,z1] (src/glst_force.cu:774-834, same term order).
+template <class G>
+__device__ inline double2 box_sum(const G &P, uint32_t x0, uint32_t y0,
+                                  uint32_t z0, uint32_t x1, uint32_t y1,
+                                  uint32_t z1, uint32_t ny, uint32_t nz) {
+  const bool xb = x0 == 0, yb = y0 == 0, zb = z0 == 0;
+  const uint32_t xm = xb ? 0 : x0 - 1, ym = yb ? 0 : y0 - 1,
+                 zm = zb ? 0 : z0 - 1;
+  auto id = [&](uint32_t x, uint32_t y, uint32_t z) {
+    return (x * ny + y) * nz + z;
+  };
+  const double2 zero = make_double2(0.0, 0.0);
+  const double2 g0 = P.at(id(x1, y1, z1));
+  const double2 g1 = xb ? zero : P.at(id(xm, y1, z1));
+  const double2 g2 = yb ? zero : P.at(id(x1, ym, z1));
+  const double2 g3 = zb ? zero : P.at(id(x1, y1, zm));
+  const double2 g4 = (xb || yb) ? zero : P.at(id(xm, ym, z1));
+  const double2 g5 = (xb || zb) ? zero : P.at(id(xm, y1, zm));
+  const double2 g6 = (yb || zb) ? zero : P.at(id(x1, ym, zm));
+  const double2 g7 = (xb || yb || zb) ? zero : P.at(id(xm, ym, zm));
+  return make_double2(g0.x - g1.x - g2.x - g3.x + g4.x + g5.x + g6.x - g7.x,
+                      g0.y - g1.y - g2.y - g3.y + g4.y + g5.y + g6.y - g7.y);
+}
+
+// Cube of Chebyshev radius r around (x,y,z), clamped to the grid
+// (src/glst_force.cu:836-859).
+template <class G>
+__device__ inline double2 cube_sum(const G &P, uint32_t x, uint32_t y,
+                                   uint32_t z, uint32_t r, uint32_t nx,
+                                   uint32_t ny, uint32_t nz) {
+  const uint32_t x0 = x < r ? 0 : x - r, y0 = y < r ? 0 : y - r,
+                 z0 = z < r ? 0 : z - r;
+  uint32_t x1 = x + r, y1 = y + r, z1 = z + r;
+  x1 = x1 >= nx ? nx - 1 : x1;
+  y1 = y1 >= ny ? ny - 1 : y1;
+  z1 = z1 >= nz ? nz - 1 : z1;
+  return box_sum(P, x0, y0, z0, x1, y1, z1, ny, nz);
+}
+
+template <bool MIXED, class G>
+__device__ inline void shell_store(const G &P, uint32_t cell, uint32_t node,
+                                   const double4 &k, uint32_t rin,
+                                   uint32_t rout, uint32_t nx, uint32_t ny,
+                                   uint32_t nz, double h, uint32_t Kp,
+                                   void *R) {
+  const uint32_t x = cell / (ny * nz), y = (cell / nz) % ny, z = cell % nz;
+  const double2 o = cube_sum(P, x, y, z, rout, nx, ny, nz);
+  const double2 in = cube_sum(P, x, y, z, rin, nx, ny, nz);
+  const double re = (o.x - in.x) * k.w, im = (o.y - in.y) * k.w;
+  const size_t off = static_cast<size_t>(cell) * Kp + node;
+  if (MIXED) {
+    // exp(+i k.c) R: the FP32 back-projection then needs only cell-local
+    // phases.
+    const double th = k.x * ((static_cast<double>(x) + 0.5) * h) +
+                      k.y * ((static_cast<double>(y) + 0.5) * h) +
+                      k.z * ((static_cast<double>(z) + 0.5) * h);
+    double s, c;
+    sincos(th, &s, &c);
+    static_cast<float2 *>(R)[off] = make_float2(
+        static_cast<float>(re * c - im * s), static_cast<float>(re * s + im * c));
+  } else {
+    static_cast<double2 *>(R)[off] = make_double2(re, im);
+  }
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(256) void rmt_lds_kernel(
+    const double2 *__restrict__ S, const double4 *__restrict__ k64,
+    const uint32_t *__restrict__ group, const uint32_t *__restrict__ r_in,
+    const uint32_t *__restrict__ r_out, uint32_t Kp, uint32_t T,
+    uint32_t ntile, uint32_t nx, uint32_t ny, uint32_t nz, double h,
+    void *__restrict__ R) {
+  extern __shared__ double2 P[];  // [T][ncell]
+  const uint32_t ncell = nx * ny * nz;
+  const uint32_t tile = xcd_remap(blockIdx.x, ntile);
+  const uint32_t node0 = tile * T;
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t e = tid; e < T * ncell; e += blockDim.x) {
+    const uint32_t t = e % T, c = e / T;
+    P[t * ncell + c] = S[static_cast<size_t>(c) * Kp + node0 + t];
+  }
+  __syncthreads();
+  // inclusive prefix along z, then y, then x (src/glst_force.cu:702-772)
+  for (uint32_t l = tid; l < T * nx * ny; l += blockDim.x) {
+    double2 *p = P + (l / (nx * ny)) * ncell + (l % (nx * ny)) * nz;
+    double2 run = make_double2(0.0, 0.0);
+    for (uint32_t z = 0; z < nz; ++z) {
+      run.x += p[z].x;
+      run.y += p[z].y;
+      p[z] = run;
+    }
+  }
+  __syncthreads();
+  for (uint32_t l = tid; l < T * nx * nz; l += blockDim.x) {
+    const uint32_t t = l / (nx * nz), xz = l % (nx * nz);
+    double2 *p = P + t * ncell + (xz / nz) * ny * nz + (xz % nz);
+    double2 run = make_double2(0.0, 0.0);
+    for (uint32_t y = 0; y < ny; ++y) {
+      run.x += p[y * nz].x;
+      run.y += p[y * nz].y;
+      p[y * nz] = run;
+    }
+  }
+  __syncthreads();
+  for (uint32_t l = tid; l < T * ny * nz; l += blockDim.x) {
+    const uint32_t t = l / (ny * nz), yz = l % (ny * nz);
+    double2 *p = P + t * ncell + yz;
+    double2 run = make_double2(0.0, 0.0);
+    for (uint32_t x = 0; x < nx; ++x) {
+      run.x += p[x * ny * nz].x;
+      run.y += p[x * ny * nz].y;
+      p[x * ny * nz] = run;
+    }
+  }
+  __syncthreads();
+  for (uint32_t e = tid; e < T * ncell; e += blockDim.x) {
+    const uint32_t t = e % T, c = e / T;
+    const uint32_t node = node0 + t;
+    const uint32_t g = group[node];
+    LdsGrid grid{P + t * ncell};
+    shell_store<MIXED>(grid, c, node, k64[node], r_in[g], r_out[g], nx, ny,
+                       nz, h, Kp, R);
+  }
+}
+
+// Global-memory fallback for grids whose table does not fit in LDS.
+__global__ void prefix_axis_kernel(double2 *__restrict__ S, uint32_t Kp,
+                                   uint32_t line0, uint32_t nlines, uint32_t n,
+                                   uint32_t stride, uint32_t inner) {
+  // line l covers cells base(l) + i*stride, i < n; base(l) enumerates the
+  // other two axes with `inner` consecutive cells per outer step.
+  const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t l = line0 + blockIdx.y;
+  if (node >= Kp || l >= nlines) return;
+  const uint32_t base = (l / inner) * inner * n + (l % inner);
+  double2 run = make_double2(0.0, 0.0);
+  for (uint32_t i = 0; i < n; ++i) {
+    const size_t o = static_cast<size_t>(base + i * stride) * Kp + node;
+    run.x += S[o].x;
+    run.y += S[o].y;
+    S[o] = run;
+  }
+}
+
+template <bool MIXED>
+__global__ void rmt_gather_kernel(const double2 *__restrict__ P,
+                                  const double4 *__restrict__ k64,
+                                  const uint32_t *__restrict__ group,
+                                  const uint32_t *__restrict__ r_in,
+                                  const uint32_t *__restrict__ r_out,
+                                  uint32_t Kp, uint32_t cell0, uint32_t nx,
+                                  uint32_t ny, uint32_t nz, double h,
+                                  void *__restrict__ R) {
+  const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t cell = cell0 + blockIdx.y;
+  if (node >= Kp || cell >= nx * ny * nz) return;
+  const uint32_t g = group[node];
+  GlobalGrid grid{P, Kp, node};
+  shell_store<MIXED>(grid, cell, node, k64[node], r_in[g], r_out[g], nx, ny,
+                     nz, h, Kp, R);
+}
+
+// ---------------------------------------------------------------------------
+// long-range back-projection
+
+constexpr uint32_t kLrBlock = 256;
+
+// FP64 (src/glst_force.cu:1060-1135): en_a = q_a sum_k Re(e^{ik.r_a} R_c(k)),
+// F_a = q_a sum_k k Im(e^{ik.r_a} R_c(k)).
+__global__ __launch_bounds__(kLrBlock) void lr_f64_kernel(
+    const double4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
+    const double4 *__restrict__ k64, const double2 *__restrict__ R,
+    uint32_t Kp, uint32_t natom, uint32_t kchunk, uint32_t ntile,
+    double4 *__restrict__ partial) {
+  const uint32_t tile = xcd_remap(blockIdx.x, ntile);
+  const uint32_t i = tile * kLrBlock + threadIdx.x;
+  const uint32_t split = blockIdx.y;
+  const uint32_t k0 = split * kchunk;
+  const uint32_t k1 = (k0 + kchunk < Kp) ? k0 + kchunk : Kp;
+  const bool valid = i < natom;
+  const uint32_t ii = valid ? i : natom - 1;
+  const double4 a = atoms[ii];
+  const double2 *Rc = R + static_cast<size_t>(cell_sorted[ii]) * Kp;
+  double fx = 0.0, fy = 0.0, fz = 0.0, en = 0.0;
+  for (uint32_t k = k0; k < k1; ++k) {
+    const double4 kn = k64[k];
+    double s, c;
+    sincos(kn.x * a.x + kn.y * a.y + kn.z * a.z, &s, &c);
+    const double2 r = Rc[k];
+    const double dre = c * r.x - s * r.y;
+    const double dim = c * r.y + s * r.x;
+    fx += dim * kn.x;
+    fy += dim * kn.y;
+    fz += dim * kn.z;
+    en += dre;
+  }
+  if (valid)
+    partial[static_cast<size_t>(split) * natom + i] =
+        make_double4(a.w * fx, a.w * fy, a.w * fz, a.w * en);
+}
+
+// Mixed: cell-local FP32 phases against exp(+i k.c) R_c(k) (float2); FP32
+// partial sums over 64 nodes are flushed to FP64.
+__global__ __launch_bounds__(kLrBlock) void lr_mixed_kernel(
+    const float4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
+    const float4 *__restrict__ krev, const float2 *__restrict__ R,
+    uint32_t Kp, uint32_t natom, uint32_t kchunk, uint32_t ntile,
+    double4 *__restrict__ partial) {
+  const uint32_t tile = xcd_remap(blockIdx.x, ntile);
+  const uint32_t i = tile * kLrBlock + threadIdx.x;
+  const uint32_t split = blockIdx.y;
+  const uint32_t k0 = split * kchunk;
+  const uint32_t k1 = (k0 + kchunk < Kp) ? k0 + kchunk : Kp;
+  const bool valid = i < natom;
+  const uint32_t ii = valid ? i : natom - 1;
+  const float4 a = atoms[ii];
+  const float4 *Rc =
+      reinterpret_cast<const float4 *>(R + static_cast<size_t>(cell_sorted[ii]) * Kp);
+  double FX = 0.0, FY = 0.0, FZ = 0.0, EN = 0.0;
+  for (uint32_t kb = k0; kb < k1; kb += GLST_LR_FLUSH) {
+    float fx = 0.f, fy = 0.f, fz = 0.f, en = 0.f;
+#pragma unroll 2
+    for (uint32_t k = kb; k < kb + GLST_LR_FLUSH; k += 2) {
+      const float4 n0 = krev[k];
+      const float4 n1 = krev[k + 1];
+      const float4 r = Rc[k >> 1];  // R_k, R_{k+1}
+      const float t0 = fract_rev(n0.x * a.x + n0.y * a.y + n0.z * a.z);
+      const float t1 = fract_rev(n1.x * a.x + n1.y * a.y + n1.z * a.z);
+      const float s0 = sin_rev(t0), c0 = cos_rev(t0);
+      const float s1 = sin_rev(t1), c1 = cos_rev(t1);
+      const float A0 = c0 * r.x - s0 * r.y, B0 = c0 * r.y + s0 * r.x;
+      const float A1 = c1 * r.z - s1 * r.w, B1 = c1 * r.w + s1 * r.z;
+      en += A0 + A1;
+      fx += n0.x * B0 + n1.x * B1;
+      fy += n0.y * B0 + n1.y * B1;
+      fz += n0.z * B0 + n1.z * B1;
+    }
+    FX += fx;
+    FY += fy;
+    FZ += fz;
+    EN += en;
+  }
+  if (valid) {
+    const double qf = static_cast<double>(a.w) * kTwoPi;  // k = 2 pi k_rev
+    partial[static_cast<size_t>(split) * natom + i] =
+        make_double4(qf * FX, qf * FY, qf * FZ, static_cast<double>(a.w) * EN);
+  }
+}
+
+__global__ void lr_reduce_kernel(const double4 *__restrict__ partial,
+                                 uint32_t nsplit, uint32_t natom,
+                                 double *__restrict__ fx,
+                                 double *__restrict__ fy,
+                                 double *__restrict__ fz,
+                                 double *__restrict__ en) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= natom) return;
+  double4 s = partial[i];
+  for (uint32_t p = 1; p < nsplit; ++p) {
+    const double4 v = partial[static_cast<size_t>(p) * natom + i];
+    s.x += v.x;
+    s.y += v.y;
+    s.z += v.z;
+    s.w += v.w;
+  }
+  fx[i] += s.x;
+  fy[i] += s.y;
+  fz[i] += s.z;
+  en[i] += s.w;
+}
+
+// ---------------------------------------------------------------------------
+// near field: bare Coulomb over the cell and its 26 neighbours, clamped at the
+// grid edge (src/glst_force.cu:1170-1544).  One wave per cell, lanes = atoms.
+
+template <bool MIXED>
+__global__ __launch_bounds__(64) void sr_kernel(
+    const void *__restrict__ atoms_v, const uint32_t *__restrict__ start,
+    uint32_t nx, uint32_t ny, uint32_t nz, double h,
+    double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
+    double *__restrict__ en) {
+  const uint32_t cell = blockIdx.x;
+  const uint32_t ix = cell / (ny * nz), iy = (cell / nz) % ny, iz = cell % nz;
+  const uint32_t a0 = start[cell], a1 = start[cell + 1];
+  const float hf = static_cast<float>(h);
+  for (uint32_t base = a0; base < a1; base += 64) {
+    const uint32_t i = base + threadIdx.x;
+    const bool valid = i < a1;
+    const uint32_t ii = valid ? i : a0;
+    double FX = 0.0, FY = 0.0, FZ = 0.0, EN = 0.0;
+    if (MIXED) {
+      const float4 *atoms = static_cast<const float4 *>(atoms_v);
+      const float4 ai = atoms[ii];
+      for (int dx = -1; dx <= 1; ++dx) {
+        if ((ix == 0 && dx < 0) || (ix == nx - 1 && dx > 0)) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+          if ((iy == 0 && dy < 0) || (iy == ny - 1 && dy > 0)) continue;
+          for (int dz = -1; dz <= 1; ++dz) {
+            if ((iz == 0 && dz < 0) || (iz == nz - 1 && dz > 0)) continue;
+            const uint32_t nb = ((ix + dx) * ny + (iy + dy)) * nz + (iz + dz);
+            const uint32_t b0 = start[nb], b1 = start[nb + 1];
+            // r_i - r_j = d_i - d_j - (cell offset) * h
+            const float ox = ai.x - static_cast<float>(dx) * hf;
+            const float oy = ai.y - static_cast<float>(dy) * hf;
+            const float oz = ai.z - static_cast<float>(dz) * hf;
+            float fx0 = 0.f, fy0 = 0.f, fz0 = 0.f, en0 = 0.f;
+            for (uint32_t j = b0; j < b1; ++j) {
+              const float4 aj = atoms[j];
+              const float rx = ox - aj.x, ry = oy - aj.y, rz = oz - aj.z;
+              float r2 = rx * rx + ry * ry + rz * rz;
+              r2 = (j == ii) ? INFINITY : r2;  // no self interaction
+              const float ir = __builtin_amdgcn_rsqf(r2);
+              const float e = ai.w * aj.w * ir;
+              const float f = e * ir * ir;
+              fx0 += f * rx;
+              fy0 += f * ry;
+              fz0 += f * rz;
+              en0 += e;
+            }
+            FX += fx0;
+            FY += fy0;
+            FZ += fz0;
+            EN += en0;
+          }
+        }
+      }
+    } else {
+      const double4 *atoms = static_cast<const double4 *>(atoms_v);
+      const double4 ai = atoms[ii];
+      for (int dx = -1; dx <= 1; ++dx) {
+        if ((ix == 0 && dx < 0) || (ix == nx - 1 && dx > 0)) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+          if ((iy == 0 && dy < 0) || (iy == ny - 1 && dy > 0)) continue;
+          for (int dz = -1; dz <= 1; ++dz) {
+            if ((iz == 0 && dz < 0) || (iz == nz - 1 && dz > 0)) continue;
+            const uint32_t nb = ((ix + dx) * ny + (iy + dy)) * nz + (iz + dz);
+            const uint32_t b0 = start[nb], b1 = start[nb + 1];
+            for (uint32_t j = b0; j < b1; ++j) {
+              const double4 aj = atoms[j];
+              const double rx = ai.x - aj.x, ry = ai.y - aj.y, rz = ai.z - aj.z;
+              double r2 = rx * rx + ry * ry + rz * rz;
+              r2 = (j == ii) ? INFINITY : r2;
+              const double ir = 1.0 / sqrt(r2);
+              const double e = ai.w * aj.w * ir;
+              const double f = e * ir * ir;
+              FX += f * rx;
+              FY += f * ry;
+              FZ += f * rz;
+              EN += e;
+            }
+          }
+        }
+      }
+    }
+    (void)hf;
+    if (valid) {
+      fx[i] += FX;
+      fy[i] += FY;
+      fz[i] += FZ;
+      en[i] += EN;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+
+__global__ void unpermute_kernel(const uint32_t *__restrict__ sorted_idx,
+                                 uint32_t natom, const double *__restrict__ fx,
+                                 const double *__restrict__ fy,
+                                 const double *__restrict__ fz,
+                                 const double *__restrict__ en,
+                                 double *__restrict__ ofx,
+                                 double *__restrict__ ofy,
+                                 double *__restrict__ ofz,
+                                 double *__restrict__ oen) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= natom) return;
+  const uint32_t o = sorted_idx[i];
+  ofx[o] = fx[i];
+  ofy[o] = fy[i];
+  ofz[o] = fz[i];
+  oen[o] = en[i];
+}
+
+constexpr uint32_t kCoulBlock = 256;
+
+__global__ __launch_bounds__(kCoulBlock) void coulomb_kernel(
+    const double *__restrict__ rx, const double *__restrict__ ry,
+    const double *__restrict__ rz, const double *__restrict__ qc,
+    uint32_t natom, double *__restrict__ fx, double *__restrict__ fy,
+    double *__restrict__ fz, double *__restrict__ en) {
+  __shared__ double4 tile[kCoulBlock];
+  const uint32_t i = blockIdx.x * kCoulBlock + threadIdx.x;
+  const uint32_t ii = i < natom ? i : 0;
+  const double xi = rx[ii], yi = ry[ii], zi = rz[ii], qi = qc[ii];
+  double FX = 0.0, FY = 0.0, FZ = 0.0, EN = 0.0;
+  for (uint32_t j0 = 0; j0 < natom; j0 += kCoulBlock) {
+    __syncthreads();
+    const uint32_t j = j0 + threadIdx.x;
+    if (j < natom) tile[threadIdx.x] = make_double4(rx[j], ry[j], rz[j], qc[j]);
+    __syncthreads();
+    const uint32_t n = (natom - j0 < kCoulBlock) ? natom - j0 : kCoulBlock;
+    for (uint32_t t = 0; t < n; ++t) {
+      const double4 b = tile[t];
+      const double dx = xi - b.x, dy = yi - b.y, dz = zi - b.z;
+      double r2 = dx * dx + dy * dy + dz * dz;
+      r2 = (j0 + t == i) ? INFINITY : r2;
+      const double ir = 1.0 / sqrt(r2);
+      const double e = qi * b.w * ir;
+      const double f = e * ir * ir;
+      FX += f * dx;
+      FY += f * dy;
+      FZ += f * dz;
+      EN += e;
+    }
+  }
+  if (i < natom) {
+    fx[i] = FX;
+    fy[i] = FY;
+    fz[i] = FZ;
+    en[i] = EN;
+  }
+}
+
+inline uint32_t blocks_for(size_t n, uint32_t b) {
+  return static_cast<uint32_t>((n + b - 1) / b);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+
+void fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  fill_u32_kernel<<<blocks_for(n, 256), 256, 0, s>>>(p, value, n);
+}
+
+void fill_f64(double *p, double value, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  fill_f64_kernel<<<blocks_for(n, 256), 256, 0, s>>>(p, value, n);
+}
+
+void bin_chunks(bool ct_float, const double *rx, const double *ry,
+                const double *rz, uint32_t natom, double inv_cell,
+                const GridShape &g, uint32_t *atom_cell, uint32_t *rank,
+                uint32_t *hist, hipStream_t s) {
+  const uint32_t nchunk = blocks_for(natom, kBinChunk);
+  if (nchunk == 0) return;
+  if (ct_float)
+    bin_chunk_kernel<float><<<nchunk, kBinChunk, 0, s>>>(
+        rx, ry, rz, natom, static_cast<float>(inv_cell), g.nx, g.ny, g.nz,
+        g.ncell, atom_cell, rank, hist);
+  else
+    bin_chunk_kernel<double><<<nchunk, kBinChunk, 0, s>>>(
+        rx, ry, rz, natom, inv_cell, g.nx, g.ny, g.nz, g.ncell, atom_cell,
+        rank, hist);
+}
+
+void hist_offsets(uint32_t *hist, uint32_t nchunk, uint32_t ncell,
+                  uint32_t *count, hipStream_t s) {
+  hist_offsets_kernel<<<blocks_for(ncell, 256), 256, 0, s>>>(hist, nchunk,
+                                                             ncell, count);
+}
+
+void cell_scan(const uint32_t *count, uint32_t ncell, uint32_t *start,
+               uint32_t *maxc, hipStream_t s) {
+  cell_scan_kernel<<<1, 1024, 0, s>>>(count, ncell, start, maxc);
+}
+
+void bin_scatter(bool mixed, const double *rx, const double *ry,
+                 const double *rz, const double *qc, uint32_t natom,
+                 const uint32_t *atom_cell, const uint32_t *rank,
+                 const uint32_t *hist, const uint32_t *start,
+                 const GridShape &g, uint32_t *sorted_idx,
+                 uint32_t *cell_sorted, void *atoms, hipStream_t s) {
+  if (natom == 0) return;
+  if (mixed)
+    bin_scatter_kernel<true><<<blocks_for(natom, 256), 256, 0, s>>>(
+        rx, ry, rz, qc, natom, atom_cell, rank, hist, start, g.ncell, g.ny,
+        g.nz, g.h, sorted_idx, cell_sorted, atoms);
+  else
+    bin_scatter_kernel<false><<<blocks_for(natom, 256), 256, 0, s>>>(
+        rx, ry, rz, qc, natom, atom_cell, rank, hist, start, g.ncell, g.ny,
+        g.nz, g.h, sorted_idx, cell_sorted, atoms);
+}
+
+void sf(bool mixed, const void *atoms, const uint32_t *start,
+        const double4 *k64, const float4 *krev, uint32_t Kp,
+        const GridShape &g, double2 *S, hipStream_t s) {
+  const dim3 grid(g.nx * g.ny, Kp / kSfBlock);
+  if (mixed)
+    sf_mixed_kernel<<<grid, kSfBlock, 0, s>>>(
+        static_cast<const float4 *>(atoms), start, k64, krev, Kp, g.ny, g.nz,
+        g.h, S);
+  else
+    sf_f64_kernel<<<grid, kSfBlock, 0, s>>>(static_cast<const double4 *>(atoms),
+                                            start, k64, Kp, g.nz, S);
+}
+
+bool rmt_lds(bool mixed, const double2 *S, const double4 *k64,
+             const uint32_t *group, const uint32_t *r_in,
+             const uint32_t *r_out, uint32_t Kp, const GridShape &g,
+             size_t lds_limit, void *R, hipStream_t s) {
+  const size_t per_node = static_cast<size_t>(g.ncell) * sizeof(double2);
+  uint32_t T = 0;
+  for (uint32_t t : {2u, 1u})
+    if (t * per_node <= lds_limit) {
+      T = t;
+      break;
+    }
+  if (T == 0) return false;
+  const uint32_t ntile = Kp / T;
+  const size_t lds = T * per_node;
+  if (mixed) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&rmt_lds_kernel<true>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                        static_cast<int>(lds));
+    rmt_lds_kernel<true><<<ntile, 256, lds, s>>>(S, k64, group, r_in, r_out,
+                                                 Kp, T, ntile, g.nx, g.ny,
+                                                 g.nz, g.h, R);
+  } else {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&rmt_lds_kernel<false>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                        static_cast<int>(lds));
+    rmt_lds_kernel<false><<<ntile, 256, lds, s>>>(S, k64, group, r_in, r_out,
+                                                  Kp, T, ntile, g.nx, g.ny,
+                                                  g.nz, g.h, R);
+  }
+  return true;
+}
+
+void rmt_global(bool mixed, double2 *S, const double4 *k64,
+                const uint32_t *group, const uint32_t *r_in,
+                const uint32_t *r_out, uint32_t Kp, const GridShape &g,
+                void *R, hipStream_t s) {
+  const uint32_t bx = blocks_for(Kp, 256);
+  constexpr uint32_t kMaxY = 65535;  // grid.y limit
+  auto prefix = [&](uint32_t nlines, uint32_t n, uint32_t stride,
+                    uint32_t inner) {
+    for (uint32_t l0 = 0; l0 < nlines; l0 += kMaxY) {
+      const uint32_t cnt = (nlines - l0 < kMaxY) ? nlines - l0 : kMaxY;
+      prefix_axis_kernel<<<dim3(bx, cnt), 256, 0, s>>>(S, Kp, l0, nlines, n,
+                                                       stride, inner);
+    }
+  };
+  prefix(g.nx * g.ny, g.nz, 1, 1);            // z lines: base (x*ny+y)*nz
+  prefix(g.nx * g.nz, g.ny, g.nz, g.nz);      // y lines: base x*ny*nz + z
+  prefix(g.ny * g.nz, g.nx, g.ny * g.nz, g.ny * g.nz);  // x lines: y*nz + z
+  for (uint32_t c0 = 0; c0 < g.ncell; c0 += kMaxY) {
+    const uint32_t cnt = (g.ncell - c0 < kMaxY) ? g.ncell - c0 : kMaxY;
+    if (mixed)
+      rmt_gather_kernel<true><<<dim3(bx, cnt), 256, 0, s>>>(
+          S, k64, group, r_in, r_out, Kp, c0, g.nx, g.ny, g.nz, g.h, R);
+    else
+      rmt_gather_kernel<false><<<dim3(bx, cnt), 256, 0, s>>>(
+          S, k64, group, r_in, r_out, Kp, c0, g.nx, g.ny, g.nz, g.h, R);
+  }
+}
+
+void lr(bool mixed, const void *atoms, const uint32_t *cell_sorted,
+        const double4 *k64, const float4 *krev, const void *R, uint32_t Kp,
+        uint32_t natom, uint32_t nsplit, double4 *partial, hipStream_t s) {
+  if (natom == 0) return;
+  const uint32_t ntile = blocks_for(natom, kLrBlock);
+  // split the node range into nsplit chunks of whole 64-node groups
+  const uint32_t groups = Kp / 64;
+  const uint32_t kchunk = ((groups + nsplit - 1) / nsplit) * 64;
+  const dim3 grid(ntile, nsplit);
+  if (mixed)
+    lr_mixed_kernel<<<grid, kLrBlock, 0, s>>>(
+        static_cast<const float4 *>(atoms), cell_sorted, krev,
+        static_cast<const float2 *>(R), Kp, natom, kchunk, ntile, partial);
+  else
+    lr_f64_kernel<<<grid, kLrBlock, 0, s>>>(
+        static_cast<const double4 *>(atoms), cell_sorted, k64,
+        static_cast<const double2 *>(R), Kp, natom, kchunk, ntile, partial);
+}
+
+void lr_reduce(const double4 *partial, uint32_t nsplit, uint32_t natom,
+               double *fx, double *fy, double *fz, double *en, hipStream_t s) {
+  if (natom == 0) return;
+  lr_reduce_kernel<<<blocks_for(natom, 256), 256, 0, s>>>(partial, nsplit,
+                                                         natom, fx, fy, fz, en);
+}
+
+void sr(bool mixed, const void *atoms, const uint32_t *start,
+        const GridShape &g, double *fx, double *fy, double *fz, double *en,
+        hipStream_t s) {
+  if (mixed)
+    sr_kernel<true><<<g.ncell, 64, 0, s>>>(atoms, start, g.nx, g.ny, g.nz, g.h,
+                                           fx, fy, fz, en);
+  else
+    sr_kernel<false><<<g.ncell, 64, 0, s>>>(atoms, start, g.nx, g.ny, g.nz,
+                                            g.h, fx, fy, fz, en);
+}
+
+void unpermute(const uint32_t *sorted_idx, uint32_t natom, const double *fx,
+               const double *fy, const double *fz, const double *en,
+               double *ofx, double *ofy, double *ofz, double *oen,
+               hipStream_t s) {
+  if (natom == 0) return;
+  unpermute_kernel<<<blocks_for(natom, 256), 256, 0, s>>>(
+      sorted_idx, natom, fx, fy, fz, en, ofx, ofy, ofz, oen);
+}
+
+void coulomb_direct(const double *rx, const double *ry, const double *rz,
+                    const double *qc, uint32_t natom, double *fx, double *fy,
+                    double *fz, double *en, hipStream_t s) {
+  if (natom == 0) return;
+  coulomb_kernel<<<blocks_for(natom, kCoulBlock), kCoulBlock, 0, s>>>(
+      rx, ry, rz, qc, natom, fx, fy, fz, en);
+}
+
+}  // namespace dev
+}  // namespace glst

@@ -1,0 +1,96 @@
+// Launchers for the GLST HIP kernels (gfx950).  Every launcher enqueues work
+// on `stream` and returns immediately; shapes are validated by the caller
+// (engine.hip) before any launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace glst {
+namespace dev {
+
+// Atoms per workgroup of the chunked stable counting sort.
+constexpr uint32_t kBinChunk = 1024;
+
+struct GridShape {
+  uint32_t nx, ny, nz, ncell;
+  double h;  // cell edge (double of the CT value)
+};
+
+// ---- binning and stable counting sort (assign_atoms) ----------------------
+void fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s);
+void fill_f64(double *p, double value, size_t n, hipStream_t s);
+
+// Cell of every atom (CT arithmetic of src/glst_force.cu:356-386), the
+// stable rank of each atom among the atoms of its chunk with the same cell,
+// and the per-chunk cell histogram hist[chunk][cell] (must be zeroed).
+void bin_chunks(bool ct_float, const double *rx, const double *ry,
+                const double *rz, uint32_t natom, double inv_cell,
+                const GridShape &g, uint32_t *atom_cell, uint32_t *rank,
+                uint32_t *hist, hipStream_t s);
+// hist[chunk][cell] -> exclusive offsets over chunks; count[cell] = total.
+void hist_offsets(uint32_t *hist, uint32_t nchunk, uint32_t ncell,
+                  uint32_t *count, hipStream_t s);
+// start[0..ncell] = exclusive scan of count; *maxc = max count.
+void cell_scan(const uint32_t *count, uint32_t ncell, uint32_t *start,
+               uint32_t *maxc, hipStream_t s);
+// Scatter atoms to cell-sorted order; writes sorted_idx, the sorted atoms'
+// cells and the sorted atom records (double4 absolute x,y,z,q for FP64;
+// float4 cell-local x,y,z,q for the mixed path).
+void bin_scatter(bool mixed, const double *rx, const double *ry,
+                 const double *rz, const double *qc, uint32_t natom,
+                 const uint32_t *atom_cell, const uint32_t *rank,
+                 const uint32_t *hist, const uint32_t *start,
+                 const GridShape &g, uint32_t *sorted_idx,
+                 uint32_t *cell_sorted, void *atoms, hipStream_t s);
+
+// ---- structure factors (calc_sf) -------------------------------------------
+// S[cell*K + k] = sum_{j in cell} q_j exp(-i k.r_j), FP64 complex.
+void sf(bool mixed, const void *atoms, const uint32_t *start,
+        const double4 *k64, const float4 *krev, uint32_t K,
+        const GridShape &g, double2 *S, hipStream_t s);
+
+// ---- remote (shell) sums (sum_rmt_sf) --------------------------------------
+// R_c(k) = W_k * sum_{c' : r_in < cheb(c,c') <= r_out} S_c'(k); the mixed
+// path stores exp(+i k.center_c) R_c(k) as float2, the FP64 path R_c(k) as
+// double2.  Returns false when the grid is too large for the LDS-resident
+// kernel (the caller then uses rmt_global).
+bool rmt_lds(bool mixed, const double2 *S, const double4 *k64,
+             const uint32_t *group, const uint32_t *r_in,
+             const uint32_t *r_out, uint32_t K, const GridShape &g,
+             size_t lds_limit, void *R, hipStream_t s);
+// Global-memory fallback: in-place 3-D prefix sums of S, then the 16-read
+// shell gather.  Destroys S.
+void rmt_global(bool mixed, double2 *S, const double4 *k64,
+                const uint32_t *group, const uint32_t *r_in,
+                const uint32_t *r_out, uint32_t K, const GridShape &g,
+                void *R, hipStream_t s);
+
+// ---- long-range back-projection (calc_lr_ef) -------------------------------
+// partial[split*natom + i] = (fx, fy, fz, en) of sorted atom i over the
+// split's node range.
+void lr(bool mixed, const void *atoms, const uint32_t *cell_sorted,
+        const double4 *k64, const float4 *krev, const void *R, uint32_t K,
+        uint32_t natom, uint32_t nsplit, double4 *partial, hipStream_t s);
+// f[i] += sum over splits (fixed order).
+void lr_reduce(const double4 *partial, uint32_t nsplit, uint32_t natom,
+               double *fx, double *fy, double *fz, double *en, hipStream_t s);
+
+// ---- near field (calc_sr_ef) -----------------------------------------------
+void sr(bool mixed, const void *atoms, const uint32_t *start,
+        const GridShape &g, double *fx, double *fy, double *fz, double *en,
+        hipStream_t s);
+
+// ---- result order (get_ef) ---------------------------------------------------
+void unpermute(const uint32_t *sorted_idx, uint32_t natom, const double *fx,
+               const double *fy, const double *fz, const double *en,
+               double *ofx, double *ofy, double *ofz, double *oen,
+               hipStream_t s);
+
+// ---- direct all-pairs Coulomb (utils/coulomb.cu) -----------------------------
+void coulomb_direct(const double *rx, const double *ry, const double *rz,
+                    const double *qc, uint32_t natom, double *fx, double *fy,
+                    double *fz, double *en, hipStream_t s);
+
+}  // namespace dev
+}  // namespace glst
