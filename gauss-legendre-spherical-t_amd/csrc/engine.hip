@@ -209,7 +209,7 @@ struct glst_engine {
 
   // atoms
   DeviceBuffer atom_cell, rank, hist, cell_count, cell_start, maxc;
-  DeviceBuffer sorted_idx, cell_sorted, atoms;
+  DeviceBuffer sorted_idx, cell_sorted, atoms64, atoms32;
   DeviceBuffer fx, fy, fz, en;
   // nodes
   DeviceBuffer k64, krev, group, r_in, r_out;
@@ -227,10 +227,10 @@ struct glst_engine {
   size_t device_bytes() const {
     const DeviceBuffer *bufs[] = {&atom_cell, &rank,  &hist,       &cell_count,
                                   &cell_start, &maxc, &sorted_idx, &cell_sorted,
-                                  &atoms,     &fx,    &fy,         &fz,
+                                  &atoms64,   &fx,    &fy,         &fz,
                                   &en,        &k64,   &krev,       &group,
                                   &r_in,      &r_out, &S,          &R,
-                                  &partial};
+                                  &partial,   &atoms32};
     size_t s = 0;
     for (auto b : bufs) s += b->bytes();
     return s;
@@ -313,7 +313,8 @@ struct glst_engine {
     maxc.resize(4);
     sorted_idx.resize(n * 4);
     cell_sorted.resize(n * 4);
-    atoms.resize(n * (mixed() ? sizeof(float4) : sizeof(double4)));
+    atoms64.resize(n * sizeof(double4));
+    atoms32.resize(mixed() ? n * sizeof(float4) : 0);
     fx.resize(n * 8);
     fy.resize(n * 8);
     fz.resize(n * 8);
@@ -363,6 +364,11 @@ struct glst_engine {
                        groups.r_out.size() * 4, hipMemcpyHostToDevice));
   }
 
+  // atom records the long-range kernels read
+  const void *lr_atoms() const {
+    return mixed() ? atoms32.as<void>() : atoms64.as<void>();
+  }
+
   // ---- the step ------------------------------------------------------------
 
   void assign_atoms(const double *rx, const double *ry, const double *rz,
@@ -389,7 +395,7 @@ struct glst_engine {
                        atom_cell.as<uint32_t>(), rank.as<uint32_t>(),
                        hist.as<uint32_t>(), cell_start.as<uint32_t>(), grid,
                        sorted_idx.as<uint32_t>(), cell_sorted.as<uint32_t>(),
-                       atoms.as<void>(), stream);
+                       atoms64.as<double4>(), atoms32.as<float4>(), stream);
       check_launch("sort");
     }
     assigned = true;
@@ -400,7 +406,7 @@ struct glst_engine {
     if (!has_lr || Kp == 0) return;  // src/glst_force.cu:667-670
     if (!assigned) throw StateError("calc_sf called before assign_atoms");
     Scope sc(timer, "sf", stream);
-    dev::sf(mixed(), atoms.as<void>(), cell_start.as<uint32_t>(),
+    dev::sf(mixed(), lr_atoms(), cell_start.as<uint32_t>(),
             k64.as<double4>(), krev.as<float4>(), Kp, grid, S.as<double2>(),
             stream);
     check_launch("sf");
@@ -430,7 +436,7 @@ struct glst_engine {
     if (!rmt_done) throw StateError("calc_lr_ef called before sum_rmt_sf");
     {
       Scope sc(timer, "lr", stream);
-      dev::lr(mixed(), atoms.as<void>(), cell_sorted.as<uint32_t>(),
+      dev::lr(mixed(), lr_atoms(), cell_sorted.as<uint32_t>(),
               k64.as<double4>(), krev.as<float4>(), R.as<void>(), Kp, natom,
               nsplit, partial.as<double4>(), stream);
       check_launch("lr");
@@ -448,7 +454,7 @@ struct glst_engine {
     if (natom == 0) return;
     if (!assigned) throw StateError("calc_sr_ef called before assign_atoms");
     Scope sc(timer, "sr", stream);
-    dev::sr(mixed(), atoms.as<void>(), cell_start.as<uint32_t>(), grid,
+    dev::sr(mixed(), atoms64.as<double4>(), cell_start.as<uint32_t>(), grid,
             fx.as<double>(), fy.as<double>(), fz.as<double>(), en.as<double>(),
             stream);
     check_launch("sr");
@@ -803,7 +809,21 @@ glst_status_t glst_get_sf(glst_handle_t h, double *re, double *im) {
   return guarded(h, [&] {
     if (!h->has_lr || h->K == 0) return;
     if (!h->sf_done) throw StateError("glst_get_sf: no structure factors");
-    h->field_to_host(h->S, false, re, im);
+    const size_t n = static_cast<size_t>(h->grid.ncell) * h->K;
+    std::vector<double> r(n), i(n);
+    h->field_to_host(h->S, false, r.data(), i.data());
+    // the field holds running sums along z; difference them back
+    const auto &g = h->grid;
+    for (uint32_t col = 0; col < g.nx * g.ny; ++col)
+      for (uint32_t z = g.nz - 1; z >= 1; --z) {
+        const size_t c = static_cast<size_t>(col) * g.nz + z;
+        for (uint32_t k = 0; k < h->K; ++k) {
+          r[c * h->K + k] -= r[(c - 1) * h->K + k];
+          i[c * h->K + k] -= i[(c - 1) * h->K + k];
+        }
+      }
+    if (re) std::memcpy(re, r.data(), n * 8);
+    if (im) std::memcpy(im, i.data(), n * 8);
   });
 }
 

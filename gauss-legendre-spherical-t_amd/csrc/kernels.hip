@@ -201,7 +201,8 @@ __global__ void bin_scatter_kernel(
     const uint32_t *__restrict__ rank, const uint32_t *__restrict__ hist,
     const uint32_t *__restrict__ start, uint32_t ncell, uint32_t ny,
     uint32_t nz, double h, uint32_t *__restrict__ sorted_idx,
-    uint32_t *__restrict__ cell_sorted, void *__restrict__ atoms) {
+    uint32_t *__restrict__ cell_sorted, double4 *__restrict__ atoms64,
+    float4 *__restrict__ atoms32) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= natom) return;
   const uint32_t c = atom_cell[i];
@@ -210,6 +211,8 @@ __global__ void bin_scatter_kernel(
       start[c] + hist[static_cast<size_t>(ch) * ncell + c] + rank[i];
   sorted_idx[dst] = i;
   cell_sorted[dst] = c;
+  const double x = rx[i], y = ry[i], z = rz[i], q = qc[i];
+  atoms64[dst] = make_double4(x, y, z, q);
   if (MIXED) {
     // Coordinates relative to the cell centre, rounded once from FP64: the
     // large part of every phase k.r is carried by the FP64 cell centre.
@@ -217,11 +220,8 @@ __global__ void bin_scatter_kernel(
     const double cx = (static_cast<double>(ix) + 0.5) * h;
     const double cy = (static_cast<double>(iy) + 0.5) * h;
     const double cz = (static_cast<double>(iz) + 0.5) * h;
-    static_cast<float4 *>(atoms)[dst] =
-        make_float4(static_cast<float>(rx[i] - cx), static_cast<float>(ry[i] - cy),
-                    static_cast<float>(rz[i] - cz), static_cast<float>(qc[i]));
-  } else {
-    static_cast<double4 *>(atoms)[dst] = make_double4(rx[i], ry[i], rz[i], qc[i]);
+    atoms32[dst] = make_float4(static_cast<float>(x - cx), static_cast<float>(y - cy),
+                               static_cast<float>(z - cz), static_cast<float>(q));
   }
 }
 
@@ -229,6 +229,10 @@ __global__ void bin_scatter_kernel(
 // structure factors
 
 constexpr uint32_t kSfBlock = 256;
+
+// Both structure-factor kernels walk one z-column of cells per lane and store
+// the running sum along z, i.e. the first pass of the summed-volume table of
+// src/glst_force.cu:702-724, so the shell-sum kernel starts at the y pass.
 
 // FP64: S_c(k) = sum_j q_j (cos(k.r_j) - i sin(k.r_j)), absolute coordinates
 // (src/glst_force.cu:601-664).  blockIdx.x = z-column, blockIdx.y = node tile.
@@ -239,6 +243,7 @@ __global__ __launch_bounds__(kSfBlock) void sf_f64_kernel(
   const uint32_t col = blockIdx.x;
   const uint32_t node = blockIdx.y * kSfBlock + threadIdx.x;
   const double4 k = k64[node];
+  double pre = 0.0, pim = 0.0;  // running z-prefix
   for (uint32_t z = 0; z < nz; ++z) {
     const uint32_t cell = col * nz + z;
     const uint32_t a0 = start[cell], a1 = start[cell + 1];
@@ -262,7 +267,9 @@ __global__ __launch_bounds__(kSfBlock) void sf_f64_kernel(
       re0 += a.w * ca;
       im0 -= a.w * sa;
     }
-    S[static_cast<size_t>(cell) * Kp + node] = make_double2(re0 + re1, im0 + im1);
+    pre += re0 + re1;
+    pim += im0 + im1;
+    S[static_cast<size_t>(cell) * Kp + node] = make_double2(pre, pim);
   }
 }
 
@@ -286,6 +293,7 @@ __global__ __launch_bounds__(kSfBlock) void sf_mixed_kernel(
   double ps, pc, ss, sc;
   sincos(kd.x * cx + kd.y * cy + kd.z * (0.5 * h), &ps, &pc);
   sincos(kd.z * h, &ss, &sc);
+  double pre = 0.0, pim = 0.0;  // running z-prefix
   for (uint32_t z = 0; z < nz; ++z) {
     const uint32_t cell = col * nz + z;
     const uint32_t a0 = start[cell], a1 = start[cell + 1];
@@ -314,8 +322,9 @@ __global__ __launch_bounds__(kSfBlock) void sf_mixed_kernel(
       IM += static_cast<double>(im0) + static_cast<double>(im1);
     }
     // (RE + i IM) * (pc - i ps)
-    S[static_cast<size_t>(cell) * Kp + node] =
-        make_double2(RE * pc + IM * ps, IM * pc - RE * ps);
+    pre += RE * pc + IM * ps;
+    pim += IM * pc - RE * ps;
+    S[static_cast<size_t>(cell) * Kp + node] = make_double2(pre, pim);
     const double npc = pc * sc - ps * ss;
     const double nps = ps * sc + pc * ss;
     pc = npc;
@@ -379,61 +388,75 @@ __device__ inline double2 cube_sum(const G &P, uint32_t x, uint32_t y,
   return box_sum(P, x0, y0, z0, x1, y1, z1, ny, nz);
 }
 
+// R_c(k) = W_k (cube_out - cube_in); the mixed path multiplies by
+// exp(+i k.c) = a b c (the cell's three FP64 axis factors) and stores FP32,
+// the FP64 path stores R_c(k).
 template <bool MIXED, class G>
-__device__ inline void shell_store(const G &P, uint32_t cell, uint32_t node,
-                                   const double4 &k, uint32_t rin,
-                                   uint32_t rout, uint32_t nx, uint32_t ny,
-                                   uint32_t nz, double h, uint32_t Kp,
+__device__ inline void shell_store(const G &P, uint32_t x, uint32_t y,
+                                   uint32_t z, uint32_t node, double w,
+                                   uint32_t rin, uint32_t rout, uint32_t nx,
+                                   uint32_t ny, uint32_t nz, double2 a,
+                                   double2 b, double2 c, uint32_t Kp,
                                    void *R) {
-  const uint32_t x = cell / (ny * nz), y = (cell / nz) % ny, z = cell % nz;
   const double2 o = cube_sum(P, x, y, z, rout, nx, ny, nz);
   const double2 in = cube_sum(P, x, y, z, rin, nx, ny, nz);
-  const double re = (o.x - in.x) * k.w, im = (o.y - in.y) * k.w;
-  const size_t off = static_cast<size_t>(cell) * Kp + node;
+  const double re = (o.x - in.x) * w, im = (o.y - in.y) * w;
+  const size_t off = static_cast<size_t>((x * ny + y) * nz + z) * Kp + node;
   if (MIXED) {
-    // exp(+i k.c) R: the FP32 back-projection then needs only cell-local
-    // phases.
-    const double th = k.x * ((static_cast<double>(x) + 0.5) * h) +
-                      k.y * ((static_cast<double>(y) + 0.5) * h) +
-                      k.z * ((static_cast<double>(z) + 0.5) * h);
-    double s, c;
-    sincos(th, &s, &c);
+    const double pr = a.x * b.x - a.y * b.y, pi = a.x * b.y + a.y * b.x;
+    const double cr = pr * c.x - pi * c.y, ci = pr * c.y + pi * c.x;
     static_cast<float2 *>(R)[off] = make_float2(
-        static_cast<float>(re * c - im * s), static_cast<float>(re * s + im * c));
+        static_cast<float>(re * cr - im * ci), static_cast<float>(re * ci + im * cr));
   } else {
     static_cast<double2 *>(R)[off] = make_double2(re, im);
   }
 }
 
+// exp(+i k_d (i + 1/2) h) for i < n (one axis of the cell-centre phase).
+__device__ inline void axis_phase(double2 *dst, double kd, double h,
+                                  uint32_t i) {
+  double s, c;
+  sincos(kd * ((static_cast<double>(i) + 0.5) * h), &s, &c);
+  *dst = make_double2(c, s);
+}
+
+constexpr uint32_t kRmtBlock = 1024;
+
+// One workgroup holds the z-prefixed table of T nodes in LDS, completes the
+// summed-volume table along y and x, and gathers the shells.
 template <bool MIXED>
-__global__ __launch_bounds__(256) void rmt_lds_kernel(
+__global__ __launch_bounds__(kRmtBlock) void rmt_lds_kernel(
     const double2 *__restrict__ S, const double4 *__restrict__ k64,
     const uint32_t *__restrict__ group, const uint32_t *__restrict__ r_in,
     const uint32_t *__restrict__ r_out, uint32_t Kp, uint32_t T,
     uint32_t ntile, uint32_t nx, uint32_t ny, uint32_t nz, double h,
     void *__restrict__ R) {
-  extern __shared__ double2 P[];  // [T][ncell]
+  extern __shared__ double2 P[];  // [T][ncell], then [T][nx+ny+nz] phases
   const uint32_t ncell = nx * ny * nz;
+  const uint32_t nax = nx + ny + nz;
+  double2 *phase = P + T * ncell;
   const uint32_t tile = xcd_remap(blockIdx.x, ntile);
   const uint32_t node0 = tile * T;
   const uint32_t tid = threadIdx.x;
-  for (uint32_t e = tid; e < T * ncell; e += blockDim.x) {
+  for (uint32_t e = tid; e < T * ncell; e += kRmtBlock) {
     const uint32_t t = e % T, c = e / T;
     P[t * ncell + c] = S[static_cast<size_t>(c) * Kp + node0 + t];
   }
-  __syncthreads();
-  // inclusive prefix along z, then y, then x (src/glst_force.cu:702-772)
-  for (uint32_t l = tid; l < T * nx * ny; l += blockDim.x) {
-    double2 *p = P + (l / (nx * ny)) * ncell + (l % (nx * ny)) * nz;
-    double2 run = make_double2(0.0, 0.0);
-    for (uint32_t z = 0; z < nz; ++z) {
-      run.x += p[z].x;
-      run.y += p[z].y;
-      p[z] = run;
+  if (MIXED) {
+    for (uint32_t e = tid; e < T * nax; e += kRmtBlock) {
+      const uint32_t t = e / nax, a = e % nax;
+      const double4 k = k64[node0 + t];
+      if (a < nx)
+        axis_phase(phase + e, k.x, h, a);
+      else if (a < nx + ny)
+        axis_phase(phase + e, k.y, h, a - nx);
+      else
+        axis_phase(phase + e, k.z, h, a - nx - ny);
     }
   }
   __syncthreads();
-  for (uint32_t l = tid; l < T * nx * nz; l += blockDim.x) {
+  // inclusive prefix along y, then x (src/glst_force.cu:726-772)
+  for (uint32_t l = tid; l < T * nx * nz; l += kRmtBlock) {
     const uint32_t t = l / (nx * nz), xz = l % (nx * nz);
     double2 *p = P + t * ncell + (xz / nz) * ny * nz + (xz % nz);
     double2 run = make_double2(0.0, 0.0);
@@ -444,7 +467,7 @@ __global__ __launch_bounds__(256) void rmt_lds_kernel(
     }
   }
   __syncthreads();
-  for (uint32_t l = tid; l < T * ny * nz; l += blockDim.x) {
+  for (uint32_t l = tid; l < T * ny * nz; l += kRmtBlock) {
     const uint32_t t = l / (ny * nz), yz = l % (ny * nz);
     double2 *p = P + t * ncell + yz;
     double2 run = make_double2(0.0, 0.0);
@@ -455,13 +478,18 @@ __global__ __launch_bounds__(256) void rmt_lds_kernel(
     }
   }
   __syncthreads();
-  for (uint32_t e = tid; e < T * ncell; e += blockDim.x) {
+  for (uint32_t e = tid; e < T * ncell; e += kRmtBlock) {
     const uint32_t t = e % T, c = e / T;
     const uint32_t node = node0 + t;
     const uint32_t g = group[node];
+    const uint32_t x = c / (ny * nz), y = (c / nz) % ny, z = c % nz;
+    const double2 *ph = phase + t * nax;
+    const double2 one = make_double2(1.0, 0.0);
     LdsGrid grid{P + t * ncell};
-    shell_store<MIXED>(grid, c, node, k64[node], r_in[g], r_out[g], nx, ny,
-                       nz, h, Kp, R);
+    shell_store<MIXED>(grid, x, y, z, node, k64[node].w, r_in[g], r_out[g],
+                       nx, ny, nz, MIXED ? ph[x] : one,
+                       MIXED ? ph[nx + y] : one, MIXED ? ph[nx + ny + z] : one,
+                       Kp, R);
   }
 }
 
@@ -497,9 +525,17 @@ __global__ void rmt_gather_kernel(const double2 *__restrict__ P,
   const uint32_t cell = cell0 + blockIdx.y;
   if (node >= Kp || cell >= nx * ny * nz) return;
   const uint32_t g = group[node];
+  const double4 k = k64[node];
+  const uint32_t x = cell / (ny * nz), y = (cell / nz) % ny, z = cell % nz;
+  double2 a = make_double2(1.0, 0.0), b = a, c = a;
+  if (MIXED) {
+    axis_phase(&a, k.x, h, x);
+    axis_phase(&b, k.y, h, y);
+    axis_phase(&c, k.z, h, z);
+  }
   GlobalGrid grid{P, Kp, node};
-  shell_store<MIXED>(grid, cell, node, k64[node], r_in[g], r_out[g], nx, ny,
-                     nz, h, Kp, R);
+  shell_store<MIXED>(grid, x, y, z, node, k.w, r_in[g], r_out[g], nx, ny, nz,
+                     a, b, c, Kp, R);
 }
 
 // ---------------------------------------------------------------------------
@@ -617,42 +653,42 @@ __global__ void lr_reduce_kernel(const double4 *__restrict__ partial,
 
 template <bool MIXED>
 __global__ __launch_bounds__(64) void sr_kernel(
-    const void *__restrict__ atoms_v, const uint32_t *__restrict__ start,
-    uint32_t nx, uint32_t ny, uint32_t nz, double h,
-    double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
+    const double4 *__restrict__ atoms, const uint32_t *__restrict__ start,
+    uint32_t nx, uint32_t ny, uint32_t nz, double *__restrict__ fx,
+    double *__restrict__ fy, double *__restrict__ fz,
     double *__restrict__ en) {
   const uint32_t cell = blockIdx.x;
   const uint32_t ix = cell / (ny * nz), iy = (cell / nz) % ny, iz = cell % nz;
   const uint32_t a0 = start[cell], a1 = start[cell + 1];
-  const float hf = static_cast<float>(h);
   for (uint32_t base = a0; base < a1; base += 64) {
     const uint32_t i = base + threadIdx.x;
     const bool valid = i < a1;
     const uint32_t ii = valid ? i : a0;
+    const double4 ai = atoms[ii];
     double FX = 0.0, FY = 0.0, FZ = 0.0, EN = 0.0;
-    if (MIXED) {
-      const float4 *atoms = static_cast<const float4 *>(atoms_v);
-      const float4 ai = atoms[ii];
-      for (int dx = -1; dx <= 1; ++dx) {
-        if ((ix == 0 && dx < 0) || (ix == nx - 1 && dx > 0)) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-          if ((iy == 0 && dy < 0) || (iy == ny - 1 && dy > 0)) continue;
-          for (int dz = -1; dz <= 1; ++dz) {
-            if ((iz == 0 && dz < 0) || (iz == nz - 1 && dz > 0)) continue;
-            const uint32_t nb = ((ix + dx) * ny + (iy + dy)) * nz + (iz + dz);
-            const uint32_t b0 = start[nb], b1 = start[nb + 1];
-            // r_i - r_j = d_i - d_j - (cell offset) * h
-            const float ox = ai.x - static_cast<float>(dx) * hf;
-            const float oy = ai.y - static_cast<float>(dy) * hf;
-            const float oz = ai.z - static_cast<float>(dz) * hf;
+    for (int dx = -1; dx <= 1; ++dx) {
+      if ((ix == 0 && dx < 0) || (ix == nx - 1 && dx > 0)) continue;
+      for (int dy = -1; dy <= 1; ++dy) {
+        if ((iy == 0 && dy < 0) || (iy == ny - 1 && dy > 0)) continue;
+        for (int dz = -1; dz <= 1; ++dz) {
+          if ((iz == 0 && dz < 0) || (iz == nz - 1 && dz > 0)) continue;
+          const uint32_t nb = ((ix + dx) * ny + (iy + dy)) * nz + (iz + dz);
+          const uint32_t b0 = start[nb], b1 = start[nb + 1];
+          if (MIXED) {
+            // Separations in FP64 (exact for close pairs, including atoms
+            // that overlap), everything after them in FP32; FP32 partial
+            // sums over one neighbour cell are flushed to FP64.
+            const float qi = static_cast<float>(ai.w);
             float fx0 = 0.f, fy0 = 0.f, fz0 = 0.f, en0 = 0.f;
             for (uint32_t j = b0; j < b1; ++j) {
-              const float4 aj = atoms[j];
-              const float rx = ox - aj.x, ry = oy - aj.y, rz = oz - aj.z;
+              const double4 aj = atoms[j];
+              const float rx = static_cast<float>(ai.x - aj.x);
+              const float ry = static_cast<float>(ai.y - aj.y);
+              const float rz = static_cast<float>(ai.z - aj.z);
               float r2 = rx * rx + ry * ry + rz * rz;
               r2 = (j == ii) ? INFINITY : r2;  // no self interaction
               const float ir = __builtin_amdgcn_rsqf(r2);
-              const float e = ai.w * aj.w * ir;
+              const float e = qi * static_cast<float>(aj.w) * ir;
               const float f = e * ir * ir;
               fx0 += f * rx;
               fy0 += f * ry;
@@ -663,20 +699,7 @@ __global__ __launch_bounds__(64) void sr_kernel(
             FY += fy0;
             FZ += fz0;
             EN += en0;
-          }
-        }
-      }
-    } else {
-      const double4 *atoms = static_cast<const double4 *>(atoms_v);
-      const double4 ai = atoms[ii];
-      for (int dx = -1; dx <= 1; ++dx) {
-        if ((ix == 0 && dx < 0) || (ix == nx - 1 && dx > 0)) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-          if ((iy == 0 && dy < 0) || (iy == ny - 1 && dy > 0)) continue;
-          for (int dz = -1; dz <= 1; ++dz) {
-            if ((iz == 0 && dz < 0) || (iz == nz - 1 && dz > 0)) continue;
-            const uint32_t nb = ((ix + dx) * ny + (iy + dy)) * nz + (iz + dz);
-            const uint32_t b0 = start[nb], b1 = start[nb + 1];
+          } else {
             for (uint32_t j = b0; j < b1; ++j) {
               const double4 aj = atoms[j];
               const double rx = ai.x - aj.x, ry = ai.y - aj.y, rz = ai.z - aj.z;
@@ -694,7 +717,6 @@ __global__ __launch_bounds__(64) void sr_kernel(
         }
       }
     }
-    (void)hf;
     if (valid) {
       fx[i] += FX;
       fy[i] += FY;
@@ -815,16 +837,17 @@ void bin_scatter(bool mixed, const double *rx, const double *ry,
                  const uint32_t *atom_cell, const uint32_t *rank,
                  const uint32_t *hist, const uint32_t *start,
                  const GridShape &g, uint32_t *sorted_idx,
-                 uint32_t *cell_sorted, void *atoms, hipStream_t s) {
+                 uint32_t *cell_sorted, double4 *atoms64, float4 *atoms32,
+                 hipStream_t s) {
   if (natom == 0) return;
   if (mixed)
     bin_scatter_kernel<true><<<blocks_for(natom, 256), 256, 0, s>>>(
         rx, ry, rz, qc, natom, atom_cell, rank, hist, start, g.ncell, g.ny,
-        g.nz, g.h, sorted_idx, cell_sorted, atoms);
+        g.nz, g.h, sorted_idx, cell_sorted, atoms64, atoms32);
   else
     bin_scatter_kernel<false><<<blocks_for(natom, 256), 256, 0, s>>>(
         rx, ry, rz, qc, natom, atom_cell, rank, hist, start, g.ncell, g.ny,
-        g.nz, g.h, sorted_idx, cell_sorted, atoms);
+        g.nz, g.h, sorted_idx, cell_sorted, atoms64, atoms32);
 }
 
 void sf(bool mixed, const void *atoms, const uint32_t *start,
@@ -844,7 +867,8 @@ bool rmt_lds(bool mixed, const double2 *S, const double4 *k64,
              const uint32_t *group, const uint32_t *r_in,
              const uint32_t *r_out, uint32_t Kp, const GridShape &g,
              size_t lds_limit, void *R, hipStream_t s) {
-  const size_t per_node = static_cast<size_t>(g.ncell) * sizeof(double2);
+  const size_t per_node = (static_cast<size_t>(g.ncell) + g.nx + g.ny + g.nz) *
+                          sizeof(double2);
   uint32_t T = 0;
   for (uint32_t t : {2u, 1u})
     if (t * per_node <= lds_limit) {
@@ -855,19 +879,17 @@ bool rmt_lds(bool mixed, const double2 *S, const double4 *k64,
   const uint32_t ntile = Kp / T;
   const size_t lds = T * per_node;
   if (mixed) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&rmt_lds_kernel<true>),
-                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                        static_cast<int>(lds));
-    rmt_lds_kernel<true><<<ntile, 256, lds, s>>>(S, k64, group, r_in, r_out,
-                                                 Kp, T, ntile, g.nx, g.ny,
-                                                 g.nz, g.h, R);
+    (void)hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&rmt_lds_kernel<true>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    rmt_lds_kernel<true><<<ntile, kRmtBlock, lds, s>>>(
+        S, k64, group, r_in, r_out, Kp, T, ntile, g.nx, g.ny, g.nz, g.h, R);
   } else {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&rmt_lds_kernel<false>),
-                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                        static_cast<int>(lds));
-    rmt_lds_kernel<false><<<ntile, 256, lds, s>>>(S, k64, group, r_in, r_out,
-                                                  Kp, T, ntile, g.nx, g.ny,
-                                                  g.nz, g.h, R);
+    (void)hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&rmt_lds_kernel<false>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    rmt_lds_kernel<false><<<ntile, kRmtBlock, lds, s>>>(
+        S, k64, group, r_in, r_out, Kp, T, ntile, g.nx, g.ny, g.nz, g.h, R);
   }
   return true;
 }
@@ -886,7 +908,7 @@ void rmt_global(bool mixed, double2 *S, const double4 *k64,
                                                        stride, inner);
     }
   };
-  prefix(g.nx * g.ny, g.nz, 1, 1);            // z lines: base (x*ny+y)*nz
+  // S arrives prefix-summed along z (structure-factor kernels)
   prefix(g.nx * g.nz, g.ny, g.nz, g.nz);      // y lines: base x*ny*nz + z
   prefix(g.ny * g.nz, g.nx, g.ny * g.nz, g.ny * g.nz);  // x lines: y*nz + z
   for (uint32_t c0 = 0; c0 < g.ncell; c0 += kMaxY) {
@@ -926,15 +948,15 @@ void lr_reduce(const double4 *partial, uint32_t nsplit, uint32_t natom,
                                                          natom, fx, fy, fz, en);
 }
 
-void sr(bool mixed, const void *atoms, const uint32_t *start,
+void sr(bool mixed, const double4 *atoms, const uint32_t *start,
         const GridShape &g, double *fx, double *fy, double *fz, double *en,
         hipStream_t s) {
   if (mixed)
-    sr_kernel<true><<<g.ncell, 64, 0, s>>>(atoms, start, g.nx, g.ny, g.nz, g.h,
-                                           fx, fy, fz, en);
+    sr_kernel<true><<<g.ncell, 64, 0, s>>>(atoms, start, g.nx, g.ny, g.nz, fx,
+                                           fy, fz, en);
   else
     sr_kernel<false><<<g.ncell, 64, 0, s>>>(atoms, start, g.nx, g.ny, g.nz,
-                                            g.h, fx, fy, fz, en);
+                                            fx, fy, fz, en);
 }
 
 void unpermute(const uint32_t *sorted_idx, uint32_t natom, const double *fx,

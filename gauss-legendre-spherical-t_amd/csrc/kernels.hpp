@@ -35,14 +35,15 @@ void hist_offsets(uint32_t *hist, uint32_t nchunk, uint32_t ncell,
 void cell_scan(const uint32_t *count, uint32_t ncell, uint32_t *start,
                uint32_t *maxc, hipStream_t s);
 // Scatter atoms to cell-sorted order; writes sorted_idx, the sorted atoms'
-// cells and the sorted atom records (double4 absolute x,y,z,q for FP64;
-// float4 cell-local x,y,z,q for the mixed path).
+// cells and the sorted atom records: double4 absolute (x,y,z,q) always, and
+// for the mixed path float4 cell-centre-relative (x,y,z,q).
 void bin_scatter(bool mixed, const double *rx, const double *ry,
                  const double *rz, const double *qc, uint32_t natom,
                  const uint32_t *atom_cell, const uint32_t *rank,
                  const uint32_t *hist, const uint32_t *start,
                  const GridShape &g, uint32_t *sorted_idx,
-                 uint32_t *cell_sorted, void *atoms, hipStream_t s);
+                 uint32_t *cell_sorted, double4 *atoms64, float4 *atoms32,
+                 hipStream_t s);
 
 // ---- structure factors (calc_sf) -------------------------------------------
 // S[cell*K + k] = sum_{j in cell} q_j exp(-i k.r_j), FP64 complex.
@@ -77,7 +78,7 @@ void lr_reduce(const double4 *partial, uint32_t nsplit, uint32_t natom,
                double *fx, double *fy, double *fz, double *en, hipStream_t s);
 
 // ---- near field (calc_sr_ef) -----------------------------------------------
-void sr(bool mixed, const void *atoms, const uint32_t *start,
+void sr(bool mixed, const double4 *atoms, const uint32_t *start,
         const GridShape &g, double *fx, double *fy, double *fz, double *en,
         hipStream_t s);
 
