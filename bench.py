@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Benchmark of the GLST force step on one MI355X.
+
+Workload: the 288,000-charge rigid-water box (96,000 SPC/E-geometry waters,
+seed 1, L = 142.0 A, rcut = 10 A, tol = 1e-4), inputs resident in HBM.  A step
+is what the reference driver times per iteration (test/full_example.cu:135-210):
+clear the accumulators, then calc_ener_force (assign -> S(k) -> shell sums ->
+back-projection -> near field -> comm), which ends with a stream
+synchronisation.
+
+Prints ONE JSON line: atom-steps/s of the mixed FP32/FP64 path (value), the
+plain FP64 path beside it, the relative force error of both against the
+direct FP64 all-pairs sum, the roofline of the dominant kernel (HIP events on
+the handle's stream) and a CPU baseline (the oracle's FP64 restatement of the
+reference path timed on a bounded sample on the host cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "gauss-legendre-spherical-t_amd"))
+sys.path.insert(0, REPO)
+
+NWATER, BOX, RCUT, TOL, SEED = 96000, 142.0, 10.0, 1e-4, 1
+PEAK_FP32_VALU_TFLOPS = 157.3   # MI355X published FP32 vector peak
+PEAK_HBM_GBS = 8000.0
+STAGES = ["bin", "sort", "sf", "rmt", "lr", "lr_reduce", "sr"]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def rel_force_err(got, ref):
+    import numpy as np
+    num = sum(float(((g - r) ** 2).sum()) for g, r in zip(got[:3], ref[:3]))
+    den = sum(float((r ** 2).sum()) for r in ref[:3])
+    return (num / den) ** 0.5
+
+
+def ref_style_errors(got, ref):
+    """utils/utils.cpp:60-168: rms(err)/rms(F_glst) per component."""
+    import numpy as np
+    out = {}
+    for name, g, r in zip(("fx", "fy", "fz", "en"), got, ref):
+        out[name] = float(np.sqrt(((r - g) ** 2).mean()) / np.sqrt((g ** 2).mean()))
+    return out
+
+
+def time_path(g, torch, d, natom, ct, steps, warmup, timing):
+    f = g.GLSTForce(natom, TOL, BOX, rcut=RCUT, ct=ct)
+    for _ in range(warmup):
+        f.zero_ef()
+        f.calc_ener_force(*d)
+    torch.cuda.synchronize()
+    f.set_timing(timing)
+    f.reset_timing()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        f.zero_ef()
+        f.calc_ener_force(*d)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    stages = {}
+    for s in STAGES:
+        ms, n = f.kernel_time(s)
+        if n:
+            stages[s] = ms / n
+    got = [t.cpu().numpy() for t in f.get_ef()]
+    info = f.info()
+    f.close()
+    return dt, stages, got, info
+
+
+def cpu_baseline(h, natom):
+    """Oracle (C, OpenMP) on a bounded sample: assign + near field in full,
+    the long-range part on every `stride`-th node, extrapolated."""
+    import numpy as np
+    from oracle import oracle as o
+    threads = min(16, os.cpu_count() or 1)
+    o.set_threads(threads)
+    t0 = time.perf_counter()
+    o.glst(*h, TOL, BOX, RCUT, far=False, near=True)
+    t_near = time.perf_counter() - t0
+    stride = 16
+    t0 = time.perf_counter()
+    o.glst(*h, TOL, BOX, RCUT, node_stride=stride, far=True, near=False)
+    t_far = (time.perf_counter() - t0) * stride
+    t_step = t_near + t_far
+    return {
+        "value": natom / t_step,
+        "unit": "atom-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle FP64 restatement of the reference path, 1 step: "
+                   f"assign+near field in full ({t_near:.2f} s), long range on "
+                   f"every {stride}th node ({t_far / stride:.2f} s) x{stride}; "
+                   f"est. {t_step:.1f} s/step"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--fp64-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="mixed path only (for rocprofv3 runs), plus one "
+                         "1 GiB elementwise copy used to calibrate counters")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs one system on one GPU; --gpus "
+              f"{args.gpus} is not supported (multi-GPU node sharding is a "
+              f"design only, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import glst_amd as g
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    h = g.make_water_box(NWATER, BOX, seed=SEED)
+    natom = h[0].size
+    d = [torch.as_tensor(a, dtype=torch.float64, device="cuda") for a in h]
+    log(f"bench: {natom} atoms, box {BOX}, rcut {RCUT}, tol {TOL}")
+
+    dt_mixed, st_mixed, got_mixed, info = time_path(
+        g, torch, d, natom, "float", args.steps, args.warmup, True)
+    log(f"mixed: {dt_mixed * 1e3:.3f} ms/step  stages {st_mixed}")
+    if args.profile:
+        # calibration: reads 1 GiB and writes 1 GiB
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile": True, "value": natom / dt_mixed,
+                          "ms_per_step": dt_mixed * 1e3,
+                          "stages_ms": st_mixed,
+                          "calibration_bytes": {"read": 1 << 30,
+                                                "write": 1 << 30}}),
+              flush=True)
+        return
+    dt_fp64, st_fp64, got_fp64, _ = time_path(
+        g, torch, d, natom, "double", args.fp64_steps, 1, True)
+    log(f"fp64 : {dt_fp64 * 1e3:.3f} ms/step  stages {st_fp64}")
+
+    ref = [t.cpu().numpy() for t in g.coulomb_direct(*d)]
+    err_mixed = rel_force_err(got_mixed, ref)
+    err_fp64 = rel_force_err(got_fp64, ref)
+    log(f"relative force error vs direct FP64 sum: mixed {err_mixed:.3e} "
+        f"fp64 {err_fp64:.3e}")
+
+    # roofline of the dominant kernel (back-projection "lr"): 20 flops and
+    # 2 transcendentals per (atom, node) pair over the padded node table.
+    K = info.num_nodes
+    Kp = (K + 255) // 256 * 256
+    pairs = natom * Kp
+    lr_ms = st_mixed.get("lr", float("nan"))
+    achieved = 20.0 * pairs / (lr_ms * 1e-3) / 1e12
+    traffic = None
+    tf = os.path.join(REPO, "profiles", "lr_mixed_288k_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+    roof = {
+        "kernel": "lr_mixed_kernel (calc_lr_ef back-projection)",
+        "bound": "valu",
+        "achieved": achieved,
+        "peak": PEAK_FP32_VALU_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": achieved / PEAK_FP32_VALU_TFLOPS,
+        "traffic": traffic,
+        "avg_ms": lr_ms,
+        "pair_rate_per_s": pairs / (lr_ms * 1e-3),
+    }
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(h, natom)
+        log(f"cpu baseline: {cpu}")
+
+    line = {
+        "metric": "atom-steps/s @288k atoms mixed vs FP64; served p99 ms; "
+                  "rel force err <=1e-5",
+        "value": natom / dt_mixed,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt_mixed * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded rigid-water box)",
+        "config": {
+            "workload": "288,000-charge water box (96,000 SPC/E-geometry "
+                        "waters), L=142 A, rcut=10 A, tol=1e-4, mixed path",
+            "natom": natom, "box": BOX, "rcut": RCUT, "tol": TOL,
+            "ncell": list(info.ncell), "num_nodes": K,
+            "parallelism": "single GPU",
+        },
+        "rel_force_err": err_mixed,
+        "rel_force_err_fp64": err_fp64,
+        "ref_style_err": ref_style_errors(got_mixed, ref),
+        "fp64_path": {"value": natom / dt_fp64, "ms_per_step": dt_fp64 * 1e3,
+                      "steps": args.fp64_steps},
+        "speedup_vs_fp64": dt_fp64 / dt_mixed,
+        "stages_ms": st_mixed,
+        "stages_ms_fp64": st_fp64,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

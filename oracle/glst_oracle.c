@@ -212,11 +212,37 @@ static int ref_roots(uint32_t n, double *xs, uint32_t *count) {
   return 0;
 }
 
+/* Odd orders (used only by the spherical product rule; the reference calls
+ * get_xgl with even orders only, and its scan does not reach the first
+ * extremum for small odd n, e.g. n = 3): Newton from
+ * x0 = cos(pi (k - 1/4) / (n + 1/2)). */
+static int odd_roots(uint32_t n, double *xs, uint32_t *count) {
+  const uint32_t m = (n - 1) / 2;
+  xs[0] = 0.0;
+  for (uint32_t k = 1; k <= m; k++) {
+    double x = cos(ORC_PI * ((double)k - 0.25) / ((double)n + 0.5));
+    for (int it = 0; it < 100; it++) {
+      double p, dp;
+      orc_legendre(x, n, &p, &dp);
+      const double dx = p / dp;
+      x -= dx;
+      if (fabs(dx) < 1e-16) break;
+    }
+    xs[m + 1 - k] = x; /* ascending */
+  }
+  *count = m + 1;
+  return 0;
+}
+
 int orc_gauss_legendre(uint32_t n, double *x, double *w, uint32_t *count) {
   double buf[400];
   uint32_t c = 0;
   if (n == 0 || n > 790) return 1;
-  if (ref_roots(n, buf, &c)) return 1;
+  if (n % 2 == 1 && n > 1) {
+    if (odd_roots(n, buf, &c)) return 1;
+  } else if (ref_roots(n, buf, &c)) {
+    return 1;
+  }
   for (uint32_t i = 0; i < c; i++) {
     double p, dp;
     orc_legendre(buf[i], n, &p, &dp);
