@@ -82,6 +82,14 @@ class GroupInfo(Structure):
 
 
 def _load():
+    # PyTorch ships its own HIP runtime and requests it by an unversioned
+    # name; loaded after ours it would be a second runtime in the process
+    # (and the second one finds no device).  Import torch first so the
+    # library binds to the runtime torch loaded (same soname).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"GLST HIP library not built: {LIB_PATH} is missing "

@@ -79,3 +79,17 @@ def test_tdesign_dir_missing_is_io_error(tmp_path):
         assert e.value.status in (2, 5)
     finally:
         g.set_tdesign_dir(None)
+
+
+def test_single_hip_runtime_in_process():
+    """Importing the bindings before torch must not load a second HIP/HSA
+    runtime (two runtimes in one process see no device)."""
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "import glst_amd, torch\n"
+        "m = open('/proc/self/maps').read().splitlines()\n"
+        "libs = {l.split()[-1] for l in m if 'libamdhip64' in l}\n"
+        "print(len(libs))\n" % os.path.join(REPO, "gauss-legendre-spherical-t_amd"))
+    out = subprocess.run(["python", "-c", code], capture_output=True, text=True,
+                         check=True, timeout=300).stdout.strip()
+    assert out == "1"
