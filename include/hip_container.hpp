@@ -92,7 +92,10 @@ template <typename T> class hip_container {
       check(hipMalloc(&p, n * sizeof(T)));
       d_.ptr_ = static_cast<T *>(p);
       d_.n_ = n;
+      // hipMemset may run asynchronously on the null stream; finish it so
+      // work queued on other (non-blocking) streams never races it.
       check(hipMemset(p, 0, n * sizeof(T)));
+      check(hipDeviceSynchronize());
     }
   }
 
