@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -186,6 +187,9 @@ struct Scope {
 
 using namespace glst;
 
+// elements allocated past the end of the remote-sum field
+constexpr size_t kRowSlack = 8;
+
 struct glst_engine {
   std::string last_error;
   int ct = GLST_CT_DOUBLE;
@@ -205,14 +209,24 @@ struct glst_engine {
 
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  // the fused step runs the near field on a second stream (default
+  // priority), overlapped with S(k) -> shell sums -> back-projection
+  hipStream_t side = nullptr;
+  hipEvent_t ev_assigned = nullptr, ev_near = nullptr;
+  DeviceBuffer near4;
+  bool overlap_near = true;
   KernelTimer timer;
 
   // atoms
   DeviceBuffer atom_cell, rank, hist, cell_count, cell_start, maxc;
   DeviceBuffer sorted_idx, cell_sorted, atoms64, atoms32;
   DeviceBuffer fx, fy, fz, en;
-  // nodes
+  // nodes (device order: direction-major inside each group)
   DeviceBuffer k64, krev, group, r_in, r_out;
+  std::vector<uint32_t> dev_of_ref;  // device index of each reference node
+  // back-projection tasks (one per group and direction) and their splits
+  DeviceBuffer t_node0, t_nr, t_koff, t_dir64, t_dir32, kap64, kap32, t_split;
+  uint32_t ntask = 0;
   // fields
   DeviceBuffer S, R, partial;
   uint32_t nsplit = 1;
@@ -221,6 +235,9 @@ struct glst_engine {
   bool rmt_done = false;
 
   ~glst_engine() {
+    if (side) (void)hipStreamDestroy(side);
+    if (ev_assigned) (void)hipEventDestroy(ev_assigned);
+    if (ev_near) (void)hipEventDestroy(ev_near);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
   }
 
@@ -230,7 +247,9 @@ struct glst_engine {
                                   &atoms64,   &fx,    &fy,         &fz,
                                   &en,        &k64,   &krev,       &group,
                                   &r_in,      &r_out, &S,          &R,
-                                  &partial,   &atoms32};
+                                  &partial,   &atoms32, &t_node0,   &t_nr,
+                                  &t_koff,    &t_dir64, &t_dir32,   &kap64,
+                                  &kap32,     &t_split, &near4};
     size_t s = 0;
     for (auto b : bufs) s += b->bytes();
     return s;
@@ -292,6 +311,9 @@ struct glst_engine {
     K = static_cast<uint32_t>(nodes.kx.size());
     Kp = (K + 255u) / 256u * 256u;
 
+    GLST_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    GLST_HIP(hipEventCreateWithFlags(&ev_assigned, hipEventDisableTiming));
+    GLST_HIP(hipEventCreateWithFlags(&ev_near, hipEventDisableTiming));
     allocate();
     upload_nodes();
     dev::fill_f64(fx.as<double>(), 0.0, natom, stream);
@@ -315,53 +337,117 @@ struct glst_engine {
     cell_sorted.resize(n * 4);
     atoms64.resize(n * sizeof(double4));
     atoms32.resize(mixed() ? n * sizeof(float4) : 0);
+    near4.resize(n * sizeof(double4));
     fx.resize(n * 8);
     fy.resize(n * 8);
     fz.resize(n * 8);
     en.resize(n * 8);
     if (Kp > 0) {
-      k64.resize(static_cast<size_t>(Kp) * sizeof(double4));
-      krev.resize(static_cast<size_t>(Kp) * sizeof(float4));
-      group.resize(static_cast<size_t>(Kp) * 4);
-      r_in.resize(groups.r_in.size() * 4);
-      r_out.resize(groups.r_out.size() * 4);
       if (has_lr) {
         const size_t field = static_cast<size_t>(grid.ncell) * Kp;
         S.resize(field * sizeof(double2));
-        R.resize(field * (mixed() ? sizeof(float2) : sizeof(double2)));
+        // slack past the last row: the back-projection prefetches ahead
+        R.resize((field + kRowSlack) *
+                 (mixed() ? sizeof(float2) : sizeof(double2)));
         const uint32_t ntile = (natom + 255) / 256;
-        const uint32_t groups64 = Kp / 64;
-        nsplit = ntile > 0 ? (4096 + ntile - 1) / ntile : 1;
-        if (nsplit > groups64) nsplit = groups64;
-        if (nsplit < 1) nsplit = 1;
+        // enough workgroups to fill the chip several times over
+        make_splits(ntile > 0 ? (4096 + ntile - 1) / ntile : 1);
         partial.resize(static_cast<size_t>(nsplit) * n * sizeof(double4));
       }
     }
   }
 
+  // Back-projection splits: contiguous task ranges that never cross an alpha
+  // group (all tasks of a split then share the radial count and kappa_i);
+  // each group gets about target * K_g / K splits of equal size.
+  std::vector<uint32_t> split_host;
+  std::vector<dev::LrGroup> lr_groups;
+  void make_splits(uint32_t target) {
+    split_host.assign(1, 0);
+    lr_groups.clear();
+    uint32_t base = 0;
+    for (size_t g = 0; g < nodes.dirs.size(); ++g) {
+      const uint32_t M = static_cast<uint32_t>(nodes.dirs[g].size());
+      const double share =
+          static_cast<double>(target) * nodes.count[g] / std::max<uint32_t>(K, 1);
+      uint32_t ns = static_cast<uint32_t>(share + 0.5);
+      ns = std::max<uint32_t>(1, std::min<uint32_t>(ns, M));
+      lr_groups.push_back({static_cast<uint32_t>(nodes.kappa[g].size()),
+                           static_cast<uint32_t>(split_host.size() - 1), ns});
+      for (uint32_t q = 1; q <= ns; ++q)
+        split_host.push_back(base + static_cast<uint32_t>(
+                                        (static_cast<uint64_t>(M) * q) / ns));
+      base += M;
+    }
+    if (split_host.size() == 1) split_host.push_back(0);  // no tasks at all
+    nsplit = static_cast<uint32_t>(split_host.size() - 1);
+  }
+
   void upload_nodes() {
     if (Kp == 0) return;
+    const double inv2pi = 1.0 / 6.283185307179586476925286766559;
     std::vector<double4> h64(Kp, make_double4(0, 0, 0, 0));
     std::vector<float4> hrev(Kp, make_float4(0, 0, 0, 0));
     std::vector<uint32_t> hg(Kp, 0);
-    const double inv2pi = 1.0 / 6.283185307179586476925286766559;
-    for (uint32_t k = 0; k < K; ++k) {
-      h64[k] = make_double4(nodes.kx[k], nodes.ky[k], nodes.kz[k], nodes.w[k]);
-      hrev[k] = make_float4(static_cast<float>(nodes.kx[k] * inv2pi),
-                            static_cast<float>(nodes.ky[k] * inv2pi),
-                            static_cast<float>(nodes.kz[k] * inv2pi), 0.f);
-      hg[k] = nodes.group[k];
+    std::vector<uint32_t> tn0, tnr, tko;
+    std::vector<double4> td64;
+    std::vector<float4> td32;
+    std::vector<double> kv64;
+    std::vector<float> kv32;
+    dev_of_ref.assign(K, 0);
+    for (size_t g = 0; g < nodes.dirs.size(); ++g) {
+      const uint32_t nr = static_cast<uint32_t>(nodes.kappa[g].size());
+      const uint32_t M = static_cast<uint32_t>(nodes.dirs[g].size());
+      const uint32_t first = nodes.first[g];
+      const uint32_t koff = static_cast<uint32_t>(kv64.size());
+      for (double kap : nodes.kappa[g]) {
+        kv64.push_back(kap);
+        kv32.push_back(static_cast<float>(kap * inv2pi));
+      }
+      for (uint32_t m = 0; m < M; ++m) {
+        const auto &sd = nodes.dirs[g][m];
+        tn0.push_back(first + m * nr);
+        tnr.push_back(nr);
+        tko.push_back(koff);
+        td64.push_back(make_double4(sd[0], sd[1], sd[2], 0.0));
+        td32.push_back(make_float4(static_cast<float>(sd[0]),
+                                   static_cast<float>(sd[1]),
+                                   static_cast<float>(sd[2]), 0.f));
+        for (uint32_t i = 0; i < nr; ++i) {
+          const uint32_t ref = first + i * M + m;  // reference order
+          const uint32_t dev = first + m * nr + i;  // direction-major
+          dev_of_ref[ref] = dev;
+          h64[dev] = make_double4(nodes.kx[ref], nodes.ky[ref], nodes.kz[ref],
+                                  nodes.w[ref]);
+          hrev[dev] = make_float4(static_cast<float>(nodes.kx[ref] * inv2pi),
+                                  static_cast<float>(nodes.ky[ref] * inv2pi),
+                                  static_cast<float>(nodes.kz[ref] * inv2pi),
+                                  0.f);
+          hg[dev] = static_cast<uint32_t>(g);
+        }
+      }
     }
-    GLST_HIP(hipMemcpy(k64.as<void>(), h64.data(), Kp * sizeof(double4),
-                       hipMemcpyHostToDevice));
-    GLST_HIP(hipMemcpy(krev.as<void>(), hrev.data(), Kp * sizeof(float4),
-                       hipMemcpyHostToDevice));
-    GLST_HIP(hipMemcpy(group.as<void>(), hg.data(), Kp * 4,
-                       hipMemcpyHostToDevice));
-    GLST_HIP(hipMemcpy(r_in.as<void>(), groups.r_in.data(),
-                       groups.r_in.size() * 4, hipMemcpyHostToDevice));
-    GLST_HIP(hipMemcpy(r_out.as<void>(), groups.r_out.data(),
-                       groups.r_out.size() * 4, hipMemcpyHostToDevice));
+    ntask = static_cast<uint32_t>(tn0.size());
+    if (split_host.empty()) make_splits(1);
+    const std::vector<uint32_t> &split = split_host;
+    auto up = [&](DeviceBuffer &b, const void *src, size_t bytes) {
+      b.resize(bytes);
+      if (bytes)
+        GLST_HIP(hipMemcpy(b.as<void>(), src, bytes, hipMemcpyHostToDevice));
+    };
+    up(k64, h64.data(), Kp * sizeof(double4));
+    up(krev, hrev.data(), Kp * sizeof(float4));
+    up(group, hg.data(), Kp * 4);
+    up(r_in, groups.r_in.data(), groups.r_in.size() * 4);
+    up(r_out, groups.r_out.data(), groups.r_out.size() * 4);
+    up(t_node0, tn0.data(), tn0.size() * 4);
+    up(t_nr, tnr.data(), tnr.size() * 4);
+    up(t_koff, tko.data(), tko.size() * 4);
+    up(t_dir64, td64.data(), td64.size() * sizeof(double4));
+    up(t_dir32, td32.data(), td32.size() * sizeof(float4));
+    up(kap64, kv64.data(), kv64.size() * 8);
+    up(kap32, kv32.data(), kv32.size() * 4);
+    up(t_split, split.data(), split.size() * 4);
   }
 
   // atom records the long-range kernels read
@@ -434,20 +520,31 @@ struct glst_engine {
   void calc_lr_ef() {
     if (!has_lr || Kp == 0 || natom == 0) return;
     if (!rmt_done) throw StateError("calc_lr_ef called before sum_rmt_sf");
-    {
-      Scope sc(timer, "lr", stream);
-      dev::lr(mixed(), lr_atoms(), cell_sorted.as<uint32_t>(),
-              k64.as<double4>(), krev.as<float4>(), R.as<void>(), Kp, natom,
-              nsplit, partial.as<double4>(), stream);
-      check_launch("lr");
-    }
-    {
-      Scope sc(timer, "lr_reduce", stream);
-      dev::lr_reduce(partial.as<double4>(), nsplit, natom, fx.as<double>(),
-                     fy.as<double>(), fz.as<double>(), en.as<double>(),
-                     stream);
-      check_launch("lr_reduce");
-    }
+    lr_partials();
+    reduce(false);
+  }
+
+  // fx..en += the back-projection partials (and the near-field record of the
+  // fused step when with_near)
+  void reduce(bool with_near) {
+    Scope sc(timer, "lr_reduce", stream);
+    const bool lr = has_lr && Kp > 0;
+    dev::lr_reduce(partial.as<double4>(), lr ? nsplit : 0, natom,
+                   with_near ? near4.as<double4>() : nullptr, fx.as<double>(),
+                   fy.as<double>(), fz.as<double>(), en.as<double>(), stream);
+    check_launch("lr_reduce");
+  }
+
+  void lr_partials() {
+    Scope sc(timer, "lr", stream);
+    const dev::LrTaskArgs ta{t_node0.as<uint32_t>(), t_nr.as<uint32_t>(),
+                             t_koff.as<uint32_t>(),  t_dir64.as<double4>(),
+                             t_dir32.as<float4>(),   kap64.as<double>(),
+                             kap32.as<float>(),      t_split.as<uint32_t>()};
+    dev::lr(mixed(), lr_atoms(), cell_sorted.as<uint32_t>(), ta, lr_groups,
+            krev.as<float4>(), R.as<void>(), Kp, natom, nsplit,
+            partial.as<double4>(), stream);
+    check_launch("lr");
   }
 
   void calc_sr_ef() {
@@ -456,19 +553,40 @@ struct glst_engine {
     Scope sc(timer, "sr", stream);
     dev::sr(mixed(), atoms64.as<double4>(), cell_start.as<uint32_t>(), grid,
             fx.as<double>(), fy.as<double>(), fz.as<double>(), en.as<double>(),
-            stream);
+            nullptr, stream);
     check_launch("sr");
   }
 
   void comm_ef() { GLST_HIP(hipStreamSynchronize(stream)); }
 
+  // The fused step: the near field (which needs only the sorted atoms) runs
+  // on the side stream while the main stream does S(k), the shell sums and
+  // the back-projection; one fixed-order reduction adds both parts.
   void step_async(const double *rx, const double *ry, const double *rz,
                   const double *qc) {
     assign_atoms(rx, ry, rz, qc);
+    if (natom == 0) return;
+    if (!overlap_near) {
+      calc_sf();
+      sum_rmt_sf();
+      calc_lr_ef();
+      calc_sr_ef();
+      return;
+    }
+    GLST_HIP(hipEventRecord(ev_assigned, stream));
+    GLST_HIP(hipStreamWaitEvent(side, ev_assigned, 0));
+    {
+      Scope sc(timer, "sr", side);
+      dev::sr(mixed(), atoms64.as<double4>(), cell_start.as<uint32_t>(), grid,
+              nullptr, nullptr, nullptr, nullptr, near4.as<double4>(), side);
+      check_launch("sr");
+    }
+    GLST_HIP(hipEventRecord(ev_near, side));
     calc_sf();
     sum_rmt_sf();
-    calc_lr_ef();
-    calc_sr_ef();
+    if (has_lr && Kp > 0) lr_partials();
+    GLST_HIP(hipStreamWaitEvent(stream, ev_near, 0));
+    reduce(true);
   }
 
   void get_ef(double *ofx, double *ofy, double *ofz, double *oen) {
@@ -527,7 +645,7 @@ struct glst_engine {
     }
     for (uint32_t c = 0; c < grid.ncell; ++c)
       for (uint32_t k = 0; k < K; ++k) {
-        const size_t s = static_cast<size_t>(c) * Kp + k;
+        const size_t s = static_cast<size_t>(c) * Kp + dev_of_ref[k];
         const size_t d = static_cast<size_t>(c) * K + k;
         if (re) re[d] = tmp[2 * s];
         if (im) im[d] = tmp[2 * s + 1];
@@ -856,6 +974,19 @@ glst_status_t glst_get_rmt(glst_handle_t h, double *re, double *im) {
     }
     if (re) std::memcpy(re, r.data(), n * 8);
     if (im) std::memcpy(im, i.data(), n * 8);
+  });
+}
+
+glst_status_t glst_set_option(glst_handle_t h, const char *name,
+                              int64_t value) {
+  NEED_HANDLE(h);
+  if (!name) return GLST_ERR_INVALID;
+  return guarded(h, [&] {
+    const std::string n = name;
+    if (n == "overlap_near")
+      h->overlap_near = value != 0;
+    else
+      throw ArgError("unknown option \"" + n + "\"");
   });
 }
 

@@ -21,7 +21,9 @@
 
 #include <cfloat>
 #include <climits>
+#include <array>
 #include <cmath>
+#include <utility>
 
 namespace glst {
 namespace dev {
@@ -42,8 +44,13 @@ __device__ inline uint32_t xcd_remap(uint32_t b, uint32_t nb) {
 #ifndef GLST_SF_FLUSH
 #define GLST_SF_FLUSH 32
 #endif
-#ifndef GLST_LR_FLUSH
-#define GLST_LR_FLUSH 64
+// back-projection form for the mixed path: 0 direction tasks, 1 flat
+// prefetching walk, 2 node-major, 3 per-group unrolled directions
+#ifndef GLST_LR_VARIANT
+#define GLST_LR_VARIANT 3
+#endif
+#ifndef GLST_LR_FLUSH_TASKS
+#define GLST_LR_FLUSH_TASKS 4
 #endif
 #ifndef GLST_ACCURATE_SINCOS
 #define GLST_ACCURATE_SINCOS 0
@@ -58,7 +65,14 @@ __device__ inline float cos_rev(float t) { return cospif(2.0f * t); }
 __device__ inline float sin_rev(float t) { return __builtin_amdgcn_sinf(t); }
 __device__ inline float cos_rev(float t) { return __builtin_amdgcn_cosf(t); }
 #endif
+#ifndef GLST_NO_FRACT
+#define GLST_NO_FRACT 0
+#endif
+#if GLST_NO_FRACT
+__device__ inline float fract_rev(float t) { return t; }
+#else
 __device__ inline float fract_rev(float t) { return __builtin_amdgcn_fractf(t); }
+#endif
 
 // ---------------------------------------------------------------------------
 // fills
@@ -420,7 +434,13 @@ __device__ inline void axis_phase(double2 *dst, double kd, double h,
   *dst = make_double2(c, s);
 }
 
-constexpr uint32_t kRmtBlock = 1024;
+#ifndef GLST_RMT_BLOCK
+#define GLST_RMT_BLOCK 1024
+#endif
+#ifndef GLST_RMT_TMAX
+#define GLST_RMT_TMAX 2
+#endif
+constexpr uint32_t kRmtBlock = GLST_RMT_BLOCK;
 
 // One workgroup holds the z-prefixed table of T nodes in LDS, completes the
 // summed-volume table along y and x, and gathers the shells.
@@ -543,34 +563,54 @@ __global__ void rmt_gather_kernel(const double2 *__restrict__ P,
 
 constexpr uint32_t kLrBlock = 256;
 
+// Nodes are stored direction-major inside each group: a task is one
+// (group, direction s) with its n_r radial nodes k_i = kappa_i s at
+// consecutive indices, so every lane projects its atom once, p = s.r, and
+// each radial node costs one multiply for the phase kappa_i p.
+struct LrTasks {
+  const uint32_t *node0;   // first node of the task
+  const uint32_t *nr;      // radial nodes in the task
+  const uint32_t *koff;    // offset of the task's kappa values
+  const double4 *dir64;    // s (FP64)
+  const float4 *dir32;     // s (FP32)
+  const double *kappa64;   // kappa (1/A)
+  const float *kappa32;    // kappa / (2 pi) (revolutions per A)
+  const uint32_t *split;   // task range of each split: [split[s], split[s+1])
+};
+
 // FP64 (src/glst_force.cu:1060-1135): en_a = q_a sum_k Re(e^{ik.r_a} R_c(k)),
-// F_a = q_a sum_k k Im(e^{ik.r_a} R_c(k)).
+// F_a = q_a sum_k k Im(e^{ik.r_a} R_c(k)), absolute coordinates.
 __global__ __launch_bounds__(kLrBlock) void lr_f64_kernel(
     const double4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
-    const double4 *__restrict__ k64, const double2 *__restrict__ R,
-    uint32_t Kp, uint32_t natom, uint32_t kchunk, uint32_t ntile,
-    double4 *__restrict__ partial) {
+    LrTasks tk, const double2 *__restrict__ R, uint32_t Kp, uint32_t natom,
+    uint32_t ntile, double4 *__restrict__ partial) {
   const uint32_t tile = xcd_remap(blockIdx.x, ntile);
   const uint32_t i = tile * kLrBlock + threadIdx.x;
   const uint32_t split = blockIdx.y;
-  const uint32_t k0 = split * kchunk;
-  const uint32_t k1 = (k0 + kchunk < Kp) ? k0 + kchunk : Kp;
   const bool valid = i < natom;
   const uint32_t ii = valid ? i : natom - 1;
   const double4 a = atoms[ii];
   const double2 *Rc = R + static_cast<size_t>(cell_sorted[ii]) * Kp;
   double fx = 0.0, fy = 0.0, fz = 0.0, en = 0.0;
-  for (uint32_t k = k0; k < k1; ++k) {
-    const double4 kn = k64[k];
-    double s, c;
-    sincos(kn.x * a.x + kn.y * a.y + kn.z * a.z, &s, &c);
-    const double2 r = Rc[k];
-    const double dre = c * r.x - s * r.y;
-    const double dim = c * r.y + s * r.x;
-    fx += dim * kn.x;
-    fy += dim * kn.y;
-    fz += dim * kn.z;
-    en += dre;
+  const uint32_t t1 = tk.split[split + 1];
+  for (uint32_t t = tk.split[split]; t < t1; ++t) {
+    const uint32_t n0 = tk.node0[t], nr = tk.nr[t];
+    const double *kap = tk.kappa64 + tk.koff[t];
+    const double4 d = tk.dir64[t];
+    const double p = d.x * a.x + d.y * a.y + d.z * a.z;
+    double e = 0.0, b = 0.0;
+    for (uint32_t j = 0; j < nr; ++j) {
+      const double kj = kap[j];
+      double sn, cs;
+      sincos(kj * p, &sn, &cs);
+      const double2 r = Rc[n0 + j];
+      e += cs * r.x - sn * r.y;
+      b += kj * (cs * r.y + sn * r.x);
+    }
+    en += e;
+    fx += d.x * b;
+    fy += d.y * b;
+    fz += d.z * b;
   }
   if (valid)
     partial[static_cast<size_t>(split) * natom + i] =
@@ -578,8 +618,296 @@ __global__ __launch_bounds__(kLrBlock) void lr_f64_kernel(
 }
 
 // Mixed: cell-local FP32 phases against exp(+i k.c) R_c(k) (float2); FP32
-// partial sums over 64 nodes are flushed to FP64.
+// partial sums over four tasks (a few dozen nodes) are flushed to FP64.
 __global__ __launch_bounds__(kLrBlock) void lr_mixed_kernel(
+    const float4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
+    LrTasks tk, const float2 *__restrict__ R, uint32_t Kp, uint32_t natom,
+    uint32_t ntile, double4 *__restrict__ partial) {
+  const uint32_t tile = xcd_remap(blockIdx.x, ntile);
+  const uint32_t i = tile * kLrBlock + threadIdx.x;
+  const uint32_t split = blockIdx.y;
+  const bool valid = i < natom;
+  const uint32_t ii = valid ? i : natom - 1;
+  const float4 a = atoms[ii];
+  const float2 *Rc = R + static_cast<size_t>(cell_sorted[ii]) * Kp;
+  double FX = 0.0, FY = 0.0, FZ = 0.0, EN = 0.0;
+  float fx = 0.f, fy = 0.f, fz = 0.f, en = 0.f;
+  uint32_t pending = 0;
+  const uint32_t t1 = tk.split[split + 1];
+  for (uint32_t t = tk.split[split]; t < t1; ++t) {
+    const uint32_t n0 = tk.node0[t], nr = tk.nr[t];
+    const float *kap = tk.kappa32 + tk.koff[t];
+    const float4 d = tk.dir32[t];
+    const float p = d.x * a.x + d.y * a.y + d.z * a.z;
+    float e0 = 0.f, b0 = 0.f, e1 = 0.f, b1 = 0.f;
+    uint32_t j = 0;
+    for (; j + 2 <= nr; j += 2) {
+      const float k0 = kap[j], k1 = kap[j + 1];
+      const float t0 = fract_rev(k0 * p), t1v = fract_rev(k1 * p);
+      const float s0 = sin_rev(t0), c0 = cos_rev(t0);
+      const float s1 = sin_rev(t1v), c1 = cos_rev(t1v);
+      const float2 r0 = Rc[n0 + j], r1 = Rc[n0 + j + 1];
+      // Re(e^{it} R) accumulated as two FMAs; Im(e^{it} R) weighted by kappa
+      e0 = fmaf(c0, r0.x, e0);
+      e0 = fmaf(-s0, r0.y, e0);
+      b0 = fmaf(k0, fmaf(c0, r0.y, s0 * r0.x), b0);
+      e1 = fmaf(c1, r1.x, e1);
+      e1 = fmaf(-s1, r1.y, e1);
+      b1 = fmaf(k1, fmaf(c1, r1.y, s1 * r1.x), b1);
+    }
+    if (j < nr) {
+      const float k0 = kap[j];
+      const float t0 = fract_rev(k0 * p);
+      const float s0 = sin_rev(t0), c0 = cos_rev(t0);
+      const float2 r0 = Rc[n0 + j];
+      e0 = fmaf(c0, r0.x, e0);
+      e0 = fmaf(-s0, r0.y, e0);
+      b0 = fmaf(k0, fmaf(c0, r0.y, s0 * r0.x), b0);
+    }
+    const float b = b0 + b1;
+    en += e0 + e1;
+    fx += d.x * b;
+    fy += d.y * b;
+    fz += d.z * b;
+    if (++pending == GLST_LR_FLUSH_TASKS) {
+      FX += fx;
+      FY += fy;
+      FZ += fz;
+      EN += en;
+      fx = fy = fz = en = 0.f;
+      pending = 0;
+    }
+  }
+  FX += fx;
+  FY += fy;
+  FZ += fz;
+  EN += en;
+  if (valid) {
+    const double qf = static_cast<double>(a.w) * kTwoPi;  // k = 2 pi k_rev
+    partial[static_cast<size_t>(split) * natom + i] =
+        make_double4(qf * FX, qf * FY, qf * FZ, static_cast<double>(a.w) * EN);
+  }
+}
+
+// Mixed, fixed radial count: every split lies inside one alpha group, so all
+// its tasks have the same number NR of radial nodes and share the same
+// kappa_i.  With NR a template parameter the kappa_i sit in registers for the
+// whole split, the NR remote sums of a direction are fetched together with
+// immediate offsets, and the radial loop unrolls completely: per node one
+// multiply for the phase, fract, sin, cos and five FMAs.
+template <int NR>
+__device__ __forceinline__ void lr_mixed_dirs(
+    const float4 &a, const float2 *__restrict__ Rc, const LrTasks &tk,
+    uint32_t tb, uint32_t te, double &FX, double &FY, double &FZ,
+    double &EN) {
+  float kap[NR];
+  const float *kp = tk.kappa32 + tk.koff[tb];
+#pragma unroll
+  for (int i = 0; i < NR; ++i) kap[i] = kp[i];
+  float fx = 0.f, fy = 0.f, fz = 0.f, en = 0.f;
+  uint32_t pending = 0;
+  for (uint32_t t = tb; t < te; ++t) {
+    const float2 *rp = Rc + tk.node0[t];
+    const float4 d = tk.dir32[t];
+    float2 r[NR];
+#pragma unroll
+    for (int i = 0; i < NR; ++i) r[i] = rp[i];
+    const float p = d.x * a.x + d.y * a.y + d.z * a.z;
+    float e0 = 0.f, e1 = 0.f, b0 = 0.f, b1 = 0.f;
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      const float tt = fract_rev(kap[i] * p);
+      const float sn = sin_rev(tt), cs = cos_rev(tt);
+      if (i & 1) {
+        e1 = fmaf(cs, r[i].x, e1);
+        e1 = fmaf(-sn, r[i].y, e1);
+        b1 = fmaf(kap[i], fmaf(cs, r[i].y, sn * r[i].x), b1);
+      } else {
+        e0 = fmaf(cs, r[i].x, e0);
+        e0 = fmaf(-sn, r[i].y, e0);
+        b0 = fmaf(kap[i], fmaf(cs, r[i].y, sn * r[i].x), b0);
+      }
+    }
+    const float b = b0 + b1;
+    en += e0 + e1;
+    fx = fmaf(d.x, b, fx);
+    fy = fmaf(d.y, b, fy);
+    fz = fmaf(d.z, b, fz);
+    if (++pending == GLST_LR_FLUSH_TASKS) {
+      FX += fx;
+      FY += fy;
+      FZ += fz;
+      EN += en;
+      fx = fy = fz = en = 0.f;
+      pending = 0;
+    }
+  }
+  FX += fx;
+  FY += fy;
+  FZ += fz;
+  EN += en;
+}
+
+// Any radial count (larger than the unrolled cases).
+__device__ __forceinline__ void lr_mixed_dirs_any(
+    const float4 &a, const float2 *__restrict__ Rc, const LrTasks &tk,
+    uint32_t tb, uint32_t te, double &FX, double &FY, double &FZ,
+    double &EN) {
+  float fx = 0.f, fy = 0.f, fz = 0.f, en = 0.f;
+  uint32_t pending = 0;
+  for (uint32_t t = tb; t < te; ++t) {
+    const uint32_t n0 = tk.node0[t], nr = tk.nr[t];
+    const float *kap = tk.kappa32 + tk.koff[t];
+    const float4 d = tk.dir32[t];
+    const float p = d.x * a.x + d.y * a.y + d.z * a.z;
+    float e = 0.f, b = 0.f;
+    for (uint32_t j = 0; j < nr; ++j) {
+      const float kj = kap[j];
+      const float tt = fract_rev(kj * p);
+      const float sn = sin_rev(tt), cs = cos_rev(tt);
+      const float2 r = Rc[n0 + j];
+      e = fmaf(cs, r.x, e);
+      e = fmaf(-sn, r.y, e);
+      b = fmaf(kj, fmaf(cs, r.y, sn * r.x), b);
+    }
+    en += e;
+    fx = fmaf(d.x, b, fx);
+    fy = fmaf(d.y, b, fy);
+    fz = fmaf(d.z, b, fz);
+    if (++pending == GLST_LR_FLUSH_TASKS) {
+      FX += fx;
+      FY += fy;
+      FZ += fz;
+      EN += en;
+      fx = fy = fz = en = 0.f;
+      pending = 0;
+    }
+  }
+  FX += fx;
+  FY += fy;
+  FZ += fz;
+  EN += en;
+}
+
+// One instantiation per radial count (NR = 0: any count); the host launches
+// one grid per alpha group, blockIdx.y + split0 being the split.
+template <int NR>
+__global__ __launch_bounds__(kLrBlock) void lr_mixed_nr_kernel(
+    const float4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
+    LrTasks tk, const float2 *__restrict__ R, uint32_t Kp, uint32_t natom,
+    uint32_t ntile, uint32_t split0, double4 *__restrict__ partial) {
+  const uint32_t tile = xcd_remap(blockIdx.x, ntile);
+  const uint32_t i = tile * kLrBlock + threadIdx.x;
+  const uint32_t split = split0 + blockIdx.y;
+  const bool valid = i < natom;
+  const uint32_t ii = valid ? i : natom - 1;
+  const float4 a = atoms[ii];
+  const float2 *Rc = R + static_cast<size_t>(cell_sorted[ii]) * Kp;
+  double FX = 0.0, FY = 0.0, FZ = 0.0, EN = 0.0;
+  const uint32_t tb = tk.split[split], te = tk.split[split + 1];
+  if (tb < te) {
+    if constexpr (NR > 0)
+      lr_mixed_dirs<NR>(a, Rc, tk, tb, te, FX, FY, FZ, EN);
+    else
+      lr_mixed_dirs_any(a, Rc, tk, tb, te, FX, FY, FZ, EN);
+  }
+  if (valid) {
+    const double qf = static_cast<double>(a.w) * kTwoPi;  // k = 2 pi k_rev
+    partial[static_cast<size_t>(split) * natom + i] =
+        make_double4(qf * FX, qf * FY, qf * FZ, static_cast<double>(a.w) * EN);
+  }
+}
+
+typedef void (*LrNrKernel)(const float4 *, const uint32_t *, LrTasks,
+                           const float2 *, uint32_t, uint32_t, uint32_t,
+                           uint32_t, double4 *);
+
+template <int... N>
+constexpr auto lr_nr_table(std::integer_sequence<int, N...>) {
+  return std::array<LrNrKernel, sizeof...(N)>{{&lr_mixed_nr_kernel<N>...}};
+}
+
+// index NR (1..kLrMaxUnrolled) -> unrolled kernel; 0 -> any radial count
+constexpr int kLrMaxUnrolled = 40;
+const auto kLrNrKernels =
+    lr_nr_table(std::make_integer_sequence<int, kLrMaxUnrolled + 1>{});
+
+// Mixed, flat form: a split's tasks cover one contiguous node range, so the
+// lane walks it node by node and the remote sum of node k+2 is fetched while
+// node k is computed (R carries a few elements of slack past the last row).
+// At each direction boundary (a uniform branch) the radial sum is folded into
+// the force along that direction and the next direction is projected.
+__global__ __launch_bounds__(kLrBlock) void lr_mixed_flat_kernel(
+    const float4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
+    LrTasks tk, const float2 *__restrict__ R, uint32_t Kp, uint32_t natom,
+    uint32_t ntile, double4 *__restrict__ partial) {
+  const uint32_t tile = xcd_remap(blockIdx.x, ntile);
+  const uint32_t i = tile * kLrBlock + threadIdx.x;
+  const uint32_t split = blockIdx.y;
+  const bool valid = i < natom;
+  const uint32_t ii = valid ? i : natom - 1;
+  const float4 a = atoms[ii];
+  const float2 *Rc = R + static_cast<size_t>(cell_sorted[ii]) * Kp;
+  double FX = 0.0, FY = 0.0, FZ = 0.0, EN = 0.0;
+  const uint32_t tb = tk.split[split], te = tk.split[split + 1];
+  if (tb < te) {
+    uint32_t t = tb;
+    uint32_t k = tk.node0[tb];
+    const uint32_t kend = tk.node0[te - 1] + tk.nr[te - 1];
+    uint32_t nr = tk.nr[t], j = 0, pending = 0;
+    const float *kap = tk.kappa32 + tk.koff[t];
+    float4 d = tk.dir32[t];
+    float p = d.x * a.x + d.y * a.y + d.z * a.z;
+    float fx = 0.f, fy = 0.f, fz = 0.f, en = 0.f, e = 0.f, b = 0.f;
+    float2 r1 = Rc[k], r2 = Rc[k + 1];
+    for (; k < kend; ++k) {
+      const float2 r = r1;
+      r1 = r2;
+      r2 = Rc[k + 2];
+      const float kj = kap[j];
+      const float tt = fract_rev(kj * p);
+      const float sn = sin_rev(tt), cs = cos_rev(tt);
+      e = fmaf(cs, r.x, e);
+      e = fmaf(-sn, r.y, e);
+      b = fmaf(kj, fmaf(cs, r.y, sn * r.x), b);
+      if (++j == nr) {
+        en += e;
+        fx += d.x * b;
+        fy += d.y * b;
+        fz += d.z * b;
+        e = b = 0.f;
+        j = 0;
+        if (++pending == GLST_LR_FLUSH_TASKS) {
+          FX += fx;
+          FY += fy;
+          FZ += fz;
+          EN += en;
+          fx = fy = fz = en = 0.f;
+          pending = 0;
+        }
+        if (++t < te) {
+          nr = tk.nr[t];
+          kap = tk.kappa32 + tk.koff[t];
+          d = tk.dir32[t];
+          p = d.x * a.x + d.y * a.y + d.z * a.z;
+        }
+      }
+    }
+    FX += fx;
+    FY += fy;
+    FZ += fz;
+    EN += en;
+  }
+  if (valid) {
+    const double qf = static_cast<double>(a.w) * kTwoPi;  // k = 2 pi k_rev
+    partial[static_cast<size_t>(split) * natom + i] =
+        make_double4(qf * FX, qf * FY, qf * FZ, static_cast<double>(a.w) * EN);
+  }
+}
+
+// Mixed, node-major form: every node carries its own k (krev); the node
+// range of a split is cut into 64-node blocks, two nodes per step.
+__global__ __launch_bounds__(kLrBlock) void lr_mixed_nodes_kernel(
     const float4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
     const float4 *__restrict__ krev, const float2 *__restrict__ R,
     uint32_t Kp, uint32_t natom, uint32_t kchunk, uint32_t ntile,
@@ -595,10 +923,10 @@ __global__ __launch_bounds__(kLrBlock) void lr_mixed_kernel(
   const float4 *Rc =
       reinterpret_cast<const float4 *>(R + static_cast<size_t>(cell_sorted[ii]) * Kp);
   double FX = 0.0, FY = 0.0, FZ = 0.0, EN = 0.0;
-  for (uint32_t kb = k0; kb < k1; kb += GLST_LR_FLUSH) {
+  for (uint32_t kb = k0; kb < k1; kb += 64) {
     float fx = 0.f, fy = 0.f, fz = 0.f, en = 0.f;
 #pragma unroll 2
-    for (uint32_t k = kb; k < kb + GLST_LR_FLUSH; k += 2) {
+    for (uint32_t k = kb; k < kb + 64; k += 2) {
       const float4 n0 = krev[k];
       const float4 n1 = krev[k + 1];
       const float4 r = Rc[k >> 1];  // R_k, R_{k+1}
@@ -619,23 +947,33 @@ __global__ __launch_bounds__(kLrBlock) void lr_mixed_kernel(
     EN += en;
   }
   if (valid) {
-    const double qf = static_cast<double>(a.w) * kTwoPi;  // k = 2 pi k_rev
+    const double qf = static_cast<double>(a.w) * kTwoPi;
     partial[static_cast<size_t>(split) * natom + i] =
         make_double4(qf * FX, qf * FY, qf * FZ, static_cast<double>(a.w) * EN);
   }
 }
 
+// f[i] += sum of the node-range partials (fixed order), plus the near-field
+// record when one is given.
 __global__ void lr_reduce_kernel(const double4 *__restrict__ partial,
                                  uint32_t nsplit, uint32_t natom,
+                                 const double4 *__restrict__ near,
                                  double *__restrict__ fx,
                                  double *__restrict__ fy,
                                  double *__restrict__ fz,
                                  double *__restrict__ en) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= natom) return;
-  double4 s = partial[i];
+  double4 s = nsplit ? partial[i] : make_double4(0.0, 0.0, 0.0, 0.0);
   for (uint32_t p = 1; p < nsplit; ++p) {
     const double4 v = partial[static_cast<size_t>(p) * natom + i];
+    s.x += v.x;
+    s.y += v.y;
+    s.z += v.z;
+    s.w += v.w;
+  }
+  if (near) {
+    const double4 v = near[i];
     s.x += v.x;
     s.y += v.y;
     s.z += v.z;
@@ -651,12 +989,14 @@ __global__ void lr_reduce_kernel(const double4 *__restrict__ partial,
 // near field: bare Coulomb over the cell and its 26 neighbours, clamped at the
 // grid edge (src/glst_force.cu:1170-1544).  One wave per cell, lanes = atoms.
 
-template <bool MIXED>
+// STORE: write (fx, fy, fz, en) to out4[i] (the caller adds it later);
+// otherwise accumulate into fx..en.
+template <bool MIXED, bool STORE>
 __global__ __launch_bounds__(64) void sr_kernel(
     const double4 *__restrict__ atoms, const uint32_t *__restrict__ start,
     uint32_t nx, uint32_t ny, uint32_t nz, double *__restrict__ fx,
     double *__restrict__ fy, double *__restrict__ fz,
-    double *__restrict__ en) {
+    double *__restrict__ en, double4 *__restrict__ out4) {
   const uint32_t cell = blockIdx.x;
   const uint32_t ix = cell / (ny * nz), iy = (cell / nz) % ny, iz = cell % nz;
   const uint32_t a0 = start[cell], a1 = start[cell + 1];
@@ -718,10 +1058,14 @@ __global__ __launch_bounds__(64) void sr_kernel(
       }
     }
     if (valid) {
-      fx[i] += FX;
-      fy[i] += FY;
-      fz[i] += FZ;
-      en[i] += EN;
+      if (STORE) {
+        out4[i] = make_double4(FX, FY, FZ, EN);
+      } else {
+        fx[i] += FX;
+        fy[i] += FY;
+        fz[i] += FZ;
+        en[i] += EN;
+      }
     }
   }
 }
@@ -870,7 +1214,7 @@ bool rmt_lds(bool mixed, const double2 *S, const double4 *k64,
   const size_t per_node = (static_cast<size_t>(g.ncell) + g.nx + g.ny + g.nz) *
                           sizeof(double2);
   uint32_t T = 0;
-  for (uint32_t t : {2u, 1u})
+  for (uint32_t t = GLST_RMT_TMAX; t >= 1; --t)
     if (t * per_node <= lds_limit) {
       T = t;
       break;
@@ -923,40 +1267,70 @@ void rmt_global(bool mixed, double2 *S, const double4 *k64,
 }
 
 void lr(bool mixed, const void *atoms, const uint32_t *cell_sorted,
-        const double4 *k64, const float4 *krev, const void *R, uint32_t Kp,
-        uint32_t natom, uint32_t nsplit, double4 *partial, hipStream_t s) {
-  if (natom == 0) return;
+        const LrTaskArgs &ta, const std::vector<LrGroup> &groups,
+        const float4 *krev, const void *R, uint32_t Kp, uint32_t natom,
+        uint32_t nsplit, double4 *partial, hipStream_t s) {
+  if (natom == 0 || nsplit == 0) return;
   const uint32_t ntile = blocks_for(natom, kLrBlock);
-  // split the node range into nsplit chunks of whole 64-node groups
-  const uint32_t groups = Kp / 64;
-  const uint32_t kchunk = ((groups + nsplit - 1) / nsplit) * 64;
   const dim3 grid(ntile, nsplit);
-  if (mixed)
-    lr_mixed_kernel<<<grid, kLrBlock, 0, s>>>(
+  const LrTasks tk{ta.node0, ta.nr, ta.koff, ta.dir64, ta.dir32,
+                   ta.kappa64, ta.kappa32, ta.split};
+  if (mixed && GLST_LR_VARIANT == 3) {
+    for (const LrGroup &gr : groups) {
+      if (gr.nsplit == 0) continue;
+      const LrNrKernel k =
+          kLrNrKernels[gr.nr <= static_cast<uint32_t>(kLrMaxUnrolled) ? gr.nr : 0];
+      k<<<dim3(ntile, gr.nsplit), kLrBlock, 0, s>>>(
+          static_cast<const float4 *>(atoms), cell_sorted, tk,
+          static_cast<const float2 *>(R), Kp, natom, ntile, gr.split0,
+          partial);
+    }
+  } else if (mixed && GLST_LR_VARIANT == 2) {
+    const uint32_t blocks64 = Kp / 64;
+    const uint32_t kchunk = ((blocks64 + nsplit - 1) / nsplit) * 64;
+    lr_mixed_nodes_kernel<<<grid, kLrBlock, 0, s>>>(
         static_cast<const float4 *>(atoms), cell_sorted, krev,
         static_cast<const float2 *>(R), Kp, natom, kchunk, ntile, partial);
+  } else if (mixed && GLST_LR_VARIANT == 1)
+    lr_mixed_flat_kernel<<<grid, kLrBlock, 0, s>>>(
+        static_cast<const float4 *>(atoms), cell_sorted, tk,
+        static_cast<const float2 *>(R), Kp, natom, ntile, partial);
+  else if (mixed)
+    lr_mixed_kernel<<<grid, kLrBlock, 0, s>>>(
+        static_cast<const float4 *>(atoms), cell_sorted, tk,
+        static_cast<const float2 *>(R), Kp, natom, ntile, partial);
   else
     lr_f64_kernel<<<grid, kLrBlock, 0, s>>>(
-        static_cast<const double4 *>(atoms), cell_sorted, k64,
-        static_cast<const double2 *>(R), Kp, natom, kchunk, ntile, partial);
+        static_cast<const double4 *>(atoms), cell_sorted, tk,
+        static_cast<const double2 *>(R), Kp, natom, ntile, partial);
 }
 
 void lr_reduce(const double4 *partial, uint32_t nsplit, uint32_t natom,
-               double *fx, double *fy, double *fz, double *en, hipStream_t s) {
+               const double4 *near, double *fx, double *fy, double *fz,
+               double *en, hipStream_t s) {
   if (natom == 0) return;
-  lr_reduce_kernel<<<blocks_for(natom, 256), 256, 0, s>>>(partial, nsplit,
-                                                         natom, fx, fy, fz, en);
+  lr_reduce_kernel<<<blocks_for(natom, 256), 256, 0, s>>>(
+      partial, nsplit, natom, near, fx, fy, fz, en);
 }
 
 void sr(bool mixed, const double4 *atoms, const uint32_t *start,
         const GridShape &g, double *fx, double *fy, double *fz, double *en,
-        hipStream_t s) {
-  if (mixed)
-    sr_kernel<true><<<g.ncell, 64, 0, s>>>(atoms, start, g.nx, g.ny, g.nz, fx,
-                                           fy, fz, en);
-  else
-    sr_kernel<false><<<g.ncell, 64, 0, s>>>(atoms, start, g.nx, g.ny, g.nz,
-                                            fx, fy, fz, en);
+        double4 *out4, hipStream_t s) {
+  if (out4) {
+    if (mixed)
+      sr_kernel<true, true><<<g.ncell, 64, 0, s>>>(atoms, start, g.nx, g.ny,
+                                                   g.nz, fx, fy, fz, en, out4);
+    else
+      sr_kernel<false, true><<<g.ncell, 64, 0, s>>>(atoms, start, g.nx, g.ny,
+                                                    g.nz, fx, fy, fz, en, out4);
+  } else {
+    if (mixed)
+      sr_kernel<true, false><<<g.ncell, 64, 0, s>>>(atoms, start, g.nx, g.ny,
+                                                    g.nz, fx, fy, fz, en, out4);
+    else
+      sr_kernel<false, false><<<g.ncell, 64, 0, s>>>(
+          atoms, start, g.nx, g.ny, g.nz, fx, fy, fz, en, out4);
+  }
 }
 
 void unpermute(const uint32_t *sorted_idx, uint32_t natom, const double *fx,

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <vector>
 
 namespace glst {
 namespace dev {
@@ -68,19 +69,39 @@ void rmt_global(bool mixed, double2 *S, const double4 *k64,
                 void *R, hipStream_t s);
 
 // ---- long-range back-projection (calc_lr_ef) -------------------------------
+// Task table of the direction-major node order: task t covers nodes
+// node0[t] .. node0[t]+nr[t]-1, all along direction dir[t], with radial
+// magnitudes kappa[koff[t] ..]; split s owns tasks split[s] .. split[s+1]-1.
+struct LrTaskArgs {
+  const uint32_t *node0, *nr, *koff;
+  const double4 *dir64;
+  const float4 *dir32;
+  const double *kappa64;
+  const float *kappa32;
+  const uint32_t *split;
+};
+// The splits of one alpha group: splits split0 .. split0+nsplit-1, all of
+// whose tasks have nr radial nodes.
+struct LrGroup {
+  uint32_t nr, split0, nsplit;
+};
 // partial[split*natom + i] = (fx, fy, fz, en) of sorted atom i over the
-// split's node range.
+// split's tasks.
 void lr(bool mixed, const void *atoms, const uint32_t *cell_sorted,
-        const double4 *k64, const float4 *krev, const void *R, uint32_t K,
-        uint32_t natom, uint32_t nsplit, double4 *partial, hipStream_t s);
-// f[i] += sum over splits (fixed order).
+        const LrTaskArgs &tasks, const std::vector<LrGroup> &groups,
+        const float4 *krev, const void *R, uint32_t Kp, uint32_t natom,
+        uint32_t nsplit, double4 *partial, hipStream_t s);
+// f[i] += sum over splits (fixed order) + near[i] (when near != nullptr).
 void lr_reduce(const double4 *partial, uint32_t nsplit, uint32_t natom,
-               double *fx, double *fy, double *fz, double *en, hipStream_t s);
+               const double4 *near, double *fx, double *fy, double *fz,
+               double *en, hipStream_t s);
 
 // ---- near field (calc_sr_ef) -----------------------------------------------
+// Accumulates into fx..en, or (out4 != nullptr) stores (fx, fy, fz, en) per
+// sorted atom into out4.
 void sr(bool mixed, const double4 *atoms, const uint32_t *start,
         const GridShape &g, double *fx, double *fy, double *fz, double *en,
-        hipStream_t s);
+        double4 *out4, hipStream_t s);
 
 // ---- result order (get_ef) ---------------------------------------------------
 void unpermute(const uint32_t *sorted_idx, uint32_t natom, const double *fx,

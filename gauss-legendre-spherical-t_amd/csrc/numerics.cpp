@@ -317,6 +317,10 @@ NodeTable build_nodes(double tol, const AlphaGroups &g,
         t.group.push_back(static_cast<uint32_t>(grp));
       }
     }
+    std::vector<double> kap;
+    for (size_t i = 0; i < gl.x.size(); ++i) kap.push_back(2.0 * a * z * gl.x[i]);
+    t.kappa.push_back(std::move(kap));
+    t.dirs.push_back(sph.s);
     t.gl.push_back(std::move(gl));
     t.sph_m.push_back(sph.m);
     t.sph_count.push_back(static_cast<uint32_t>(sph.s.size()));

@@ -80,6 +80,10 @@ struct NodeTable {
   std::vector<uint32_t> first, count;  // per group
   std::vector<GLRule> gl;
   std::vector<uint32_t> sph_m, sph_count;
+  // per group: radial magnitudes kappa_i = 2 a z x_i and the half-sphere
+  // directions s_m, so node (i, m) has k = kappa_i s_m
+  std::vector<std::vector<double>> kappa;
+  std::vector<std::vector<std::array<double, 3>>> dirs;
 };
 
 // cubature<CT>::initialize (src/cubature.cu:99-159): nodes ordered

@@ -122,6 +122,7 @@ def _load():
         "glst_get_sf": [P, P, P],
         "glst_get_rmt": [P, P, P],
         "glst_set_timing": [P, c_int],
+        "glst_set_option": [P, c_char_p, ctypes.c_int64],
         "glst_kernel_time": [P, c_char_p, POINTER(d), POINTER(u)],
         "glst_reset_timing": [P],
         "glst_coulomb_direct": [P, P, P, P, u, P, P, P, P, P],
@@ -409,6 +410,9 @@ class GLSTForce:
 
     def rmt(self):
         return self._field(_lib.glst_get_rmt)
+
+    def set_option(self, name: str, value: int):
+        self._c(_lib.glst_set_option(self._h, name.encode(), int(value)))
 
     # timing
     def set_timing(self, on: bool):

@@ -174,6 +174,13 @@ glst_status_t glst_get_sf(glst_handle_t h, double *re, double *im);
  * after sum_rmt_sf (rmt_sum_re_/rmt_sum_im_, ncell x num_nodes). */
 glst_status_t glst_get_rmt(glst_handle_t h, double *re, double *im);
 
+/* Tuning options of a handle:
+ *   "overlap_near" (default 1): calc_ener_force runs the near field on a
+ *   second stream of the handle, overlapped with S(k) / shell sums /
+ *   back-projection; 0 runs the stages one after another. */
+glst_status_t glst_set_option(glst_handle_t h, const char *name,
+                              int64_t value);
+
 /* Per-kernel timing with HIP events on the handle's stream.  When enabled,
  * every launch of the named kernel is bracketed by events; glst_kernel_time
  * returns the summed milliseconds and launch count since the last reset.
