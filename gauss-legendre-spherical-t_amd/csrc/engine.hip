@@ -187,9 +187,6 @@ struct Scope {
 
 using namespace glst;
 
-// elements allocated past the end of the remote-sum field
-constexpr size_t kRowSlack = 8;
-
 struct glst_engine {
   std::string last_error;
   int ct = GLST_CT_DOUBLE;
@@ -215,6 +212,7 @@ struct glst_engine {
   hipEvent_t ev_assigned = nullptr, ev_near = nullptr;
   DeviceBuffer near4;
   bool overlap_near = true;
+  int sf_variant = 1;  // mixed S(k): 0 lane per node, 1 lane per direction
   KernelTimer timer;
 
   // atoms
@@ -320,6 +318,8 @@ struct glst_engine {
     dev::fill_f64(fy.as<double>(), 0.0, natom, stream);
     dev::fill_f64(fz.as<double>(), 0.0, natom, stream);
     dev::fill_f64(en.as<double>(), 0.0, natom, stream);
+    // the mixed structure-factor kernel never writes the padded node columns
+    dev::fill_f64(S.as<double>(), 0.0, S.bytes() / sizeof(double), stream);
     check_launch("init");
     GLST_HIP(hipStreamSynchronize(stream));
   }
@@ -346,9 +346,7 @@ struct glst_engine {
       if (has_lr) {
         const size_t field = static_cast<size_t>(grid.ncell) * Kp;
         S.resize(field * sizeof(double2));
-        // slack past the last row: the back-projection prefetches ahead
-        R.resize((field + kRowSlack) *
-                 (mixed() ? sizeof(float2) : sizeof(double2)));
+        R.resize(field * (mixed() ? sizeof(float2) : sizeof(double2)));
         const uint32_t ntile = (natom + 255) / 256;
         // enough workgroups to fill the chip several times over
         make_splits(ntile > 0 ? (4096 + ntile - 1) / ntile : 1);
@@ -362,9 +360,11 @@ struct glst_engine {
   // each group gets about target * K_g / K splits of equal size.
   std::vector<uint32_t> split_host;
   std::vector<dev::LrGroup> lr_groups;
+  std::vector<dev::SfGroup> sf_groups;
   void make_splits(uint32_t target) {
     split_host.assign(1, 0);
     lr_groups.clear();
+    sf_groups.clear();
     uint32_t base = 0;
     for (size_t g = 0; g < nodes.dirs.size(); ++g) {
       const uint32_t M = static_cast<uint32_t>(nodes.dirs[g].size());
@@ -374,6 +374,8 @@ struct glst_engine {
       ns = std::max<uint32_t>(1, std::min<uint32_t>(ns, M));
       lr_groups.push_back({static_cast<uint32_t>(nodes.kappa[g].size()),
                            static_cast<uint32_t>(split_host.size() - 1), ns});
+      sf_groups.push_back(
+          {static_cast<uint32_t>(nodes.kappa[g].size()), base, M});
       for (uint32_t q = 1; q <= ns; ++q)
         split_host.push_back(base + static_cast<uint32_t>(
                                         (static_cast<uint64_t>(M) * q) / ns));
@@ -492,9 +494,9 @@ struct glst_engine {
     if (!has_lr || Kp == 0) return;  // src/glst_force.cu:667-670
     if (!assigned) throw StateError("calc_sf called before assign_atoms");
     Scope sc(timer, "sf", stream);
-    dev::sf(mixed(), lr_atoms(), cell_start.as<uint32_t>(),
-            k64.as<double4>(), krev.as<float4>(), Kp, grid, S.as<double2>(),
-            stream);
+    dev::sf(mixed(), sf_variant, lr_atoms(), cell_start.as<uint32_t>(),
+            k64.as<double4>(), krev.as<float4>(), task_args(), sf_groups, Kp,
+            grid, S.as<double2>(), stream);
     check_launch("sf");
     sf_done = true;
     rmt_done = false;
@@ -535,14 +537,18 @@ struct glst_engine {
     check_launch("lr_reduce");
   }
 
+  dev::LrTaskArgs task_args() const {
+    return {t_node0.as<uint32_t>(), t_nr.as<uint32_t>(),
+            t_koff.as<uint32_t>(),  t_dir64.as<double4>(),
+            t_dir32.as<float4>(),   kap64.as<double>(),
+            kap32.as<float>(),      t_split.as<uint32_t>()};
+  }
+
   void lr_partials() {
     Scope sc(timer, "lr", stream);
-    const dev::LrTaskArgs ta{t_node0.as<uint32_t>(), t_nr.as<uint32_t>(),
-                             t_koff.as<uint32_t>(),  t_dir64.as<double4>(),
-                             t_dir32.as<float4>(),   kap64.as<double>(),
-                             kap32.as<float>(),      t_split.as<uint32_t>()};
+    const dev::LrTaskArgs ta = task_args();
     dev::lr(mixed(), lr_atoms(), cell_sorted.as<uint32_t>(), ta, lr_groups,
-            krev.as<float4>(), R.as<void>(), Kp, natom, nsplit,
+            R.as<void>(), Kp, natom, nsplit,
             partial.as<double4>(), stream);
     check_launch("lr");
   }
@@ -930,8 +936,29 @@ glst_status_t glst_get_sf(glst_handle_t h, double *re, double *im) {
     const size_t n = static_cast<size_t>(h->grid.ncell) * h->K;
     std::vector<double> r(n), i(n);
     h->field_to_host(h->S, false, r.data(), i.data());
-    // the field holds running sums along z; difference them back
     const auto &g = h->grid;
+    if (h->mixed()) {
+      // the field holds cell-local sums; apply exp(-i k.c)
+      for (uint32_t c = 0; c < g.ncell; ++c) {
+        const uint32_t x = c / (g.ny * g.nz), y = (c / g.nz) % g.ny,
+                       z = c % g.nz;
+        const double cx = (x + 0.5) * g.h, cy = (y + 0.5) * g.h,
+                     cz = (z + 0.5) * g.h;
+        for (uint32_t k = 0; k < h->K; ++k) {
+          const double th =
+              h->nodes.kx[k] * cx + h->nodes.ky[k] * cy + h->nodes.kz[k] * cz;
+          const double cs = std::cos(th), sn = std::sin(th);
+          const size_t o = static_cast<size_t>(c) * h->K + k;
+          const double a = r[o], b = i[o];
+          r[o] = a * cs + b * sn;
+          i[o] = b * cs - a * sn;
+        }
+      }
+      if (re) std::memcpy(re, r.data(), n * 8);
+      if (im) std::memcpy(im, i.data(), n * 8);
+      return;
+    }
+    // the FP64 field holds running sums along z; difference them back
     for (uint32_t col = 0; col < g.nx * g.ny; ++col)
       for (uint32_t z = g.nz - 1; z >= 1; --z) {
         const size_t c = static_cast<size_t>(col) * g.nz + z;
@@ -985,7 +1012,10 @@ glst_status_t glst_set_option(glst_handle_t h, const char *name,
     const std::string n = name;
     if (n == "overlap_near")
       h->overlap_near = value != 0;
-    else
+    else if (n == "sf_variant") {
+      if (value != 0 && value != 1) throw ArgError("sf_variant must be 0 or 1");
+      h->sf_variant = static_cast<int>(value);
+    } else
       throw ArgError("unknown option \"" + n + "\"");
   });
 }

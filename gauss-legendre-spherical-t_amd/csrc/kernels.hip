@@ -40,14 +40,10 @@ __device__ inline uint32_t xcd_remap(uint32_t b, uint32_t nb) {
   return x * q + (x < r ? x : r) + i;
 }
 
-// Tuning knobs (compile-time): FP32 partial-sum lengths before the FP64 flush.
+// Tuning knobs (compile-time): FP32 partial-sum lengths before the FP64 flush
+// (atoms for the node-per-lane S(k), directions for the back-projection).
 #ifndef GLST_SF_FLUSH
 #define GLST_SF_FLUSH 32
-#endif
-// back-projection form for the mixed path: 0 direction tasks, 1 flat
-// prefetching walk, 2 node-major, 3 per-group unrolled directions
-#ifndef GLST_LR_VARIANT
-#define GLST_LR_VARIANT 3
 #endif
 #ifndef GLST_LR_FLUSH_TASKS
 #define GLST_LR_FLUSH_TASKS 4
@@ -57,7 +53,13 @@ __device__ inline uint32_t xcd_remap(uint32_t b, uint32_t nb) {
 #endif
 
 // sin(2 pi t), cos(2 pi t) for t in revolutions.  The hardware V_SIN_F32 /
-// V_COS_F32 take revolutions directly.
+// V_COS_F32 take revolutions directly and reduce the argument themselves:
+// on gfx950 their absolute error stays below 1.3e-7 for |t| up to 64
+// revolutions (measured against FP64 sin/cos over 4M uniform samples); the
+// ISA's input domain is |t| <= 256.  The phases here are k.d with d the
+// offset from the cell centre, a few revolutions at most for atoms inside
+// the box (|t| > 256 needs an atom hundreds of Angstrom outside it), so no
+// V_FRACT_F32 is spent (GLST_NO_FRACT=0 restores it).
 #if GLST_ACCURATE_SINCOS
 __device__ inline float sin_rev(float t) { return sinpif(2.0f * t); }
 __device__ inline float cos_rev(float t) { return cospif(2.0f * t); }
@@ -66,7 +68,7 @@ __device__ inline float sin_rev(float t) { return __builtin_amdgcn_sinf(t); }
 __device__ inline float cos_rev(float t) { return __builtin_amdgcn_cosf(t); }
 #endif
 #ifndef GLST_NO_FRACT
-#define GLST_NO_FRACT 0
+#define GLST_NO_FRACT 1
 #endif
 #if GLST_NO_FRACT
 __device__ inline float fract_rev(float t) { return t; }
@@ -292,22 +294,13 @@ __global__ __launch_bounds__(kSfBlock) void sf_f64_kernel(
 // which take revolutions; FP32 partial sums are flushed to FP64 every 32 atoms;
 // the cell phase exp(-i k.c) is applied in FP64 and advanced along z by an
 // FP64 rotation.
-__global__ __launch_bounds__(kSfBlock) void sf_mixed_kernel(
+__global__ __launch_bounds__(kSfBlock) void sf_mixed_node_kernel(
     const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
-    const double4 *__restrict__ k64, const float4 *__restrict__ krev,
-    uint32_t Kp, uint32_t ny, uint32_t nz, double h,
+    const float4 *__restrict__ krev, uint32_t Kp, uint32_t nz,
     double2 *__restrict__ S) {
   const uint32_t col = blockIdx.x;
   const uint32_t node = blockIdx.y * kSfBlock + threadIdx.x;
   const float4 kr = krev[node];
-  const double4 kd = k64[node];
-  const uint32_t ix = col / ny, iy = col % ny;
-  const double cx = (static_cast<double>(ix) + 0.5) * h;
-  const double cy = (static_cast<double>(iy) + 0.5) * h;
-  double ps, pc, ss, sc;
-  sincos(kd.x * cx + kd.y * cy + kd.z * (0.5 * h), &ps, &pc);
-  sincos(kd.z * h, &ss, &sc);
-  double pre = 0.0, pim = 0.0;  // running z-prefix
   for (uint32_t z = 0; z < nz; ++z) {
     const uint32_t cell = col * nz + z;
     const uint32_t a0 = start[cell], a1 = start[cell + 1];
@@ -321,28 +314,21 @@ __global__ __launch_bounds__(kSfBlock) void sf_mixed_kernel(
         const float4 b = atoms[j + 1];
         const float ta = fract_rev(kr.x * a.x + kr.y * a.y + kr.z * a.z);
         const float tb = fract_rev(kr.x * b.x + kr.y * b.y + kr.z * b.z);
-        re0 += a.w * cos_rev(ta);
-        im0 -= a.w * sin_rev(ta);
-        re1 += b.w * cos_rev(tb);
-        im1 -= b.w * sin_rev(tb);
+        re0 = fmaf(a.w, cos_rev(ta), re0);
+        im0 = fmaf(-a.w, sin_rev(ta), im0);
+        re1 = fmaf(b.w, cos_rev(tb), re1);
+        im1 = fmaf(-b.w, sin_rev(tb), im1);
       }
       if (j < je) {
         const float4 a = atoms[j];
         const float ta = fract_rev(kr.x * a.x + kr.y * a.y + kr.z * a.z);
-        re0 += a.w * cos_rev(ta);
-        im0 -= a.w * sin_rev(ta);
+        re0 = fmaf(a.w, cos_rev(ta), re0);
+        im0 = fmaf(-a.w, sin_rev(ta), im0);
       }
       RE += static_cast<double>(re0) + static_cast<double>(re1);
       IM += static_cast<double>(im0) + static_cast<double>(im1);
     }
-    // (RE + i IM) * (pc - i ps)
-    pre += RE * pc + IM * ps;
-    pim += IM * pc - RE * ps;
-    S[static_cast<size_t>(cell) * Kp + node] = make_double2(pre, pim);
-    const double npc = pc * sc - ps * ss;
-    const double nps = ps * sc + pc * ss;
-    pc = npc;
-    ps = nps;
+    S[static_cast<size_t>(cell) * Kp + node] = make_double2(RE, IM);
   }
 }
 
@@ -458,10 +444,6 @@ __global__ __launch_bounds__(kRmtBlock) void rmt_lds_kernel(
   const uint32_t tile = xcd_remap(blockIdx.x, ntile);
   const uint32_t node0 = tile * T;
   const uint32_t tid = threadIdx.x;
-  for (uint32_t e = tid; e < T * ncell; e += kRmtBlock) {
-    const uint32_t t = e % T, c = e / T;
-    P[t * ncell + c] = S[static_cast<size_t>(c) * Kp + node0 + t];
-  }
   if (MIXED) {
     for (uint32_t e = tid; e < T * nax; e += kRmtBlock) {
       const uint32_t t = e / nax, a = e % nax;
@@ -472,6 +454,35 @@ __global__ __launch_bounds__(kRmtBlock) void rmt_lds_kernel(
         axis_phase(phase + e, k.y, h, a - nx);
       else
         axis_phase(phase + e, k.z, h, a - nx - ny);
+    }
+    __syncthreads();
+    // S holds cell-local sums: S_c = exp(-i k.c) * local, then z prefix
+    for (uint32_t e = tid; e < T * ncell; e += kRmtBlock) {
+      const uint32_t t = e % T, c = e / T;
+      const double2 v = S[static_cast<size_t>(c) * Kp + node0 + t];
+      const uint32_t x = c / (ny * nz), y = (c / nz) % ny, z = c % nz;
+      const double2 *ph = phase + t * nax;
+      const double2 a = ph[x], b = ph[nx + y], cz = ph[nx + ny + z];
+      const double pr = a.x * b.x - a.y * b.y, pi = a.x * b.y + a.y * b.x;
+      const double er = pr * cz.x - pi * cz.y, ei = pr * cz.y + pi * cz.x;
+      // v * conj(e)
+      P[t * ncell + c] = make_double2(v.x * er + v.y * ei, v.y * er - v.x * ei);
+    }
+    __syncthreads();
+    for (uint32_t l = tid; l < T * nx * ny; l += kRmtBlock) {
+      double2 *p = P + (l / (nx * ny)) * ncell + (l % (nx * ny)) * nz;
+      double2 run = make_double2(0.0, 0.0);
+      for (uint32_t z = 0; z < nz; ++z) {
+        run.x += p[z].x;
+        run.y += p[z].y;
+        p[z] = run;
+      }
+    }
+  } else {
+    // the FP64 structure-factor kernel already stored running sums along z
+    for (uint32_t e = tid; e < T * ncell; e += kRmtBlock) {
+      const uint32_t t = e % T, c = e / T;
+      P[t * ncell + c] = S[static_cast<size_t>(c) * Kp + node0 + t];
     }
   }
   __syncthreads();
@@ -617,78 +628,6 @@ __global__ __launch_bounds__(kLrBlock) void lr_f64_kernel(
         make_double4(a.w * fx, a.w * fy, a.w * fz, a.w * en);
 }
 
-// Mixed: cell-local FP32 phases against exp(+i k.c) R_c(k) (float2); FP32
-// partial sums over four tasks (a few dozen nodes) are flushed to FP64.
-__global__ __launch_bounds__(kLrBlock) void lr_mixed_kernel(
-    const float4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
-    LrTasks tk, const float2 *__restrict__ R, uint32_t Kp, uint32_t natom,
-    uint32_t ntile, double4 *__restrict__ partial) {
-  const uint32_t tile = xcd_remap(blockIdx.x, ntile);
-  const uint32_t i = tile * kLrBlock + threadIdx.x;
-  const uint32_t split = blockIdx.y;
-  const bool valid = i < natom;
-  const uint32_t ii = valid ? i : natom - 1;
-  const float4 a = atoms[ii];
-  const float2 *Rc = R + static_cast<size_t>(cell_sorted[ii]) * Kp;
-  double FX = 0.0, FY = 0.0, FZ = 0.0, EN = 0.0;
-  float fx = 0.f, fy = 0.f, fz = 0.f, en = 0.f;
-  uint32_t pending = 0;
-  const uint32_t t1 = tk.split[split + 1];
-  for (uint32_t t = tk.split[split]; t < t1; ++t) {
-    const uint32_t n0 = tk.node0[t], nr = tk.nr[t];
-    const float *kap = tk.kappa32 + tk.koff[t];
-    const float4 d = tk.dir32[t];
-    const float p = d.x * a.x + d.y * a.y + d.z * a.z;
-    float e0 = 0.f, b0 = 0.f, e1 = 0.f, b1 = 0.f;
-    uint32_t j = 0;
-    for (; j + 2 <= nr; j += 2) {
-      const float k0 = kap[j], k1 = kap[j + 1];
-      const float t0 = fract_rev(k0 * p), t1v = fract_rev(k1 * p);
-      const float s0 = sin_rev(t0), c0 = cos_rev(t0);
-      const float s1 = sin_rev(t1v), c1 = cos_rev(t1v);
-      const float2 r0 = Rc[n0 + j], r1 = Rc[n0 + j + 1];
-      // Re(e^{it} R) accumulated as two FMAs; Im(e^{it} R) weighted by kappa
-      e0 = fmaf(c0, r0.x, e0);
-      e0 = fmaf(-s0, r0.y, e0);
-      b0 = fmaf(k0, fmaf(c0, r0.y, s0 * r0.x), b0);
-      e1 = fmaf(c1, r1.x, e1);
-      e1 = fmaf(-s1, r1.y, e1);
-      b1 = fmaf(k1, fmaf(c1, r1.y, s1 * r1.x), b1);
-    }
-    if (j < nr) {
-      const float k0 = kap[j];
-      const float t0 = fract_rev(k0 * p);
-      const float s0 = sin_rev(t0), c0 = cos_rev(t0);
-      const float2 r0 = Rc[n0 + j];
-      e0 = fmaf(c0, r0.x, e0);
-      e0 = fmaf(-s0, r0.y, e0);
-      b0 = fmaf(k0, fmaf(c0, r0.y, s0 * r0.x), b0);
-    }
-    const float b = b0 + b1;
-    en += e0 + e1;
-    fx += d.x * b;
-    fy += d.y * b;
-    fz += d.z * b;
-    if (++pending == GLST_LR_FLUSH_TASKS) {
-      FX += fx;
-      FY += fy;
-      FZ += fz;
-      EN += en;
-      fx = fy = fz = en = 0.f;
-      pending = 0;
-    }
-  }
-  FX += fx;
-  FY += fy;
-  FZ += fz;
-  EN += en;
-  if (valid) {
-    const double qf = static_cast<double>(a.w) * kTwoPi;  // k = 2 pi k_rev
-    partial[static_cast<size_t>(split) * natom + i] =
-        make_double4(qf * FX, qf * FY, qf * FZ, static_cast<double>(a.w) * EN);
-  }
-}
-
 // Mixed, fixed radial count: every split lies inside one alpha group, so all
 // its tasks have the same number NR of radial nodes and share the same
 // kappa_i.  With NR a template parameter the kappa_i sit in registers for the
@@ -832,126 +771,123 @@ constexpr int kLrMaxUnrolled = 40;
 const auto kLrNrKernels =
     lr_nr_table(std::make_integer_sequence<int, kLrMaxUnrolled + 1>{});
 
-// Mixed, flat form: a split's tasks cover one contiguous node range, so the
-// lane walks it node by node and the remote sum of node k+2 is fetched while
-// node k is computed (R carries a few elements of slack past the last row).
-// At each direction boundary (a uniform branch) the radial sum is folded into
-// the force along that direction and the next direction is projected.
-__global__ __launch_bounds__(kLrBlock) void lr_mixed_flat_kernel(
-    const float4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
-    LrTasks tk, const float2 *__restrict__ R, uint32_t Kp, uint32_t natom,
-    uint32_t ntile, double4 *__restrict__ partial) {
-  const uint32_t tile = xcd_remap(blockIdx.x, ntile);
-  const uint32_t i = tile * kLrBlock + threadIdx.x;
-  const uint32_t split = blockIdx.y;
-  const bool valid = i < natom;
-  const uint32_t ii = valid ? i : natom - 1;
-  const float4 a = atoms[ii];
-  const float2 *Rc = R + static_cast<size_t>(cell_sorted[ii]) * Kp;
-  double FX = 0.0, FY = 0.0, FZ = 0.0, EN = 0.0;
-  const uint32_t tb = tk.split[split], te = tk.split[split + 1];
-  if (tb < te) {
-    uint32_t t = tb;
-    uint32_t k = tk.node0[tb];
-    const uint32_t kend = tk.node0[te - 1] + tk.nr[te - 1];
-    uint32_t nr = tk.nr[t], j = 0, pending = 0;
-    const float *kap = tk.kappa32 + tk.koff[t];
-    float4 d = tk.dir32[t];
-    float p = d.x * a.x + d.y * a.y + d.z * a.z;
-    float fx = 0.f, fy = 0.f, fz = 0.f, en = 0.f, e = 0.f, b = 0.f;
-    float2 r1 = Rc[k], r2 = Rc[k + 1];
-    for (; k < kend; ++k) {
-      const float2 r = r1;
-      r1 = r2;
-      r2 = Rc[k + 2];
-      const float kj = kap[j];
-      const float tt = fract_rev(kj * p);
-      const float sn = sin_rev(tt), cs = cos_rev(tt);
-      e = fmaf(cs, r.x, e);
-      e = fmaf(-sn, r.y, e);
-      b = fmaf(kj, fmaf(cs, r.y, sn * r.x), b);
-      if (++j == nr) {
-        en += e;
-        fx += d.x * b;
-        fy += d.y * b;
-        fz += d.z * b;
-        e = b = 0.f;
-        j = 0;
-        if (++pending == GLST_LR_FLUSH_TASKS) {
-          FX += fx;
-          FY += fy;
-          FZ += fz;
-          EN += en;
-          fx = fy = fz = en = 0.f;
-          pending = 0;
+// Mixed structure factors, one instantiation per radial count NR: lane = one
+// direction s of an alpha group, one wave per (cell, 64 directions), atom
+// records are wave-uniform.  Per atom the lane projects p = s.d once; each of
+// the NR radial nodes then costs one multiply for its phase kappa_i p, fract,
+// sin, cos and two FMAs.  The kappa_i are wave-uniform registers.  The lane
+// stores the cell-local sums sum_j q_j exp(-i k.d_j) in FP64 (FP32 partial
+// sums over at most 128 atoms); the shell-sum kernel applies the FP64
+// cell-centre phase exp(-i k.c).
+constexpr uint32_t kSfDirs = 64;
+constexpr uint32_t kSfChunk = 128;
+
+template <int NR>
+__global__ __launch_bounds__(kSfDirs) void sf_mixed_nr_kernel(
+    const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
+    LrTasks tk, uint32_t task0, uint32_t ntask, uint32_t Kp, uint32_t nz,
+    double2 *__restrict__ S) {
+  const uint32_t t = task0 + blockIdx.y * kSfDirs + threadIdx.x;
+  const bool active = t < task0 + ntask;
+  const uint32_t tt = active ? t : task0;
+  float kap[NR];
+  const float *kp = tk.kappa32 + tk.koff[task0];
+#pragma unroll
+  for (int i = 0; i < NR; ++i) kap[i] = kp[i];
+  const float4 d = tk.dir32[tt];
+  const uint32_t n0 = tk.node0[tt];
+  {
+    const uint32_t cell = blockIdx.x;
+    const uint32_t a0 = start[cell], a1 = start[cell + 1];
+    double2 *out = S + static_cast<size_t>(cell) * Kp + n0;
+    if (a0 == a1) {
+      if (active) {
+#pragma unroll
+        for (int i = 0; i < NR; ++i) out[i] = make_double2(0.0, 0.0);
+      }
+      return;
+    }
+    for (uint32_t jb = a0; jb < a1; jb += kSfChunk) {
+      const uint32_t je = (jb + kSfChunk < a1) ? jb + kSfChunk : a1;
+      float re[NR], im[NR];
+#pragma unroll
+      for (int i = 0; i < NR; ++i) re[i] = im[i] = 0.f;
+      for (uint32_t j = jb; j < je; ++j) {
+        const float4 a = atoms[j];
+        const float p = d.x * a.x + d.y * a.y + d.z * a.z;
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+          const float ph = fract_rev(kap[i] * p);
+          re[i] = fmaf(a.w, cos_rev(ph), re[i]);
+          im[i] = fmaf(-a.w, sin_rev(ph), im[i]);
         }
-        if (++t < te) {
-          nr = tk.nr[t];
-          kap = tk.kappa32 + tk.koff[t];
-          d = tk.dir32[t];
-          p = d.x * a.x + d.y * a.y + d.z * a.z;
+      }
+      if (active) {
+        if (jb == a0) {
+#pragma unroll
+          for (int i = 0; i < NR; ++i)
+            out[i] = make_double2(static_cast<double>(re[i]),
+                                  static_cast<double>(im[i]));
+        } else {  // cells denser than one chunk: add in FP64
+#pragma unroll
+          for (int i = 0; i < NR; ++i) {
+            const double2 o = out[i];
+            out[i] = make_double2(o.x + static_cast<double>(re[i]),
+                                  o.y + static_cast<double>(im[i]));
+          }
         }
       }
     }
-    FX += fx;
-    FY += fy;
-    FZ += fz;
-    EN += en;
-  }
-  if (valid) {
-    const double qf = static_cast<double>(a.w) * kTwoPi;  // k = 2 pi k_rev
-    partial[static_cast<size_t>(split) * natom + i] =
-        make_double4(qf * FX, qf * FY, qf * FZ, static_cast<double>(a.w) * EN);
   }
 }
 
-// Mixed, node-major form: every node carries its own k (krev); the node
-// range of a split is cut into 64-node blocks, two nodes per step.
-__global__ __launch_bounds__(kLrBlock) void lr_mixed_nodes_kernel(
-    const float4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
-    const float4 *__restrict__ krev, const float2 *__restrict__ R,
-    uint32_t Kp, uint32_t natom, uint32_t kchunk, uint32_t ntile,
-    double4 *__restrict__ partial) {
-  const uint32_t tile = xcd_remap(blockIdx.x, ntile);
-  const uint32_t i = tile * kLrBlock + threadIdx.x;
-  const uint32_t split = blockIdx.y;
-  const uint32_t k0 = split * kchunk;
-  const uint32_t k1 = (k0 + kchunk < Kp) ? k0 + kchunk : Kp;
-  const bool valid = i < natom;
-  const uint32_t ii = valid ? i : natom - 1;
-  const float4 a = atoms[ii];
-  const float4 *Rc =
-      reinterpret_cast<const float4 *>(R + static_cast<size_t>(cell_sorted[ii]) * Kp);
-  double FX = 0.0, FY = 0.0, FZ = 0.0, EN = 0.0;
-  for (uint32_t kb = k0; kb < k1; kb += 64) {
-    float fx = 0.f, fy = 0.f, fz = 0.f, en = 0.f;
-#pragma unroll 2
-    for (uint32_t k = kb; k < kb + 64; k += 2) {
-      const float4 n0 = krev[k];
-      const float4 n1 = krev[k + 1];
-      const float4 r = Rc[k >> 1];  // R_k, R_{k+1}
-      const float t0 = fract_rev(n0.x * a.x + n0.y * a.y + n0.z * a.z);
-      const float t1 = fract_rev(n1.x * a.x + n1.y * a.y + n1.z * a.z);
-      const float s0 = sin_rev(t0), c0 = cos_rev(t0);
-      const float s1 = sin_rev(t1), c1 = cos_rev(t1);
-      const float A0 = c0 * r.x - s0 * r.y, B0 = c0 * r.y + s0 * r.x;
-      const float A1 = c1 * r.z - s1 * r.w, B1 = c1 * r.w + s1 * r.z;
-      en += A0 + A1;
-      fx += n0.x * B0 + n1.x * B1;
-      fy += n0.y * B0 + n1.y * B1;
-      fz += n0.z * B0 + n1.z * B1;
+// Any radial count (beyond the unrolled cases).
+__global__ __launch_bounds__(kSfDirs) void sf_mixed_any_kernel(
+    const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
+    LrTasks tk, uint32_t task0, uint32_t ntask, uint32_t Kp, uint32_t nz,
+    double2 *__restrict__ S) {
+  const uint32_t t = task0 + blockIdx.y * kSfDirs + threadIdx.x;
+  const bool active = t < task0 + ntask;
+  const uint32_t tt = active ? t : task0;
+  const float *kap = tk.kappa32 + tk.koff[task0];
+  const uint32_t nr = tk.nr[task0];
+  const float4 d = tk.dir32[tt];
+  const uint32_t n0 = tk.node0[tt];
+  {
+    const uint32_t cell = blockIdx.x;
+    const uint32_t a0 = start[cell], a1 = start[cell + 1];
+    double2 *out = S + static_cast<size_t>(cell) * Kp + n0;
+    for (uint32_t i = 0; i < nr; ++i) {
+      double RE = 0.0, IM = 0.0;
+      for (uint32_t jb = a0; jb < a1; jb += kSfChunk) {
+        const uint32_t je = (jb + kSfChunk < a1) ? jb + kSfChunk : a1;
+        float re = 0.f, im = 0.f;
+        for (uint32_t j = jb; j < je; ++j) {
+          const float4 a = atoms[j];
+          const float ph =
+              fract_rev(kap[i] * (d.x * a.x + d.y * a.y + d.z * a.z));
+          re = fmaf(a.w, cos_rev(ph), re);
+          im = fmaf(-a.w, sin_rev(ph), im);
+        }
+        RE += re;
+        IM += im;
+      }
+      if (active) out[i] = make_double2(RE, IM);
     }
-    FX += fx;
-    FY += fy;
-    FZ += fz;
-    EN += en;
-  }
-  if (valid) {
-    const double qf = static_cast<double>(a.w) * kTwoPi;
-    partial[static_cast<size_t>(split) * natom + i] =
-        make_double4(qf * FX, qf * FY, qf * FZ, static_cast<double>(a.w) * EN);
   }
 }
+
+typedef void (*SfNrKernel)(const float4 *, const uint32_t *, LrTasks,
+                           uint32_t, uint32_t, uint32_t, uint32_t, double2 *);
+
+template <int... N>
+constexpr auto sf_nr_table(std::integer_sequence<int, N...>) {
+  return std::array<SfNrKernel, sizeof...(N)>{{&sf_mixed_nr_kernel<N + 1>...}};
+}
+
+constexpr int kSfMaxUnrolled = 40;
+const auto kSfNrKernels =
+    sf_nr_table(std::make_integer_sequence<int, kSfMaxUnrolled>{});
 
 // f[i] += sum of the node-range partials (fixed order), plus the near-field
 // record when one is given.
@@ -1194,17 +1130,36 @@ void bin_scatter(bool mixed, const double *rx, const double *ry,
         g.nz, g.h, sorted_idx, cell_sorted, atoms64, atoms32);
 }
 
-void sf(bool mixed, const void *atoms, const uint32_t *start,
-        const double4 *k64, const float4 *krev, uint32_t Kp,
-        const GridShape &g, double2 *S, hipStream_t s) {
-  const dim3 grid(g.nx * g.ny, Kp / kSfBlock);
-  if (mixed)
-    sf_mixed_kernel<<<grid, kSfBlock, 0, s>>>(
-        static_cast<const float4 *>(atoms), start, k64, krev, Kp, g.ny, g.nz,
-        g.h, S);
-  else
+void sf(bool mixed, int variant, const void *atoms, const uint32_t *start,
+        const double4 *k64, const float4 *krev, const LrTaskArgs &ta,
+        const std::vector<SfGroup> &groups, uint32_t Kp, const GridShape &g,
+        double2 *S, hipStream_t s) {
+  if (!mixed) {
+    const dim3 grid(g.nx * g.ny, Kp / kSfBlock);
     sf_f64_kernel<<<grid, kSfBlock, 0, s>>>(static_cast<const double4 *>(atoms),
                                             start, k64, Kp, g.nz, S);
+    return;
+  }
+  if (variant == 0) {
+    const dim3 grid(g.nx * g.ny, Kp / kSfBlock);
+    sf_mixed_node_kernel<<<grid, kSfBlock, 0, s>>>(
+        static_cast<const float4 *>(atoms), start, krev, Kp, g.nz, S);
+    return;
+  }
+  const LrTasks tk{ta.node0, ta.nr,      ta.koff,   ta.dir64,
+                   ta.dir32, ta.kappa64, ta.kappa32, ta.split};
+  for (const SfGroup &gr : groups) {
+    if (gr.ntask == 0) continue;
+    const dim3 grid(g.ncell, (gr.ntask + kSfDirs - 1) / kSfDirs);
+    if (gr.nr >= 1 && gr.nr <= static_cast<uint32_t>(kSfMaxUnrolled))
+      kSfNrKernels[gr.nr - 1]<<<grid, kSfDirs, 0, s>>>(
+          static_cast<const float4 *>(atoms), start, tk, gr.task0, gr.ntask,
+          Kp, g.nz, S);
+    else
+      sf_mixed_any_kernel<<<grid, kSfDirs, 0, s>>>(
+          static_cast<const float4 *>(atoms), start, tk, gr.task0, gr.ntask,
+          Kp, g.nz, S);
+  }
 }
 
 bool rmt_lds(bool mixed, const double2 *S, const double4 *k64,
@@ -1268,14 +1223,14 @@ void rmt_global(bool mixed, double2 *S, const double4 *k64,
 
 void lr(bool mixed, const void *atoms, const uint32_t *cell_sorted,
         const LrTaskArgs &ta, const std::vector<LrGroup> &groups,
-        const float4 *krev, const void *R, uint32_t Kp, uint32_t natom,
+        const void *R, uint32_t Kp, uint32_t natom,
         uint32_t nsplit, double4 *partial, hipStream_t s) {
   if (natom == 0 || nsplit == 0) return;
   const uint32_t ntile = blocks_for(natom, kLrBlock);
   const dim3 grid(ntile, nsplit);
   const LrTasks tk{ta.node0, ta.nr, ta.koff, ta.dir64, ta.dir32,
                    ta.kappa64, ta.kappa32, ta.split};
-  if (mixed && GLST_LR_VARIANT == 3) {
+  if (mixed) {
     for (const LrGroup &gr : groups) {
       if (gr.nsplit == 0) continue;
       const LrNrKernel k =
@@ -1285,21 +1240,7 @@ void lr(bool mixed, const void *atoms, const uint32_t *cell_sorted,
           static_cast<const float2 *>(R), Kp, natom, ntile, gr.split0,
           partial);
     }
-  } else if (mixed && GLST_LR_VARIANT == 2) {
-    const uint32_t blocks64 = Kp / 64;
-    const uint32_t kchunk = ((blocks64 + nsplit - 1) / nsplit) * 64;
-    lr_mixed_nodes_kernel<<<grid, kLrBlock, 0, s>>>(
-        static_cast<const float4 *>(atoms), cell_sorted, krev,
-        static_cast<const float2 *>(R), Kp, natom, kchunk, ntile, partial);
-  } else if (mixed && GLST_LR_VARIANT == 1)
-    lr_mixed_flat_kernel<<<grid, kLrBlock, 0, s>>>(
-        static_cast<const float4 *>(atoms), cell_sorted, tk,
-        static_cast<const float2 *>(R), Kp, natom, ntile, partial);
-  else if (mixed)
-    lr_mixed_kernel<<<grid, kLrBlock, 0, s>>>(
-        static_cast<const float4 *>(atoms), cell_sorted, tk,
-        static_cast<const float2 *>(R), Kp, natom, ntile, partial);
-  else
+  } else
     lr_f64_kernel<<<grid, kLrBlock, 0, s>>>(
         static_cast<const double4 *>(atoms), cell_sorted, tk,
         static_cast<const double2 *>(R), Kp, natom, ntile, partial);

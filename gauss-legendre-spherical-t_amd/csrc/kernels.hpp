@@ -47,10 +47,21 @@ void bin_scatter(bool mixed, const double *rx, const double *ry,
                  hipStream_t s);
 
 // ---- structure factors (calc_sf) -------------------------------------------
-// S[cell*K + k] = sum_{j in cell} q_j exp(-i k.r_j), FP64 complex.
-void sf(bool mixed, const void *atoms, const uint32_t *start,
-        const double4 *k64, const float4 *krev, uint32_t K,
-        const GridShape &g, double2 *S, hipStream_t s);
+struct LrTaskArgs;
+// Tasks (directions) of one alpha group for the mixed structure factors.
+struct SfGroup {
+  uint32_t nr, task0, ntask;
+};
+// FP64: S[cell*Kp + k] = running sum along z of sum_{j in cell} q_j
+// exp(-i k.r_j).  Mixed: S[cell*Kp + k] = sum_{j in cell} q_j exp(-i k.d_j)
+// with d_j the position relative to the cell centre.  Mixed variant 0: one
+// lane per node (krev, all Kp columns written); variant 1: one lane per
+// direction of an alpha group, the group's radii unrolled (padded node
+// columns are not written).
+void sf(bool mixed, int variant, const void *atoms, const uint32_t *start,
+        const double4 *k64, const float4 *krev, const LrTaskArgs &tasks,
+        const std::vector<SfGroup> &groups, uint32_t Kp, const GridShape &g,
+        double2 *S, hipStream_t s);
 
 // ---- remote (shell) sums (sum_rmt_sf) --------------------------------------
 // R_c(k) = W_k * sum_{c' : r_in < cheb(c,c') <= r_out} S_c'(k); the mixed
@@ -89,7 +100,7 @@ struct LrGroup {
 // split's tasks.
 void lr(bool mixed, const void *atoms, const uint32_t *cell_sorted,
         const LrTaskArgs &tasks, const std::vector<LrGroup> &groups,
-        const float4 *krev, const void *R, uint32_t Kp, uint32_t natom,
+        const void *R, uint32_t Kp, uint32_t natom,
         uint32_t nsplit, double4 *partial, hipStream_t s);
 // f[i] += sum over splits (fixed order) + near[i] (when near != nullptr).
 void lr_reduce(const double4 *partial, uint32_t nsplit, uint32_t natom,

@@ -162,3 +162,57 @@ def test_accumulate_and_stages(cuda, water1k, ct):
     again = _host(f.get_ef())
     for a, b in zip(one, again):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("sf_variant", [0, 1])
+def test_fields_mixed_match_oracle(cuda, water1k, sf_variant):
+    """Mixed-precision S(k) and shell sums R(k) against the FP64 oracle's
+    fields, for both S(k) kernels (bar: 1e-5 of the field's largest entry,
+    FP32 phases of cell-local coordinates)."""
+    import torch
+    import glst_amd as g
+    from oracle import oracle as o
+    rx, ry, rz, qc = water1k
+    tol = 1e-4
+    f = g.GLSTForce(rx.size, tol, 31.04, rcut=9.0, ct="float")
+    f.set_option("sf_variant", sf_variant)
+    f.assign_atoms(*_dev([rx, ry, rz, qc], torch))
+    f.calc_sf()
+    sre, sim = f.sf()
+    f.sum_rmt_sf()
+    rre, rim = f.rmt()
+    _, (osre, osim, orre, orim) = o.glst(rx, ry, rz, qc, tol, 31.04, 9.0,
+                                         fields=True)
+    scale = np.abs(osre).max()
+    assert np.abs(sre - osre).max() <= 1e-5 * scale
+    assert np.abs(sim - osim).max() <= 1e-5 * scale
+    rscale = np.abs(orre).max()
+    assert np.abs(rre - orre).max() <= 1e-5 * rscale
+    assert np.abs(rim - orim).max() <= 1e-5 * rscale
+
+
+def test_mixed_options_agree(cuda, water1k):
+    """Every combination of the tuning options gives the same forces to FP32
+    rounding (the near field's overlap changes no arithmetic at all)."""
+    import torch
+    import glst_amd as g
+    rx, ry, rz, qc = water1k
+    d = _dev([rx, ry, rz, qc], torch)
+    out = {}
+    for sv in (0, 1):
+        for ov in (0, 1):
+            f = g.GLSTForce(rx.size, 1e-5, 31.04, rcut=9.0, ct="float")
+            f.set_option("sf_variant", sv)
+            f.set_option("overlap_near", ov)
+            f.calc_ener_force(*d)
+            out[sv, ov] = _host(f.get_ef())
+            f.close()
+    for sv in (0, 1):
+        for a, b in zip(out[sv, 0], out[sv, 1]):
+            np.testing.assert_array_equal(a, b)
+    assert rel_err(out[1, 0][:3], out[0, 0][:3]) < 1e-6
+    f = g.GLSTForce(rx.size, 1e-5, 31.04, rcut=9.0, ct="float")
+    with pytest.raises(Exception):
+        f.set_option("sf_variant", 2)
+    with pytest.raises(Exception):
+        f.set_option("no_such_option", 1)
