@@ -156,8 +156,12 @@ def main():
     log(f"relative force error vs direct FP64 sum: mixed {err_mixed:.3e} "
         f"fp64 {err_fp64:.3e}")
 
-    # roofline of the dominant kernel (back-projection "lr"): 20 flops and
-    # 2 transcendentals per (atom, node) pair over the padded node table.
+    # Roofline of the dominant kernel, the back-projection (calc_lr_ef:
+    # lr_mixed_nr_kernel<NR>, one launch per alpha group; avg_ms is the sum
+    # of the group launches of one step, HIP events on the handle's stream).
+    # Algorithmic work: 20 flops (sin and cos counted as one each) per
+    # (atom, node) pair over the padded node table, the count of the
+    # reference's per-pair formulation.
     K = info.num_nodes
     Kp = (K + 255) // 256 * 256
     pairs = natom * Kp
@@ -169,7 +173,8 @@ def main():
         with open(tf) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
     roof = {
-        "kernel": "lr_mixed_kernel (calc_lr_ef back-projection)",
+        "kernel": "lr_mixed_nr_kernel<NR> (calc_lr_ef back-projection, "
+                  "all alpha-group launches of one step)",
         "bound": "valu",
         "achieved": achieved,
         "peak": PEAK_FP32_VALU_TFLOPS,

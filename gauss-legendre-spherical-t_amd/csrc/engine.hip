@@ -211,8 +211,10 @@ struct glst_engine {
   hipStream_t side = nullptr;
   hipEvent_t ev_assigned = nullptr, ev_near = nullptr;
   DeviceBuffer near4;
-  bool overlap_near = true;
-  int sf_variant = 1;  // mixed S(k): 0 lane per node, 1 lane per direction
+  // near field on the side stream: 0 off, 1 alongside S(k) onwards,
+  // 2 alongside the shell sums onwards
+  int overlap_near = 1;
+  int rmt_shape = 0;  // shell sums: 0 auto, 1 one node per workgroup, 2 global
   KernelTimer timer;
 
   // atoms
@@ -220,13 +222,14 @@ struct glst_engine {
   DeviceBuffer sorted_idx, cell_sorted, atoms64, atoms32;
   DeviceBuffer fx, fy, fz, en;
   // nodes (device order: direction-major inside each group)
-  DeviceBuffer k64, krev, group, r_in, r_out;
+  DeviceBuffer k64, group, r_in, r_out;
   std::vector<uint32_t> dev_of_ref;  // device index of each reference node
   // back-projection tasks (one per group and direction) and their splits
   DeviceBuffer t_node0, t_nr, t_koff, t_dir64, t_dir32, kap64, kap32, t_split;
   uint32_t ntask = 0;
   // fields
   DeviceBuffer S, R, partial;
+  DeviceBuffer rmt_work;  // FP64 tables of the global shell-sum path
   uint32_t nsplit = 1;
   bool assigned = false;
   bool sf_done = false;
@@ -243,7 +246,7 @@ struct glst_engine {
     const DeviceBuffer *bufs[] = {&atom_cell, &rank,  &hist,       &cell_count,
                                   &cell_start, &maxc, &sorted_idx, &cell_sorted,
                                   &atoms64,   &fx,    &fy,         &fz,
-                                  &en,        &k64,   &krev,       &group,
+                                  &en,        &k64,   &rmt_work,   &group,
                                   &r_in,      &r_out, &S,          &R,
                                   &partial,   &atoms32, &t_node0,   &t_nr,
                                   &t_koff,    &t_dir64, &t_dir32,   &kap64,
@@ -345,7 +348,7 @@ struct glst_engine {
     if (Kp > 0) {
       if (has_lr) {
         const size_t field = static_cast<size_t>(grid.ncell) * Kp;
-        S.resize(field * sizeof(double2));
+        S.resize(field * (mixed() ? sizeof(float2) : sizeof(double2)));
         R.resize(field * (mixed() ? sizeof(float2) : sizeof(double2)));
         const uint32_t ntile = (natom + 255) / 256;
         // enough workgroups to fill the chip several times over
@@ -389,7 +392,6 @@ struct glst_engine {
     if (Kp == 0) return;
     const double inv2pi = 1.0 / 6.283185307179586476925286766559;
     std::vector<double4> h64(Kp, make_double4(0, 0, 0, 0));
-    std::vector<float4> hrev(Kp, make_float4(0, 0, 0, 0));
     std::vector<uint32_t> hg(Kp, 0);
     std::vector<uint32_t> tn0, tnr, tko;
     std::vector<double4> td64;
@@ -421,10 +423,6 @@ struct glst_engine {
           dev_of_ref[ref] = dev;
           h64[dev] = make_double4(nodes.kx[ref], nodes.ky[ref], nodes.kz[ref],
                                   nodes.w[ref]);
-          hrev[dev] = make_float4(static_cast<float>(nodes.kx[ref] * inv2pi),
-                                  static_cast<float>(nodes.ky[ref] * inv2pi),
-                                  static_cast<float>(nodes.kz[ref] * inv2pi),
-                                  0.f);
           hg[dev] = static_cast<uint32_t>(g);
         }
       }
@@ -438,7 +436,6 @@ struct glst_engine {
         GLST_HIP(hipMemcpy(b.as<void>(), src, bytes, hipMemcpyHostToDevice));
     };
     up(k64, h64.data(), Kp * sizeof(double4));
-    up(krev, hrev.data(), Kp * sizeof(float4));
     up(group, hg.data(), Kp * 4);
     up(r_in, groups.r_in.data(), groups.r_in.size() * 4);
     up(r_out, groups.r_out.data(), groups.r_out.size() * 4);
@@ -494,9 +491,8 @@ struct glst_engine {
     if (!has_lr || Kp == 0) return;  // src/glst_force.cu:667-670
     if (!assigned) throw StateError("calc_sf called before assign_atoms");
     Scope sc(timer, "sf", stream);
-    dev::sf(mixed(), sf_variant, lr_atoms(), cell_start.as<uint32_t>(),
-            k64.as<double4>(), krev.as<float4>(), task_args(), sf_groups, Kp,
-            grid, S.as<double2>(), stream);
+    dev::sf(mixed(), lr_atoms(), cell_start.as<uint32_t>(), k64.as<double4>(),
+            task_args(), sf_groups, Kp, grid, S.as<void>(), stream);
     check_launch("sf");
     sf_done = true;
     rmt_done = false;
@@ -506,14 +502,17 @@ struct glst_engine {
     if (!has_lr || Kp == 0) return;
     if (!sf_done) throw StateError("sum_rmt_sf called before calc_sf");
     Scope sc(timer, "rmt", stream);
-    if (!dev::rmt_lds(mixed(), S.as<double2>(), k64.as<double4>(),
-                      group.as<uint32_t>(), r_in.as<uint32_t>(),
-                      r_out.as<uint32_t>(), Kp, grid, lds_limit,
-                      R.as<void>(), stream)) {
-      dev::rmt_global(mixed(), S.as<double2>(), k64.as<double4>(),
-                      group.as<uint32_t>(), r_in.as<uint32_t>(),
-                      r_out.as<uint32_t>(), Kp, grid, R.as<void>(), stream);
-      sf_done = false;  // S now holds prefix sums
+    if (rmt_shape == 2 ||
+        !dev::rmt_lds(mixed(), rmt_shape == 1 ? 1 : 2, S.as<void>(),
+                      k64.as<double4>(), group.as<uint32_t>(),
+                      r_in.as<uint32_t>(), r_out.as<uint32_t>(), Kp, grid,
+                      lds_limit, R.as<void>(), stream)) {
+      // tables too large for LDS: FP64 work field, allocated on first use
+      rmt_work.resize(static_cast<size_t>(grid.ncell) * Kp * sizeof(double2));
+      dev::rmt_global(mixed(), S.as<void>(), rmt_work.as<double2>(),
+                      k64.as<double4>(), group.as<uint32_t>(),
+                      r_in.as<uint32_t>(), r_out.as<uint32_t>(), Kp, grid,
+                      R.as<void>(), stream);
     }
     check_launch("rmt");
     rmt_done = true;
@@ -579,16 +578,24 @@ struct glst_engine {
       calc_sr_ef();
       return;
     }
-    GLST_HIP(hipEventRecord(ev_assigned, stream));
-    GLST_HIP(hipStreamWaitEvent(side, ev_assigned, 0));
-    {
-      Scope sc(timer, "sr", side);
-      dev::sr(mixed(), atoms64.as<double4>(), cell_start.as<uint32_t>(), grid,
-              nullptr, nullptr, nullptr, nullptr, near4.as<double4>(), side);
-      check_launch("sr");
-    }
-    GLST_HIP(hipEventRecord(ev_near, side));
+    auto launch_near = [&] {
+      GLST_HIP(hipEventRecord(ev_assigned, stream));
+      GLST_HIP(hipStreamWaitEvent(side, ev_assigned, 0));
+      {
+        Scope sc(timer, "sr", side);
+        dev::sr(mixed(), atoms64.as<double4>(), cell_start.as<uint32_t>(),
+                grid, nullptr, nullptr, nullptr, nullptr, near4.as<double4>(),
+                side);
+        check_launch("sr");
+      }
+      GLST_HIP(hipEventRecord(ev_near, side));
+    };
+    // Started with S(k) (mode 1) the near field fills the chip's slack
+    // through the whole far field; started after S(k) (mode 2) it delays
+    // the LDS-heavy shell-sum workgroups and ends up slower.
+    if (overlap_near == 1 || !(has_lr && Kp > 0)) launch_near();
     calc_sf();
+    if (overlap_near == 2 && has_lr && Kp > 0) launch_near();
     sum_rmt_sf();
     if (has_lr && Kp > 0) lr_partials();
     GLST_HIP(hipStreamWaitEvent(stream, ev_near, 0));
@@ -634,16 +641,22 @@ struct glst_engine {
                             hipMemcpyDeviceToHost, stream));
   }
 
-  void field_to_host(const DeviceBuffer &buf, bool single, double *re,
-                     double *im) {
+  // Device field (R: c*Kp + k; S: node-pair tiles) to host arrays of
+  // ncell x K in reference node order.  FP32 fields hold `fstride` floats
+  // per element, the first two being (re, im).
+  void field_to_host(const DeviceBuffer &buf, bool single, uint32_t fstride,
+                     bool tiled, double *re, double *im) {
     const size_t n = static_cast<size_t>(grid.ncell) * Kp;
     std::vector<double> tmp(2 * n);
     if (single) {
-      std::vector<float> f(2 * n);
+      std::vector<float> f(fstride * n);
       GLST_HIP(hipMemcpyAsync(f.data(), buf.as<void>(), f.size() * 4,
                               hipMemcpyDeviceToHost, stream));
       GLST_HIP(hipStreamSynchronize(stream));
-      for (size_t i = 0; i < 2 * n; ++i) tmp[i] = f[i];
+      for (size_t i = 0; i < n; ++i) {
+        tmp[2 * i] = f[fstride * i];
+        tmp[2 * i + 1] = f[fstride * i + 1];
+      }
     } else {
       GLST_HIP(hipMemcpyAsync(tmp.data(), buf.as<void>(), tmp.size() * 8,
                               hipMemcpyDeviceToHost, stream));
@@ -651,7 +664,8 @@ struct glst_engine {
     }
     for (uint32_t c = 0; c < grid.ncell; ++c)
       for (uint32_t k = 0; k < K; ++k) {
-        const size_t s = static_cast<size_t>(c) * Kp + dev_of_ref[k];
+        const size_t s = tiled ? dev::sf_slot(dev_of_ref[k], c, grid.ncell)
+                               : static_cast<size_t>(c) * Kp + dev_of_ref[k];
         const size_t d = static_cast<size_t>(c) * K + k;
         if (re) re[d] = tmp[2 * s];
         if (im) im[d] = tmp[2 * s + 1];
@@ -935,7 +949,7 @@ glst_status_t glst_get_sf(glst_handle_t h, double *re, double *im) {
     if (!h->sf_done) throw StateError("glst_get_sf: no structure factors");
     const size_t n = static_cast<size_t>(h->grid.ncell) * h->K;
     std::vector<double> r(n), i(n);
-    h->field_to_host(h->S, false, r.data(), i.data());
+    h->field_to_host(h->S, h->mixed(), 2, true, r.data(), i.data());
     const auto &g = h->grid;
     if (h->mixed()) {
       // the field holds cell-local sums; apply exp(-i k.c)
@@ -979,7 +993,7 @@ glst_status_t glst_get_rmt(glst_handle_t h, double *re, double *im) {
     if (!h->rmt_done) throw StateError("glst_get_rmt: no remote sums");
     const size_t n = static_cast<size_t>(h->grid.ncell) * h->K;
     std::vector<double> r(n), i(n);
-    h->field_to_host(h->R, h->mixed(), r.data(), i.data());
+    h->field_to_host(h->R, h->mixed(), 2, false, r.data(), i.data());
     if (h->mixed()) {
       // undo the stored exp(+i k.c) factor in FP64
       const auto &g = h->grid;
@@ -1010,11 +1024,13 @@ glst_status_t glst_set_option(glst_handle_t h, const char *name,
   if (!name) return GLST_ERR_INVALID;
   return guarded(h, [&] {
     const std::string n = name;
-    if (n == "overlap_near")
-      h->overlap_near = value != 0;
-    else if (n == "sf_variant") {
-      if (value != 0 && value != 1) throw ArgError("sf_variant must be 0 or 1");
-      h->sf_variant = static_cast<int>(value);
+    if (n == "overlap_near") {
+      if (value < 0 || value > 2) throw ArgError("overlap_near must be 0, 1 or 2");
+      h->overlap_near = static_cast<int>(value);
+    }
+    else if (n == "rmt_shape") {
+      if (value < 0 || value > 2) throw ArgError("rmt_shape must be 0, 1 or 2");
+      h->rmt_shape = static_cast<int>(value);
     } else
       throw ArgError("unknown option \"" + n + "\"");
   });

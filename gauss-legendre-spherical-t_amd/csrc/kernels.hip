@@ -23,6 +23,7 @@
 #include <climits>
 #include <array>
 #include <cmath>
+#include <type_traits>
 #include <utility>
 
 namespace glst {
@@ -246,15 +247,16 @@ __global__ void bin_scatter_kernel(
 
 constexpr uint32_t kSfBlock = 256;
 
-// Both structure-factor kernels walk one z-column of cells per lane and store
-// the running sum along z, i.e. the first pass of the summed-volume table of
-// src/glst_force.cu:702-724, so the shell-sum kernel starts at the y pass.
+// The FP64 structure-factor kernel walks one z-column of cells per lane and
+// stores the running sum along z, i.e. the first pass of the summed-volume
+// table of src/glst_force.cu:702-724, so the shell-sum kernel starts at the y
+// pass.  S is stored in node-pair tiles (sf_slot).
 
 // FP64: S_c(k) = sum_j q_j (cos(k.r_j) - i sin(k.r_j)), absolute coordinates
 // (src/glst_force.cu:601-664).  blockIdx.x = z-column, blockIdx.y = node tile.
 __global__ __launch_bounds__(kSfBlock) void sf_f64_kernel(
     const double4 *__restrict__ atoms, const uint32_t *__restrict__ start,
-    const double4 *__restrict__ k64, uint32_t Kp, uint32_t nz,
+    const double4 *__restrict__ k64, uint32_t nz, uint32_t ncell,
     double2 *__restrict__ S) {
   const uint32_t col = blockIdx.x;
   const uint32_t node = blockIdx.y * kSfBlock + threadIdx.x;
@@ -285,60 +287,13 @@ __global__ __launch_bounds__(kSfBlock) void sf_f64_kernel(
     }
     pre += re0 + re1;
     pim += im0 + im1;
-    S[static_cast<size_t>(cell) * Kp + node] = make_double2(pre, pim);
+    S[sf_slot(node, cell, ncell)] = make_double2(pre, pim);
   }
 }
 
-// Mixed: the phase splits as k.r_j = k.c + k.d_j with c the cell centre.
-// k.d_j (|k.d_j| < a few revolutions) runs in FP32 on the hardware sin/cos,
-// which take revolutions; FP32 partial sums are flushed to FP64 every 32 atoms;
-// the cell phase exp(-i k.c) is applied in FP64 and advanced along z by an
-// FP64 rotation.
-__global__ __launch_bounds__(kSfBlock) void sf_mixed_node_kernel(
-    const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
-    const float4 *__restrict__ krev, uint32_t Kp, uint32_t nz,
-    double2 *__restrict__ S) {
-  const uint32_t col = blockIdx.x;
-  const uint32_t node = blockIdx.y * kSfBlock + threadIdx.x;
-  const float4 kr = krev[node];
-  for (uint32_t z = 0; z < nz; ++z) {
-    const uint32_t cell = col * nz + z;
-    const uint32_t a0 = start[cell], a1 = start[cell + 1];
-    double RE = 0.0, IM = 0.0;
-    for (uint32_t jb = a0; jb < a1; jb += GLST_SF_FLUSH) {
-      const uint32_t je = (jb + GLST_SF_FLUSH < a1) ? jb + GLST_SF_FLUSH : a1;
-      float re0 = 0.f, im0 = 0.f, re1 = 0.f, im1 = 0.f;
-      uint32_t j = jb;
-      for (; j + 2 <= je; j += 2) {
-        const float4 a = atoms[j];
-        const float4 b = atoms[j + 1];
-        const float ta = fract_rev(kr.x * a.x + kr.y * a.y + kr.z * a.z);
-        const float tb = fract_rev(kr.x * b.x + kr.y * b.y + kr.z * b.z);
-        re0 = fmaf(a.w, cos_rev(ta), re0);
-        im0 = fmaf(-a.w, sin_rev(ta), im0);
-        re1 = fmaf(b.w, cos_rev(tb), re1);
-        im1 = fmaf(-b.w, sin_rev(tb), im1);
-      }
-      if (j < je) {
-        const float4 a = atoms[j];
-        const float ta = fract_rev(kr.x * a.x + kr.y * a.y + kr.z * a.z);
-        re0 = fmaf(a.w, cos_rev(ta), re0);
-        im0 = fmaf(-a.w, sin_rev(ta), im0);
-      }
-      RE += static_cast<double>(re0) + static_cast<double>(re1);
-      IM += static_cast<double>(im0) + static_cast<double>(im1);
-    }
-    S[static_cast<size_t>(cell) * Kp + node] = make_double2(RE, IM);
-  }
-}
 
 // ---------------------------------------------------------------------------
 // shell sums
-
-struct LdsGrid {
-  const double2 *p;
-  __device__ double2 at(uint32_t c) const { return p[c]; }
-};
 
 struct GlobalGrid {
   const double2 *p;
@@ -388,25 +343,32 @@ __device__ inline double2 cube_sum(const G &P, uint32_t x, uint32_t y,
   return box_sum(P, x0, y0, z0, x1, y1, z1, ny, nz);
 }
 
+// Mixed-path remote sum of one (cell, node): exp(+i k.c) R_c(k), FP64 in,
+// FP32 out.
+__device__ inline void store_rmt_mixed(void *R, size_t off, double re,
+                                       double im) {
+  static_cast<float2 *>(R)[off] =
+      make_float2(static_cast<float>(re), static_cast<float>(im));
+}
+
 // R_c(k) = W_k (cube_out - cube_in); the mixed path multiplies by
 // exp(+i k.c) = a b c (the cell's three FP64 axis factors) and stores FP32,
 // the FP64 path stores R_c(k).
 template <bool MIXED, class G>
 __device__ inline void shell_store(const G &P, uint32_t x, uint32_t y,
-                                   uint32_t z, uint32_t node, double w,
+                                   uint32_t z, uint32_t node, const double4 &k,
                                    uint32_t rin, uint32_t rout, uint32_t nx,
                                    uint32_t ny, uint32_t nz, double2 a,
                                    double2 b, double2 c, uint32_t Kp,
                                    void *R) {
   const double2 o = cube_sum(P, x, y, z, rout, nx, ny, nz);
   const double2 in = cube_sum(P, x, y, z, rin, nx, ny, nz);
-  const double re = (o.x - in.x) * w, im = (o.y - in.y) * w;
+  const double re = (o.x - in.x) * k.w, im = (o.y - in.y) * k.w;
   const size_t off = static_cast<size_t>((x * ny + y) * nz + z) * Kp + node;
   if (MIXED) {
     const double pr = a.x * b.x - a.y * b.y, pi = a.x * b.y + a.y * b.x;
     const double cr = pr * c.x - pi * c.y, ci = pr * c.y + pi * c.x;
-    static_cast<float2 *>(R)[off] = make_float2(
-        static_cast<float>(re * cr - im * ci), static_cast<float>(re * ci + im * cr));
+    store_rmt_mixed(R, off, re * cr - im * ci, re * ci + im * cr);
   } else {
     static_cast<double2 *>(R)[off] = make_double2(re, im);
   }
@@ -420,19 +382,93 @@ __device__ inline void axis_phase(double2 *dst, double kd, double h,
   *dst = make_double2(c, s);
 }
 
-#ifndef GLST_RMT_BLOCK
-#define GLST_RMT_BLOCK 1024
-#endif
-#ifndef GLST_RMT_TMAX
-#define GLST_RMT_TMAX 2
-#endif
-constexpr uint32_t kRmtBlock = GLST_RMT_BLOCK;
+// ---- LDS-resident tables (grids whose table of one node fits in LDS) ----
 
-// One workgroup holds the z-prefixed table of T nodes in LDS, completes the
-// summed-volume table along y and x, and gathers the shells.
+constexpr uint32_t kRmtBlock = 1024;
+// Values held in registers per thread while a line of the table is scanned,
+// and S slots loaded per thread before the first is stored.
+constexpr uint32_t kRmtBatch = 16;
+constexpr uint32_t kRmtLoadBatch = 4;
+
+// Inclusive running sum along one line of the LDS table (n elements, stride
+// elements apart), in batches of kRmtBatch.
+__device__ inline void lds_scan(double2 *p, uint32_t n, uint32_t stride) {
+  double2 run = make_double2(0.0, 0.0);
+  for (uint32_t i0 = 0; i0 < n; i0 += kRmtBatch) {
+    const uint32_t nb = (n - i0 < kRmtBatch) ? n - i0 : kRmtBatch;
+    double2 v[kRmtBatch];
+#pragma unroll
+    for (uint32_t u = 0; u < kRmtBatch; ++u)  // clamped: always in the line
+      v[u] = p[(i0 + (u < nb ? u : nb - 1)) * stride];
+#pragma unroll
+    for (uint32_t u = 0; u < kRmtBatch; ++u)
+      if (u < nb) {
+        run.x += v[u].x;
+        run.y += v[u].y;
+        p[(i0 + u) * stride] = run;
+      }
+  }
+}
+
+// The x/z part of a cube of Chebyshev radius r around (x, ., z), clamped to
+// the grid: table offsets of the x-planes x1 and x0-1, the z-rows z1 and
+// z0-1, and whether x0 / z0 touch the grid's low face.
+struct CubeXZ {
+  uint32_t bx1, bxm, z1, zm;
+  bool xb, zb;
+};
+
+__device__ inline CubeXZ cube_xz(uint32_t x, uint32_t z, uint32_t r,
+                                 uint32_t nx, uint32_t ny, uint32_t nz) {
+  const uint32_t x0 = x < r ? 0 : x - r, z0 = z < r ? 0 : z - r;
+  const uint32_t x1 = (x + r >= nx) ? nx - 1 : x + r;
+  CubeXZ c;
+  c.xb = x0 == 0;
+  c.zb = z0 == 0;
+  c.bx1 = x1 * ny * nz;
+  c.bxm = (c.xb ? 0 : x0 - 1) * ny * nz;
+  c.z1 = (z + r >= nz) ? nz - 1 : z + r;
+  c.zm = c.zb ? 0 : z0 - 1;
+  return c;
+}
+
+// Cube sum at row y from the summed-volume table P: the inclusion-exclusion
+// of box_sum with the same term order; every corner is read (all offsets lie
+// inside the table) and those outside the grid are selected away.
+__device__ inline double2 cube_y(const double2 *P, const CubeXZ &c,
+                                 uint32_t y, uint32_t r, uint32_t ny,
+                                 uint32_t nz) {
+  const uint32_t y0 = y < r ? 0 : y - r;
+  const uint32_t y1 = (y + r >= ny) ? ny - 1 : y + r;
+  const bool yb = y0 == 0;
+  const uint32_t o1 = y1 * nz, om = (yb ? 0 : y0 - 1) * nz;
+  const double2 r0 = P[c.bx1 + o1 + c.z1];  // (x1, y1, z1)
+  const double2 r1 = P[c.bxm + o1 + c.z1];  // (x0-1, y1, z1)
+  const double2 r2 = P[c.bx1 + om + c.z1];  // (x1, y0-1, z1)
+  const double2 r3 = P[c.bx1 + o1 + c.zm];  // (x1, y1, z0-1)
+  const double2 r4 = P[c.bxm + om + c.z1];  // (x0-1, y0-1, z1)
+  const double2 r5 = P[c.bxm + o1 + c.zm];  // (x0-1, y1, z0-1)
+  const double2 r6 = P[c.bx1 + om + c.zm];  // (x1, y0-1, z0-1)
+  const double2 r7 = P[c.bxm + om + c.zm];  // (x0-1, y0-1, z0-1)
+  const double2 zero = make_double2(0.0, 0.0);
+  const double2 g1 = c.xb ? zero : r1, g2 = yb ? zero : r2,
+                g3 = c.zb ? zero : r3, g4 = (c.xb || yb) ? zero : r4,
+                g5 = (c.xb || c.zb) ? zero : r5, g6 = (yb || c.zb) ? zero : r6,
+                g7 = (c.xb || yb || c.zb) ? zero : r7;
+  return make_double2(r0.x - g1.x - g2.x - g3.x + g4.x + g5.x + g6.x - g7.x,
+                      r0.y - g1.y - g2.y - g3.y + g4.y + g5.y + g6.y - g7.y);
+}
+
+// One workgroup holds the summed-volume tables of T (1 or 2) consecutive
+// nodes in LDS.  (1) It reads the nodes' S slots, one contiguous run of the
+// node-pair tiles, converting to FP64; (2) mixed path: turns the cell-local
+// sums into S_c = exp(-i k.c) local with the FP64 axis factors and takes the
+// running sum along z (the FP64 kernel stored z-running sums already);
+// (3) completes the table along y and x; (4) gathers the shells, lanes on
+// consecutive (node, z), every thread walking y.
 template <bool MIXED>
 __global__ __launch_bounds__(kRmtBlock) void rmt_lds_kernel(
-    const double2 *__restrict__ S, const double4 *__restrict__ k64,
+    const void *__restrict__ Sv, const double4 *__restrict__ k64,
     const uint32_t *__restrict__ group, const uint32_t *__restrict__ r_in,
     const uint32_t *__restrict__ r_out, uint32_t Kp, uint32_t T,
     uint32_t ntile, uint32_t nx, uint32_t ny, uint32_t nz, double h,
@@ -444,6 +480,56 @@ __global__ __launch_bounds__(kRmtBlock) void rmt_lds_kernel(
   const uint32_t tile = xcd_remap(blockIdx.x, ntile);
   const uint32_t node0 = tile * T;
   const uint32_t tid = threadIdx.x;
+  // (1) slot pair (2m, 2m+1) of cell c sits at index 2 (m ncell + c)
+  const size_t run0 = static_cast<size_t>(node0 >> 1) * ncell;
+  if (T == 2) {
+    for (uint32_t c0 = tid; c0 < ncell; c0 += kRmtBlock * kRmtLoadBatch) {
+      double2 lo[kRmtLoadBatch], hi[kRmtLoadBatch];
+#pragma unroll
+      for (uint32_t u = 0; u < kRmtLoadBatch; ++u) {
+        const uint32_t c0u = c0 + u * kRmtBlock;
+        const uint32_t c = c0u < ncell ? c0u : ncell - 1;
+        if (MIXED) {
+          const float4 v = static_cast<const float4 *>(Sv)[run0 + c];
+          lo[u] = make_double2(v.x, v.y);
+          hi[u] = make_double2(v.z, v.w);
+        } else {
+          const double2 *p = static_cast<const double2 *>(Sv) + 2 * (run0 + c);
+          lo[u] = p[0];
+          hi[u] = p[1];
+        }
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < kRmtLoadBatch; ++u) {
+        const uint32_t c = c0 + u * kRmtBlock;
+        if (c < ncell) {
+          P[c] = lo[u];
+          P[ncell + c] = hi[u];
+        }
+      }
+    }
+  } else {
+    const uint32_t par = node0 & 1;
+    for (uint32_t c0 = tid; c0 < ncell; c0 += kRmtBlock * kRmtLoadBatch) {
+      double2 v[kRmtLoadBatch];
+#pragma unroll
+      for (uint32_t u = 0; u < kRmtLoadBatch; ++u) {
+        const uint32_t c0u = c0 + u * kRmtBlock;
+        const size_t slot = 2 * (run0 + (c0u < ncell ? c0u : ncell - 1)) + par;
+        if (MIXED) {
+          const float2 f = static_cast<const float2 *>(Sv)[slot];
+          v[u] = make_double2(f.x, f.y);
+        } else {
+          v[u] = static_cast<const double2 *>(Sv)[slot];
+        }
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < kRmtLoadBatch; ++u) {
+        const uint32_t c = c0 + u * kRmtBlock;
+        if (c < ncell) P[c] = v[u];
+      }
+    }
+  }
   if (MIXED) {
     for (uint32_t e = tid; e < T * nax; e += kRmtBlock) {
       const uint32_t t = e / nax, a = e % nax;
@@ -455,76 +541,132 @@ __global__ __launch_bounds__(kRmtBlock) void rmt_lds_kernel(
       else
         axis_phase(phase + e, k.z, h, a - nx - ny);
     }
-    __syncthreads();
-    // S holds cell-local sums: S_c = exp(-i k.c) * local, then z prefix
-    for (uint32_t e = tid; e < T * ncell; e += kRmtBlock) {
-      const uint32_t t = e % T, c = e / T;
-      const double2 v = S[static_cast<size_t>(c) * Kp + node0 + t];
-      const uint32_t x = c / (ny * nz), y = (c / nz) % ny, z = c % nz;
-      const double2 *ph = phase + t * nax;
-      const double2 a = ph[x], b = ph[nx + y], cz = ph[nx + ny + z];
-      const double pr = a.x * b.x - a.y * b.y, pi = a.x * b.y + a.y * b.x;
-      const double er = pr * cz.x - pi * cz.y, ei = pr * cz.y + pi * cz.x;
-      // v * conj(e)
-      P[t * ncell + c] = make_double2(v.x * er + v.y * ei, v.y * er - v.x * ei);
-    }
-    __syncthreads();
-    for (uint32_t l = tid; l < T * nx * ny; l += kRmtBlock) {
-      double2 *p = P + (l / (nx * ny)) * ncell + (l % (nx * ny)) * nz;
-      double2 run = make_double2(0.0, 0.0);
-      for (uint32_t z = 0; z < nz; ++z) {
-        run.x += p[z].x;
-        run.y += p[z].y;
-        p[z] = run;
-      }
-    }
-  } else {
-    // the FP64 structure-factor kernel already stored running sums along z
-    for (uint32_t e = tid; e < T * ncell; e += kRmtBlock) {
-      const uint32_t t = e % T, c = e / T;
-      P[t * ncell + c] = S[static_cast<size_t>(c) * Kp + node0 + t];
-    }
   }
   __syncthreads();
-  // inclusive prefix along y, then x (src/glst_force.cu:726-772)
+  if (MIXED) {
+    // (2) z-lines, neighbouring lanes on neighbouring lines
+    for (uint32_t l = tid; l < T * nx * ny; l += kRmtBlock) {
+      const uint32_t t = l / (nx * ny), xy = l - t * (nx * ny);
+      const uint32_t x = xy / ny, y = xy - x * ny;
+      const double2 *ph = phase + t * nax;
+      const double2 a = ph[x], b = ph[nx + y];
+      const double pr = a.x * b.x - a.y * b.y, pi = a.x * b.y + a.y * b.x;
+      const double2 *pz = ph + nx + ny;
+      double2 *p = P + t * ncell + xy * nz;
+      double2 run = make_double2(0.0, 0.0);
+      for (uint32_t z0 = 0; z0 < nz; z0 += kRmtBatch) {
+        const uint32_t nb = (nz - z0 < kRmtBatch) ? nz - z0 : kRmtBatch;
+        double2 v[kRmtBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kRmtBatch; ++u)  // clamped: always in the line
+          v[u] = p[z0 + (u < nb ? u : nb - 1)];
+#pragma unroll
+        for (uint32_t u = 0; u < kRmtBatch; ++u)
+          if (u < nb) {
+            const double2 c = pz[z0 + u];
+            const double er = pr * c.x - pi * c.y, ei = pr * c.y + pi * c.x;
+            // v * conj(e), accumulated along z
+            run.x += v[u].x * er + v[u].y * ei;
+            run.y += v[u].y * er - v[u].x * ei;
+            p[z0 + u] = run;
+          }
+      }
+    }
+    __syncthreads();
+  }
+  // (3) inclusive prefix along y, then x (src/glst_force.cu:726-772)
   for (uint32_t l = tid; l < T * nx * nz; l += kRmtBlock) {
     const uint32_t t = l / (nx * nz), xz = l % (nx * nz);
-    double2 *p = P + t * ncell + (xz / nz) * ny * nz + (xz % nz);
-    double2 run = make_double2(0.0, 0.0);
-    for (uint32_t y = 0; y < ny; ++y) {
-      run.x += p[y * nz].x;
-      run.y += p[y * nz].y;
-      p[y * nz] = run;
-    }
+    lds_scan(P + t * ncell + (xz / nz) * ny * nz + (xz % nz), ny, nz);
   }
   __syncthreads();
   for (uint32_t l = tid; l < T * ny * nz; l += kRmtBlock) {
     const uint32_t t = l / (ny * nz), yz = l % (ny * nz);
-    double2 *p = P + t * ncell + yz;
-    double2 run = make_double2(0.0, 0.0);
-    for (uint32_t x = 0; x < nx; ++x) {
-      run.x += p[x * ny * nz].x;
-      run.y += p[x * ny * nz].y;
-      p[x * ny * nz] = run;
-    }
+    lds_scan(P + t * ncell + yz, nx, ny * nz);
   }
   __syncthreads();
-  for (uint32_t e = tid; e < T * ncell; e += kRmtBlock) {
-    const uint32_t t = e % T, c = e / T;
+  // (4) shells; y is cut into segments so that the workgroup stays busy
+  const uint32_t lines = T * nx * nz;
+  uint32_t nseg = kRmtBlock / lines;
+  nseg = nseg < 1 ? 1 : (nseg > ny ? ny : nseg);
+  const uint32_t ylen = (ny + nseg - 1) / nseg;
+  for (uint32_t task = tid; task < lines * nseg; task += kRmtBlock) {
+    const uint32_t t = task % T;
+    uint32_t q = task / T;
+    const uint32_t z = q % nz;
+    q /= nz;
+    const uint32_t x = q % nx, seg = q / nx;
     const uint32_t node = node0 + t;
     const uint32_t g = group[node];
-    const uint32_t x = c / (ny * nz), y = (c / nz) % ny, z = c % nz;
+    const uint32_t rin = r_in[g], rout = r_out[g];
+    const double w = k64[node].w;
+    const double2 *Pt = P + t * ncell;
+    const CubeXZ co = cube_xz(x, z, rout, nx, ny, nz);
+    const CubeXZ ci = cube_xz(x, z, rin, nx, ny, nz);
     const double2 *ph = phase + t * nax;
-    const double2 one = make_double2(1.0, 0.0);
-    LdsGrid grid{P + t * ncell};
-    shell_store<MIXED>(grid, x, y, z, node, k64[node].w, r_in[g], r_out[g],
-                       nx, ny, nz, MIXED ? ph[x] : one,
-                       MIXED ? ph[nx + y] : one, MIXED ? ph[nx + ny + z] : one,
-                       Kp, R);
+    const uint32_t yb = seg * ylen;
+    const uint32_t ye = (yb + ylen < ny) ? yb + ylen : ny;
+    for (uint32_t y = yb; y < ye; ++y) {
+      const double2 o = cube_y(Pt, co, y, rout, ny, nz);
+      const double2 in = cube_y(Pt, ci, y, rin, ny, nz);
+      const double re = (o.x - in.x) * w, im = (o.y - in.y) * w;
+      const size_t off = static_cast<size_t>((x * ny + y) * nz + z) * Kp + node;
+      if (MIXED) {
+        // exp(+i k.c) R_c(k)
+        const double2 a = ph[x], b = ph[nx + y], c = ph[nx + ny + z];
+        const double pr = a.x * b.x - a.y * b.y, pi = a.x * b.y + a.y * b.x;
+        const double er = pr * c.x - pi * c.y, ei = pr * c.y + pi * c.x;
+        store_rmt_mixed(R, off, re * er - im * ei, re * ei + im * er);
+      } else {
+        static_cast<double2 *>(R)[off] = make_double2(re, im);
+      }
+    }
   }
 }
 
-// Global-memory fallback for grids whose table does not fit in LDS.
+// ---- global-memory tables (grids too large for LDS) ----
+
+// G[c*Kp + k] = the node-pair-tiled S converted to FP64 cell-major order;
+// the mixed path applies exp(-i k.c) to the cell-local sums and takes the
+// running sum along z (the FP64 slots hold z-running sums already).  One
+// thread per (node, z-column).
+template <bool MIXED>
+__global__ void rmt_global_load_kernel(const void *__restrict__ Sv,
+                                       const double4 *__restrict__ k64,
+                                       double2 *__restrict__ G, uint32_t Kp,
+                                       uint32_t col0, uint32_t nx,
+                                       uint32_t ny, uint32_t nz, double h) {
+  const uint32_t node = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t col = col0 + blockIdx.y;
+  if (node >= Kp || col >= nx * ny) return;
+  const uint32_t ncell = nx * ny * nz;
+  if (!MIXED) {
+    for (uint32_t z = 0; z < nz; ++z) {
+      const uint32_t cell = col * nz + z;
+      G[static_cast<size_t>(cell) * Kp + node] =
+          static_cast<const double2 *>(Sv)[sf_slot(node, cell, ncell)];
+    }
+    return;
+  }
+  const uint32_t x = col / ny, y = col % ny;
+  const double4 k = k64[node];
+  double2 a, b;
+  axis_phase(&a, k.x, h, x);
+  axis_phase(&b, k.y, h, y);
+  const double pr = a.x * b.x - a.y * b.y, pi = a.x * b.y + a.y * b.x;
+  double2 run = make_double2(0.0, 0.0);
+  for (uint32_t z = 0; z < nz; ++z) {
+    const uint32_t cell = col * nz + z;
+    double2 c;
+    axis_phase(&c, k.z, h, z);
+    const double er = pr * c.x - pi * c.y, ei = pr * c.y + pi * c.x;
+    const float2 v = static_cast<const float2 *>(Sv)[sf_slot(node, cell, ncell)];
+    run.x += v.x * er + v.y * ei;
+    run.y += v.y * er - v.x * ei;
+    G[static_cast<size_t>(cell) * Kp + node] = run;
+  }
+}
+
 __global__ void prefix_axis_kernel(double2 *__restrict__ S, uint32_t Kp,
                                    uint32_t line0, uint32_t nlines, uint32_t n,
                                    uint32_t stride, uint32_t inner) {
@@ -565,7 +707,7 @@ __global__ void rmt_gather_kernel(const double2 *__restrict__ P,
     axis_phase(&c, k.z, h, z);
   }
   GlobalGrid grid{P, Kp, node};
-  shell_store<MIXED>(grid, x, y, z, node, k.w, r_in[g], r_out[g], nx, ny, nz,
+  shell_store<MIXED>(grid, x, y, z, node, k, r_in[g], r_out[g], nx, ny, nz,
                      a, b, c, Kp, R);
 }
 
@@ -633,7 +775,7 @@ __global__ __launch_bounds__(kLrBlock) void lr_f64_kernel(
 // kappa_i.  With NR a template parameter the kappa_i sit in registers for the
 // whole split, the NR remote sums of a direction are fetched together with
 // immediate offsets, and the radial loop unrolls completely: per node one
-// multiply for the phase, fract, sin, cos and five FMAs.
+// multiply for the phase, sin, cos, one multiply and four FMAs.
 template <int NR>
 __device__ __forceinline__ void lr_mixed_dirs(
     const float4 &a, const float2 *__restrict__ Rc, const LrTasks &tk,
@@ -775,19 +917,25 @@ const auto kLrNrKernels =
 // direction s of an alpha group, one wave per (cell, 64 directions), atom
 // records are wave-uniform.  Per atom the lane projects p = s.d once; each of
 // the NR radial nodes then costs one multiply for its phase kappa_i p, fract,
-// sin, cos and two FMAs.  The kappa_i are wave-uniform registers.  The lane
-// stores the cell-local sums sum_j q_j exp(-i k.d_j) in FP64 (FP32 partial
-// sums over at most 128 atoms); the shell-sum kernel applies the FP64
-// cell-centre phase exp(-i k.c).
+// sin and cos and two FMAs.  The kappa_i are wave-uniform registers.  The
+// lane stores the cell-local sums sum_j q_j exp(-i k.d_j) as FP32 pairs in
+// the node-pair tiles of sf_slot (FP32 running sums over at most 128 atoms,
+// denser cells add their further chunks to the stored value); the
+// shell-sum kernel applies the FP64 cell-centre phase exp(-i k.c).
 constexpr uint32_t kSfDirs = 64;
 constexpr uint32_t kSfChunk = 128;
 
 template <int NR>
 __global__ __launch_bounds__(kSfDirs) void sf_mixed_nr_kernel(
     const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
-    LrTasks tk, uint32_t task0, uint32_t ntask, uint32_t Kp, uint32_t nz,
-    double2 *__restrict__ S) {
-  const uint32_t t = task0 + blockIdx.y * kSfDirs + threadIdx.x;
+    LrTasks tk, uint32_t task0, uint32_t ntask, uint32_t ncell,
+    float2 *__restrict__ S) {
+  // (cell, direction chunk) of this workgroup, numbered so that each XCD
+  // runs consecutive cells (their node-pair slots share cache lines)
+  const uint32_t wg = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x,
+                                gridDim.x * gridDim.y);
+  const uint32_t cell = wg % ncell;
+  const uint32_t t = task0 + (wg / ncell) * kSfDirs + threadIdx.x;
   const bool active = t < task0 + ntask;
   const uint32_t tt = active ? t : task0;
   float kap[NR];
@@ -797,13 +945,12 @@ __global__ __launch_bounds__(kSfDirs) void sf_mixed_nr_kernel(
   const float4 d = tk.dir32[tt];
   const uint32_t n0 = tk.node0[tt];
   {
-    const uint32_t cell = blockIdx.x;
     const uint32_t a0 = start[cell], a1 = start[cell + 1];
-    double2 *out = S + static_cast<size_t>(cell) * Kp + n0;
     if (a0 == a1) {
       if (active) {
 #pragma unroll
-        for (int i = 0; i < NR; ++i) out[i] = make_double2(0.0, 0.0);
+        for (int i = 0; i < NR; ++i)
+          S[sf_slot(n0 + i, cell, ncell)] = make_float2(0.f, 0.f);
       }
       return;
     }
@@ -826,14 +973,12 @@ __global__ __launch_bounds__(kSfDirs) void sf_mixed_nr_kernel(
         if (jb == a0) {
 #pragma unroll
           for (int i = 0; i < NR; ++i)
-            out[i] = make_double2(static_cast<double>(re[i]),
-                                  static_cast<double>(im[i]));
-        } else {  // cells denser than one chunk: add in FP64
+            S[sf_slot(n0 + i, cell, ncell)] = make_float2(re[i], im[i]);
+        } else {  // cells denser than one chunk
 #pragma unroll
           for (int i = 0; i < NR; ++i) {
-            const double2 o = out[i];
-            out[i] = make_double2(o.x + static_cast<double>(re[i]),
-                                  o.y + static_cast<double>(im[i]));
+            float2 &o = S[sf_slot(n0 + i, cell, ncell)];
+            o = make_float2(o.x + re[i], o.y + im[i]);
           }
         }
       }
@@ -844,9 +989,14 @@ __global__ __launch_bounds__(kSfDirs) void sf_mixed_nr_kernel(
 // Any radial count (beyond the unrolled cases).
 __global__ __launch_bounds__(kSfDirs) void sf_mixed_any_kernel(
     const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
-    LrTasks tk, uint32_t task0, uint32_t ntask, uint32_t Kp, uint32_t nz,
-    double2 *__restrict__ S) {
-  const uint32_t t = task0 + blockIdx.y * kSfDirs + threadIdx.x;
+    LrTasks tk, uint32_t task0, uint32_t ntask, uint32_t ncell,
+    float2 *__restrict__ S) {
+  // (cell, direction chunk) of this workgroup, numbered so that each XCD
+  // runs consecutive cells (their node-pair slots share cache lines)
+  const uint32_t wg = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x,
+                                gridDim.x * gridDim.y);
+  const uint32_t cell = wg % ncell;
+  const uint32_t t = task0 + (wg / ncell) * kSfDirs + threadIdx.x;
   const bool active = t < task0 + ntask;
   const uint32_t tt = active ? t : task0;
   const float *kap = tk.kappa32 + tk.koff[task0];
@@ -854,9 +1004,7 @@ __global__ __launch_bounds__(kSfDirs) void sf_mixed_any_kernel(
   const float4 d = tk.dir32[tt];
   const uint32_t n0 = tk.node0[tt];
   {
-    const uint32_t cell = blockIdx.x;
     const uint32_t a0 = start[cell], a1 = start[cell + 1];
-    double2 *out = S + static_cast<size_t>(cell) * Kp + n0;
     for (uint32_t i = 0; i < nr; ++i) {
       double RE = 0.0, IM = 0.0;
       for (uint32_t jb = a0; jb < a1; jb += kSfChunk) {
@@ -872,13 +1020,15 @@ __global__ __launch_bounds__(kSfDirs) void sf_mixed_any_kernel(
         RE += re;
         IM += im;
       }
-      if (active) out[i] = make_double2(RE, IM);
+      if (active)
+        S[sf_slot(n0 + i, cell, ncell)] =
+            make_float2(static_cast<float>(RE), static_cast<float>(IM));
     }
   }
 }
 
 typedef void (*SfNrKernel)(const float4 *, const uint32_t *, LrTasks,
-                           uint32_t, uint32_t, uint32_t, uint32_t, double2 *);
+                           uint32_t, uint32_t, uint32_t, float2 *);
 
 template <int... N>
 constexpr auto sf_nr_table(std::integer_sequence<int, N...>) {
@@ -923,86 +1073,143 @@ __global__ void lr_reduce_kernel(const double4 *__restrict__ partial,
 
 // ---------------------------------------------------------------------------
 // near field: bare Coulomb over the cell and its 26 neighbours, clamped at the
-// grid edge (src/glst_force.cu:1170-1544).  One wave per cell, lanes = atoms.
+// grid edge (src/glst_force.cu:1170-1544).
+//
+// One 256-thread workgroup per cell.  A cell of n atoms is covered S =
+// floor(256/n) times: lane (i, s) takes atom i against every S-th atom
+// (phase s) of each neighbour cell, so a cell of 85 atoms keeps 255 lanes
+// busy instead of 128 lanes in two 64-atom passes (denser cells run in
+// 256-atom chunks with S = 1).  Neighbour atoms are read with vector loads,
+// several in flight per lane.  The S partial sums of an atom are added in
+// LDS in a fixed order.
+constexpr uint32_t kSrBlock = 256;
+constexpr uint32_t kSrMaxPhase = 16;
+
+// (F, e) of one pair; r = r_i - r_j, e = q_i q_j / |r|, F = e r / |r|^2.
+// Mixed: the separation in FP64, the rest in FP32.
+template <bool MIXED>
+struct PairAcc;
+
+template <>
+struct PairAcc<true> {
+  float fx = 0.f, fy = 0.f, fz = 0.f, en = 0.f;
+  __device__ void add(const double4 &ai, float qi, const double4 &aj,
+                      bool self) {
+    const float rx = static_cast<float>(ai.x - aj.x);
+    const float ry = static_cast<float>(ai.y - aj.y);
+    const float rz = static_cast<float>(ai.z - aj.z);
+    float r2 = rx * rx + ry * ry + rz * rz;
+    r2 = self ? INFINITY : r2;  // no self interaction
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    const float e = qi * static_cast<float>(aj.w) * ir;
+    const float f = e * ir * ir;
+    fx += f * rx;
+    fy += f * ry;
+    fz += f * rz;
+    en += e;
+  }
+};
+
+template <>
+struct PairAcc<false> {
+  double fx = 0.0, fy = 0.0, fz = 0.0, en = 0.0;
+  __device__ void add(const double4 &ai, double qi, const double4 &aj,
+                      bool self) {
+    const double rx = ai.x - aj.x, ry = ai.y - aj.y, rz = ai.z - aj.z;
+    double r2 = rx * rx + ry * ry + rz * rz;
+    r2 = self ? INFINITY : r2;
+    const double ir = 1.0 / sqrt(r2);
+    const double e = qi * aj.w * ir;
+    const double f = e * ir * ir;
+    fx += f * rx;
+    fy += f * ry;
+    fz += f * rz;
+    en += e;
+  }
+};
 
 // STORE: write (fx, fy, fz, en) to out4[i] (the caller adds it later);
 // otherwise accumulate into fx..en.
 template <bool MIXED, bool STORE>
-__global__ __launch_bounds__(64) void sr_kernel(
+__global__ __launch_bounds__(kSrBlock) void sr_kernel(
     const double4 *__restrict__ atoms, const uint32_t *__restrict__ start,
     uint32_t nx, uint32_t ny, uint32_t nz, double *__restrict__ fx,
     double *__restrict__ fy, double *__restrict__ fz,
     double *__restrict__ en, double4 *__restrict__ out4) {
-  const uint32_t cell = blockIdx.x;
+  __shared__ double4 part[kSrBlock];
+  const uint32_t ncell = nx * ny * nz;
+  const uint32_t cell = xcd_remap(blockIdx.x, ncell);
   const uint32_t ix = cell / (ny * nz), iy = (cell / nz) % ny, iz = cell % nz;
   const uint32_t a0 = start[cell], a1 = start[cell + 1];
-  for (uint32_t base = a0; base < a1; base += 64) {
-    const uint32_t i = base + threadIdx.x;
-    const bool valid = i < a1;
+  const uint32_t n = a1 - a0;
+  if (n == 0) return;  // uniform over the workgroup
+  uint32_t S = n >= kSrBlock ? 1 : kSrBlock / n;
+  S = S > kSrMaxPhase ? kSrMaxPhase : S;
+  const uint32_t chunk = kSrBlock / S;  // atoms per pass
+  const uint32_t tid = threadIdx.x;
+  const uint32_t li = tid / S, s = tid - li * S;
+  // waves whose lanes all lie past chunk * S have nothing to do
+  const bool wave_busy = (tid & ~63u) < chunk * S;
+  for (uint32_t base = a0; base < a1; base += chunk) {
+    const uint32_t i = base + li;
+    const bool valid = li < chunk && i < a1;
     const uint32_t ii = valid ? i : a0;
-    const double4 ai = atoms[ii];
     double FX = 0.0, FY = 0.0, FZ = 0.0, EN = 0.0;
-    for (int dx = -1; dx <= 1; ++dx) {
-      if ((ix == 0 && dx < 0) || (ix == nx - 1 && dx > 0)) continue;
-      for (int dy = -1; dy <= 1; ++dy) {
-        if ((iy == 0 && dy < 0) || (iy == ny - 1 && dy > 0)) continue;
-        for (int dz = -1; dz <= 1; ++dz) {
-          if ((iz == 0 && dz < 0) || (iz == nz - 1 && dz > 0)) continue;
-          const uint32_t nb = ((ix + dx) * ny + (iy + dy)) * nz + (iz + dz);
-          const uint32_t b0 = start[nb], b1 = start[nb + 1];
-          if (MIXED) {
-            // Separations in FP64 (exact for close pairs, including atoms
-            // that overlap), everything after them in FP32; FP32 partial
-            // sums over one neighbour cell are flushed to FP64.
-            const float qi = static_cast<float>(ai.w);
-            float fx0 = 0.f, fy0 = 0.f, fz0 = 0.f, en0 = 0.f;
-            for (uint32_t j = b0; j < b1; ++j) {
-              const double4 aj = atoms[j];
-              const float rx = static_cast<float>(ai.x - aj.x);
-              const float ry = static_cast<float>(ai.y - aj.y);
-              const float rz = static_cast<float>(ai.z - aj.z);
-              float r2 = rx * rx + ry * ry + rz * rz;
-              r2 = (j == ii) ? INFINITY : r2;  // no self interaction
-              const float ir = __builtin_amdgcn_rsqf(r2);
-              const float e = qi * static_cast<float>(aj.w) * ir;
-              const float f = e * ir * ir;
-              fx0 += f * rx;
-              fy0 += f * ry;
-              fz0 += f * rz;
-              en0 += e;
+    if (wave_busy) {
+      const double4 ai = atoms[ii];
+      using Q = typename std::conditional<MIXED, float, double>::type;
+      const Q qi = static_cast<Q>(ai.w);
+      for (int dx = -1; dx <= 1; ++dx) {
+        if ((ix == 0 && dx < 0) || (ix == nx - 1 && dx > 0)) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+          if ((iy == 0 && dy < 0) || (iy == ny - 1 && dy > 0)) continue;
+          for (int dz = -1; dz <= 1; ++dz) {
+            if ((iz == 0 && dz < 0) || (iz == nz - 1 && dz > 0)) continue;
+            const uint32_t nb = ((ix + dx) * ny + (iy + dy)) * nz + (iz + dz);
+            const uint32_t b1 = start[nb + 1];
+            uint32_t j = start[nb] + s;
+            // Mixed: FP32 sums over one neighbour cell, flushed to FP64.
+            PairAcc<MIXED> acc;
+            for (; j + 3 * S < b1; j += 4 * S) {
+              const double4 q0 = atoms[j], q1 = atoms[j + S],
+                            q2 = atoms[j + 2 * S], q3 = atoms[j + 3 * S];
+              acc.add(ai, qi, q0, j == ii);
+              acc.add(ai, qi, q1, j + S == ii);
+              acc.add(ai, qi, q2, j + 2 * S == ii);
+              acc.add(ai, qi, q3, j + 3 * S == ii);
             }
-            FX += fx0;
-            FY += fy0;
-            FZ += fz0;
-            EN += en0;
-          } else {
-            for (uint32_t j = b0; j < b1; ++j) {
-              const double4 aj = atoms[j];
-              const double rx = ai.x - aj.x, ry = ai.y - aj.y, rz = ai.z - aj.z;
-              double r2 = rx * rx + ry * ry + rz * rz;
-              r2 = (j == ii) ? INFINITY : r2;
-              const double ir = 1.0 / sqrt(r2);
-              const double e = ai.w * aj.w * ir;
-              const double f = e * ir * ir;
-              FX += f * rx;
-              FY += f * ry;
-              FZ += f * rz;
-              EN += e;
-            }
+            for (; j < b1; j += S) acc.add(ai, qi, atoms[j], j == ii);
+            FX += acc.fx;
+            FY += acc.fy;
+            FZ += acc.fz;
+            EN += acc.en;
           }
         }
       }
     }
-    if (valid) {
+    part[tid] = valid ? make_double4(FX, FY, FZ, EN)
+                      : make_double4(0.0, 0.0, 0.0, 0.0);
+    __syncthreads();
+    if (tid < chunk && base + tid < a1) {
+      double4 t = part[tid * S];
+      for (uint32_t u = 1; u < S; ++u) {
+        const double4 v = part[tid * S + u];
+        t.x += v.x;
+        t.y += v.y;
+        t.z += v.z;
+        t.w += v.w;
+      }
+      const uint32_t o = base + tid;
       if (STORE) {
-        out4[i] = make_double4(FX, FY, FZ, EN);
+        out4[o] = t;
       } else {
-        fx[i] += FX;
-        fy[i] += FY;
-        fz[i] += FZ;
-        en[i] += EN;
+        fx[o] += t.x;
+        fy[o] += t.y;
+        fz[o] += t.z;
+        en[o] += t.w;
       }
     }
+    __syncthreads();
   }
 }
 
@@ -1130,20 +1337,15 @@ void bin_scatter(bool mixed, const double *rx, const double *ry,
         g.nz, g.h, sorted_idx, cell_sorted, atoms64, atoms32);
 }
 
-void sf(bool mixed, int variant, const void *atoms, const uint32_t *start,
-        const double4 *k64, const float4 *krev, const LrTaskArgs &ta,
+void sf(bool mixed, const void *atoms, const uint32_t *start,
+        const double4 *k64, const LrTaskArgs &ta,
         const std::vector<SfGroup> &groups, uint32_t Kp, const GridShape &g,
-        double2 *S, hipStream_t s) {
+        void *S, hipStream_t s) {
   if (!mixed) {
     const dim3 grid(g.nx * g.ny, Kp / kSfBlock);
     sf_f64_kernel<<<grid, kSfBlock, 0, s>>>(static_cast<const double4 *>(atoms),
-                                            start, k64, Kp, g.nz, S);
-    return;
-  }
-  if (variant == 0) {
-    const dim3 grid(g.nx * g.ny, Kp / kSfBlock);
-    sf_mixed_node_kernel<<<grid, kSfBlock, 0, s>>>(
-        static_cast<const float4 *>(atoms), start, krev, Kp, g.nz, S);
+                                            start, k64, g.nz, g.ncell,
+                                            static_cast<double2 *>(S));
     return;
   }
   const LrTasks tk{ta.node0, ta.nr,      ta.koff,   ta.dir64,
@@ -1154,28 +1356,23 @@ void sf(bool mixed, int variant, const void *atoms, const uint32_t *start,
     if (gr.nr >= 1 && gr.nr <= static_cast<uint32_t>(kSfMaxUnrolled))
       kSfNrKernels[gr.nr - 1]<<<grid, kSfDirs, 0, s>>>(
           static_cast<const float4 *>(atoms), start, tk, gr.task0, gr.ntask,
-          Kp, g.nz, S);
+          g.ncell, static_cast<float2 *>(S));
     else
       sf_mixed_any_kernel<<<grid, kSfDirs, 0, s>>>(
           static_cast<const float4 *>(atoms), start, tk, gr.task0, gr.ntask,
-          Kp, g.nz, S);
+          g.ncell, static_cast<float2 *>(S));
   }
 }
 
-bool rmt_lds(bool mixed, const double2 *S, const double4 *k64,
+bool rmt_lds(bool mixed, uint32_t tmax, const void *S, const double4 *k64,
              const uint32_t *group, const uint32_t *r_in,
              const uint32_t *r_out, uint32_t Kp, const GridShape &g,
              size_t lds_limit, void *R, hipStream_t s) {
   const size_t per_node = (static_cast<size_t>(g.ncell) + g.nx + g.ny + g.nz) *
                           sizeof(double2);
-  uint32_t T = 0;
-  for (uint32_t t = GLST_RMT_TMAX; t >= 1; --t)
-    if (t * per_node <= lds_limit) {
-      T = t;
-      break;
-    }
-  if (T == 0) return false;
-  const uint32_t ntile = Kp / T;
+  const uint32_t T = (tmax >= 2 && 2 * per_node <= lds_limit) ? 2 : 1;
+  if (T * per_node > lds_limit) return false;
+  const uint32_t ntile = Kp / T;  // Kp is even
   const size_t lds = T * per_node;
   if (mixed) {
     (void)hipFuncSetAttribute(
@@ -1193,31 +1390,40 @@ bool rmt_lds(bool mixed, const double2 *S, const double4 *k64,
   return true;
 }
 
-void rmt_global(bool mixed, double2 *S, const double4 *k64,
+void rmt_global(bool mixed, const void *S, double2 *G, const double4 *k64,
                 const uint32_t *group, const uint32_t *r_in,
                 const uint32_t *r_out, uint32_t Kp, const GridShape &g,
                 void *R, hipStream_t s) {
   const uint32_t bx = blocks_for(Kp, 256);
   constexpr uint32_t kMaxY = 65535;  // grid.y limit
+  const uint32_t ncol = g.nx * g.ny;
+  for (uint32_t c0 = 0; c0 < ncol; c0 += kMaxY) {
+    const uint32_t cnt = (ncol - c0 < kMaxY) ? ncol - c0 : kMaxY;
+    if (mixed)
+      rmt_global_load_kernel<true><<<dim3(bx, cnt), 256, 0, s>>>(
+          S, k64, G, Kp, c0, g.nx, g.ny, g.nz, g.h);
+    else
+      rmt_global_load_kernel<false><<<dim3(bx, cnt), 256, 0, s>>>(
+          S, k64, G, Kp, c0, g.nx, g.ny, g.nz, g.h);
+  }
   auto prefix = [&](uint32_t nlines, uint32_t n, uint32_t stride,
                     uint32_t inner) {
     for (uint32_t l0 = 0; l0 < nlines; l0 += kMaxY) {
       const uint32_t cnt = (nlines - l0 < kMaxY) ? nlines - l0 : kMaxY;
-      prefix_axis_kernel<<<dim3(bx, cnt), 256, 0, s>>>(S, Kp, l0, nlines, n,
+      prefix_axis_kernel<<<dim3(bx, cnt), 256, 0, s>>>(G, Kp, l0, nlines, n,
                                                        stride, inner);
     }
   };
-  // S arrives prefix-summed along z (structure-factor kernels)
-  prefix(g.nx * g.nz, g.ny, g.nz, g.nz);      // y lines: base x*ny*nz + z
-  prefix(g.ny * g.nz, g.nx, g.ny * g.nz, g.ny * g.nz);  // x lines: y*nz + z
+  prefix(g.nx * g.nz, g.ny, g.nz, g.nz);                // y lines
+  prefix(g.ny * g.nz, g.nx, g.ny * g.nz, g.ny * g.nz);  // x lines
   for (uint32_t c0 = 0; c0 < g.ncell; c0 += kMaxY) {
     const uint32_t cnt = (g.ncell - c0 < kMaxY) ? g.ncell - c0 : kMaxY;
     if (mixed)
       rmt_gather_kernel<true><<<dim3(bx, cnt), 256, 0, s>>>(
-          S, k64, group, r_in, r_out, Kp, c0, g.nx, g.ny, g.nz, g.h, R);
+          G, k64, group, r_in, r_out, Kp, c0, g.nx, g.ny, g.nz, g.h, R);
     else
       rmt_gather_kernel<false><<<dim3(bx, cnt), 256, 0, s>>>(
-          S, k64, group, r_in, r_out, Kp, c0, g.nx, g.ny, g.nz, g.h, R);
+          G, k64, group, r_in, r_out, Kp, c0, g.nx, g.ny, g.nz, g.h, R);
   }
 }
 
@@ -1257,19 +1463,20 @@ void lr_reduce(const double4 *partial, uint32_t nsplit, uint32_t natom,
 void sr(bool mixed, const double4 *atoms, const uint32_t *start,
         const GridShape &g, double *fx, double *fy, double *fz, double *en,
         double4 *out4, hipStream_t s) {
+  const uint32_t nb = g.ncell;
   if (out4) {
     if (mixed)
-      sr_kernel<true, true><<<g.ncell, 64, 0, s>>>(atoms, start, g.nx, g.ny,
-                                                   g.nz, fx, fy, fz, en, out4);
-    else
-      sr_kernel<false, true><<<g.ncell, 64, 0, s>>>(atoms, start, g.nx, g.ny,
+      sr_kernel<true, true><<<nb, kSrBlock, 0, s>>>(atoms, start, g.nx, g.ny,
                                                     g.nz, fx, fy, fz, en, out4);
+    else
+      sr_kernel<false, true><<<nb, kSrBlock, 0, s>>>(
+          atoms, start, g.nx, g.ny, g.nz, fx, fy, fz, en, out4);
   } else {
     if (mixed)
-      sr_kernel<true, false><<<g.ncell, 64, 0, s>>>(atoms, start, g.nx, g.ny,
-                                                    g.nz, fx, fy, fz, en, out4);
+      sr_kernel<true, false><<<nb, kSrBlock, 0, s>>>(
+          atoms, start, g.nx, g.ny, g.nz, fx, fy, fz, en, out4);
     else
-      sr_kernel<false, false><<<g.ncell, 64, 0, s>>>(
+      sr_kernel<false, false><<<nb, kSrBlock, 0, s>>>(
           atoms, start, g.nx, g.ny, g.nz, fx, fy, fz, en, out4);
   }
 }

@@ -47,36 +47,45 @@ void bin_scatter(bool mixed, const double *rx, const double *ry,
                  hipStream_t s);
 
 // ---- structure factors (calc_sf) -------------------------------------------
+// S(k) is stored in node-pair tiles: the slots of nodes 2m and 2m+1 of all
+// cells form one contiguous run, cell-major inside it, so the shell-sum
+// workgroup of a node pair reads one contiguous block.
+__host__ __device__ inline size_t sf_slot(uint32_t node, uint32_t cell,
+                                          uint32_t ncell) {
+  return ((static_cast<size_t>(node >> 1) * ncell + cell) << 1) + (node & 1);
+}
+
 struct LrTaskArgs;
 // Tasks (directions) of one alpha group for the mixed structure factors.
 struct SfGroup {
   uint32_t nr, task0, ntask;
 };
-// FP64: S[cell*Kp + k] = running sum along z of sum_{j in cell} q_j
-// exp(-i k.r_j).  Mixed: S[cell*Kp + k] = sum_{j in cell} q_j exp(-i k.d_j)
-// with d_j the position relative to the cell centre.  Mixed variant 0: one
-// lane per node (krev, all Kp columns written); variant 1: one lane per
-// direction of an alpha group, the group's radii unrolled (padded node
-// columns are not written).
-void sf(bool mixed, int variant, const void *atoms, const uint32_t *start,
-        const double4 *k64, const float4 *krev, const LrTaskArgs &tasks,
+// FP64: S[sf_slot(k, c)] (double2) = running sum along z of
+// sum_{j in cell} q_j exp(-i k.r_j).  Mixed: S[sf_slot(k, c)] (float2) =
+// sum_{j in c} q_j exp(-i k.d_j) with d_j the position relative to the cell
+// centre; one lane per direction of an alpha group, the group's radii
+// unrolled (padded node columns are not written).
+void sf(bool mixed, const void *atoms, const uint32_t *start,
+        const double4 *k64, const LrTaskArgs &tasks,
         const std::vector<SfGroup> &groups, uint32_t Kp, const GridShape &g,
-        double2 *S, hipStream_t s);
+        void *S, hipStream_t s);
 
 // ---- remote (shell) sums (sum_rmt_sf) --------------------------------------
 // R_c(k) = W_k * sum_{c' : r_in < cheb(c,c') <= r_out} S_c'(k); the mixed
 // path stores exp(+i k.center_c) R_c(k) as float2, the FP64 path R_c(k) as
-// double2.  Returns false when the grid is too large for the LDS-resident
-// kernel (the caller then uses rmt_global).
-bool rmt_lds(bool mixed, const double2 *S, const double4 *k64,
+// double2, both at R[c*Kp + k].  LDS-resident tables of tmax (1 or 2) nodes
+// per workgroup; returns false when the table of one node does not fit in
+// lds_limit bytes (the caller then uses rmt_global).
+bool rmt_lds(bool mixed, uint32_t tmax, const void *S, const double4 *k64,
              const uint32_t *group, const uint32_t *r_in,
-             const uint32_t *r_out, uint32_t K, const GridShape &g,
+             const uint32_t *r_out, uint32_t Kp, const GridShape &g,
              size_t lds_limit, void *R, hipStream_t s);
-// Global-memory fallback: in-place 3-D prefix sums of S, then the 16-read
-// shell gather.  Destroys S.
-void rmt_global(bool mixed, double2 *S, const double4 *k64,
+// Global-memory tables: G (ncell*Kp double2, scratch) receives S in FP64
+// cell-major order, is prefix-summed in place, and the shells are gathered
+// from it.  S is left unchanged.
+void rmt_global(bool mixed, const void *S, double2 *G, const double4 *k64,
                 const uint32_t *group, const uint32_t *r_in,
-                const uint32_t *r_out, uint32_t K, const GridShape &g,
+                const uint32_t *r_out, uint32_t Kp, const GridShape &g,
                 void *R, hipStream_t s);
 
 // ---- long-range back-projection (calc_lr_ef) -------------------------------

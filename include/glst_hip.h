@@ -176,11 +176,13 @@ glst_status_t glst_get_rmt(glst_handle_t h, double *re, double *im);
 
 /* Tuning options of a handle:
  *   "overlap_near" (default 1): calc_ener_force runs the near field on a
- *   second stream of the handle, overlapped with S(k) / shell sums /
- *   back-projection; 0 runs the stages one after another.
- *   "sf_variant" (default 1, mixed precision only): 0 computes S(k) with one
- *   lane per node, 1 with one lane per direction of an alpha group and the
- *   group's radial nodes unrolled.  Results agree to FP32 rounding. */
+ *   second stream of the handle, started with the structure factors (1) or
+ *   after them, overlapping the shell sums and the back-projection (2); 0
+ *   runs the stages one after another.  The results are bitwise identical.
+ *   "rmt_shape" (default 0): shell sums with LDS-resident tables of two
+ *   nodes per workgroup where they fit, else one (0); always one node per
+ *   workgroup (1); or the global-memory tables otherwise used when one
+ *   node's table does not fit in LDS (2).  Results agree to FP64 rounding. */
 glst_status_t glst_set_option(glst_handle_t h, const char *name,
                               int64_t value);
 

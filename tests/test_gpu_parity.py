@@ -75,13 +75,17 @@ def test_binning_edge_cases(cuda):
         np.testing.assert_array_equal(cc, occ)
 
 
-def test_fields_fp64_match_oracle(cuda, water1k):
+@pytest.mark.parametrize("rmt_shape", [0, 1, 2])
+def test_fields_fp64_match_oracle(cuda, water1k, rmt_shape):
+    """S(k) and R(k) of the FP64 path against the oracle, for every shell-sum
+    kernel shape (2 = the global-memory tables of large grids)."""
     import torch
     import glst_amd as g
     from oracle import oracle as o
     rx, ry, rz, qc = water1k
     tol = 1e-4
     f = g.GLSTForce(rx.size, tol, 31.04, rcut=9.0, ct="double")
+    f.set_option("rmt_shape", rmt_shape)
     f.assign_atoms(*_dev([rx, ry, rz, qc], torch))
     f.calc_sf()
     sre, sim = f.sf()
@@ -164,18 +168,18 @@ def test_accumulate_and_stages(cuda, water1k, ct):
         np.testing.assert_array_equal(a, b)
 
 
-@pytest.mark.parametrize("sf_variant", [0, 1])
-def test_fields_mixed_match_oracle(cuda, water1k, sf_variant):
+@pytest.mark.parametrize("rmt_shape", [0, 1, 2])
+def test_fields_mixed_match_oracle(cuda, water1k, rmt_shape):
     """Mixed-precision S(k) and shell sums R(k) against the FP64 oracle's
-    fields, for both S(k) kernels (bar: 1e-5 of the field's largest entry,
-    FP32 phases of cell-local coordinates)."""
+    fields, for every shell-sum shape (bar: 1e-5 of the field's largest
+    entry: FP32 phases of cell-local coordinates, FP32 S(k) slots)."""
     import torch
     import glst_amd as g
     from oracle import oracle as o
     rx, ry, rz, qc = water1k
     tol = 1e-4
     f = g.GLSTForce(rx.size, tol, 31.04, rcut=9.0, ct="float")
-    f.set_option("sf_variant", sf_variant)
+    f.set_option("rmt_shape", rmt_shape)
     f.assign_atoms(*_dev([rx, ry, rz, qc], torch))
     f.calc_sf()
     sre, sim = f.sf()
@@ -192,27 +196,31 @@ def test_fields_mixed_match_oracle(cuda, water1k, sf_variant):
 
 
 def test_mixed_options_agree(cuda, water1k):
-    """Every combination of the tuning options gives the same forces to FP32
-    rounding (the near field's overlap changes no arithmetic at all)."""
+    """Every combination of the tuning options gives the same forces: the
+    near field's overlap changes no arithmetic at all, the shell-sum shapes
+    agree to FP64 rounding before the FP32 store."""
     import torch
     import glst_amd as g
     rx, ry, rz, qc = water1k
     d = _dev([rx, ry, rz, qc], torch)
     out = {}
-    for sv in (0, 1):
-        for ov in (0, 1):
+    for rs in (0, 1, 2):
+        for ov in (0, 1, 2):
             f = g.GLSTForce(rx.size, 1e-5, 31.04, rcut=9.0, ct="float")
-            f.set_option("sf_variant", sv)
+            f.set_option("rmt_shape", rs)
             f.set_option("overlap_near", ov)
             f.calc_ener_force(*d)
-            out[sv, ov] = _host(f.get_ef())
+            out[rs, ov] = _host(f.get_ef())
             f.close()
-    for sv in (0, 1):
-        for a, b in zip(out[sv, 0], out[sv, 1]):
-            np.testing.assert_array_equal(a, b)
-    assert rel_err(out[1, 0][:3], out[0, 0][:3]) < 1e-6
+    for rs in (0, 1, 2):
+        for ov in (1, 2):
+            for a, b in zip(out[rs, 0], out[rs, ov]):
+                np.testing.assert_array_equal(a, b)
+        assert rel_err(out[rs, 0][:3], out[0, 0][:3]) < 1e-7
     f = g.GLSTForce(rx.size, 1e-5, 31.04, rcut=9.0, ct="float")
     with pytest.raises(Exception):
-        f.set_option("sf_variant", 2)
+        f.set_option("rmt_shape", 3)
+    with pytest.raises(Exception):
+        f.set_option("overlap_near", 3)
     with pytest.raises(Exception):
         f.set_option("no_such_option", 1)
