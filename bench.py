@@ -171,7 +171,7 @@ def main():
     tf = os.path.join(REPO, "profiles", "lr_mixed_288k_traffic.json")
     if os.path.exists(tf):
         with open(tf) as fh:
-            traffic = json.load(fh).get("bytes_per_launch")
+            traffic = json.load(fh).get("bytes_per_step")
     roof = {
         "kernel": "lr_mixed_nr_kernel<NR> (calc_lr_ef back-projection, "
                   "all alpha-group launches of one step)",
