@@ -1,9 +1,7 @@
 // The GLST engine behind the C ABI of include/glst_hip.h: one handle = one
 // system on one GPU with one HIP stream.  Host setup follows
 // src/glst_force.cu:178-327; the force step follows :431-1614.
-#include "glst_hip.h"
-#include "kernels.hpp"
-#include "numerics.hpp"
+#include "runtime.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -19,40 +17,12 @@
 
 namespace glst {
 
-struct HipError : std::runtime_error {
-  using std::runtime_error::runtime_error;
-};
-struct StateError : std::runtime_error {
-  using std::runtime_error::runtime_error;
-};
-struct ArgError : std::runtime_error {
-  using std::runtime_error::runtime_error;
-};
-struct AllocError : std::runtime_error {
-  using std::runtime_error::runtime_error;
-};
-
-#define GLST_HIP(call)                                                        \
-  do {                                                                        \
-    hipError_t e_ = (call);                                                   \
-    if (e_ != hipSuccess)                                                     \
-      throw HipError(std::string("HIP error: ") + hipGetErrorString(e_) +    \
-                     " at " + __FILE__ + ":" + std::to_string(__LINE__));     \
-  } while (0)
-
-static void check_launch(const char *what) {
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess)
-    throw HipError(std::string("HIP launch error in ") + what + ": " +
-                   hipGetErrorString(e));
-}
-
 static std::mutex g_mutex;
 static std::string g_last_error;
 static std::string g_tdesign_dir;
 static bool g_tdesign_init = false;
 
-static std::string tdesign_dir() {
+std::string tdesign_dir() {
   std::lock_guard<std::mutex> lock(g_mutex);
   if (!g_tdesign_init) {
     const char *env = std::getenv("GLST_TDESIGN_DIR");
@@ -62,126 +32,10 @@ static std::string tdesign_dir() {
   return g_tdesign_dir;
 }
 
-class DeviceBuffer {
- public:
-  DeviceBuffer() = default;
-  DeviceBuffer(const DeviceBuffer &) = delete;
-  DeviceBuffer &operator=(const DeviceBuffer &) = delete;
-  ~DeviceBuffer() { release(); }
-  void resize(size_t bytes) {
-    if (bytes == bytes_) return;
-    release();
-    if (bytes == 0) return;
-    void *p = nullptr;
-    hipError_t e = hipMalloc(&p, bytes);
-    if (e != hipSuccess) {
-      (void)hipGetLastError();
-      throw AllocError("hipMalloc of " + std::to_string(bytes) +
-                       " bytes failed: " + hipGetErrorString(e));
-    }
-    ptr_ = p;
-    bytes_ = bytes;
-  }
-  void release() {
-    if (ptr_) (void)hipFree(ptr_);
-    ptr_ = nullptr;
-    bytes_ = 0;
-  }
-  template <typename T>
-  T *as() const {
-    return static_cast<T *>(ptr_);
-  }
-  size_t bytes() const { return bytes_; }
-
- private:
-  void *ptr_ = nullptr;
-  size_t bytes_ = 0;
-};
-
-class KernelTimer {
- public:
-  ~KernelTimer() {
-    for (auto &kv : stats_)
-      for (auto &pr : kv.second.pending) {
-        (void)hipEventDestroy(pr.first);
-        (void)hipEventDestroy(pr.second);
-      }
-    for (auto e : pool_) (void)hipEventDestroy(e);
-  }
-  bool enabled = false;
-  hipEvent_t take() {
-    if (!pool_.empty()) {
-      hipEvent_t e = pool_.back();
-      pool_.pop_back();
-      return e;
-    }
-    hipEvent_t e;
-    GLST_HIP(hipEventCreate(&e));
-    return e;
-  }
-  void begin(const char *name, hipStream_t s) {
-    if (!enabled) return;
-    cur_name_ = name;
-    cur_start_ = take();
-    GLST_HIP(hipEventRecord(cur_start_, s));
-  }
-  // Called from a destructor: never throws; a failed record drops the
-  // sample.
-  void end(hipStream_t s) noexcept {
-    if (!enabled) return;
-    try {
-      hipEvent_t stop = take();
-      if (hipEventRecord(stop, s) == hipSuccess)
-        stats_[cur_name_].pending.emplace_back(cur_start_, stop);
-    } catch (...) {
-    }
-  }
-  void collect() {
-    for (auto &kv : stats_) {
-      for (auto &pr : kv.second.pending) {
-        GLST_HIP(hipEventSynchronize(pr.second));
-        float ms = 0.f;
-        GLST_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
-        kv.second.ms += ms;
-        kv.second.n += 1;
-        pool_.push_back(pr.first);
-        pool_.push_back(pr.second);
-      }
-      kv.second.pending.clear();
-    }
-  }
-  void get(const std::string &name, double &ms, uint32_t &n) {
-    collect();
-    auto it = stats_.find(name);
-    ms = it == stats_.end() ? 0.0 : it->second.ms;
-    n = it == stats_.end() ? 0 : it->second.n;
-  }
-  void reset() {
-    collect();
-    stats_.clear();
-  }
-
- private:
-  struct Stat {
-    double ms = 0.0;
-    uint32_t n = 0;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
-  };
-  std::map<std::string, Stat> stats_;
-  std::vector<hipEvent_t> pool_;
-  std::string cur_name_;
-  hipEvent_t cur_start_{};
-};
-
-struct Scope {
-  KernelTimer &t;
-  hipStream_t s;
-  Scope(KernelTimer &timer, const char *name, hipStream_t stream)
-      : t(timer), s(stream) {
-    t.begin(name, s);
-  }
-  ~Scope() { t.end(s); }
-};
+void set_global_error(const std::string &m) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  g_last_error = m;
+}
 
 }  // namespace glst
 
@@ -258,11 +112,7 @@ struct glst_engine {
 
   void init(int ct_, uint32_t natom_, double tol_, double bx, double by,
             double bz, double rcut, void *stream_) {
-    if (ct_ != GLST_CT_DOUBLE && ct_ != GLST_CT_FLOAT)
-      throw ArgError("ct must be GLST_CT_DOUBLE or GLST_CT_FLOAT");
-    if (!(bx > 0 && by > 0 && bz > 0 && rcut > 0))
-      throw ArgError("box dimensions and rcut must be positive");
-    if (!(tol_ > 0)) throw ArgError("tol must be positive");
+    check_system_args(ct_, bx, by, bz, rcut, tol_);
     ct = ct_;
     natom = natom_;
     tol = tol_;
@@ -283,34 +133,17 @@ struct glst_engine {
       own_stream = true;
     }
 
-    // Grid in CT precision (src/glst_force.cu:223-241).
-    double cell;
-    if (mixed()) {
-      grid_dims<float>(bx, by, bz, rcut, grid.nx, grid.ny, grid.nz);
-      const float cf = static_cast<float>(rcut);
-      cell = static_cast<double>(cf);
-      inv_cell = static_cast<double>(1.0f / cf);
-    } else {
-      grid_dims<double>(bx, by, bz, rcut, grid.nx, grid.ny, grid.nz);
-      cell = rcut;
-      inv_cell = 1.0 / rcut;
-    }
-    grid.h = cell;
-    const uint64_t nc = static_cast<uint64_t>(grid.nx) * grid.ny * grid.nz;
-    if (nc == 0 || nc > 0x7FFFFFFFull)
-      throw ArgError("cell grid of " + std::to_string(nc) +
-                     " cells is out of range");
-    grid.ncell = static_cast<uint32_t>(nc);
-    has_lr = grid.nx > 2 && grid.ny > 2 && grid.nz > 2;
-
-    // Alpha groups and cubature (src/glst_force.cu:1642-1724,
-    // src/cubature.cu:99-159).
-    groups = alpha_groups(grid.nx, grid.ny, grid.nz, cell, cell, cell, tol);
-    const std::string dir = tdesign_dir();
-    spherical_rule = dir.empty() ? 0 : 1;
-    nodes = build_nodes(tol, groups, dir);
-    K = static_cast<uint32_t>(nodes.kx.size());
-    Kp = (K + 255u) / 256u * 256u;
+    // Grid in CT precision (src/glst_force.cu:223-241), alpha groups and
+    // cubature (src/glst_force.cu:1642-1724, src/cubature.cu:99-159).
+    SystemSetup su = setup_system(ct_, bx, by, bz, rcut, tol_);
+    grid = su.grid;
+    inv_cell = su.inv_cell;
+    has_lr = su.has_lr;
+    groups = std::move(su.groups);
+    nodes = std::move(su.nodes);
+    spherical_rule = su.spherical_rule;
+    K = su.K;
+    Kp = su.Kp;
 
     GLST_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
     GLST_HIP(hipEventCreateWithFlags(&ev_assigned, hipEventDisableTiming));
@@ -352,100 +185,47 @@ struct glst_engine {
         R.resize(field * (mixed() ? sizeof(float2) : sizeof(double2)));
         const uint32_t ntile = (natom + 255) / 256;
         // enough workgroups to fill the chip several times over
-        make_splits(ntile > 0 ? (4096 + ntile - 1) / ntile : 1);
+        set_splits(ntile > 0 ? (4096 + ntile - 1) / ntile : 1);
         partial.resize(static_cast<size_t>(nsplit) * n * sizeof(double4));
       }
     }
   }
 
-  // Back-projection splits: contiguous task ranges that never cross an alpha
-  // group (all tasks of a split then share the radial count and kappa_i);
-  // each group gets about target * K_g / K splits of equal size.
+  // Back-projection splits (make_splits in runtime.hpp).
   std::vector<uint32_t> split_host;
   std::vector<dev::LrGroup> lr_groups;
   std::vector<dev::SfGroup> sf_groups;
-  void make_splits(uint32_t target) {
-    split_host.assign(1, 0);
-    lr_groups.clear();
-    sf_groups.clear();
-    uint32_t base = 0;
-    for (size_t g = 0; g < nodes.dirs.size(); ++g) {
-      const uint32_t M = static_cast<uint32_t>(nodes.dirs[g].size());
-      const double share =
-          static_cast<double>(target) * nodes.count[g] / std::max<uint32_t>(K, 1);
-      uint32_t ns = static_cast<uint32_t>(share + 0.5);
-      ns = std::max<uint32_t>(1, std::min<uint32_t>(ns, M));
-      lr_groups.push_back({static_cast<uint32_t>(nodes.kappa[g].size()),
-                           static_cast<uint32_t>(split_host.size() - 1), ns});
-      sf_groups.push_back(
-          {static_cast<uint32_t>(nodes.kappa[g].size()), base, M});
-      for (uint32_t q = 1; q <= ns; ++q)
-        split_host.push_back(base + static_cast<uint32_t>(
-                                        (static_cast<uint64_t>(M) * q) / ns));
-      base += M;
-    }
-    if (split_host.size() == 1) split_host.push_back(0);  // no tasks at all
-    nsplit = static_cast<uint32_t>(split_host.size() - 1);
+  void set_splits(uint32_t target) {
+    SplitPlan P = glst::make_splits(nodes, K, target);
+    split_host = std::move(P.split);
+    lr_groups = std::move(P.lr_groups);
+    sf_groups = std::move(P.sf_groups);
+    nsplit = P.nsplit;
   }
 
   void upload_nodes() {
     if (Kp == 0) return;
-    const double inv2pi = 1.0 / 6.283185307179586476925286766559;
-    std::vector<double4> h64(Kp, make_double4(0, 0, 0, 0));
-    std::vector<uint32_t> hg(Kp, 0);
-    std::vector<uint32_t> tn0, tnr, tko;
-    std::vector<double4> td64;
-    std::vector<float4> td32;
-    std::vector<double> kv64;
-    std::vector<float> kv32;
-    dev_of_ref.assign(K, 0);
-    for (size_t g = 0; g < nodes.dirs.size(); ++g) {
-      const uint32_t nr = static_cast<uint32_t>(nodes.kappa[g].size());
-      const uint32_t M = static_cast<uint32_t>(nodes.dirs[g].size());
-      const uint32_t first = nodes.first[g];
-      const uint32_t koff = static_cast<uint32_t>(kv64.size());
-      for (double kap : nodes.kappa[g]) {
-        kv64.push_back(kap);
-        kv32.push_back(static_cast<float>(kap * inv2pi));
-      }
-      for (uint32_t m = 0; m < M; ++m) {
-        const auto &sd = nodes.dirs[g][m];
-        tn0.push_back(first + m * nr);
-        tnr.push_back(nr);
-        tko.push_back(koff);
-        td64.push_back(make_double4(sd[0], sd[1], sd[2], 0.0));
-        td32.push_back(make_float4(static_cast<float>(sd[0]),
-                                   static_cast<float>(sd[1]),
-                                   static_cast<float>(sd[2]), 0.f));
-        for (uint32_t i = 0; i < nr; ++i) {
-          const uint32_t ref = first + i * M + m;  // reference order
-          const uint32_t dev = first + m * nr + i;  // direction-major
-          dev_of_ref[ref] = dev;
-          h64[dev] = make_double4(nodes.kx[ref], nodes.ky[ref], nodes.kz[ref],
-                                  nodes.w[ref]);
-          hg[dev] = static_cast<uint32_t>(g);
-        }
-      }
-    }
-    ntask = static_cast<uint32_t>(tn0.size());
-    if (split_host.empty()) make_splits(1);
+    NodeLayout L = layout_nodes(nodes, K, Kp);
+    dev_of_ref = std::move(L.dev_of_ref);
+    ntask = static_cast<uint32_t>(L.node0.size());
+    if (split_host.empty()) set_splits(1);
     const std::vector<uint32_t> &split = split_host;
     auto up = [&](DeviceBuffer &b, const void *src, size_t bytes) {
       b.resize(bytes);
       if (bytes)
         GLST_HIP(hipMemcpy(b.as<void>(), src, bytes, hipMemcpyHostToDevice));
     };
-    up(k64, h64.data(), Kp * sizeof(double4));
-    up(group, hg.data(), Kp * 4);
+    up(k64, L.k64.data(), Kp * sizeof(double4));
+    up(group, L.group.data(), Kp * 4);
     up(r_in, groups.r_in.data(), groups.r_in.size() * 4);
     up(r_out, groups.r_out.data(), groups.r_out.size() * 4);
-    up(t_node0, tn0.data(), tn0.size() * 4);
-    up(t_nr, tnr.data(), tnr.size() * 4);
-    up(t_koff, tko.data(), tko.size() * 4);
-    up(t_dir64, td64.data(), td64.size() * sizeof(double4));
-    up(t_dir32, td32.data(), td32.size() * sizeof(float4));
-    up(kap64, kv64.data(), kv64.size() * 8);
-    up(kap32, kv32.data(), kv32.size() * 4);
+    up(t_node0, L.node0.data(), L.node0.size() * 4);
+    up(t_nr, L.nr.data(), L.nr.size() * 4);
+    up(t_koff, L.koff.data(), L.koff.size() * 4);
+    up(t_dir64, L.dir64.data(), L.dir64.size() * sizeof(double4));
+    up(t_dir32, L.dir32.data(), L.dir32.size() * sizeof(float4));
+    up(kap64, L.kappa64.data(), L.kappa64.size() * 8);
+    up(kap32, L.kappa32.data(), L.kappa32.size() * 4);
     up(t_split, split.data(), split.size() * 4);
   }
 
@@ -677,11 +457,6 @@ struct glst_engine {
 // C ABI
 
 namespace {
-
-void set_global_error(const std::string &m) {
-  std::lock_guard<std::mutex> lock(g_mutex);
-  g_last_error = m;
-}
 
 void record_error(glst_engine *h, const std::string &m) {
   if (h)

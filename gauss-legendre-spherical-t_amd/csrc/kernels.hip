@@ -105,15 +105,16 @@ __device__ inline uint32_t cell_coord(double r, CT inv, uint32_t n) {
   return static_cast<uint32_t>(c) < n ? static_cast<uint32_t>(c) : n - 1u;
 }
 
+// One chunk of 1024 atoms of one system (pointers at the system's first
+// atom and histogram).
 template <typename CT>
-__global__ __launch_bounds__(1024) void bin_chunk_kernel(
+__device__ __forceinline__ void bin_chunk_body(
     const double *__restrict__ rx, const double *__restrict__ ry,
     const double *__restrict__ rz, uint32_t natom, CT inv, uint32_t nx,
     uint32_t ny, uint32_t nz, uint32_t ncell, uint32_t *__restrict__ atom_cell,
-    uint32_t *__restrict__ rank, uint32_t *__restrict__ hist) {
+    uint32_t *__restrict__ rank, uint32_t *__restrict__ hist, uint32_t chunk) {
   __shared__ unsigned long long keys[kBinChunk];
   const uint32_t tid = threadIdx.x;
-  const uint32_t chunk = blockIdx.x;
   const uint32_t i = chunk * kBinChunk + tid;
   uint32_t cell = 0xFFFFFFFFu;  // sorts after every real cell
   if (i < natom) {
@@ -159,10 +160,21 @@ __global__ __launch_bounds__(1024) void bin_chunk_kernel(
   }
 }
 
-__global__ void hist_offsets_kernel(uint32_t *__restrict__ hist,
-                                    uint32_t nchunk, uint32_t ncell,
-                                    uint32_t *__restrict__ count) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+template <typename CT>
+__global__ __launch_bounds__(1024) void bin_chunk_kernel(
+    const double *__restrict__ rx, const double *__restrict__ ry,
+    const double *__restrict__ rz, uint32_t natom, CT inv, uint32_t nx,
+    uint32_t ny, uint32_t nz, uint32_t ncell, uint32_t *__restrict__ atom_cell,
+    uint32_t *__restrict__ rank, uint32_t *__restrict__ hist) {
+  bin_chunk_body<CT>(rx, ry, rz, natom, inv, nx, ny, nz, ncell, atom_cell,
+                     rank, hist, blockIdx.x);
+}
+
+__device__ __forceinline__ void hist_offsets_body(uint32_t *__restrict__ hist,
+                                                  uint32_t nchunk,
+                                                  uint32_t ncell,
+                                                  uint32_t *__restrict__ count,
+                                                  uint32_t c) {
   if (c >= ncell) return;
   uint32_t run = 0;
   for (uint32_t ch = 0; ch < nchunk; ++ch) {
@@ -174,9 +186,19 @@ __global__ void hist_offsets_kernel(uint32_t *__restrict__ hist,
   count[c] = run;
 }
 
-__global__ __launch_bounds__(1024) void cell_scan_kernel(
+__global__ void hist_offsets_kernel(uint32_t *__restrict__ hist,
+                                    uint32_t nchunk, uint32_t ncell,
+                                    uint32_t *__restrict__ count) {
+  hist_offsets_body(hist, nchunk, ncell, count,
+                    blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// start[c] = base + exclusive scan of count (start[ncell] too when
+// write_end); *maxc = max count.
+__device__ __forceinline__ void cell_scan_body(
     const uint32_t *__restrict__ count, uint32_t ncell,
-    uint32_t *__restrict__ start, uint32_t *__restrict__ maxc) {
+    uint32_t *__restrict__ start, uint32_t *__restrict__ maxc, uint32_t base,
+    bool write_end) {
   __shared__ uint32_t sums[1024];
   __shared__ uint32_t maxs[1024];
   const uint32_t tid = threadIdx.x;
@@ -199,19 +221,28 @@ __global__ __launch_bounds__(1024) void cell_scan_kernel(
     maxs[tid] = mx > maxs[tid] ? mx : maxs[tid];
     __syncthreads();
   }
-  uint32_t run = sums[tid] - s;  // exclusive prefix of this segment
+  uint32_t run = base + sums[tid] - s;  // exclusive prefix of this segment
   for (uint32_t c = b; c < e; ++c) {
     start[c] = run;
     run += count[c];
   }
   if (tid == 1023) {
-    start[ncell] = sums[1023];
+    if (write_end) start[ncell] = base + sums[1023];
     *maxc = maxs[1023];
   }
 }
 
+__global__ __launch_bounds__(1024) void cell_scan_kernel(
+    const uint32_t *__restrict__ count, uint32_t ncell,
+    uint32_t *__restrict__ start, uint32_t *__restrict__ maxc) {
+  cell_scan_body(count, ncell, start, maxc, 0u, true);
+}
+
+// Atom i of one system (inputs, atom_cell, rank, hist and start at the
+// system's first atom / histogram / cell; start holds batch-wide slots);
+// sorted_idx receives atom0 + i, cell_sorted the system-local cell.
 template <bool MIXED>
-__global__ void bin_scatter_kernel(
+__device__ __forceinline__ void bin_scatter_body(
     const double *__restrict__ rx, const double *__restrict__ ry,
     const double *__restrict__ rz, const double *__restrict__ qc,
     uint32_t natom, const uint32_t *__restrict__ atom_cell,
@@ -219,14 +250,13 @@ __global__ void bin_scatter_kernel(
     const uint32_t *__restrict__ start, uint32_t ncell, uint32_t ny,
     uint32_t nz, double h, uint32_t *__restrict__ sorted_idx,
     uint32_t *__restrict__ cell_sorted, double4 *__restrict__ atoms64,
-    float4 *__restrict__ atoms32) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    float4 *__restrict__ atoms32, uint32_t i, uint32_t atom0) {
   if (i >= natom) return;
   const uint32_t c = atom_cell[i];
   const uint32_t ch = i / kBinChunk;
   const uint32_t dst =
       start[c] + hist[static_cast<size_t>(ch) * ncell + c] + rank[i];
-  sorted_idx[dst] = i;
+  sorted_idx[dst] = atom0 + i;
   cell_sorted[dst] = c;
   const double x = rx[i], y = ry[i], z = rz[i], q = qc[i];
   atoms64[dst] = make_double4(x, y, z, q);
@@ -242,6 +272,21 @@ __global__ void bin_scatter_kernel(
   }
 }
 
+template <bool MIXED>
+__global__ void bin_scatter_kernel(
+    const double *__restrict__ rx, const double *__restrict__ ry,
+    const double *__restrict__ rz, const double *__restrict__ qc,
+    uint32_t natom, const uint32_t *__restrict__ atom_cell,
+    const uint32_t *__restrict__ rank, const uint32_t *__restrict__ hist,
+    const uint32_t *__restrict__ start, uint32_t ncell, uint32_t ny,
+    uint32_t nz, double h, uint32_t *__restrict__ sorted_idx,
+    uint32_t *__restrict__ cell_sorted, double4 *__restrict__ atoms64,
+    float4 *__restrict__ atoms32) {
+  bin_scatter_body<MIXED>(rx, ry, rz, qc, natom, atom_cell, rank, hist, start,
+                          ncell, ny, nz, h, sorted_idx, cell_sorted, atoms64,
+                          atoms32, blockIdx.x * blockDim.x + threadIdx.x, 0u);
+}
+
 // ---------------------------------------------------------------------------
 // structure factors
 
@@ -254,12 +299,10 @@ constexpr uint32_t kSfBlock = 256;
 
 // FP64: S_c(k) = sum_j q_j (cos(k.r_j) - i sin(k.r_j)), absolute coordinates
 // (src/glst_force.cu:601-664).  blockIdx.x = z-column, blockIdx.y = node tile.
-__global__ __launch_bounds__(kSfBlock) void sf_f64_kernel(
+__device__ __forceinline__ void sf_f64_body(
     const double4 *__restrict__ atoms, const uint32_t *__restrict__ start,
     const double4 *__restrict__ k64, uint32_t nz, uint32_t ncell,
-    double2 *__restrict__ S) {
-  const uint32_t col = blockIdx.x;
-  const uint32_t node = blockIdx.y * kSfBlock + threadIdx.x;
+    double2 *__restrict__ S, uint32_t col, uint32_t node) {
   const double4 k = k64[node];
   double pre = 0.0, pim = 0.0;  // running z-prefix
   for (uint32_t z = 0; z < nz; ++z) {
@@ -289,6 +332,14 @@ __global__ __launch_bounds__(kSfBlock) void sf_f64_kernel(
     pim += im0 + im1;
     S[sf_slot(node, cell, ncell)] = make_double2(pre, pim);
   }
+}
+
+__global__ __launch_bounds__(kSfBlock) void sf_f64_kernel(
+    const double4 *__restrict__ atoms, const uint32_t *__restrict__ start,
+    const double4 *__restrict__ k64, uint32_t nz, uint32_t ncell,
+    double2 *__restrict__ S) {
+  sf_f64_body(atoms, start, k64, nz, ncell, S, blockIdx.x,
+              blockIdx.y * kSfBlock + threadIdx.x);
 }
 
 
@@ -467,17 +518,16 @@ __device__ inline double2 cube_y(const double2 *P, const CubeXZ &c,
 // (3) completes the table along y and x; (4) gathers the shells, lanes on
 // consecutive (node, z), every thread walking y.
 template <bool MIXED>
-__global__ __launch_bounds__(kRmtBlock) void rmt_lds_kernel(
+__device__ __forceinline__ void rmt_lds_body(
     const void *__restrict__ Sv, const double4 *__restrict__ k64,
     const uint32_t *__restrict__ group, const uint32_t *__restrict__ r_in,
     const uint32_t *__restrict__ r_out, uint32_t Kp, uint32_t T,
-    uint32_t ntile, uint32_t nx, uint32_t ny, uint32_t nz, double h,
-    void *__restrict__ R) {
+    uint32_t nx, uint32_t ny, uint32_t nz, double h, void *__restrict__ R,
+    uint32_t tile) {
   extern __shared__ double2 P[];  // [T][ncell], then [T][nx+ny+nz] phases
   const uint32_t ncell = nx * ny * nz;
   const uint32_t nax = nx + ny + nz;
   double2 *phase = P + T * ncell;
-  const uint32_t tile = xcd_remap(blockIdx.x, ntile);
   const uint32_t node0 = tile * T;
   const uint32_t tid = threadIdx.x;
   // (1) slot pair (2m, 2m+1) of cell c sits at index 2 (m ncell + c)
@@ -624,6 +674,17 @@ __global__ __launch_bounds__(kRmtBlock) void rmt_lds_kernel(
   }
 }
 
+template <bool MIXED>
+__global__ __launch_bounds__(kRmtBlock) void rmt_lds_kernel(
+    const void *__restrict__ Sv, const double4 *__restrict__ k64,
+    const uint32_t *__restrict__ group, const uint32_t *__restrict__ r_in,
+    const uint32_t *__restrict__ r_out, uint32_t Kp, uint32_t T,
+    uint32_t ntile, uint32_t nx, uint32_t ny, uint32_t nz, double h,
+    void *__restrict__ R) {
+  rmt_lds_body<MIXED>(Sv, k64, group, r_in, r_out, Kp, T, nx, ny, nz, h, R,
+                      xcd_remap(blockIdx.x, ntile));
+}
+
 // ---- global-memory tables (grids too large for LDS) ----
 
 // G[c*Kp + k] = the node-pair-tiled S converted to FP64 cell-major order;
@@ -733,13 +794,15 @@ struct LrTasks {
 
 // FP64 (src/glst_force.cu:1060-1135): en_a = q_a sum_k Re(e^{ik.r_a} R_c(k)),
 // F_a = q_a sum_k k Im(e^{ik.r_a} R_c(k)), absolute coordinates.
-__global__ __launch_bounds__(kLrBlock) void lr_f64_kernel(
+// Atom tile `tile` of one system (atoms, cell_sorted at its first atom, R at
+// its field) over the tasks of split `split`; the result goes to partial row
+// `prow` (partial at the system's partial block).
+__device__ __forceinline__ void lr_f64_body(
     const double4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
-    LrTasks tk, const double2 *__restrict__ R, uint32_t Kp, uint32_t natom,
-    uint32_t ntile, double4 *__restrict__ partial) {
-  const uint32_t tile = xcd_remap(blockIdx.x, ntile);
+    const LrTasks &tk, const double2 *__restrict__ R, uint32_t Kp,
+    uint32_t natom, uint32_t tile, uint32_t split, uint32_t prow,
+    double4 *__restrict__ partial) {
   const uint32_t i = tile * kLrBlock + threadIdx.x;
-  const uint32_t split = blockIdx.y;
   const bool valid = i < natom;
   const uint32_t ii = valid ? i : natom - 1;
   const double4 a = atoms[ii];
@@ -766,8 +829,16 @@ __global__ __launch_bounds__(kLrBlock) void lr_f64_kernel(
     fz += d.z * b;
   }
   if (valid)
-    partial[static_cast<size_t>(split) * natom + i] =
+    partial[static_cast<size_t>(prow) * natom + i] =
         make_double4(a.w * fx, a.w * fy, a.w * fz, a.w * en);
+}
+
+__global__ __launch_bounds__(kLrBlock) void lr_f64_kernel(
+    const double4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
+    LrTasks tk, const double2 *__restrict__ R, uint32_t Kp, uint32_t natom,
+    uint32_t ntile, double4 *__restrict__ partial) {
+  lr_f64_body(atoms, cell_sorted, tk, R, Kp, natom,
+              xcd_remap(blockIdx.x, ntile), blockIdx.y, blockIdx.y, partial);
 }
 
 // Mixed, fixed radial count: every split lies inside one alpha group, so all
@@ -873,13 +944,12 @@ __device__ __forceinline__ void lr_mixed_dirs_any(
 // One instantiation per radial count (NR = 0: any count); the host launches
 // one grid per alpha group, blockIdx.y + split0 being the split.
 template <int NR>
-__global__ __launch_bounds__(kLrBlock) void lr_mixed_nr_kernel(
+__device__ __forceinline__ void lr_mixed_nr_body(
     const float4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
-    LrTasks tk, const float2 *__restrict__ R, uint32_t Kp, uint32_t natom,
-    uint32_t ntile, uint32_t split0, double4 *__restrict__ partial) {
-  const uint32_t tile = xcd_remap(blockIdx.x, ntile);
+    const LrTasks &tk, const float2 *__restrict__ R, uint32_t Kp,
+    uint32_t natom, uint32_t tile, uint32_t split, uint32_t prow,
+    double4 *__restrict__ partial) {
   const uint32_t i = tile * kLrBlock + threadIdx.x;
-  const uint32_t split = split0 + blockIdx.y;
   const bool valid = i < natom;
   const uint32_t ii = valid ? i : natom - 1;
   const float4 a = atoms[ii];
@@ -894,9 +964,19 @@ __global__ __launch_bounds__(kLrBlock) void lr_mixed_nr_kernel(
   }
   if (valid) {
     const double qf = static_cast<double>(a.w) * kTwoPi;  // k = 2 pi k_rev
-    partial[static_cast<size_t>(split) * natom + i] =
+    partial[static_cast<size_t>(prow) * natom + i] =
         make_double4(qf * FX, qf * FY, qf * FZ, static_cast<double>(a.w) * EN);
   }
+}
+
+template <int NR>
+__global__ __launch_bounds__(kLrBlock) void lr_mixed_nr_kernel(
+    const float4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
+    LrTasks tk, const float2 *__restrict__ R, uint32_t Kp, uint32_t natom,
+    uint32_t ntile, uint32_t split0, double4 *__restrict__ partial) {
+  const uint32_t split = split0 + blockIdx.y;
+  lr_mixed_nr_body<NR>(atoms, cell_sorted, tk, R, Kp, natom,
+                       xcd_remap(blockIdx.x, ntile), split, split, partial);
 }
 
 typedef void (*LrNrKernel)(const float4 *, const uint32_t *, LrTasks,
@@ -925,17 +1005,14 @@ const auto kLrNrKernels =
 constexpr uint32_t kSfDirs = 64;
 constexpr uint32_t kSfChunk = 128;
 
+// Cell `cell` of one system (start at its first cell, S at its field) and
+// direction chunk `chunk` of the group whose tasks are task0 .. +ntask.
 template <int NR>
-__global__ __launch_bounds__(kSfDirs) void sf_mixed_nr_kernel(
+__device__ __forceinline__ void sf_mixed_nr_body(
     const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
-    LrTasks tk, uint32_t task0, uint32_t ntask, uint32_t ncell,
-    float2 *__restrict__ S) {
-  // (cell, direction chunk) of this workgroup, numbered so that each XCD
-  // runs consecutive cells (their node-pair slots share cache lines)
-  const uint32_t wg = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x,
-                                gridDim.x * gridDim.y);
-  const uint32_t cell = wg % ncell;
-  const uint32_t t = task0 + (wg / ncell) * kSfDirs + threadIdx.x;
+    const LrTasks &tk, uint32_t task0, uint32_t ntask, uint32_t ncell,
+    float2 *__restrict__ S, uint32_t cell, uint32_t chunk) {
+  const uint32_t t = task0 + chunk * kSfDirs + threadIdx.x;
   const bool active = t < task0 + ntask;
   const uint32_t tt = active ? t : task0;
   float kap[NR];
@@ -987,16 +1064,11 @@ __global__ __launch_bounds__(kSfDirs) void sf_mixed_nr_kernel(
 }
 
 // Any radial count (beyond the unrolled cases).
-__global__ __launch_bounds__(kSfDirs) void sf_mixed_any_kernel(
+__device__ __forceinline__ void sf_mixed_any_body(
     const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
-    LrTasks tk, uint32_t task0, uint32_t ntask, uint32_t ncell,
-    float2 *__restrict__ S) {
-  // (cell, direction chunk) of this workgroup, numbered so that each XCD
-  // runs consecutive cells (their node-pair slots share cache lines)
-  const uint32_t wg = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x,
-                                gridDim.x * gridDim.y);
-  const uint32_t cell = wg % ncell;
-  const uint32_t t = task0 + (wg / ncell) * kSfDirs + threadIdx.x;
+    const LrTasks &tk, uint32_t task0, uint32_t ntask, uint32_t ncell,
+    float2 *__restrict__ S, uint32_t cell, uint32_t chunk) {
+  const uint32_t t = task0 + chunk * kSfDirs + threadIdx.x;
   const bool active = t < task0 + ntask;
   const uint32_t tt = active ? t : task0;
   const float *kap = tk.kappa32 + tk.koff[task0];
@@ -1027,28 +1099,43 @@ __global__ __launch_bounds__(kSfDirs) void sf_mixed_any_kernel(
   }
 }
 
+// (cell, direction chunk) of a workgroup, numbered so that each XCD runs
+// consecutive cells (their node-pair slots share cache lines).
+template <int NR>
+__global__ __launch_bounds__(kSfDirs) void sf_mixed_nr_kernel(
+    const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
+    LrTasks tk, uint32_t task0, uint32_t ntask, uint32_t ncell,
+    float2 *__restrict__ S) {
+  const uint32_t wg = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x,
+                                gridDim.x * gridDim.y);
+  if constexpr (NR > 0)
+    sf_mixed_nr_body<NR>(atoms, start, tk, task0, ntask, ncell, S, wg % ncell,
+                         wg / ncell);
+  else
+    sf_mixed_any_body(atoms, start, tk, task0, ntask, ncell, S, wg % ncell,
+                      wg / ncell);
+}
+
 typedef void (*SfNrKernel)(const float4 *, const uint32_t *, LrTasks,
                            uint32_t, uint32_t, uint32_t, float2 *);
 
+// index NR (1..kSfMaxUnrolled) -> unrolled kernel; 0 -> any radial count
 template <int... N>
 constexpr auto sf_nr_table(std::integer_sequence<int, N...>) {
-  return std::array<SfNrKernel, sizeof...(N)>{{&sf_mixed_nr_kernel<N + 1>...}};
+  return std::array<SfNrKernel, sizeof...(N)>{{&sf_mixed_nr_kernel<N>...}};
 }
 
 constexpr int kSfMaxUnrolled = 40;
 const auto kSfNrKernels =
-    sf_nr_table(std::make_integer_sequence<int, kSfMaxUnrolled>{});
+    sf_nr_table(std::make_integer_sequence<int, kSfMaxUnrolled + 1>{});
 
 // f[i] += sum of the node-range partials (fixed order), plus the near-field
 // record when one is given.
-__global__ void lr_reduce_kernel(const double4 *__restrict__ partial,
-                                 uint32_t nsplit, uint32_t natom,
-                                 const double4 *__restrict__ near,
-                                 double *__restrict__ fx,
-                                 double *__restrict__ fy,
-                                 double *__restrict__ fz,
-                                 double *__restrict__ en) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void lr_reduce_body(
+    const double4 *__restrict__ partial, uint32_t nsplit, uint32_t natom,
+    const double4 *__restrict__ near, double *__restrict__ fx,
+    double *__restrict__ fy, double *__restrict__ fz, double *__restrict__ en,
+    uint32_t i) {
   if (i >= natom) return;
   double4 s = nsplit ? partial[i] : make_double4(0.0, 0.0, 0.0, 0.0);
   for (uint32_t p = 1; p < nsplit; ++p) {
@@ -1069,6 +1156,17 @@ __global__ void lr_reduce_kernel(const double4 *__restrict__ partial,
   fy[i] += s.y;
   fz[i] += s.z;
   en[i] += s.w;
+}
+
+__global__ void lr_reduce_kernel(const double4 *__restrict__ partial,
+                                 uint32_t nsplit, uint32_t natom,
+                                 const double4 *__restrict__ near,
+                                 double *__restrict__ fx,
+                                 double *__restrict__ fy,
+                                 double *__restrict__ fz,
+                                 double *__restrict__ en) {
+  lr_reduce_body(partial, nsplit, natom, near, fx, fy, fz, en,
+                 blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1130,15 +1228,15 @@ struct PairAcc<false> {
 
 // STORE: write (fx, fy, fz, en) to out4[i] (the caller adds it later);
 // otherwise accumulate into fx..en.
+// Cell `cell` of one system (start at the system's first cell, holding
+// batch-wide atom slots; atoms and outputs batch-wide).
 template <bool MIXED, bool STORE>
-__global__ __launch_bounds__(kSrBlock) void sr_kernel(
+__device__ __forceinline__ void sr_body(
     const double4 *__restrict__ atoms, const uint32_t *__restrict__ start,
     uint32_t nx, uint32_t ny, uint32_t nz, double *__restrict__ fx,
     double *__restrict__ fy, double *__restrict__ fz,
-    double *__restrict__ en, double4 *__restrict__ out4) {
+    double *__restrict__ en, double4 *__restrict__ out4, uint32_t cell) {
   __shared__ double4 part[kSrBlock];
-  const uint32_t ncell = nx * ny * nz;
-  const uint32_t cell = xcd_remap(blockIdx.x, ncell);
   const uint32_t ix = cell / (ny * nz), iy = (cell / nz) % ny, iz = cell % nz;
   const uint32_t a0 = start[cell], a1 = start[cell + 1];
   const uint32_t n = a1 - a0;
@@ -1213,6 +1311,16 @@ __global__ __launch_bounds__(kSrBlock) void sr_kernel(
   }
 }
 
+template <bool MIXED, bool STORE>
+__global__ __launch_bounds__(kSrBlock) void sr_kernel(
+    const double4 *__restrict__ atoms, const uint32_t *__restrict__ start,
+    uint32_t nx, uint32_t ny, uint32_t nz, double *__restrict__ fx,
+    double *__restrict__ fy, double *__restrict__ fz,
+    double *__restrict__ en, double4 *__restrict__ out4) {
+  sr_body<MIXED, STORE>(atoms, start, nx, ny, nz, fx, fy, fz, en, out4,
+                        xcd_remap(blockIdx.x, nx * ny * nz));
+}
+
 // ---------------------------------------------------------------------------
 
 __global__ void unpermute_kernel(const uint32_t *__restrict__ sorted_idx,
@@ -1275,6 +1383,215 @@ __global__ __launch_bounds__(kCoulBlock) void coulomb_kernel(
 
 inline uint32_t blocks_for(size_t n, uint32_t b) {
   return static_cast<uint32_t>((n + b - 1) / b);
+}
+
+// ---------------------------------------------------------------------------
+// batched systems: every workgroup finds its run (system, and alpha group
+// where the launch is per radial count) by binary search over the runs'
+// first workgroups, then runs the single-system body on its system's slice.
+
+__device__ __forceinline__ uint32_t run_of(const Seg *__restrict__ seg,
+                                           uint32_t nseg, uint32_t blk) {
+  uint32_t lo = 0, hi = nseg;  // the last run with blk0 <= blk
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (seg[mid].blk0 <= blk)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <typename CT>
+__global__ __launch_bounds__(1024) void bin_chunk_batch_kernel(
+    const double *__restrict__ rx, const double *__restrict__ ry,
+    const double *__restrict__ rz, const SysDev *__restrict__ sys,
+    const Seg *__restrict__ seg, uint32_t nseg, uint32_t *__restrict__ atom_cell,
+    uint32_t *__restrict__ rank, uint32_t *__restrict__ hist) {
+  const Seg g = seg[run_of(seg, nseg, blockIdx.x)];
+  const SysDev &d = sys[g.sys];
+  bin_chunk_body<CT>(rx + d.atom0, ry + d.atom0, rz + d.atom0, d.natom,
+                     static_cast<CT>(d.inv_cell), d.nx, d.ny, d.nz, d.ncell,
+                     atom_cell + d.atom0, rank + d.atom0, hist + d.hist0,
+                     blockIdx.x - g.blk0);
+}
+
+__global__ void hist_offsets_batch_kernel(uint32_t *__restrict__ hist,
+                                          const SysDev *__restrict__ sys,
+                                          const Seg *__restrict__ seg,
+                                          uint32_t nseg,
+                                          uint32_t *__restrict__ count) {
+  const Seg g = seg[run_of(seg, nseg, blockIdx.x)];
+  const SysDev &d = sys[g.sys];
+  hist_offsets_body(hist + d.hist0, d.nchunk, d.ncell, count + d.cell0,
+                    (blockIdx.x - g.blk0) * blockDim.x + threadIdx.x);
+}
+
+__global__ __launch_bounds__(1024) void cell_scan_batch_kernel(
+    const uint32_t *__restrict__ count, const SysDev *__restrict__ sys,
+    uint32_t nsys, uint32_t *__restrict__ start, uint32_t *__restrict__ maxc) {
+  const SysDev &d = sys[blockIdx.x];
+  // start[cell0 + ncell] is the next system's first slot; the last system
+  // writes the batch's end
+  cell_scan_body(count + d.cell0, d.ncell, start + d.cell0, maxc + blockIdx.x,
+                 d.atom0, blockIdx.x + 1 == nsys);
+}
+
+template <bool MIXED>
+__global__ void bin_scatter_batch_kernel(
+    const double *__restrict__ rx, const double *__restrict__ ry,
+    const double *__restrict__ rz, const double *__restrict__ qc,
+    const SysDev *__restrict__ sys, const Seg *__restrict__ seg, uint32_t nseg,
+    const uint32_t *__restrict__ atom_cell, const uint32_t *__restrict__ rank,
+    const uint32_t *__restrict__ hist, const uint32_t *__restrict__ start,
+    uint32_t *__restrict__ sorted_idx, uint32_t *__restrict__ cell_sorted,
+    double4 *__restrict__ atoms64, float4 *__restrict__ atoms32) {
+  const Seg g = seg[run_of(seg, nseg, blockIdx.x)];
+  const SysDev &d = sys[g.sys];
+  bin_scatter_body<MIXED>(
+      rx + d.atom0, ry + d.atom0, rz + d.atom0, qc + d.atom0, d.natom,
+      atom_cell + d.atom0, rank + d.atom0, hist + d.hist0, start + d.cell0,
+      d.ncell, d.ny, d.nz, d.h, sorted_idx, cell_sorted, atoms64, atoms32,
+      (blockIdx.x - g.blk0) * blockDim.x + threadIdx.x, d.atom0);
+}
+
+__global__ __launch_bounds__(kSfBlock) void sf_f64_batch_kernel(
+    const double4 *__restrict__ atoms, const uint32_t *__restrict__ start,
+    const double4 *__restrict__ k64, const SysDev *__restrict__ sys,
+    const Seg *__restrict__ seg, uint32_t nseg, double2 *__restrict__ S) {
+  const Seg g = seg[run_of(seg, nseg, blockIdx.x)];
+  const SysDev &d = sys[g.sys];
+  const uint32_t local = blockIdx.x - g.blk0, ncol = d.nx * d.ny;
+  sf_f64_body(atoms, start + d.cell0, k64 + d.node0, d.nz, d.ncell,
+              S + d.field0, local % ncol,
+              (local / ncol) * kSfBlock + threadIdx.x);
+}
+
+template <int NR>
+__global__ __launch_bounds__(kSfDirs) void sf_mixed_batch_kernel(
+    const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
+    LrTasks tk, const SysDev *__restrict__ sys, const Seg *__restrict__ seg,
+    uint32_t nseg, float2 *__restrict__ S) {
+  const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
+  const Seg g = seg[run_of(seg, nseg, wg)];
+  const SysDev &d = sys[g.sys];
+  const uint32_t local = wg - g.blk0;
+  if constexpr (NR > 0)
+    sf_mixed_nr_body<NR>(atoms, start + d.cell0, tk, g.a, g.b, d.ncell,
+                         S + d.field0, local % d.ncell, local / d.ncell);
+  else
+    sf_mixed_any_body(atoms, start + d.cell0, tk, g.a, g.b, d.ncell,
+                      S + d.field0, local % d.ncell, local / d.ncell);
+}
+
+typedef void (*SfBatchKernel)(const float4 *, const uint32_t *, LrTasks,
+                              const SysDev *, const Seg *, uint32_t, float2 *);
+
+template <int... N>
+constexpr auto sf_batch_table(std::integer_sequence<int, N...>) {
+  return std::array<SfBatchKernel, sizeof...(N)>{
+      {&sf_mixed_batch_kernel<N>...}};
+}
+
+const auto kSfBatchKernels =
+    sf_batch_table(std::make_integer_sequence<int, kSfMaxUnrolled + 1>{});
+
+template <bool MIXED>
+__global__ __launch_bounds__(kRmtBlock) void rmt_lds_batch_kernel(
+    const void *__restrict__ Sv, const double4 *__restrict__ k64,
+    const uint32_t *__restrict__ group, const uint32_t *__restrict__ r_in,
+    const uint32_t *__restrict__ r_out, const SysDev *__restrict__ sys,
+    const Seg *__restrict__ seg, uint32_t nseg, void *__restrict__ R) {
+  const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
+  const Seg g = seg[run_of(seg, nseg, wg)];
+  const SysDev &d = sys[g.sys];
+  const void *Ss;
+  void *Rs;
+  if (MIXED) {
+    Ss = static_cast<const float2 *>(Sv) + d.field0;
+    Rs = static_cast<float2 *>(R) + d.field0;
+  } else {
+    Ss = static_cast<const double2 *>(Sv) + d.field0;
+    Rs = static_cast<double2 *>(R) + d.field0;
+  }
+  rmt_lds_body<MIXED>(Ss, k64 + d.node0, group + d.node0, r_in + d.group0,
+                      r_out + d.group0, d.Kp, d.T, d.nx, d.ny, d.nz, d.h, Rs,
+                      wg - g.blk0);
+}
+
+__global__ __launch_bounds__(kLrBlock) void lr_f64_batch_kernel(
+    const double4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
+    LrTasks tk, const double2 *__restrict__ R, const SysDev *__restrict__ sys,
+    const Seg *__restrict__ seg, uint32_t nseg,
+    double4 *__restrict__ partial) {
+  const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
+  const Seg g = seg[run_of(seg, nseg, wg)];
+  const SysDev &d = sys[g.sys];
+  const uint32_t local = wg - g.blk0;
+  const uint32_t prow = local / d.ntile;
+  lr_f64_body(atoms + d.atom0, cell_sorted + d.atom0, tk, R + d.field0, d.Kp,
+              d.natom, local % d.ntile, d.split0 + prow, prow,
+              partial + d.partial0);
+}
+
+template <int NR>
+__global__ __launch_bounds__(kLrBlock) void lr_mixed_batch_kernel(
+    const float4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
+    LrTasks tk, const float2 *__restrict__ R, const SysDev *__restrict__ sys,
+    const Seg *__restrict__ seg, uint32_t nseg,
+    double4 *__restrict__ partial) {
+  const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
+  const Seg g = seg[run_of(seg, nseg, wg)];
+  const SysDev &d = sys[g.sys];
+  const uint32_t local = wg - g.blk0;
+  const uint32_t split = g.a + local / d.ntile;
+  lr_mixed_nr_body<NR>(atoms + d.atom0, cell_sorted + d.atom0, tk,
+                       R + d.field0, d.Kp, d.natom, local % d.ntile, split,
+                       split - d.split0, partial + d.partial0);
+}
+
+typedef void (*LrBatchKernel)(const float4 *, const uint32_t *, LrTasks,
+                              const float2 *, const SysDev *, const Seg *,
+                              uint32_t, double4 *);
+
+template <int... N>
+constexpr auto lr_batch_table(std::integer_sequence<int, N...>) {
+  return std::array<LrBatchKernel, sizeof...(N)>{
+      {&lr_mixed_batch_kernel<N>...}};
+}
+
+const auto kLrBatchKernels =
+    lr_batch_table(std::make_integer_sequence<int, kLrMaxUnrolled + 1>{});
+
+__global__ void lr_reduce_batch_kernel(const double4 *__restrict__ partial,
+                                       const SysDev *__restrict__ sys,
+                                       const Seg *__restrict__ seg,
+                                       uint32_t nseg,
+                                       const double4 *__restrict__ near,
+                                       double *__restrict__ fx,
+                                       double *__restrict__ fy,
+                                       double *__restrict__ fz,
+                                       double *__restrict__ en) {
+  const Seg g = seg[run_of(seg, nseg, blockIdx.x)];
+  const SysDev &d = sys[g.sys];
+  lr_reduce_body(partial + d.partial0, d.nsplit, d.natom,
+                 near ? near + d.atom0 : nullptr, fx + d.atom0, fy + d.atom0,
+                 fz + d.atom0, en + d.atom0,
+                 (blockIdx.x - g.blk0) * blockDim.x + threadIdx.x);
+}
+
+template <bool MIXED, bool STORE>
+__global__ __launch_bounds__(kSrBlock) void sr_batch_kernel(
+    const double4 *__restrict__ atoms, const uint32_t *__restrict__ start,
+    const SysDev *__restrict__ sys, const Seg *__restrict__ seg, uint32_t nseg,
+    double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
+    double *__restrict__ en, double4 *__restrict__ out4) {
+  const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
+  const Seg g = seg[run_of(seg, nseg, wg)];
+  const SysDev &d = sys[g.sys];
+  sr_body<MIXED, STORE>(atoms, start + d.cell0, d.nx, d.ny, d.nz, fx, fy, fz,
+                        en, out4, wg - g.blk0);
 }
 
 }  // namespace
@@ -1353,14 +1670,10 @@ void sf(bool mixed, const void *atoms, const uint32_t *start,
   for (const SfGroup &gr : groups) {
     if (gr.ntask == 0) continue;
     const dim3 grid(g.ncell, (gr.ntask + kSfDirs - 1) / kSfDirs);
-    if (gr.nr >= 1 && gr.nr <= static_cast<uint32_t>(kSfMaxUnrolled))
-      kSfNrKernels[gr.nr - 1]<<<grid, kSfDirs, 0, s>>>(
-          static_cast<const float4 *>(atoms), start, tk, gr.task0, gr.ntask,
-          g.ncell, static_cast<float2 *>(S));
-    else
-      sf_mixed_any_kernel<<<grid, kSfDirs, 0, s>>>(
-          static_cast<const float4 *>(atoms), start, tk, gr.task0, gr.ntask,
-          g.ncell, static_cast<float2 *>(S));
+    kSfNrKernels[gr.nr <= static_cast<uint32_t>(kSfMaxUnrolled) ? gr.nr : 0]
+        <<<grid, kSfDirs, 0, s>>>(static_cast<const float4 *>(atoms), start,
+                                  tk, gr.task0, gr.ntask, g.ncell,
+                                  static_cast<float2 *>(S));
   }
 }
 
@@ -1496,6 +1809,144 @@ void coulomb_direct(const double *rx, const double *ry, const double *rz,
   if (natom == 0) return;
   coulomb_kernel<<<blocks_for(natom, kCoulBlock), kCoulBlock, 0, s>>>(
       rx, ry, rz, qc, natom, fx, fy, fz, en);
+}
+
+// ---- batched launchers -----------------------------------------------------
+
+void bin_chunks_batch(bool ct_float, const double *rx, const double *ry,
+                      const double *rz, const SysDev *sys,
+                      const BatchLaunch &L, uint32_t *atom_cell,
+                      uint32_t *rank, uint32_t *hist, hipStream_t s) {
+  if (L.nblk == 0) return;
+  if (ct_float)
+    bin_chunk_batch_kernel<float><<<L.nblk, kBinChunk, 0, s>>>(
+        rx, ry, rz, sys, L.seg, L.nseg, atom_cell, rank, hist);
+  else
+    bin_chunk_batch_kernel<double><<<L.nblk, kBinChunk, 0, s>>>(
+        rx, ry, rz, sys, L.seg, L.nseg, atom_cell, rank, hist);
+}
+
+void hist_offsets_batch(uint32_t *hist, const SysDev *sys,
+                        const BatchLaunch &L, uint32_t *count, hipStream_t s) {
+  if (L.nblk == 0) return;
+  hist_offsets_batch_kernel<<<L.nblk, 256, 0, s>>>(hist, sys, L.seg, L.nseg,
+                                                   count);
+}
+
+void cell_scan_batch(const uint32_t *count, const SysDev *sys, uint32_t nsys,
+                     uint32_t *start, uint32_t *maxc, hipStream_t s) {
+  if (nsys == 0) return;
+  cell_scan_batch_kernel<<<nsys, 1024, 0, s>>>(count, sys, nsys, start, maxc);
+}
+
+void bin_scatter_batch(bool mixed, const double *rx, const double *ry,
+                       const double *rz, const double *qc, const SysDev *sys,
+                       const BatchLaunch &L, const uint32_t *atom_cell,
+                       const uint32_t *rank, const uint32_t *hist,
+                       const uint32_t *start, uint32_t *sorted_idx,
+                       uint32_t *cell_sorted, double4 *atoms64,
+                       float4 *atoms32, hipStream_t s) {
+  if (L.nblk == 0) return;
+  if (mixed)
+    bin_scatter_batch_kernel<true><<<L.nblk, 256, 0, s>>>(
+        rx, ry, rz, qc, sys, L.seg, L.nseg, atom_cell, rank, hist, start,
+        sorted_idx, cell_sorted, atoms64, atoms32);
+  else
+    bin_scatter_batch_kernel<false><<<L.nblk, 256, 0, s>>>(
+        rx, ry, rz, qc, sys, L.seg, L.nseg, atom_cell, rank, hist, start,
+        sorted_idx, cell_sorted, atoms64, atoms32);
+}
+
+void sf_batch(bool mixed, const void *atoms, const uint32_t *start,
+              const double4 *k64, const LrTaskArgs &ta, const SysDev *sys,
+              const std::vector<BatchLaunch> &launches, void *S,
+              hipStream_t s) {
+  const LrTasks tk{ta.node0, ta.nr,      ta.koff,   ta.dir64,
+                   ta.dir32, ta.kappa64, ta.kappa32, ta.split};
+  for (const BatchLaunch &L : launches) {
+    if (L.nblk == 0) continue;
+    if (!mixed) {
+      sf_f64_batch_kernel<<<L.nblk, kSfBlock, 0, s>>>(
+          static_cast<const double4 *>(atoms), start, k64, sys, L.seg, L.nseg,
+          static_cast<double2 *>(S));
+      continue;
+    }
+    kSfBatchKernels[L.nr <= static_cast<uint32_t>(kSfMaxUnrolled) ? L.nr : 0]
+        <<<L.nblk, kSfDirs, 0, s>>>(static_cast<const float4 *>(atoms), start,
+                                    tk, sys, L.seg, L.nseg,
+                                    static_cast<float2 *>(S));
+  }
+}
+
+void rmt_batch(bool mixed, const void *S, const double4 *k64,
+               const uint32_t *group, const uint32_t *r_in,
+               const uint32_t *r_out, const SysDev *sys, const BatchLaunch &L,
+               size_t lds_bytes, void *R, hipStream_t s) {
+  if (L.nblk == 0) return;
+  if (mixed) {
+    (void)hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&rmt_lds_batch_kernel<true>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_bytes));
+    rmt_lds_batch_kernel<true><<<L.nblk, kRmtBlock, lds_bytes, s>>>(
+        S, k64, group, r_in, r_out, sys, L.seg, L.nseg, R);
+  } else {
+    (void)hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&rmt_lds_batch_kernel<false>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_bytes));
+    rmt_lds_batch_kernel<false><<<L.nblk, kRmtBlock, lds_bytes, s>>>(
+        S, k64, group, r_in, r_out, sys, L.seg, L.nseg, R);
+  }
+}
+
+void lr_batch(bool mixed, const void *atoms, const uint32_t *cell_sorted,
+              const LrTaskArgs &ta, const SysDev *sys,
+              const std::vector<BatchLaunch> &launches, const void *R,
+              double4 *partial, hipStream_t s) {
+  const LrTasks tk{ta.node0, ta.nr,      ta.koff,   ta.dir64,
+                   ta.dir32, ta.kappa64, ta.kappa32, ta.split};
+  for (const BatchLaunch &L : launches) {
+    if (L.nblk == 0) continue;
+    if (!mixed) {
+      lr_f64_batch_kernel<<<L.nblk, kLrBlock, 0, s>>>(
+          static_cast<const double4 *>(atoms), cell_sorted, tk,
+          static_cast<const double2 *>(R), sys, L.seg, L.nseg, partial);
+      continue;
+    }
+    kLrBatchKernels[L.nr <= static_cast<uint32_t>(kLrMaxUnrolled) ? L.nr : 0]
+        <<<L.nblk, kLrBlock, 0, s>>>(static_cast<const float4 *>(atoms),
+                                     cell_sorted, tk,
+                                     static_cast<const float2 *>(R), sys,
+                                     L.seg, L.nseg, partial);
+  }
+}
+
+void lr_reduce_batch(const double4 *partial, const SysDev *sys,
+                     const BatchLaunch &L, const double4 *near, double *fx,
+                     double *fy, double *fz, double *en, hipStream_t s) {
+  if (L.nblk == 0) return;
+  lr_reduce_batch_kernel<<<L.nblk, 256, 0, s>>>(partial, sys, L.seg, L.nseg,
+                                                near, fx, fy, fz, en);
+}
+
+void sr_batch(bool mixed, const double4 *atoms, const uint32_t *start,
+              const SysDev *sys, const BatchLaunch &L, double *fx, double *fy,
+              double *fz, double *en, double4 *out4, hipStream_t s) {
+  if (L.nblk == 0) return;
+  if (out4) {
+    if (mixed)
+      sr_batch_kernel<true, true><<<L.nblk, kSrBlock, 0, s>>>(
+          atoms, start, sys, L.seg, L.nseg, fx, fy, fz, en, out4);
+    else
+      sr_batch_kernel<false, true><<<L.nblk, kSrBlock, 0, s>>>(
+          atoms, start, sys, L.seg, L.nseg, fx, fy, fz, en, out4);
+  } else {
+    if (mixed)
+      sr_batch_kernel<true, false><<<L.nblk, kSrBlock, 0, s>>>(
+          atoms, start, sys, L.seg, L.nseg, fx, fy, fz, en, out4);
+    else
+      sr_batch_kernel<false, false><<<L.nblk, kSrBlock, 0, s>>>(
+          atoms, start, sys, L.seg, L.nseg, fx, fy, fz, en, out4);
+  }
 }
 
 }  // namespace dev

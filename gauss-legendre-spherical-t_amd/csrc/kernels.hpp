@@ -129,6 +129,80 @@ void unpermute(const uint32_t *sorted_idx, uint32_t natom, const double *fx,
                double *ofx, double *ofy, double *ofz, double *oen,
                hipStream_t s);
 
+// ---- batched systems ----------------------------------------------------------
+// Many independent systems in one set of launches.  Every system owns a
+// contiguous range of the batch's atoms, cells, nodes, tasks, splits, S/R
+// slots, histogram and partial rows; a batched launch covers all systems,
+// each workgroup finding its system through a run table (Seg).
+
+// One system of a batch as the batched kernels see it.
+struct SysDev {
+  uint32_t natom, atom0;              // atoms [atom0, atom0 + natom)
+  uint32_t nx, ny, nz, ncell, cell0;  // cells [cell0, cell0 + ncell)
+  uint32_t nchunk;                    // binning chunks of kBinChunk atoms
+  uint32_t Kp, node0;                 // nodes [node0, node0 + Kp)
+  uint32_t group0;                    // alpha groups at r_in/r_out[group0..]
+  uint32_t ntile;                     // back-projection atom tiles (256)
+  uint32_t nsplit, split0;            // splits [split0, split0 + nsplit)
+  uint32_t T;                         // shell-sum nodes per workgroup
+  uint32_t pad;
+  uint64_t hist0;                     // histogram [nchunk][ncell]
+  uint64_t field0;                    // S and R slots [field0, + ncell*Kp)
+  uint64_t partial0;                  // partial rows [nsplit][natom]
+  double h, inv_cell;                 // cell edge, CT(1)/CT(rcut) widened
+};
+
+// A run of a batched launch's workgroups that belongs to one system (and,
+// for launches per radial count, one alpha group): workgroups blk0 up to the
+// next run's blk0.  a, b: launch-specific (S(k): first task and task count
+// of the group; back-projection: first split and split count of the group).
+struct Seg {
+  uint32_t blk0, sys, a, b;
+};
+
+// One launch of a batched stage: device run table, runs, workgroups, and
+// for S(k) / back-projection the radial count of its groups.
+struct BatchLaunch {
+  const Seg *seg;
+  uint32_t nseg, nblk, nr;
+};
+
+void bin_chunks_batch(bool ct_float, const double *rx, const double *ry,
+                      const double *rz, const SysDev *sys,
+                      const BatchLaunch &L, uint32_t *atom_cell,
+                      uint32_t *rank, uint32_t *hist, hipStream_t s);
+void hist_offsets_batch(uint32_t *hist, const SysDev *sys,
+                        const BatchLaunch &L, uint32_t *count, hipStream_t s);
+// one workgroup per system; maxc[s] = largest cell of system s
+void cell_scan_batch(const uint32_t *count, const SysDev *sys, uint32_t nsys,
+                     uint32_t *start, uint32_t *maxc, hipStream_t s);
+void bin_scatter_batch(bool mixed, const double *rx, const double *ry,
+                       const double *rz, const double *qc, const SysDev *sys,
+                       const BatchLaunch &L, const uint32_t *atom_cell,
+                       const uint32_t *rank, const uint32_t *hist,
+                       const uint32_t *start, uint32_t *sorted_idx,
+                       uint32_t *cell_sorted, double4 *atoms64,
+                       float4 *atoms32, hipStream_t s);
+// mixed: one launch per radial count (L.nr); FP64: one launch (L.nr unused)
+void sf_batch(bool mixed, const void *atoms, const uint32_t *start,
+              const double4 *k64, const LrTaskArgs &tasks, const SysDev *sys,
+              const std::vector<BatchLaunch> &launches, void *S,
+              hipStream_t s);
+void rmt_batch(bool mixed, const void *S, const double4 *k64,
+               const uint32_t *group, const uint32_t *r_in,
+               const uint32_t *r_out, const SysDev *sys, const BatchLaunch &L,
+               size_t lds_bytes, void *R, hipStream_t s);
+void lr_batch(bool mixed, const void *atoms, const uint32_t *cell_sorted,
+              const LrTaskArgs &tasks, const SysDev *sys,
+              const std::vector<BatchLaunch> &launches, const void *R,
+              double4 *partial, hipStream_t s);
+void lr_reduce_batch(const double4 *partial, const SysDev *sys,
+                     const BatchLaunch &L, const double4 *near, double *fx,
+                     double *fy, double *fz, double *en, hipStream_t s);
+void sr_batch(bool mixed, const double4 *atoms, const uint32_t *start,
+              const SysDev *sys, const BatchLaunch &L, double *fx, double *fy,
+              double *fz, double *en, double4 *out4, hipStream_t s);
+
 // ---- direct all-pairs Coulomb (utils/coulomb.cu) -----------------------------
 void coulomb_direct(const double *rx, const double *ry, const double *rz,
                     const double *qc, uint32_t natom, double *fx, double *fy,

@@ -1,0 +1,358 @@
+// Host runtime shared by the single-system engine (engine.hip) and the
+// batched engine (batch.hip): error types, device buffers, kernel timing,
+// and the host setup of one system (grid, alpha groups, cubature nodes in
+// device order, back-projection splits).
+#pragma once
+
+#include "glst_hip.h"
+#include "kernels.hpp"
+#include "numerics.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace glst {
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct StateError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct ArgError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct AllocError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define GLST_HIP(call)                                                        \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      throw HipError(std::string("HIP error: ") + hipGetErrorString(e_) +    \
+                     " at " + __FILE__ + ":" + std::to_string(__LINE__));     \
+  } while (0)
+
+inline void check_launch(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    throw HipError(std::string("HIP launch error in ") + what + ": " +
+                   hipGetErrorString(e));
+}
+
+class DeviceBuffer {
+ public:
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer &) = delete;
+  DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+  ~DeviceBuffer() { release(); }
+  void resize(size_t bytes) {
+    if (bytes == bytes_) return;
+    release();
+    if (bytes == 0) return;
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw AllocError("hipMalloc of " + std::to_string(bytes) +
+                       " bytes failed: " + hipGetErrorString(e));
+    }
+    ptr_ = p;
+    bytes_ = bytes;
+  }
+  void release() {
+    if (ptr_) (void)hipFree(ptr_);
+    ptr_ = nullptr;
+    bytes_ = 0;
+  }
+  template <typename T>
+  T *as() const {
+    return static_cast<T *>(ptr_);
+  }
+  size_t bytes() const { return bytes_; }
+
+ private:
+  void *ptr_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+class KernelTimer {
+ public:
+  ~KernelTimer() {
+    for (auto &kv : stats_)
+      for (auto &pr : kv.second.pending) {
+        (void)hipEventDestroy(pr.first);
+        (void)hipEventDestroy(pr.second);
+      }
+    for (auto e : pool_) (void)hipEventDestroy(e);
+  }
+  bool enabled = false;
+  hipEvent_t take() {
+    if (!pool_.empty()) {
+      hipEvent_t e = pool_.back();
+      pool_.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    GLST_HIP(hipEventCreate(&e));
+    return e;
+  }
+  void begin(const char *name, hipStream_t s) {
+    if (!enabled) return;
+    cur_name_ = name;
+    cur_start_ = take();
+    GLST_HIP(hipEventRecord(cur_start_, s));
+  }
+  // Called from a destructor: never throws; a failed record drops the
+  // sample.
+  void end(hipStream_t s) noexcept {
+    if (!enabled) return;
+    try {
+      hipEvent_t stop = take();
+      if (hipEventRecord(stop, s) == hipSuccess)
+        stats_[cur_name_].pending.emplace_back(cur_start_, stop);
+    } catch (...) {
+    }
+  }
+  void collect() {
+    for (auto &kv : stats_) {
+      for (auto &pr : kv.second.pending) {
+        GLST_HIP(hipEventSynchronize(pr.second));
+        float ms = 0.f;
+        GLST_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+        kv.second.ms += ms;
+        kv.second.n += 1;
+        pool_.push_back(pr.first);
+        pool_.push_back(pr.second);
+      }
+      kv.second.pending.clear();
+    }
+  }
+  void get(const std::string &name, double &ms, uint32_t &n) {
+    collect();
+    auto it = stats_.find(name);
+    ms = it == stats_.end() ? 0.0 : it->second.ms;
+    n = it == stats_.end() ? 0 : it->second.n;
+  }
+  void reset() {
+    collect();
+    stats_.clear();
+  }
+
+ private:
+  struct Stat {
+    double ms = 0.0;
+    uint32_t n = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  };
+  std::map<std::string, Stat> stats_;
+  std::vector<hipEvent_t> pool_;
+  std::string cur_name_;
+  hipEvent_t cur_start_{};
+};
+
+struct Scope {
+  KernelTimer &t;
+  hipStream_t s;
+  Scope(KernelTimer &timer, const char *name, hipStream_t stream)
+      : t(timer), s(stream) {
+    t.begin(name, s);
+  }
+  ~Scope() { t.end(s); }
+};
+
+// Process-wide last error (functions without a handle) and the t-design
+// directory (GLST_TDESIGN_DIR or glst_set_tdesign_dir); engine.hip.
+void set_global_error(const std::string &m);
+std::string tdesign_dir();
+
+// Runs f, mapping exceptions to status codes; the message goes to *err (a
+// handle's last error) or, with err == nullptr, to the process-wide one.
+template <class F>
+glst_status_t run_guarded(std::string *err, F &&f) {
+  auto record = [&](const std::string &m) {
+    if (err)
+      *err = m;
+    else
+      set_global_error(m);
+  };
+  try {
+    f();
+    if (err) err->clear();
+    return GLST_OK;
+  } catch (const ArgError &e) {
+    record(e.what());
+    return GLST_ERR_INVALID;
+  } catch (const StateError &e) {
+    record(e.what());
+    return GLST_ERR_STATE;
+  } catch (const AllocError &e) {
+    record(e.what());
+    return GLST_ERR_ALLOC;
+  } catch (const HipError &e) {
+    record(e.what());
+    return GLST_ERR_HIP;
+  } catch (const std::exception &e) {
+    const std::string m = e.what();
+    record(m);
+    if (m.find("open") != std::string::npos ||
+        m.find("Malformed") != std::string::npos ||
+        m.find("directory") != std::string::npos)
+      return GLST_ERR_IO;
+    return GLST_ERR_NUMERICS;
+  }
+}
+
+// ---- host setup of one system ------------------------------------------------
+
+// Grid, alpha groups and cubature of one system (src/glst_force.cu:178-327,
+// src/cubature.cu:99-159).  The grid uses CT arithmetic (ct = GLST_CT_*).
+struct SystemSetup {
+  dev::GridShape grid{};
+  double inv_cell = 0.0;  // CT(1)/CT(rcut), widened
+  bool has_lr = false;    // all n_d > 2 (src/glst_force.cu:667-670)
+  AlphaGroups groups;
+  NodeTable nodes;
+  uint32_t K = 0, Kp = 0;  // nodes, padded to a multiple of 256
+  int spherical_rule = 0;  // 0 product rule, 1 design files
+};
+
+inline void check_system_args(int ct, double bx, double by, double bz,
+                              double rcut, double tol) {
+  if (ct != GLST_CT_DOUBLE && ct != GLST_CT_FLOAT)
+    throw ArgError("ct must be GLST_CT_DOUBLE or GLST_CT_FLOAT");
+  if (!(bx > 0 && by > 0 && bz > 0 && rcut > 0))
+    throw ArgError("box dimensions and rcut must be positive");
+  if (!(tol > 0)) throw ArgError("tol must be positive");
+}
+
+inline SystemSetup setup_system(int ct, double bx, double by, double bz,
+                                double rcut, double tol) {
+  check_system_args(ct, bx, by, bz, rcut, tol);
+  SystemSetup S;
+  double cell;
+  if (ct == GLST_CT_FLOAT) {
+    grid_dims<float>(bx, by, bz, rcut, S.grid.nx, S.grid.ny, S.grid.nz);
+    const float cf = static_cast<float>(rcut);
+    cell = static_cast<double>(cf);
+    S.inv_cell = static_cast<double>(1.0f / cf);
+  } else {
+    grid_dims<double>(bx, by, bz, rcut, S.grid.nx, S.grid.ny, S.grid.nz);
+    cell = rcut;
+    S.inv_cell = 1.0 / rcut;
+  }
+  S.grid.h = cell;
+  const uint64_t nc = static_cast<uint64_t>(S.grid.nx) * S.grid.ny * S.grid.nz;
+  if (nc == 0 || nc > 0x7FFFFFFFull)
+    throw ArgError("cell grid of " + std::to_string(nc) +
+                   " cells is out of range");
+  S.grid.ncell = static_cast<uint32_t>(nc);
+  S.has_lr = S.grid.nx > 2 && S.grid.ny > 2 && S.grid.nz > 2;
+  S.groups = alpha_groups(S.grid.nx, S.grid.ny, S.grid.nz, cell, cell, cell,
+                          tol);
+  const std::string dir = tdesign_dir();
+  S.spherical_rule = dir.empty() ? 0 : 1;
+  S.nodes = build_nodes(tol, S.groups, dir);
+  S.K = static_cast<uint32_t>(S.nodes.kx.size());
+  S.Kp = (S.K + 255u) / 256u * 256u;
+  return S;
+}
+
+// Node and task tables in device order.  Inside each alpha group the nodes
+// are direction-major: task (group, direction m) covers nodes first + m*nr
+// .. + nr - 1 with k = kappa_i s_m.  Node indices are local to the system;
+// koff indexes kappa64/kappa32 of the system.
+struct NodeLayout {
+  std::vector<double4> k64;     // (kx, ky, kz, W) per device node, Kp
+  std::vector<uint32_t> group;  // group per device node, Kp
+  std::vector<uint32_t> node0, nr, koff;  // per task
+  std::vector<double4> dir64;
+  std::vector<float4> dir32;
+  std::vector<double> kappa64;  // kappa_i per group
+  std::vector<float> kappa32;   // kappa_i / 2 pi
+  std::vector<uint32_t> dev_of_ref;  // device index of each reference node
+};
+
+inline NodeLayout layout_nodes(const NodeTable &nodes, uint32_t K,
+                               uint32_t Kp) {
+  NodeLayout L;
+  const double inv2pi = 1.0 / 6.283185307179586476925286766559;
+  L.k64.assign(Kp, make_double4(0, 0, 0, 0));
+  L.group.assign(Kp, 0);
+  L.dev_of_ref.assign(K, 0);
+  for (size_t g = 0; g < nodes.dirs.size(); ++g) {
+    const uint32_t nr = static_cast<uint32_t>(nodes.kappa[g].size());
+    const uint32_t M = static_cast<uint32_t>(nodes.dirs[g].size());
+    const uint32_t first = nodes.first[g];
+    const uint32_t koff = static_cast<uint32_t>(L.kappa64.size());
+    for (double kap : nodes.kappa[g]) {
+      L.kappa64.push_back(kap);
+      L.kappa32.push_back(static_cast<float>(kap * inv2pi));
+    }
+    for (uint32_t m = 0; m < M; ++m) {
+      const auto &sd = nodes.dirs[g][m];
+      L.node0.push_back(first + m * nr);
+      L.nr.push_back(nr);
+      L.koff.push_back(koff);
+      L.dir64.push_back(make_double4(sd[0], sd[1], sd[2], 0.0));
+      L.dir32.push_back(make_float4(static_cast<float>(sd[0]),
+                                    static_cast<float>(sd[1]),
+                                    static_cast<float>(sd[2]), 0.f));
+      for (uint32_t i = 0; i < nr; ++i) {
+        const uint32_t ref = first + i * M + m;  // reference order
+        const uint32_t dev = first + m * nr + i;  // direction-major
+        L.dev_of_ref[ref] = dev;
+        L.k64[dev] = make_double4(nodes.kx[ref], nodes.ky[ref], nodes.kz[ref],
+                                  nodes.w[ref]);
+        L.group[dev] = static_cast<uint32_t>(g);
+      }
+    }
+  }
+  return L;
+}
+
+// Back-projection splits: contiguous task ranges that never cross an alpha
+// group (all tasks of a split then share the radial count and kappa_i);
+// each group gets about target * K_g / K splits of equal size.  Task and
+// split indices are local to the system.
+struct SplitPlan {
+  std::vector<uint32_t> split;         // nsplit + 1 task boundaries
+  std::vector<dev::LrGroup> lr_groups;  // per group: nr, first split, splits
+  std::vector<dev::SfGroup> sf_groups;  // per group: nr, first task, tasks
+  uint32_t nsplit = 0;
+};
+
+inline SplitPlan make_splits(const NodeTable &nodes, uint32_t K,
+                             uint32_t target) {
+  SplitPlan P;
+  P.split.assign(1, 0);
+  uint32_t base = 0;
+  for (size_t g = 0; g < nodes.dirs.size(); ++g) {
+    const uint32_t M = static_cast<uint32_t>(nodes.dirs[g].size());
+    const double share =
+        static_cast<double>(target) * nodes.count[g] / std::max<uint32_t>(K, 1);
+    uint32_t ns = static_cast<uint32_t>(share + 0.5);
+    ns = std::max<uint32_t>(1, std::min<uint32_t>(ns, M));
+    P.lr_groups.push_back({static_cast<uint32_t>(nodes.kappa[g].size()),
+                           static_cast<uint32_t>(P.split.size() - 1), ns});
+    P.sf_groups.push_back(
+        {static_cast<uint32_t>(nodes.kappa[g].size()), base, M});
+    for (uint32_t q = 1; q <= ns; ++q)
+      P.split.push_back(base + static_cast<uint32_t>(
+                                   (static_cast<uint64_t>(M) * q) / ns));
+    base += M;
+  }
+  if (P.split.size() == 1) P.split.push_back(0);  // no tasks at all
+  P.nsplit = static_cast<uint32_t>(P.split.size() - 1);
+  return P;
+}
+
+}  // namespace glst

@@ -136,6 +136,19 @@ def _load():
         "glst_grid_dims": [c_int, d, d, d, d, POINTER(u), POINTER(u),
                            POINTER(u)],
         "glst_make_water_box": [u, u, d, c_uint64, P, P, P, P],
+        # batched engine (include/glst_batch.h)
+        "glst_batch_create": [c_int, u, P, P, P, d, P, POINTER(P)],
+        "glst_batch_destroy": [P],
+        "glst_batch_device_bytes": [c_int, u, P, P, P, d, POINTER(c_size_t)],
+        "glst_batch_num_systems": [P, POINTER(u)],
+        "glst_batch_atom_offset": [P, u, POINTER(u)],
+        "glst_batch_get_info": [P, u, POINTER(Info)],
+        "glst_batch_calc_ener_force": [P, P, P, P, P],
+        "glst_batch_calc_ener_force_async": [P, P, P, P, P],
+        "glst_batch_get_ef": [P, P, P, P, P],
+        "glst_batch_zero_ef": [P],
+        "glst_batch_set_option": [P, c_char_p, ctypes.c_int64],
+        "glst_batch_set_stream": [P, P],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -147,6 +160,8 @@ def _load():
     lib.glst_global_last_error.restype = c_char_p
     lib.glst_version.argtypes = []
     lib.glst_version.restype = c_char_p
+    lib.glst_batch_last_error.argtypes = [P]
+    lib.glst_batch_last_error.restype = c_char_p
     return lib
 
 
@@ -427,3 +442,112 @@ class GLSTForce:
 
     def reset_timing(self):
         self._c(_lib.glst_reset_timing(self._h))
+
+
+# ---- the batched engine -----------------------------------------------------
+
+def _batch_arrays(natoms, boxes, rcuts):
+    import numpy as np
+    n = np.ascontiguousarray(natoms, dtype=np.uint32)
+    b = np.asarray(boxes, dtype=np.float64)
+    if b.ndim == 1:
+        b = np.repeat(b[:, None], 3, axis=1)
+    b = np.ascontiguousarray(b.reshape(len(n), 3))
+    r = np.ascontiguousarray(np.broadcast_to(
+        np.asarray(rcuts, dtype=np.float64), (len(n),)))
+    return n, b, r
+
+
+def batch_device_bytes(natoms, boxes, rcuts, tol: float,
+                       ct: str = "double") -> int:
+    """Device bytes a batch of these systems would hold (host-only)."""
+    n, b, r = _batch_arrays(natoms, boxes, rcuts)
+    out = c_size_t()
+    _check(_lib.glst_batch_device_bytes(
+        CT_FLOAT if ct in ("float", CT_FLOAT) else CT_DOUBLE, len(n), _ptr(n),
+        _ptr(b), _ptr(r), tol, byref(out)))
+    return out.value
+
+
+class GLSTBatch:
+    """Many independent systems, each a ``glst_force<CT>``, computed together
+    (``include/glst_batch.h``).  ``natoms[s]`` atoms in a box ``boxes[s]``
+    (a scalar edge or three edges) with cell edge ``rcuts[s]``; inputs and
+    outputs are packed: system s owns atoms ``offset(s) .. offset(s) +
+    natoms[s] - 1``."""
+
+    def __init__(self, natoms, boxes, rcuts, tol: float, ct: str = "double",
+                 stream=None):
+        import torch
+        n, b, r = _batch_arrays(natoms, boxes, rcuts)
+        self.natoms = [int(v) for v in n]
+        self.ct = CT_FLOAT if ct in ("float", CT_FLOAT) else CT_DOUBLE
+        if stream is None and torch.cuda.is_available():
+            stream = torch.cuda.current_stream().cuda_stream
+        h = c_void_p()
+        _check(_lib.glst_batch_create(self.ct, len(n), _ptr(n), _ptr(b),
+                                      _ptr(r), tol, c_void_p(stream),
+                                      byref(h)))
+        self._h = h
+        self.natom = int(sum(self.natoms))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.glst_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _c(self, status):
+        if status != 0:
+            msg = _lib.glst_batch_last_error(self._h) if self._h else b""
+            if not msg:
+                msg = _lib.glst_global_last_error()
+            raise GLSTError(status, (msg or b"").decode())
+
+    def num_systems(self) -> int:
+        v = c_uint32()
+        self._c(_lib.glst_batch_num_systems(self._h, byref(v)))
+        return v.value
+
+    def offset(self, s: int) -> int:
+        v = c_uint32()
+        self._c(_lib.glst_batch_atom_offset(self._h, s, byref(v)))
+        return v.value
+
+    def info(self, s: int) -> Info:
+        i = Info()
+        self._c(_lib.glst_batch_get_info(self._h, s, byref(i)))
+        return i
+
+    def calc_ener_force(self, rx, ry, rz, qc):
+        self._c(_lib.glst_batch_calc_ener_force(self._h, _ptr(rx), _ptr(ry),
+                                                _ptr(rz), _ptr(qc)))
+
+    def calc_ener_force_async(self, rx, ry, rz, qc):
+        self._c(_lib.glst_batch_calc_ener_force_async(
+            self._h, _ptr(rx), _ptr(ry), _ptr(rz), _ptr(qc)))
+
+    def get_ef(self):
+        """(fx, fy, fz, en) of all systems, packed, original order."""
+        import torch
+        out = [torch.empty(self.natom, dtype=torch.float64, device="cuda")
+               for _ in range(4)]
+        self._c(_lib.glst_batch_get_ef(self._h, *[_ptr(o) for o in out]))
+        return tuple(out)
+
+    def zero_ef(self):
+        self._c(_lib.glst_batch_zero_ef(self._h))
+
+    def set_option(self, name: str, value: int):
+        self._c(_lib.glst_batch_set_option(self._h, name.encode(), int(value)))

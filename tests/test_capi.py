@@ -19,10 +19,14 @@ def declared(header):
         r"(?:glst_status_t|const char\s*\*)\s*(glst_[a-z0-9_]+)\s*\(", text)))
 
 
+HEADERS = ["glst_hip.h", "glst_batch.h", "glst_io.h"]
+
+
 def test_every_declared_symbol_is_exported():
     lib = g.library()
-    names = declared("glst_hip.h")
-    assert len(names) > 30
+    names = [n for h in HEADERS for n in declared(h)]
+    assert len(declared("glst_hip.h")) > 30
+    assert len(declared("glst_batch.h")) >= 12
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
 
@@ -31,7 +35,8 @@ def test_exported_symbols_via_nm():
     out = subprocess.run(["nm", "-D", "--defined-only", g.LIB_PATH],
                          capture_output=True, text=True, check=True).stdout
     exported = set(re.findall(r" T (glst_\w+)", out))
-    assert set(declared("glst_hip.h")) <= exported
+    for h in HEADERS:
+        assert set(declared(h)) <= exported, h
 
 
 def test_version():
@@ -93,3 +98,42 @@ def test_single_hip_runtime_in_process():
     out = subprocess.run(["python", "-c", code], capture_output=True, text=True,
                          check=True, timeout=300).stdout.strip()
     assert out == "1"
+
+
+# ---- batched engine: host-side planning (no GPU needed) ----------------------
+
+def test_batch_device_bytes_grows_with_systems():
+    one = g.batch_device_bytes([3000], [31.04], 9.0, 1e-4)
+    two = g.batch_device_bytes([3000, 3000], [31.04, 31.04], 9.0, 1e-4)
+    assert 0 < one < two
+    # the mixed path stores FP32 fields
+    assert g.batch_device_bytes([3000], [31.04], 9.0, 1e-4, ct="float") < one
+
+
+def test_batch_device_bytes_no_far_field_system():
+    # a 14.4 A box with rcut 9 has a 1x1x1 grid: near field only, no fields
+    small = g.batch_device_bytes([300], [14.4], 9.0, 1e-4)
+    assert 0 < small < 1_000_000
+
+
+@pytest.mark.parametrize("natoms,boxes,rcuts,tol", [
+    ([], [], [], 1e-4),            # no systems
+    ([0], [31.0], [9.0], 1e-4),    # a system without atoms
+    ([30], [-1.0], [9.0], 1e-4),   # negative box
+    ([30], [31.0], [0.0], 1e-4),   # zero rcut
+    ([30], [31.0], [9.0], 0.0),    # zero tol
+])
+def test_batch_device_bytes_rejects_bad_arguments(natoms, boxes, rcuts, tol):
+    import numpy as np
+    with pytest.raises((g.GLSTError, ValueError)) as e:
+        g.batch_device_bytes(np.array(natoms, dtype=np.uint32),
+                             np.array(boxes, dtype=np.float64),
+                             np.array(rcuts, dtype=np.float64), tol)
+    if isinstance(e.value, g.GLSTError):
+        assert e.value.status == 1
+
+
+def test_batch_create_rejects_bad_arguments_before_the_device():
+    with pytest.raises(g.GLSTError) as e:
+        g.GLSTBatch([30, 30], [31.0, -2.0], 9.0, 1e-4, stream=0)
+    assert e.value.status == 1

@@ -1,0 +1,141 @@
+"""Batched multi-system engine (include/glst_batch.h) on the GPU.
+
+Every system of a batch must give what a single-system handle gives for it
+alone: the same kernels on the same data, differing only in how the
+back-projection's direction ranges are split (FP64: summation order, bar
+1e-12; mixed: FP32 partial-sum boundaries, bar 1e-6), and the FP64 batch
+must match the oracle like the single path does (1e-10).  Systems are water
+boxes at 0.0334 molecules/A^3 with rcut 9 A, from a 1x1x1 grid (no far
+field) to 5x5x5.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+NWATERS = [100, 300, 1000, 2000, 500, 1500]
+RCUT = 9.0
+
+
+def _box(nw):
+    return (nw / 0.0334) ** (1.0 / 3.0)
+
+
+def _systems(nwaters, seed0=11):
+    import glst_amd as g
+    out = []
+    for k, nw in enumerate(nwaters):
+        L = _box(nw)
+        out.append((L, g.make_water_box(nw, L, seed=seed0 + k)))
+    return out
+
+
+def _pack(systems, torch):
+    cols = [np.concatenate([h[i] for _, h in systems]) for i in range(4)]
+    return [torch.as_tensor(c, dtype=torch.float64, device="cuda")
+            for c in cols]
+
+
+def _single(systems, tol, ct, torch):
+    import glst_amd as g
+    res = []
+    for L, h in systems:
+        d = [torch.as_tensor(a, dtype=torch.float64, device="cuda") for a in h]
+        f = g.GLSTForce(h[0].size, tol, L, rcut=RCUT, ct=ct)
+        f.calc_ener_force(*d)
+        res.append([t.cpu().numpy() for t in f.get_ef()])
+        f.close()
+    return res
+
+
+@pytest.fixture(scope="module")
+def systems():
+    return _systems(NWATERS)
+
+
+@pytest.mark.parametrize("ct,bar", [("double", 1e-12), ("float", 1e-6)])
+def test_batch_matches_single_handles(cuda, systems, ct, bar):
+    import torch
+    import glst_amd as g
+    tol = 1e-4
+    natoms = [h[0].size for _, h in systems]
+    b = g.GLSTBatch(natoms, [L for L, _ in systems], RCUT, tol, ct=ct)
+    assert b.num_systems() == len(systems)
+    b.calc_ener_force(*_pack(systems, torch))
+    got = [t.cpu().numpy() for t in b.get_ef()]
+    ref = _single(systems, tol, ct, torch)
+    for s, (L, h) in enumerate(systems):
+        o = b.offset(s)
+        assert o == sum(natoms[:s])
+        part = [a[o:o + natoms[s]] for a in got]
+        assert rel_err(part[:3], ref[s][:3]) < bar, s
+        assert rel_err(part[3:], ref[s][3:]) < bar, s
+        inf = b.info(s)
+        assert inf.natom == natoms[s]
+        assert tuple(inf.ncell) == g.grid_dims(1 if ct == "float" else 0,
+                                               L, L, L, RCUT)
+    # the smallest box has no far field (its grid is 1x1x1)
+    assert b.info(0).has_long_range == 0
+    assert b.info(3).has_long_range == 1
+    b.close()
+
+
+def test_batch_fp64_matches_oracle(cuda):
+    import torch
+    import glst_amd as g
+    from oracle import oracle as o
+    systems = _systems([300, 700, 1000], seed0=5)
+    tol = 1e-5
+    natoms = [h[0].size for _, h in systems]
+    b = g.GLSTBatch(natoms, [L for L, _ in systems], RCUT, tol, ct="double")
+    b.calc_ener_force(*_pack(systems, torch))
+    got = [t.cpu().numpy() for t in b.get_ef()]
+    for s, (L, h) in enumerate(systems):
+        o0 = b.offset(s)
+        part = [a[o0:o0 + natoms[s]] for a in got]
+        ref = o.glst(*h, tol, L, RCUT)
+        assert rel_err(part[:3], ref[:3]) < 1e-10, s
+        assert rel_err(part[3:], ref[3:]) < 1e-10, s
+    b.close()
+
+
+@pytest.mark.parametrize("ct", ["double", "float"])
+def test_batch_accumulates_overlap_modes_agree(cuda, systems, ct):
+    """Outputs accumulate like the reference and zero_ef clears them; the
+    near field on the side stream changes no arithmetic."""
+    import torch
+    import glst_amd as g
+    natoms = [h[0].size for _, h in systems]
+    d = _pack(systems, torch)
+    b = g.GLSTBatch(natoms, [L for L, _ in systems], RCUT, 1e-4, ct=ct)
+    b.calc_ener_force(*d)
+    one = [t.cpu().numpy() for t in b.get_ef()]
+    b.calc_ener_force(*d)
+    two = [t.cpu().numpy() for t in b.get_ef()]
+    for x, y in zip(one, two):
+        np.testing.assert_allclose(y, 2 * x, rtol=1e-12, atol=1e-12)
+    b.zero_ef()
+    b.set_option("overlap_near", 0)
+    b.calc_ener_force(*d)
+    seq = [t.cpu().numpy() for t in b.get_ef()]
+    for x, y in zip(one, seq):
+        np.testing.assert_array_equal(x, y)
+    with pytest.raises(g.GLSTError):
+        b.set_option("overlap_near", 2)
+    b.close()
+
+
+def test_batch_single_system_equals_handle(cuda):
+    """A batch of one is the single-system engine."""
+    import torch
+    import glst_amd as g
+    systems = _systems([1000], seed0=3)
+    L, h = systems[0]
+    b = g.GLSTBatch([h[0].size], [L], RCUT, 1e-5, ct="double")
+    b.calc_ener_force(*_pack(systems, torch))
+    got = [t.cpu().numpy() for t in b.get_ef()]
+    ref = _single(systems, 1e-5, "double", torch)[0]
+    assert rel_err(got[:3], ref[:3]) < 1e-12
+    b.close()
