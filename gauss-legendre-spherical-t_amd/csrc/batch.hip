@@ -1,134 +1,103 @@
-// The batched multi-system engine behind include/glst_batch.h.  Every system
-// is set up exactly as a single glst_force (runtime.hpp: grid, alpha groups,
-// cubature, device node order, back-projection splits); the batch then packs
-// all systems' atoms, cells, nodes, tasks, splits and fields into shared
-// arenas and runs each stage of the force step as one launch over all
-// systems (S(k) and the back-projection: one launch per radial node count).
-// A workgroup finds its system through a run table of (first workgroup,
-// system, group data) records.
+// The batched multi-system engine behind include/glst_batch.h (see
+// batch.hpp for the design).
 #include "glst_batch.h"
-#include "runtime.hpp"
+#include "batch.hpp"
 
 #include <cstring>
-#include <map>
 #include <new>
 
 using namespace glst;
 
 namespace {
 
-struct Plan {
-  std::vector<dev::Seg> seg;
-  uint32_t nblk = 0;
-  uint32_t nr = 0;
-  void add(uint32_t sys, uint32_t blocks, uint32_t a = 0, uint32_t b = 0) {
-    if (blocks == 0) return;
-    seg.push_back({nblk, sys, a, b});
-    nblk += blocks;
-  }
-};
-
 uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
-// Host layout of a whole batch: per-system setups, SysDev records, arena
-// sizes, packed node/task tables and the run tables of every launch.
-struct BatchLayout {
-  int ct = GLST_CT_DOUBLE;
-  std::vector<SystemSetup> su;
-  std::vector<dev::SysDev> sys;
-  std::vector<SplitPlan> splits;
-  std::vector<std::vector<uint32_t>> dev_of_ref;
-  uint64_t natom = 0, ncell = 0, hist = 0, nodes = 0, field = 0, partial = 0;
-  uint32_t ngroup = 0;
-  size_t lds = 0;  // dynamic LDS of the shell-sum launch
-  // packed tables
-  std::vector<double4> k64;
-  std::vector<uint32_t> group, r_in, r_out;
-  std::vector<uint32_t> t_node0, t_nr, t_koff, t_split;
-  std::vector<double4> t_dir64;
-  std::vector<float4> t_dir32;
-  std::vector<double> kap64;
-  std::vector<float> kap32;
-  // launches
-  Plan bin, hist_p, scatter, sr, reduce, rmt, sf_f64, lr_f64;
-  std::map<uint32_t, Plan> sf_nr, lr_nr;  // radial count (0: any) -> plan
+}  // namespace
 
-  bool mixed() const { return ct == GLST_CT_FLOAT; }
+namespace glst {
 
-  size_t device_bytes() const {
-    const size_t n = natom, c = ncell;
-    size_t b = 0;
-    b += n * 4 * 4;                          // atom_cell, rank, sorted, cell
-    b += n * sizeof(double4) * 2;            // atoms64, near4
-    b += mixed() ? n * sizeof(float4) : 0;   // atoms32
-    b += n * 8 * 4;                          // fx..en
-    b += hist * 4 + (c + 1) * 4 + c * 4 + sys.size() * 4;
-    b += nodes * (sizeof(double4) + 4);
-    b += (r_in.size() + r_out.size()) * 4;
-    b += t_node0.size() * (12 + sizeof(double4) + sizeof(float4));
-    b += kap64.size() * 12 + t_split.size() * 4;
-    b += field * (mixed() ? 8 : 16) * 2;     // S and R
-    b += partial * sizeof(double4);
-    b += sys.size() * sizeof(dev::SysDev);
-    size_t nseg = bin.seg.size() + hist_p.seg.size() + scatter.seg.size() +
-                  sr.seg.size() + reduce.seg.size() + rmt.seg.size() +
-                  sf_f64.seg.size() + lr_f64.seg.size();
-    for (auto &kv : sf_nr) nseg += kv.second.seg.size();
-    for (auto &kv : lr_nr) nseg += kv.second.seg.size();
-    b += nseg * sizeof(dev::Seg);
-    return b;
-  }
-};
+std::shared_ptr<const BatchSystem> make_batch_system(int ct, uint32_t natom,
+                                                     const double box[3],
+                                                     double rcut, double tol) {
+  if (natom == 0) throw ArgError("a system without atoms");
+  auto b = std::make_shared<BatchSystem>();
+  b->natom = natom;
+  b->su = setup_system(ct, box[0], box[1], box[2], rcut, tol);
+  if (b->su.has_lr && b->su.K > 0)
+    b->nl = layout_nodes(b->su.nodes, b->su.K, b->su.Kp);
+  return b;
+}
 
-BatchLayout plan_batch(int ct, uint32_t nsys, const uint32_t *natom,
-                       const double *box, const double *rcut, double tol,
+size_t BatchLayout::device_bytes() const {
+  const size_t n = natom, c = ncell;
+  size_t b = 0;
+  b += n * 4 * 4;                         // atom_cell, rank, sorted, cell
+  b += n * sizeof(double4) * 2;           // atoms64, near4
+  b += mixed() ? n * sizeof(float4) : 0;  // atoms32
+  b += n * 8 * 4;                         // fx..en
+  b += hist * 4 + (c + 1) * 4 + c * 4 + sys.size() * 4;
+  b += nodes * (sizeof(double4) + 4);
+  b += (r_in.size() + r_out.size()) * 4;
+  b += t_node0.size() * (12 + sizeof(double4) + sizeof(float4));
+  b += kap64.size() * 12 + t_split.size() * 4;
+  b += field * (mixed() ? 8 : 16) * 2;  // S and R
+  b += partial * sizeof(double4);
+  b += sys.size() * sizeof(dev::SysDev);
+  size_t nseg = bin.seg.size() + hist_p.seg.size() + scatter.seg.size() +
+                sr.seg.size() + reduce.seg.size() + rmt.seg.size() +
+                sf_f64.seg.size() + lr_f64.seg.size();
+  for (auto &kv : sf_nr) nseg += kv.second.seg.size();
+  for (auto &kv : lr_nr) nseg += kv.second.seg.size();
+  b += nseg * sizeof(dev::Seg);
+  return b;
+}
+
+BatchLayout pack_batch(int ct,
+                       const std::vector<std::shared_ptr<const BatchSystem>> &cs,
                        size_t lds_limit) {
-  if (nsys == 0) throw ArgError("a batch needs at least one system");
-  if (!natom || !box || !rcut) throw ArgError("null system description");
+  if (cs.empty()) throw ArgError("a batch needs at least one system");
+  const uint32_t nsys = static_cast<uint32_t>(cs.size());
   BatchLayout B;
   B.ct = ct;
-  B.su.reserve(nsys);
-  for (uint32_t s = 0; s < nsys; ++s) {
-    if (natom[s] == 0)
-      throw ArgError("system " + std::to_string(s) + " has no atoms");
-    B.su.push_back(setup_system(ct, box[3 * s], box[3 * s + 1],
-                                box[3 * s + 2], rcut[s], tol));
-  }
+  B.cs = cs;
   // back-projection splits: enough workgroups over the whole batch
   uint64_t tiles = 0;
-  for (uint32_t s = 0; s < nsys; ++s)
-    if (B.su[s].has_lr && B.su[s].K > 0) tiles += div_up(natom[s], 256);
+  for (const auto &c : cs)
+    if (c->su.has_lr && c->su.K > 0) tiles += div_up(c->natom, 256);
   const uint32_t target =
       tiles ? static_cast<uint32_t>(std::max<uint64_t>(1, (4096 + tiles - 1) / tiles))
             : 1;
+  std::vector<SplitPlan> splits(nsys);
   uint32_t task_base = 0, split_base = 0;
   for (uint32_t s = 0; s < nsys; ++s) {
-    const SystemSetup &u = B.su[s];
+    const SystemSetup &u = cs[s]->su;
+    const uint32_t na = cs[s]->natom;
     const bool lr = u.has_lr && u.K > 0;
     dev::SysDev d{};
-    d.natom = natom[s];
+    d.natom = na;
     d.atom0 = static_cast<uint32_t>(B.natom);
     d.nx = u.grid.nx;
     d.ny = u.grid.ny;
     d.nz = u.grid.nz;
     d.ncell = u.grid.ncell;
     d.cell0 = static_cast<uint32_t>(B.ncell);
-    d.nchunk = div_up(natom[s], dev::kBinChunk);
+    d.nchunk = div_up(na, dev::kBinChunk);
     d.hist0 = B.hist;
     d.h = u.grid.h;
     d.inv_cell = u.inv_cell;
-    d.ntile = div_up(natom[s], 256);
+    d.ntile = div_up(na, 256);
     d.group0 = B.ngroup;
     d.Kp = lr ? u.Kp : 0;
     d.node0 = static_cast<uint32_t>(B.nodes);
     d.field0 = B.field;
     d.T = 1;
-    B.natom += natom[s];
+    d.split0 = split_base;
+    d.partial0 = B.partial;
+    B.natom += na;
     B.ncell += u.grid.ncell;
     B.hist += static_cast<uint64_t>(d.nchunk) * u.grid.ncell;
     if (B.natom > 0xFFFFFFFFull || B.ncell > 0x7FFFFFFFull)
       throw ArgError("batch exceeds 2^32 atoms or 2^31 cells");
-    SplitPlan P;
     if (lr) {
       const size_t per_node =
           (static_cast<size_t>(u.grid.ncell) + u.grid.nx + u.grid.ny +
@@ -140,7 +109,8 @@ BatchLayout plan_batch(int ct, uint32_t nsys, const uint32_t *natom,
                        "use a single-system handle");
       d.T = 2 * per_node <= lds_limit ? 2 : 1;
       B.lds = std::max(B.lds, d.T * per_node);
-      NodeLayout L = layout_nodes(u.nodes, u.K, u.Kp);
+      const NodeLayout &L = cs[s]->nl;
+      SplitPlan &P = splits[s];
       P = make_splits(u.nodes, u.K, target);
       const uint32_t koff = static_cast<uint32_t>(B.kap64.size());
       B.k64.insert(B.k64.end(), L.k64.begin(), L.k64.end());
@@ -154,29 +124,21 @@ BatchLayout plan_batch(int ct, uint32_t nsys, const uint32_t *natom,
       }
       B.kap64.insert(B.kap64.end(), L.kappa64.begin(), L.kappa64.end());
       B.kap32.insert(B.kap32.end(), L.kappa32.begin(), L.kappa32.end());
-      B.dev_of_ref.push_back(std::move(L.dev_of_ref));
       d.nsplit = P.nsplit;
-      d.split0 = split_base;
-      d.partial0 = B.partial;
-      // global split boundaries (task indices of the batch)
+      // split boundaries as task indices of the batch
       for (uint32_t q = 0; q < P.nsplit; ++q)
         B.t_split.push_back(task_base + P.split[q]);
       B.nodes += u.Kp;
       B.field += static_cast<uint64_t>(u.grid.ncell) * u.Kp;
-      B.partial += static_cast<uint64_t>(P.nsplit) * natom[s];
+      B.partial += static_cast<uint64_t>(P.nsplit) * na;
       task_base += static_cast<uint32_t>(L.node0.size());
       split_base += P.nsplit;
-    } else {
-      B.dev_of_ref.emplace_back();
-      d.nsplit = 0;
-      d.split0 = split_base;
-      d.partial0 = B.partial;
     }
     B.r_in.insert(B.r_in.end(), u.groups.r_in.begin(), u.groups.r_in.end());
-    B.r_out.insert(B.r_out.end(), u.groups.r_out.begin(), u.groups.r_out.end());
+    B.r_out.insert(B.r_out.end(), u.groups.r_out.begin(),
+                   u.groups.r_out.end());
     B.ngroup += static_cast<uint32_t>(u.groups.r_in.size());
     B.sys.push_back(d);
-    B.splits.push_back(std::move(P));
   }
   B.t_split.push_back(task_base);  // end of the last split
   // run tables
@@ -191,7 +153,7 @@ BatchLayout plan_batch(int ct, uint32_t nsys, const uint32_t *natom,
     B.rmt.add(s, d.Kp / d.T);
     B.sf_f64.add(s, d.nx * d.ny * (d.Kp / 256));
     B.lr_f64.add(s, d.ntile * d.nsplit);
-    const SplitPlan &P = B.splits[s];
+    const SplitPlan &P = splits[s];
     const uint32_t task0 = B.t_split[d.split0];  // first task of the system
     for (size_t g = 0; g < P.sf_groups.size(); ++g) {
       const dev::SfGroup &sg = P.sf_groups[g];
@@ -209,247 +171,240 @@ BatchLayout plan_batch(int ct, uint32_t nsys, const uint32_t *natom,
   return B;
 }
 
+}  // namespace glst
+
+glst_batch::~glst_batch() {
+  if (side) (void)hipStreamDestroy(side);
+  if (ev_assigned) (void)hipEventDestroy(ev_assigned);
+  if (ev_near) (void)hipEventDestroy(ev_near);
+  if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+void glst_batch::open(void *stream_) {
+  GLST_HIP(hipGetDevice(&device));
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                            device) == hipSuccess &&
+      lds > 0)
+    lds_limit = static_cast<size_t>(lds);
+  if (stream_) {
+    stream = static_cast<hipStream_t>(stream_);
+  } else {
+    GLST_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    own_stream = true;
+  }
+  GLST_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+  GLST_HIP(hipEventCreateWithFlags(&ev_assigned, hipEventDisableTiming));
+  GLST_HIP(hipEventCreateWithFlags(&ev_near, hipEventDisableTiming));
+}
+
+namespace {
+
+template <typename T>
+void upload(DeviceBuffer &b, const std::vector<T> &v, hipStream_t s) {
+  b.reserve(v.size() * sizeof(T));
+  if (!v.empty())
+    GLST_HIP(hipMemcpyAsync(b.as<void>(), v.data(), v.size() * sizeof(T),
+                            hipMemcpyHostToDevice, s));
+}
+
 }  // namespace
 
-struct glst_batch {
-  std::string last_error;
-  BatchLayout B;
-  int device = 0;
-  size_t lds_limit = 65536;
-  hipStream_t stream = nullptr, side = nullptr;
-  bool own_stream = false;
-  hipEvent_t ev_assigned = nullptr, ev_near = nullptr;
-  int overlap_near = 1;
-  bool assigned = false;
-  // arenas
-  DeviceBuffer d_sys, d_seg;
-  DeviceBuffer atom_cell, rank, hist, cell_count, cell_start, maxc;
-  DeviceBuffer sorted_idx, cell_sorted, atoms64, atoms32, near4;
-  DeviceBuffer fx, fy, fz, en;
-  DeviceBuffer k64, group, r_in, r_out, t_node0, t_nr, t_koff, t_dir64,
-      t_dir32, kap64, kap32, t_split;
-  DeviceBuffer S, R, partial;
-  // launches (run tables inside d_seg)
-  dev::BatchLaunch L_bin{}, L_hist{}, L_scatter{}, L_sr{}, L_reduce{},
-      L_rmt{};
-  std::vector<dev::BatchLaunch> L_sf, L_lr;
-
-  bool mixed() const { return B.mixed(); }
-  uint32_t nsys() const { return static_cast<uint32_t>(B.sys.size()); }
-
-  ~glst_batch() {
-    if (side) (void)hipStreamDestroy(side);
-    if (ev_assigned) (void)hipEventDestroy(ev_assigned);
-    if (ev_near) (void)hipEventDestroy(ev_near);
-    if (own_stream && stream) (void)hipStreamDestroy(stream);
+void glst_batch::configure(
+    int ct, std::vector<std::shared_ptr<const glst::BatchSystem>> cs) {
+  // the previous batch's work must be done before its tables change
+  GLST_HIP(hipStreamSynchronize(stream));
+  B = pack_batch(ct, cs, lds_limit);
+  const size_t N = B.natom, C = B.ncell;
+  atom_cell.reserve(N * 4);
+  rank.reserve(N * 4);
+  sorted_idx.reserve(N * 4);
+  cell_sorted.reserve(N * 4);
+  atoms64.reserve(N * sizeof(double4));
+  if (mixed()) atoms32.reserve(N * sizeof(float4));
+  near4.reserve(N * sizeof(double4));
+  for (DeviceBuffer *b : {&fx, &fy, &fz, &en}) b->reserve(N * 8);
+  hist.reserve(std::max<uint64_t>(B.hist, 1) * 4);
+  cell_count.reserve(C * 4);
+  cell_start.reserve((C + 1) * 4);
+  maxc.reserve(nsys() * 4);
+  const size_t fbytes = mixed() ? sizeof(float2) : sizeof(double2);
+  S.reserve(B.field * fbytes);
+  R.reserve(B.field * fbytes);
+  partial.reserve(B.partial * sizeof(double4));
+  upload(d_sys, B.sys, stream);
+  upload(k64, B.k64, stream);
+  upload(group, B.group, stream);
+  upload(r_in, B.r_in, stream);
+  upload(r_out, B.r_out, stream);
+  upload(t_node0, B.t_node0, stream);
+  upload(t_nr, B.t_nr, stream);
+  upload(t_koff, B.t_koff, stream);
+  upload(t_dir64, B.t_dir64, stream);
+  upload(t_dir32, B.t_dir32, stream);
+  upload(kap64, B.kap64, stream);
+  upload(kap32, B.kap32, stream);
+  upload(t_split, B.t_split, stream);
+  // all run tables in one buffer; launches hold offsets until it exists
+  std::vector<dev::Seg> all;
+  std::vector<std::pair<dev::BatchLaunch *, size_t>> where;
+  auto stage = [&](const Plan &p, dev::BatchLaunch &L) {
+    L.nseg = static_cast<uint32_t>(p.seg.size());
+    L.nblk = p.nblk;
+    L.nr = p.nr;
+    where.emplace_back(&L, all.size());
+    all.insert(all.end(), p.seg.begin(), p.seg.end());
+  };
+  L_sf.clear();
+  L_lr.clear();
+  // sized first so the pointers taken by `stage` stay valid
+  L_sf.resize(mixed() ? B.sf_nr.size() : 1);
+  L_lr.resize(mixed() ? B.lr_nr.size() : 1);
+  stage(B.bin, L_bin);
+  stage(B.hist_p, L_hist);
+  stage(B.scatter, L_scatter);
+  stage(B.sr, L_sr);
+  stage(B.reduce, L_reduce);
+  stage(B.rmt, L_rmt);
+  if (mixed()) {
+    size_t i = 0;
+    for (auto &kv : B.sf_nr) stage(kv.second, L_sf[i++]);
+    i = 0;
+    for (auto &kv : B.lr_nr) stage(kv.second, L_lr[i++]);
+  } else {
+    stage(B.sf_f64, L_sf[0]);
+    stage(B.lr_f64, L_lr[0]);
   }
-
-  template <typename T>
-  void up(DeviceBuffer &b, const std::vector<T> &v) {
-    b.resize(v.size() * sizeof(T));
-    if (!v.empty())
-      GLST_HIP(hipMemcpy(b.as<void>(), v.data(), v.size() * sizeof(T),
-                         hipMemcpyHostToDevice));
-  }
-
-  void init(int ct, uint32_t n, const uint32_t *natom, const double *box,
-            const double *rcut, double tol, void *stream_) {
-    GLST_HIP(hipGetDevice(&device));
-    int lds = 0;
-    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
-                              device) == hipSuccess &&
-        lds > 0)
-      lds_limit = static_cast<size_t>(lds);
-    B = plan_batch(ct, n, natom, box, rcut, tol, lds_limit);
-    if (stream_) {
-      stream = static_cast<hipStream_t>(stream_);
-    } else {
-      GLST_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-      own_stream = true;
-    }
-    GLST_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
-    GLST_HIP(hipEventCreateWithFlags(&ev_assigned, hipEventDisableTiming));
-    GLST_HIP(hipEventCreateWithFlags(&ev_near, hipEventDisableTiming));
-
-    const size_t N = B.natom, C = B.ncell;
-    atom_cell.resize(N * 4);
-    rank.resize(N * 4);
-    sorted_idx.resize(N * 4);
-    cell_sorted.resize(N * 4);
-    atoms64.resize(N * sizeof(double4));
-    atoms32.resize(mixed() ? N * sizeof(float4) : 0);
-    near4.resize(N * sizeof(double4));
-    fx.resize(N * 8);
-    fy.resize(N * 8);
-    fz.resize(N * 8);
-    en.resize(N * 8);
-    hist.resize(std::max<uint64_t>(B.hist, 1) * 4);
-    cell_count.resize(C * 4);
-    cell_start.resize((C + 1) * 4);
-    maxc.resize(nsys() * 4);
-    S.resize(B.field * (mixed() ? sizeof(float2) : sizeof(double2)));
-    R.resize(B.field * (mixed() ? sizeof(float2) : sizeof(double2)));
-    partial.resize(B.partial * sizeof(double4));
-    up(d_sys, B.sys);
-    up(k64, B.k64);
-    up(group, B.group);
-    up(r_in, B.r_in);
-    up(r_out, B.r_out);
-    up(t_node0, B.t_node0);
-    up(t_nr, B.t_nr);
-    up(t_koff, B.t_koff);
-    up(t_dir64, B.t_dir64);
-    up(t_dir32, B.t_dir32);
-    up(kap64, B.kap64);
-    up(kap32, B.kap32);
-    up(t_split, B.t_split);
-    // all run tables in one buffer
-    std::vector<dev::Seg> all;
-    std::vector<std::pair<const Plan *, dev::BatchLaunch *>> fix;
-    auto stage = [&](const Plan &p, dev::BatchLaunch &L) {
-      L.nseg = static_cast<uint32_t>(p.seg.size());
-      L.nblk = p.nblk;
-      L.nr = p.nr;
-      L.seg = reinterpret_cast<const dev::Seg *>(all.size());  // offset
-      all.insert(all.end(), p.seg.begin(), p.seg.end());
-    };
-    stage(B.bin, L_bin);
-    stage(B.hist_p, L_hist);
-    stage(B.scatter, L_scatter);
-    stage(B.sr, L_sr);
-    stage(B.reduce, L_reduce);
-    stage(B.rmt, L_rmt);
-    if (mixed()) {
-      for (auto &kv : B.sf_nr) {
-        L_sf.emplace_back();
-        stage(kv.second, L_sf.back());
-      }
-      for (auto &kv : B.lr_nr) {
-        L_lr.emplace_back();
-        stage(kv.second, L_lr.back());
-      }
-    } else {
-      L_sf.emplace_back();
-      stage(B.sf_f64, L_sf.back());
-      L_lr.emplace_back();
-      stage(B.lr_f64, L_lr.back());
-    }
-    up(d_seg, all);
-    const dev::Seg *base = d_seg.as<dev::Seg>();
-    auto rebase = [&](dev::BatchLaunch &L) {
-      L.seg = base + reinterpret_cast<uintptr_t>(L.seg);
-    };
-    for (dev::BatchLaunch *L :
-         {&L_bin, &L_hist, &L_scatter, &L_sr, &L_reduce, &L_rmt})
-      rebase(*L);
-    for (auto &L : L_sf) rebase(L);
-    for (auto &L : L_lr) rebase(L);
-    // the S(k) kernels never write padded node slots
-    dev::fill_f64(S.as<double>(), 0.0, S.bytes() / sizeof(double), stream);
-    zero_ef();
-    check_launch("batch init");
-    GLST_HIP(hipStreamSynchronize(stream));
-  }
-
-  dev::LrTaskArgs task_args() const {
-    return {t_node0.as<uint32_t>(), t_nr.as<uint32_t>(),
-            t_koff.as<uint32_t>(),  t_dir64.as<double4>(),
-            t_dir32.as<float4>(),   kap64.as<double>(),
-            kap32.as<float>(),      t_split.as<uint32_t>()};
-  }
-
-  void zero_ef() {
-    for (DeviceBuffer *b : {&fx, &fy, &fz, &en})
-      dev::fill_f64(b->as<double>(), 0.0, B.natom, stream);
-    check_launch("zero_ef");
-  }
-
-  void near_field(hipStream_t s) {
-    dev::sr_batch(mixed(), atoms64.as<double4>(), cell_start.as<uint32_t>(),
-                  d_sys.as<dev::SysDev>(), L_sr, nullptr, nullptr, nullptr,
-                  nullptr, near4.as<double4>(), s);
-    check_launch("near field");
-  }
-
-  void step_async(const double *rx, const double *ry, const double *rz,
-                  const double *qc) {
-    if (!rx || !ry || !rz || !qc)
-      throw ArgError("calc_ener_force: null coordinate or charge pointer");
-    const dev::SysDev *sys = d_sys.as<dev::SysDev>();
-    // assign_atoms of every system
-    dev::fill_u32(hist.as<uint32_t>(), 0u, std::max<uint64_t>(B.hist, 1),
+  upload(d_seg, all, stream);
+  for (auto &w : where) w.first->seg = d_seg.as<dev::Seg>() + w.second;
+  // the S(k) kernels never write padded node slots
+  if (B.field)
+    dev::fill_f64(S.as<double>(), 0.0, B.field * fbytes / sizeof(double),
                   stream);
-    dev::bin_chunks_batch(mixed(), rx, ry, rz, sys, L_bin,
-                          atom_cell.as<uint32_t>(), rank.as<uint32_t>(),
-                          hist.as<uint32_t>(), stream);
-    dev::hist_offsets_batch(hist.as<uint32_t>(), sys, L_hist,
-                            cell_count.as<uint32_t>(), stream);
-    dev::cell_scan_batch(cell_count.as<uint32_t>(), sys, nsys(),
-                         cell_start.as<uint32_t>(), maxc.as<uint32_t>(),
-                         stream);
-    dev::bin_scatter_batch(mixed(), rx, ry, rz, qc, sys, L_scatter,
-                           atom_cell.as<uint32_t>(), rank.as<uint32_t>(),
-                           hist.as<uint32_t>(), cell_start.as<uint32_t>(),
-                           sorted_idx.as<uint32_t>(),
-                           cell_sorted.as<uint32_t>(), atoms64.as<double4>(),
-                           atoms32.as<float4>(), stream);
-    check_launch("assign_atoms");
-    assigned = true;
-    if (overlap_near) {
-      GLST_HIP(hipEventRecord(ev_assigned, stream));
-      GLST_HIP(hipStreamWaitEvent(side, ev_assigned, 0));
-      near_field(side);
-      GLST_HIP(hipEventRecord(ev_near, side));
-    }
-    // far field
-    const void *atoms =
-        mixed() ? atoms32.as<void>() : static_cast<const void *>(atoms64.as<void>());
-    const dev::LrTaskArgs ta = task_args();
-    dev::sf_batch(mixed(), atoms, cell_start.as<uint32_t>(), k64.as<double4>(),
-                  ta, sys, L_sf, S.as<void>(), stream);
-    check_launch("calc_sf");
-    dev::rmt_batch(mixed(), S.as<void>(), k64.as<double4>(),
-                   group.as<uint32_t>(), r_in.as<uint32_t>(),
-                   r_out.as<uint32_t>(), sys, L_rmt, B.lds, R.as<void>(),
-                   stream);
-    check_launch("sum_rmt_sf");
-    dev::lr_batch(mixed(), atoms, cell_sorted.as<uint32_t>(), ta, sys, L_lr,
-                  R.as<void>(), partial.as<double4>(), stream);
-    check_launch("calc_lr_ef");
-    if (overlap_near)
-      GLST_HIP(hipStreamWaitEvent(stream, ev_near, 0));
-    else
-      near_field(stream);
-    dev::lr_reduce_batch(partial.as<double4>(), sys, L_reduce,
-                         near4.as<double4>(), fx.as<double>(), fy.as<double>(),
-                         fz.as<double>(), en.as<double>(), stream);
-    check_launch("reduce");
-  }
+  zero_ef();
+  assigned = false;
+  check_launch("batch configure");
+  // host vectors uploaded above must outlive the copies
+  GLST_HIP(hipStreamSynchronize(stream));
+}
 
-  void get_ef(double *ofx, double *ofy, double *ofz, double *oen) {
-    if (!ofx || !ofy || !ofz || !oen)
-      throw ArgError("get_ef: null output pointer");
-    if (!assigned) throw StateError("get_ef called before calc_ener_force");
-    dev::unpermute(sorted_idx.as<uint32_t>(), static_cast<uint32_t>(B.natom),
-                   fx.as<double>(), fy.as<double>(), fz.as<double>(),
-                   en.as<double>(), ofx, ofy, ofz, oen, stream);
-    check_launch("get_ef");
-    GLST_HIP(hipStreamSynchronize(stream));
-  }
+dev::LrTaskArgs glst_batch::task_args() const {
+  return {t_node0.as<uint32_t>(), t_nr.as<uint32_t>(),
+          t_koff.as<uint32_t>(),  t_dir64.as<double4>(),
+          t_dir32.as<float4>(),   kap64.as<double>(),
+          kap32.as<float>(),      t_split.as<uint32_t>()};
+}
 
-  size_t device_bytes() const {
-    const DeviceBuffer *bufs[] = {
-        &d_sys,   &d_seg,   &atom_cell, &rank,   &hist,    &cell_count,
-        &cell_start, &maxc, &sorted_idx, &cell_sorted, &atoms64, &atoms32,
-        &near4,   &fx,      &fy,        &fz,     &en,      &k64,
-        &group,   &r_in,    &r_out,     &t_node0, &t_nr,   &t_koff,
-        &t_dir64, &t_dir32, &kap64,     &kap32,  &t_split, &S,
-        &R,       &partial};
-    size_t b = 0;
-    for (auto p : bufs) b += p->bytes();
-    return b;
+void glst_batch::zero_ef() {
+  for (DeviceBuffer *b : {&fx, &fy, &fz, &en})
+    dev::fill_f64(b->as<double>(), 0.0, B.natom, stream);
+  check_launch("zero_ef");
+}
+
+void glst_batch::near_field(hipStream_t s) {
+  dev::sr_batch(mixed(), atoms64.as<double4>(), cell_start.as<uint32_t>(),
+                d_sys.as<dev::SysDev>(), L_sr, nullptr, nullptr, nullptr,
+                nullptr, near4.as<double4>(), s);
+  check_launch("near field");
+}
+
+void glst_batch::step_async(const double *rx, const double *ry,
+                            const double *rz, const double *qc) {
+  if (!rx || !ry || !rz || !qc)
+    throw ArgError("calc_ener_force: null coordinate or charge pointer");
+  const dev::SysDev *sys = d_sys.as<dev::SysDev>();
+  // assign_atoms of every system
+  dev::fill_u32(hist.as<uint32_t>(), 0u, std::max<uint64_t>(B.hist, 1),
+                stream);
+  dev::bin_chunks_batch(mixed(), rx, ry, rz, sys, L_bin,
+                        atom_cell.as<uint32_t>(), rank.as<uint32_t>(),
+                        hist.as<uint32_t>(), stream);
+  dev::hist_offsets_batch(hist.as<uint32_t>(), sys, L_hist,
+                          cell_count.as<uint32_t>(), stream);
+  dev::cell_scan_batch(cell_count.as<uint32_t>(), sys, nsys(),
+                       cell_start.as<uint32_t>(), maxc.as<uint32_t>(), stream);
+  dev::bin_scatter_batch(mixed(), rx, ry, rz, qc, sys, L_scatter,
+                         atom_cell.as<uint32_t>(), rank.as<uint32_t>(),
+                         hist.as<uint32_t>(), cell_start.as<uint32_t>(),
+                         sorted_idx.as<uint32_t>(), cell_sorted.as<uint32_t>(),
+                         atoms64.as<double4>(), atoms32.as<float4>(), stream);
+  check_launch("assign_atoms");
+  assigned = true;
+  if (overlap_near) {
+    GLST_HIP(hipEventRecord(ev_assigned, stream));
+    GLST_HIP(hipStreamWaitEvent(side, ev_assigned, 0));
+    near_field(side);
+    GLST_HIP(hipEventRecord(ev_near, side));
   }
-};
+  // far field
+  const void *atoms = mixed() ? atoms32.as<void>()
+                              : static_cast<const void *>(atoms64.as<void>());
+  const dev::LrTaskArgs ta = task_args();
+  dev::sf_batch(mixed(), atoms, cell_start.as<uint32_t>(), k64.as<double4>(),
+                ta, sys, L_sf, S.as<void>(), stream);
+  check_launch("calc_sf");
+  dev::rmt_batch(mixed(), S.as<void>(), k64.as<double4>(), group.as<uint32_t>(),
+                 r_in.as<uint32_t>(), r_out.as<uint32_t>(), sys, L_rmt, B.lds,
+                 R.as<void>(), stream);
+  check_launch("sum_rmt_sf");
+  dev::lr_batch(mixed(), atoms, cell_sorted.as<uint32_t>(), ta, sys, L_lr,
+                R.as<void>(), partial.as<double4>(), stream);
+  check_launch("calc_lr_ef");
+  if (overlap_near)
+    GLST_HIP(hipStreamWaitEvent(stream, ev_near, 0));
+  else
+    near_field(stream);
+  dev::lr_reduce_batch(partial.as<double4>(), sys, L_reduce,
+                       near4.as<double4>(), fx.as<double>(), fy.as<double>(),
+                       fz.as<double>(), en.as<double>(), stream);
+  check_launch("reduce");
+}
+
+void glst_batch::get_ef_async(double *ofx, double *ofy, double *ofz,
+                              double *oen) {
+  if (!ofx || !ofy || !ofz || !oen)
+    throw ArgError("get_ef: null output pointer");
+  if (!assigned) throw StateError("get_ef called before calc_ener_force");
+  dev::unpermute(sorted_idx.as<uint32_t>(), static_cast<uint32_t>(B.natom),
+                 fx.as<double>(), fy.as<double>(), fz.as<double>(),
+                 en.as<double>(), ofx, ofy, ofz, oen, stream);
+  check_launch("get_ef");
+}
+
+size_t glst_batch::device_bytes() const {
+  const DeviceBuffer *bufs[] = {
+      &d_sys,   &d_seg,   &atom_cell, &rank,    &hist,    &cell_count,
+      &cell_start, &maxc, &sorted_idx, &cell_sorted, &atoms64, &atoms32,
+      &near4,   &fx,      &fy,        &fz,      &en,      &k64,
+      &group,   &r_in,    &r_out,     &t_node0, &t_nr,    &t_koff,
+      &t_dir64, &t_dir32, &kap64,     &kap32,   &t_split, &S,
+      &R,       &partial};
+  size_t b = 0;
+  for (auto p : bufs) b += p->bytes();
+  return b;
+}
+
+namespace {
+
+std::vector<std::shared_ptr<const BatchSystem>> make_systems(
+    int ct, uint32_t nsys, const uint32_t *natom, const double *box,
+    const double *rcut, double tol) {
+  if (nsys == 0) throw ArgError("a batch needs at least one system");
+  if (!natom || !box || !rcut) throw ArgError("null system description");
+  for (uint32_t s = 0; s < nsys; ++s) {
+    check_system_args(ct, box[3 * s], box[3 * s + 1], box[3 * s + 2], rcut[s],
+                      tol);
+    if (natom[s] == 0)
+      throw ArgError("system " + std::to_string(s) + " has no atoms");
+  }
+  std::vector<std::shared_ptr<const BatchSystem>> cs;
+  cs.reserve(nsys);
+  for (uint32_t s = 0; s < nsys; ++s)
+    cs.push_back(make_batch_system(ct, natom[s], box + 3 * s, rcut[s], tol));
+  return cs;
+}
+
+}  // namespace
 
 #define NEED_BATCH(b)                                                         \
   if (!(b)) {                                                                 \
@@ -470,12 +425,9 @@ glst_status_t glst_batch_create(int ct, uint32_t nsys, const uint32_t *natom,
   glst_batch *b = new (std::nothrow) glst_batch();
   if (!b) return GLST_ERR_ALLOC;
   const glst_status_t st = run_guarded(nullptr, [&] {
-    if (nsys == 0) throw ArgError("a batch needs at least one system");
-    if (!natom || !box || !rcut) throw ArgError("null system description");
-    for (uint32_t s = 0; s < nsys; ++s)
-      check_system_args(ct, box[3 * s], box[3 * s + 1], box[3 * s + 2],
-                        rcut[s], tol);
-    b->init(ct, nsys, natom, box, rcut, tol, stream);
+    auto cs = make_systems(ct, nsys, natom, box, rcut, tol);
+    b->open(stream);
+    b->configure(ct, std::move(cs));
   });
   if (st != GLST_OK) {
     delete b;
@@ -500,13 +452,9 @@ glst_status_t glst_batch_device_bytes(int ct, uint32_t nsys,
     return GLST_ERR_INVALID;
   }
   return run_guarded(nullptr, [&] {
-    if (nsys == 0) throw ArgError("a batch needs at least one system");
-    if (!natom || !box || !rcut) throw ArgError("null system description");
-    for (uint32_t s = 0; s < nsys; ++s)
-      check_system_args(ct, box[3 * s], box[3 * s + 1], box[3 * s + 2],
-                        rcut[s], tol);
     // LDS capacity of gfx950 (160 KiB per workgroup)
-    *bytes = plan_batch(ct, nsys, natom, box, rcut, tol, 160 * 1024)
+    *bytes = pack_batch(ct, make_systems(ct, nsys, natom, box, rcut, tol),
+                        160 * 1024)
                  .device_bytes();
   });
 }
@@ -534,7 +482,7 @@ glst_status_t glst_batch_get_info(glst_batch_t b, uint32_t sys,
   return run_guarded(&b->last_error, [&] {
     if (!info) throw ArgError("null output");
     if (sys >= b->nsys()) throw ArgError("system index out of range");
-    const SystemSetup &u = b->B.su[sys];
+    const SystemSetup &u = b->B.cs[sys]->su;
     std::memset(info, 0, sizeof(*info));
     info->natom = b->B.sys[sys].natom;
     info->ncell[0] = u.grid.nx;
@@ -582,8 +530,10 @@ glst_status_t glst_batch_calc_ener_force(glst_batch_t b, const double *d_rx,
 glst_status_t glst_batch_get_ef(glst_batch_t b, double *d_fx, double *d_fy,
                                 double *d_fz, double *d_en) {
   NEED_BATCH(b);
-  return run_guarded(&b->last_error,
-                     [&] { b->get_ef(d_fx, d_fy, d_fz, d_en); });
+  return run_guarded(&b->last_error, [&] {
+    b->get_ef_async(d_fx, d_fy, d_fz, d_en);
+    GLST_HIP(hipStreamSynchronize(b->stream));
+  });
 }
 
 glst_status_t glst_batch_zero_ef(glst_batch_t b) {

@@ -1,0 +1,120 @@
+// The batched multi-system engine (include/glst_batch.h), shared by its C
+// ABI (batch.hip) and the serving loop (serve.hip).
+//
+// Every system is set up exactly as a single glst_force (runtime.hpp: grid,
+// alpha groups, cubature, device node order) and cached as a BatchSystem; a
+// batch packs any list of such systems into shared arenas (atoms, cells,
+// nodes, tasks, splits, S/R fields, histogram, partial rows) and runs each
+// stage of the force step as one launch over all of them (S(k) and the
+// back-projection: one launch per radial node count).  A workgroup finds its
+// system through a run table of (first workgroup, system, group data)
+// records.  The arenas only grow, so re-packing a batch for a new list of
+// systems allocates nothing once they are large enough.
+#pragma once
+
+#include "runtime.hpp"
+
+#include <map>
+#include <memory>
+
+namespace glst {
+
+// One system's host setup, reusable by any number of batches.
+struct BatchSystem {
+  uint32_t natom = 0;
+  SystemSetup su;
+  NodeLayout nl;  // device node/task order (empty without a far field)
+};
+
+std::shared_ptr<const BatchSystem> make_batch_system(int ct, uint32_t natom,
+                                                     const double box[3],
+                                                     double rcut, double tol);
+
+// A launch of a batched stage under construction: run table and workgroups.
+struct Plan {
+  std::vector<dev::Seg> seg;
+  uint32_t nblk = 0;
+  uint32_t nr = 0;
+  void add(uint32_t sys, uint32_t blocks, uint32_t a = 0, uint32_t b = 0) {
+    if (blocks == 0) return;
+    seg.push_back({nblk, sys, a, b});
+    nblk += blocks;
+  }
+};
+
+// Host layout of a whole batch: SysDev records, arena sizes, packed
+// node/task tables and the run tables of every launch.
+struct BatchLayout {
+  int ct = GLST_CT_DOUBLE;
+  std::vector<std::shared_ptr<const BatchSystem>> cs;
+  std::vector<dev::SysDev> sys;
+  uint64_t natom = 0, ncell = 0, hist = 0, nodes = 0, field = 0, partial = 0;
+  uint32_t ngroup = 0;
+  size_t lds = 0;  // dynamic LDS of the shell-sum launch
+  std::vector<double4> k64;
+  std::vector<uint32_t> group, r_in, r_out;
+  std::vector<uint32_t> t_node0, t_nr, t_koff, t_split;
+  std::vector<double4> t_dir64;
+  std::vector<float4> t_dir32;
+  std::vector<double> kap64;
+  std::vector<float> kap32;
+  Plan bin, hist_p, scatter, sr, reduce, rmt, sf_f64, lr_f64;
+  std::map<uint32_t, Plan> sf_nr, lr_nr;  // radial count (0: any) -> plan
+
+  bool mixed() const { return ct == GLST_CT_FLOAT; }
+  size_t device_bytes() const;  // arena bytes this layout needs
+};
+
+// Packs systems (all of precision ct) into one batch layout.  lds_limit:
+// the device's LDS per workgroup; a system whose one-node cell table does not
+// fit is rejected.
+BatchLayout pack_batch(int ct,
+                       const std::vector<std::shared_ptr<const BatchSystem>> &cs,
+                       size_t lds_limit);
+
+}  // namespace glst
+
+struct glst_batch {
+  std::string last_error;
+  glst::BatchLayout B;
+  int device = 0;
+  size_t lds_limit = 65536;
+  hipStream_t stream = nullptr, side = nullptr;
+  bool own_stream = false;
+  hipEvent_t ev_assigned = nullptr, ev_near = nullptr;
+  int overlap_near = 1;
+  bool assigned = false;
+  // arenas (grow-only)
+  glst::DeviceBuffer d_sys, d_seg;
+  glst::DeviceBuffer atom_cell, rank, hist, cell_count, cell_start, maxc;
+  glst::DeviceBuffer sorted_idx, cell_sorted, atoms64, atoms32, near4;
+  glst::DeviceBuffer fx, fy, fz, en;
+  glst::DeviceBuffer k64, group, r_in, r_out, t_node0, t_nr, t_koff, t_dir64,
+      t_dir32, kap64, kap32, t_split;
+  glst::DeviceBuffer S, R, partial;
+  // launches (run tables inside d_seg)
+  glst::dev::BatchLaunch L_bin{}, L_hist{}, L_scatter{}, L_sr{}, L_reduce{},
+      L_rmt{};
+  std::vector<glst::dev::BatchLaunch> L_sf, L_lr;
+
+  bool mixed() const { return B.mixed(); }
+  uint32_t nsys() const { return static_cast<uint32_t>(B.sys.size()); }
+
+  ~glst_batch();
+  // device, streams and events (once)
+  void open(void *stream_);
+  // (re)pack for these systems: uploads tables and run tables, grows arenas,
+  // clears S and the accumulators
+  void configure(int ct,
+                 std::vector<std::shared_ptr<const glst::BatchSystem>> cs);
+  void zero_ef();
+  void step_async(const double *rx, const double *ry, const double *rz,
+                  const double *qc);
+  // unpermuted outputs in packed original order (stream-ordered, no sync)
+  void get_ef_async(double *ofx, double *ofy, double *ofz, double *oen);
+  size_t device_bytes() const;
+
+ private:
+  glst::dev::LrTaskArgs task_args() const;
+  void near_field(hipStream_t s);
+};

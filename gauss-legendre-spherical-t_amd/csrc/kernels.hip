@@ -1594,6 +1594,11 @@ __global__ __launch_bounds__(kSrBlock) void sr_batch_kernel(
                         en, out4, wg - g.blk0);
 }
 
+__global__ void copy_runs_kernel(const CopyRun *__restrict__ runs) {
+  const CopyRun r = runs[blockIdx.x];
+  for (uint32_t i = threadIdx.x; i < r.n; i += blockDim.x) r.dst[i] = r.src[i];
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1947,6 +1952,11 @@ void sr_batch(bool mixed, const double4 *atoms, const uint32_t *start,
       sr_batch_kernel<false, false><<<L.nblk, kSrBlock, 0, s>>>(
           atoms, start, sys, L.seg, L.nseg, fx, fy, fz, en, out4);
   }
+}
+
+void copy_runs(const CopyRun *runs, uint32_t nruns, hipStream_t s) {
+  if (nruns == 0) return;
+  copy_runs_kernel<<<nruns, 256, 0, s>>>(runs);
 }
 
 }  // namespace dev

@@ -203,6 +203,15 @@ void sr_batch(bool mixed, const double4 *atoms, const uint32_t *start,
               const SysDev *sys, const BatchLaunch &L, double *fx, double *fy,
               double *fz, double *en, double4 *out4, hipStream_t s);
 
+// ---- copies of many arrays in one launch (serving loop) -----------------------
+struct CopyRun {
+  const double *src;
+  double *dst;
+  uint32_t n;
+};
+// dst[i] = src[i], i < n, for every run (one workgroup per run)
+void copy_runs(const CopyRun *runs, uint32_t nruns, hipStream_t s);
+
 // ---- direct all-pairs Coulomb (utils/coulomb.cu) -----------------------------
 void coulomb_direct(const double *rx, const double *ry, const double *rz,
                     const double *qc, uint32_t natom, double *fx, double *fy,

@@ -54,9 +54,19 @@ class DeviceBuffer {
   DeviceBuffer(const DeviceBuffer &) = delete;
   DeviceBuffer &operator=(const DeviceBuffer &) = delete;
   ~DeviceBuffer() { release(); }
+  // exactly `bytes` (reallocates on any change)
   void resize(size_t bytes) {
     if (bytes == bytes_) return;
     release();
+    allocate(bytes);
+  }
+  // at least `bytes`; never shrinks (buffers reused across batches)
+  void reserve(size_t bytes) {
+    if (bytes <= bytes_) return;
+    release();
+    allocate(bytes);
+  }
+  void allocate(size_t bytes) {
     if (bytes == 0) return;
     void *p = nullptr;
     hipError_t e = hipMalloc(&p, bytes);

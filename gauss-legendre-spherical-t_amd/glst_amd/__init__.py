@@ -149,6 +149,15 @@ def _load():
         "glst_batch_zero_ef": [P],
         "glst_batch_set_option": [P, c_char_p, ctypes.c_int64],
         "glst_batch_set_stream": [P, P],
+        # serving loop (include/glst_serve.h)
+        "glst_server_create": [c_int, d, c_size_t, u, P, POINTER(P)],
+        "glst_server_destroy": [P],
+        "glst_server_submit": [P, u, P, d, P, P, P, P, P, P, P, P,
+                               POINTER(c_uint64)],
+        "glst_server_step": [P, POINTER(u)],
+        "glst_server_completed": [P, P, u, POINTER(u)],
+        "glst_server_pending": [P, POINTER(u)],
+        "glst_server_get_stats": [P, P],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -162,6 +171,8 @@ def _load():
     lib.glst_version.restype = c_char_p
     lib.glst_batch_last_error.argtypes = [P]
     lib.glst_batch_last_error.restype = c_char_p
+    lib.glst_server_last_error.argtypes = [P]
+    lib.glst_server_last_error.restype = c_char_p
     return lib
 
 
@@ -551,3 +562,82 @@ class GLSTBatch:
 
     def set_option(self, name: str, value: int):
         self._c(_lib.glst_batch_set_option(self._h, name.encode(), int(value)))
+
+
+# ---- the serving loop -------------------------------------------------------
+
+class ServerStats(Structure):
+    _fields_ = [(n, c_uint64) for n in (
+        "submitted", "rejected", "completed", "batches", "reconfigures",
+        "setups", "setup_hits", "atoms")]
+
+
+class GLSTServer:
+    """Continuous-batching server over the batched engine
+    (``include/glst_serve.h``).  ``submit`` queues one system's force request
+    (device arrays, borrowed until it completes; outputs overwritten) and
+    returns a ticket; ``step`` computes the oldest queued requests that fit
+    ``budget_bytes`` (at most ``max_batch``) as one batch."""
+
+    def __init__(self, tol: float, budget_bytes: int, max_batch: int = 0,
+                 ct: str = "double", stream=None):
+        import torch
+        self.ct = CT_FLOAT if ct in ("float", CT_FLOAT) else CT_DOUBLE
+        if stream is None and torch.cuda.is_available():
+            stream = torch.cuda.current_stream().cuda_stream
+        h = c_void_p()
+        _check(_lib.glst_server_create(self.ct, tol, int(budget_bytes),
+                                       int(max_batch), c_void_p(stream),
+                                       byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.glst_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, status):
+        if status != 0:
+            msg = _lib.glst_server_last_error(self._h) if self._h else b""
+            if not msg:
+                msg = _lib.glst_global_last_error()
+            raise GLSTError(status, (msg or b"").decode())
+
+    def submit(self, box, rcut: float, rx, ry, rz, qc, fx, fy, fz, en) -> int:
+        import numpy as np
+        if isinstance(box, (int, float)):
+            box = (float(box),) * 3
+        b = np.ascontiguousarray(box, dtype=np.float64)
+        t = c_uint64()
+        self._c(_lib.glst_server_submit(
+            self._h, rx.numel(), _ptr(b), rcut, _ptr(rx), _ptr(ry), _ptr(rz),
+            _ptr(qc), _ptr(fx), _ptr(fy), _ptr(fz), _ptr(en), byref(t)))
+        return t.value
+
+    def step(self) -> int:
+        n = c_uint32()
+        self._c(_lib.glst_server_step(self._h, byref(n)))
+        return n.value
+
+    def completed(self, cap: int = 4096):
+        import numpy as np
+        out = np.zeros(cap, dtype=np.uint64)
+        n = c_uint32()
+        self._c(_lib.glst_server_completed(self._h, _ptr(out), cap, byref(n)))
+        return [int(v) for v in out[: n.value]]
+
+    def pending(self) -> int:
+        n = c_uint32()
+        self._c(_lib.glst_server_pending(self._h, byref(n)))
+        return n.value
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        self._c(_lib.glst_server_get_stats(self._h, ctypes.addressof(st)))
+        return {n: getattr(st, n) for n, _ in ServerStats._fields_}

@@ -1,0 +1,78 @@
+/*
+ * glst_serve.h — serving loop over the batched engine.
+ *
+ * The reference has no counterpart (it computes one system per call of
+ * glst_force::calc_ener_force, src/glst_force.cu:1604-1614).  A server takes
+ * force requests for independent systems, each with the semantics of one
+ * glst_force<CT> step (own box and rcut; the server's CT and tol), and
+ * answers them by continuous batching: every glst_server_step packs the
+ * oldest queued requests that fit the server's device-memory budget (and
+ * max_batch) into one batch and computes them together.
+ *
+ * Per-shape setups (cell grid, alpha groups, cubature and its device order)
+ * are cached by (natom, box, rcut), so repeat requests for a resident system
+ * (an MD trajectory: same shape, new positions) skip all host setup; when a
+ * step's batch has the same composition as the previous one the batch
+ * tables are reused as well.  Requests whose own footprint exceeds the budget
+ * are rejected at submission.
+ *
+ * Inputs and outputs are caller-owned device FP64 arrays of natom entries,
+ * borrowed until the request completes; outputs are overwritten (fx, fy, fz,
+ * en in the request's atom order).  One host thread drives a server.
+ */
+#ifndef GLST_SERVE_H
+#define GLST_SERVE_H
+
+#include "glst_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct glst_server *glst_server_t;
+
+typedef struct glst_server_stats {
+  uint64_t submitted;     /* requests accepted */
+  uint64_t rejected;      /* requests refused at submission (budget) */
+  uint64_t completed;     /* requests answered */
+  uint64_t batches;       /* glst_server_step calls that computed a batch */
+  uint64_t reconfigures;  /* batches whose composition changed */
+  uint64_t setups;        /* per-shape setups built */
+  uint64_t setup_hits;    /* submissions served from the setup cache */
+  uint64_t atoms;         /* atoms of all completed requests */
+} glst_server_stats_t;
+
+/* budget_bytes: device memory the server's batch may hold; max_batch: most
+ * requests per batch (0 = no limit). */
+glst_status_t glst_server_create(int ct, double tol, size_t budget_bytes,
+                                 uint32_t max_batch, void *stream,
+                                 glst_server_t *out);
+glst_status_t glst_server_destroy(glst_server_t srv);
+
+/* Queue one request; *ticket identifies it in glst_server_completed.
+ * GLST_ERR_ALLOC: the request alone needs more than the budget. */
+glst_status_t glst_server_submit(glst_server_t srv, uint32_t natom,
+                                 const double *box, double rcut,
+                                 const double *d_rx, const double *d_ry,
+                                 const double *d_rz, const double *d_qc,
+                                 double *d_fx, double *d_fy, double *d_fz,
+                                 double *d_en, uint64_t *ticket);
+
+/* Compute one batch of queued requests (FIFO within the budget); returns
+ * after their outputs are written.  *ncompleted = requests answered (0 when
+ * the queue was empty). */
+glst_status_t glst_server_step(glst_server_t srv, uint32_t *ncompleted);
+
+/* Tickets answered since the last call, oldest first (up to cap; *n set). */
+glst_status_t glst_server_completed(glst_server_t srv, uint64_t *tickets,
+                                    uint32_t cap, uint32_t *n);
+glst_status_t glst_server_pending(glst_server_t srv, uint32_t *n);
+glst_status_t glst_server_get_stats(glst_server_t srv,
+                                    glst_server_stats_t *stats);
+const char *glst_server_last_error(glst_server_t srv);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GLST_SERVE_H */

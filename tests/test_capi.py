@@ -19,7 +19,7 @@ def declared(header):
         r"(?:glst_status_t|const char\s*\*)\s*(glst_[a-z0-9_]+)\s*\(", text)))
 
 
-HEADERS = ["glst_hip.h", "glst_batch.h", "glst_io.h"]
+HEADERS = ["glst_hip.h", "glst_batch.h", "glst_serve.h", "glst_io.h"]
 
 
 def test_every_declared_symbol_is_exported():
@@ -137,3 +137,11 @@ def test_batch_create_rejects_bad_arguments_before_the_device():
     with pytest.raises(g.GLSTError) as e:
         g.GLSTBatch([30, 30], [31.0, -2.0], 9.0, 1e-4, stream=0)
     assert e.value.status == 1
+
+
+def test_server_create_rejects_bad_arguments_before_the_device():
+    for kw in (dict(tol=0.0, budget_bytes=1 << 30),
+               dict(tol=1e-4, budget_bytes=0)):
+        with pytest.raises(g.GLSTError) as e:
+            g.GLSTServer(stream=0, **kw)
+        assert e.value.status == 1
