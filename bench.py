@@ -11,8 +11,9 @@ synchronisation.
 Prints ONE JSON line: atom-steps/s of the mixed FP32/FP64 path (value), the
 plain FP64 path beside it, the relative force error of both against the
 direct FP64 all-pairs sum, the roofline of the dominant kernel (HIP events on
-the handle's stream) and a CPU baseline (the oracle's FP64 restatement of the
-reference path timed on a bounded sample on the host cores).
+the handle's stream), a CPU baseline (the oracle's FP64 restatement of the
+reference path timed on a bounded sample on the host cores) and the served
+p50/p99 latency of the continuous-batching server under Poisson arrivals.
 """
 import argparse
 import json
@@ -103,6 +104,96 @@ def cpu_baseline(h, natom):
     }
 
 
+def serve_bench(g, torch, np, n_requests=2000, seed=5):
+    """Served latency of the continuous-batching server (mixed path).
+
+    Requests are systems of the batch size mix: waters log-uniform in
+    [100, 10000] at 0.0334 waters/A^3, rcut 9 A, tol 1e-4.  30 % of the
+    requests are repeat MD steps of 16 resident systems (positions perturbed
+    by N(0, 0.01 A) per request); the rest are new shapes, each one's setup
+    built at submission.  Arrivals are Poisson at 75 % of the capacity
+    measured on a back-to-back burst of the same mix.  Latency = completion
+    time - scheduled arrival time (queueing and setup included)."""
+    rng = np.random.default_rng(seed)
+    pool = []
+    for k in range(48):
+        nw = int(round(np.exp(rng.uniform(np.log(100), np.log(10000)))))
+        L = (nw / 0.0334) ** (1 / 3)
+        h = g.make_water_box(nw, L, seed=1000 + k)
+        d = [torch.as_tensor(a, dtype=torch.float64, device="cuda") for a in h]
+        o = [torch.empty(h[0].size, dtype=torch.float64, device="cuda")
+             for _ in range(4)]
+        pool.append((L, d, o))
+    resident, fresh = pool[:16], pool[16:]
+    jitter = [0]
+
+    def make_request():
+        if rng.uniform() < 0.3:
+            L, d, o = resident[rng.integers(len(resident))]
+            pos = [torch.clamp(x + 0.01 * torch.randn_like(x), min=0.0)
+                   for x in d[:3]]
+            return L, pos + [d[3]], o
+        L, d, o = fresh[rng.integers(len(fresh))]
+        jitter[0] += 1  # a new shape: its setup is built at submission
+        return L * (1.0 + 1e-9 * jitter[0]), d, o
+
+    srv = g.GLSTServer(TOL, 64 << 30, ct="float")
+    for L, d, o in resident:  # resident systems are set up once
+        srv.submit(L, 9.0, *d, *o)
+    while srv.pending():
+        srv.step()
+    srv.completed(1 << 16)
+    torch.cuda.synchronize()
+    # capacity: a burst of the mix, back to back
+    burst = [make_request() for _ in range(256)]
+    t0 = time.perf_counter()
+    for L, d, o in burst:
+        srv.submit(L, 9.0, *d, *o)
+    while srv.pending():
+        srv.step()
+    cap = len(burst) / (time.perf_counter() - t0)
+    srv.completed(1 << 16)
+    # Poisson arrivals at 75 % of capacity
+    rate = 0.75 * cap
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    reqs = [make_request() for _ in range(n_requests)]
+    st0 = srv.stats()
+    lat, when = [], {}
+    i = 0
+    t0 = time.perf_counter()
+    while len(lat) < n_requests:
+        now = time.perf_counter() - t0
+        while i < n_requests and arrivals[i] <= now:
+            L, d, o = reqs[i]
+            when[srv.submit(L, 9.0, *d, *o)] = arrivals[i]
+            i += 1
+        if srv.pending():
+            srv.step()
+            t = time.perf_counter() - t0
+            for tk in srv.completed(1 << 16):
+                lat.append(t - when.pop(tk))
+        elif i < n_requests:
+            time.sleep(max(0.0, arrivals[i] - (time.perf_counter() - t0)))
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat = np.array(lat) * 1e3
+    return {
+        "workload": "Poisson arrivals of water systems, 100-10000 waters "
+                    "(log-uniform), rcut 9, tol 1e-4, mixed; 30 % repeat "
+                    "steps of 16 resident systems",
+        "requests": n_requests,
+        "rate_req_s": rate,
+        "capacity_req_s": cap,
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "mean_batch": (st["completed"] - st0["completed"]) /
+                      max(1, st["batches"] - st0["batches"]),
+        "atom_steps_per_s": (st["atoms"] - st0["atoms"]) / wall,
+        "new_setups": st["setups"] - st0["setups"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +201,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the served-latency measurement")
     ap.add_argument("--profile", action="store_true",
                     help="mixed path only (for rocprofv3 runs), plus one "
                          "1 GiB elementwise copy used to calibrate counters")
@@ -185,6 +278,11 @@ def main():
         "pair_rate_per_s": pairs / (lr_ms * 1e-3),
     }
 
+    served = None
+    if not args.no_serve:
+        served = serve_bench(g, torch, np)
+        log(f"served: {served}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(h, natom)
@@ -221,6 +319,7 @@ def main():
         "stages_ms_fp64": st_fp64,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "served": served,
     }
     print(json.dumps(line), flush=True)
 

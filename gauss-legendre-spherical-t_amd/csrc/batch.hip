@@ -16,15 +16,21 @@ uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 namespace glst {
 
-std::shared_ptr<const BatchSystem> make_batch_system(int ct, uint32_t natom,
-                                                     const double box[3],
-                                                     double rcut, double tol) {
+std::shared_ptr<const ShapeSetup> make_shape_setup(int ct, const double box[3],
+                                                   double rcut, double tol) {
+  auto p = std::make_shared<ShapeSetup>();
+  p->su = setup_system(ct, box[0], box[1], box[2], rcut, tol);
+  if (p->su.has_lr && p->su.K > 0)
+    p->nl = layout_nodes(p->su.nodes, p->su.K, p->su.Kp);
+  return p;
+}
+
+std::shared_ptr<const BatchSystem> make_batch_system(
+    uint32_t natom, std::shared_ptr<const ShapeSetup> shape) {
   if (natom == 0) throw ArgError("a system without atoms");
   auto b = std::make_shared<BatchSystem>();
   b->natom = natom;
-  b->su = setup_system(ct, box[0], box[1], box[2], rcut, tol);
-  if (b->su.has_lr && b->su.K > 0)
-    b->nl = layout_nodes(b->su.nodes, b->su.K, b->su.Kp);
+  b->shape = std::move(shape);
   return b;
 }
 
@@ -63,14 +69,14 @@ BatchLayout pack_batch(int ct,
   // back-projection splits: enough workgroups over the whole batch
   uint64_t tiles = 0;
   for (const auto &c : cs)
-    if (c->su.has_lr && c->su.K > 0) tiles += div_up(c->natom, 256);
+    if (c->su().has_lr && c->su().K > 0) tiles += div_up(c->natom, 256);
   const uint32_t target =
       tiles ? static_cast<uint32_t>(std::max<uint64_t>(1, (4096 + tiles - 1) / tiles))
             : 1;
   std::vector<SplitPlan> splits(nsys);
   uint32_t task_base = 0, split_base = 0;
   for (uint32_t s = 0; s < nsys; ++s) {
-    const SystemSetup &u = cs[s]->su;
+    const SystemSetup &u = cs[s]->su();
     const uint32_t na = cs[s]->natom;
     const bool lr = u.has_lr && u.K > 0;
     dev::SysDev d{};
@@ -109,7 +115,7 @@ BatchLayout pack_batch(int ct,
                        "use a single-system handle");
       d.T = 2 * per_node <= lds_limit ? 2 : 1;
       B.lds = std::max(B.lds, d.T * per_node);
-      const NodeLayout &L = cs[s]->nl;
+      const NodeLayout &L = cs[s]->nl();
       SplitPlan &P = splits[s];
       P = make_splits(u.nodes, u.K, target);
       const uint32_t koff = static_cast<uint32_t>(B.kap64.size());
@@ -400,7 +406,8 @@ std::vector<std::shared_ptr<const BatchSystem>> make_systems(
   std::vector<std::shared_ptr<const BatchSystem>> cs;
   cs.reserve(nsys);
   for (uint32_t s = 0; s < nsys; ++s)
-    cs.push_back(make_batch_system(ct, natom[s], box + 3 * s, rcut[s], tol));
+    cs.push_back(make_batch_system(
+        natom[s], make_shape_setup(ct, box + 3 * s, rcut[s], tol)));
   return cs;
 }
 
@@ -482,7 +489,7 @@ glst_status_t glst_batch_get_info(glst_batch_t b, uint32_t sys,
   return run_guarded(&b->last_error, [&] {
     if (!info) throw ArgError("null output");
     if (sys >= b->nsys()) throw ArgError("system index out of range");
-    const SystemSetup &u = b->B.cs[sys]->su;
+    const SystemSetup &u = b->B.cs[sys]->su();
     std::memset(info, 0, sizeof(*info));
     info->natom = b->B.sys[sys].natom;
     info->ncell[0] = u.grid.nx;

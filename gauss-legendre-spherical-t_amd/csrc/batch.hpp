@@ -19,16 +19,27 @@
 
 namespace glst {
 
-// One system's host setup, reusable by any number of batches.
-struct BatchSystem {
-  uint32_t natom = 0;
+// The host setup of a cell grid: it depends only on the grid dimensions,
+// the cell edge, CT and tol (not on the atoms or the exact box), so systems
+// of one grid share it.
+struct ShapeSetup {
   SystemSetup su;
   NodeLayout nl;  // device node/task order (empty without a far field)
 };
 
-std::shared_ptr<const BatchSystem> make_batch_system(int ct, uint32_t natom,
-                                                     const double box[3],
-                                                     double rcut, double tol);
+std::shared_ptr<const ShapeSetup> make_shape_setup(int ct, const double box[3],
+                                                   double rcut, double tol);
+
+// One system of a batch: its atom count and grid setup.
+struct BatchSystem {
+  uint32_t natom = 0;
+  std::shared_ptr<const ShapeSetup> shape;
+  const SystemSetup &su() const { return shape->su; }
+  const NodeLayout &nl() const { return shape->nl; }
+};
+
+std::shared_ptr<const BatchSystem> make_batch_system(
+    uint32_t natom, std::shared_ptr<const ShapeSetup> shape);
 
 // A launch of a batched stage under construction: run table and workgroups.
 struct Plan {

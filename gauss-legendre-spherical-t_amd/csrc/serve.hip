@@ -15,17 +15,47 @@ using namespace glst;
 
 namespace {
 
-// Setups cached by shape: (natom, box edges, rcut) under the server's CT and
-// tol.  Bounded; least recently used entries go first.
-using ShapeKey = std::tuple<uint32_t, double, double, double, double>;
+// Grid setups cached by (grid dimensions, cell edge) under the server's CT
+// and tol: systems of one grid share them whatever their atoms or exact box.
+// Batch systems (a setup plus an atom count) are cached by (grid, cell edge,
+// natom), so a resident system's repeat requests reuse the same object and a
+// batch of the same composition is not re-packed.  Both caches are bounded;
+// least recently used entries go first (queued requests and the configured
+// batch hold their own references).
+using GridKey = std::tuple<uint32_t, uint32_t, uint32_t, double>;
+using SysKey = std::tuple<uint32_t, uint32_t, uint32_t, double, uint32_t>;
 
-struct CacheEntry {
-  std::shared_ptr<const BatchSystem> sys;
-  size_t bytes = 0;  // device footprint of a batch of this system alone
-  std::list<ShapeKey>::iterator lru;
+template <class Key, class Value>
+struct Lru {
+  std::map<Key, std::pair<Value, typename std::list<Key>::iterator>> map;
+  std::list<Key> order;  // front = most recent
+  size_t cap;
+  explicit Lru(size_t c) : cap(c) {}
+  Value *find(const Key &k) {
+    auto it = map.find(k);
+    if (it == map.end()) return nullptr;
+    order.splice(order.begin(), order, it->second.second);
+    return &it->second.first;
+  }
+  Value &insert(const Key &k, Value v) {
+    order.push_front(k);
+    auto &slot = map.emplace(k, std::make_pair(std::move(v), order.begin()))
+                     .first->second.first;
+    while (map.size() > cap) {
+      map.erase(order.back());
+      order.pop_back();
+    }
+    return slot;
+  }
 };
 
-constexpr size_t kMaxCachedShapes = 4096;
+struct SysEntry {
+  std::shared_ptr<const BatchSystem> sys;
+  size_t bytes = 0;  // device footprint of a batch of this system alone
+};
+
+constexpr size_t kMaxCachedGrids = 1024;
+constexpr size_t kMaxCachedSystems = 8192;
 
 struct Request {
   uint64_t ticket;
@@ -46,8 +76,8 @@ struct glst_server {
   glst_batch batch;
   bool batch_open = false;
   std::vector<const BatchSystem *> composition;  // of the configured batch
-  std::map<ShapeKey, CacheEntry> cache;
-  std::list<ShapeKey> lru;  // front = most recent
+  Lru<GridKey, std::shared_ptr<const ShapeSetup>> grids{kMaxCachedGrids};
+  Lru<SysKey, SysEntry> systems{kMaxCachedSystems};
   std::deque<Request> queue;
   std::vector<uint64_t> done;
   uint64_t next_ticket = 1;
@@ -56,28 +86,31 @@ struct glst_server {
   DeviceBuffer in[4], out[4], runs;
   std::vector<dev::CopyRun> host_runs;
 
-  CacheEntry &lookup(uint32_t natom, const double *box, double rcut) {
-    const ShapeKey key{natom, box[0], box[1], box[2], rcut};
-    auto it = cache.find(key);
-    if (it != cache.end()) {
-      lru.splice(lru.begin(), lru, it->second.lru);
+  SysEntry &lookup(uint32_t natom, const double *box, double rcut) {
+    uint32_t n[3];
+    if (ct == GLST_CT_FLOAT)
+      grid_dims<float>(box[0], box[1], box[2], rcut, n[0], n[1], n[2]);
+    else
+      grid_dims<double>(box[0], box[1], box[2], rcut, n[0], n[1], n[2]);
+    const SysKey skey{n[0], n[1], n[2], rcut, natom};
+    if (SysEntry *e = systems.find(skey)) {
       ++stats.setup_hits;
-      return it->second;
+      return *e;
     }
-    CacheEntry e;
-    e.sys = make_batch_system(ct, natom, box, rcut, tol);
+    const GridKey gkey{n[0], n[1], n[2], rcut};
+    std::shared_ptr<const ShapeSetup> shape;
+    if (auto *g = grids.find(gkey)) {
+      shape = *g;
+      ++stats.setup_hits;
+    } else {
+      shape = grids.insert(gkey, make_shape_setup(ct, box, rcut, tol));
+      ++stats.setups;
+    }
+    SysEntry e;
+    e.sys = make_batch_system(natom, shape);
     e.bytes = pack_batch(ct, {e.sys}, batch.lds_limit).device_bytes() +
               8 * sizeof(double) * static_cast<size_t>(natom);
-    ++stats.setups;
-    lru.push_front(key);
-    e.lru = lru.begin();
-    auto ins = cache.emplace(key, std::move(e)).first;
-    while (cache.size() > kMaxCachedShapes) {
-      // queued requests hold their own reference to their setup
-      cache.erase(lru.back());
-      lru.pop_back();
-    }
-    return ins->second;
+    return systems.insert(skey, std::move(e));
   }
 
   void submit(uint32_t natom, const double *box, double rcut,
@@ -88,7 +121,7 @@ struct glst_server {
     for (int k = 0; k < 4; ++k)
       if (!inp[k] || !outp[k]) throw ArgError("null input or output array");
     check_system_args(ct, box[0], box[1], box[2], rcut, tol);
-    CacheEntry &e = lookup(natom, box, rcut);
+    SysEntry &e = lookup(natom, box, rcut);
     if (e.bytes > budget) {
       ++stats.rejected;
       throw AllocError("request needs " + std::to_string(e.bytes) +
