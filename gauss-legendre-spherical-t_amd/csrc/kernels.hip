@@ -1,22 +1,26 @@
 // HIP kernels of the GLST force step, written for gfx950 (CDNA4, wave64).
 //
-// Mapping choices (one MI355X holds the whole system):
+// Mapping choices:
 //  * assign_atoms: a deterministic, atomics-free stable counting sort.  Each
 //    1024-atom chunk is bitonic-sorted by (cell, index) in LDS, chunk
 //    histograms are scanned per cell, and atoms scatter straight to their
 //    final slot.  The order inside a cell is by original index, which is what
 //    a stable radix sort keyed by cell produces.
-//  * calc_sf: lane = cubature node, the wave walks a z-column of cells; the
-//    atom records are wave-uniform (scalar loads), so the inner loop is pure
-//    VALU + transcendental work with no LDS traffic or barriers.
+//  * calc_sf (mixed): one wave per (cell, 64 directions of an alpha group),
+//    lane = direction, the group's radial magnitudes unrolled as registers;
+//    atom records are wave-uniform scalar loads, so the inner loop is VALU
+//    and transcendental work only.  S is stored in node-pair tiles.
 //  * sum_rmt_sf: a workgroup holds the whole cell grid of one or two nodes in
-//    LDS and does the 3-D summed-volume table and the shell gather there, so
-//    the C x K field is read once and written once.
+//    LDS (read as one contiguous run of the node-pair tiles) and does the
+//    3-D summed-volume table and the shell gather there.
 //  * calc_lr_ef: lane = atom, 64 consecutive cell-sorted atoms per wave (no
-//    lanes idle at cell boundaries); node records are wave-uniform scalar
-//    loads, remote sums are per-lane loads that coalesce to the 1-2 cells a
-//    wave spans.
-//  * calc_sr_ef: one wave per cell, neighbour atoms as wave-uniform loads.
+//    lanes idle at cell boundaries), one launch per alpha group with its
+//    radial magnitudes unrolled; remote sums are per-lane loads that
+//    coalesce to the 1-2 cells a wave spans.
+//  * calc_sr_ef: one 256-thread workgroup per cell, a cell of n atoms
+//    covered floor(256/n) times by interleaved neighbour phases.
+//  * Every kernel body is a __device__ function shared by the single-system
+//    kernel and its batched twin, which looks its system up in a run table.
 #include "kernels.hpp"
 
 #include <cfloat>

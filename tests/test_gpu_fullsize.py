@@ -1,0 +1,83 @@
+"""The engine at the full workload sizes, checked through properties that do
+not need the oracle to finish at that size:
+
+* the GLST interaction is antisymmetric pair by pair (near pairs exactly; far
+  pairs through the symmetric Chebyshev shells and the odd sin kernel), so
+  the net force sum_i F_i vanishes up to rounding;
+* the mixed path agrees with the FP64 path well inside the method error;
+* at 288k charges the FP64 path is within 1e-5 of the direct FP64 all-pairs
+  sum, and the energies agree likewise;
+* binning at full size reproduces the bit-exact CT arithmetic (checked with
+  the oracle's vectorised binning, which runs at any size).
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(h, tol, L, rcut, ct, torch):
+    import glst_amd as g
+    d = [torch.as_tensor(a, dtype=torch.float64, device="cuda") for a in h]
+    f = g.GLSTForce(h[0].size, tol, L, rcut=rcut, ct=ct)
+    f.calc_ener_force(*d)
+    out = [t.cpu().numpy() for t in f.get_ef()]
+    info = f.info()
+    cells = f.cells()
+    f.close()
+    return out, info, cells
+
+
+def _net_force(F):
+    tot = np.abs(np.stack(F[:3])).sum()
+    return np.abs(np.array([c.sum() for c in F[:3]])).max() / tot
+
+
+@pytest.fixture(scope="module")
+def water288k():
+    import glst_amd as g
+    return g.make_water_box(96000, 142.0, seed=1)
+
+
+def test_288k_fp64_vs_direct_sum_and_mixed(cuda, water288k):
+    import torch
+    import glst_amd as g
+    from oracle import oracle as o
+    h = water288k
+    f64, info, cells = _run(h, 1e-4, 142.0, 10.0, "double", torch)
+    mix, _, _ = _run(h, 1e-4, 142.0, 10.0, "float", torch)
+    d = [torch.as_tensor(a, dtype=torch.float64, device="cuda") for a in h]
+    ref = [t.cpu().numpy() for t in g.coulomb_direct(*d)]
+    assert tuple(info.ncell) == (15, 15, 15) and info.num_nodes > 0
+    assert rel_err(f64[:3], ref[:3]) < 1e-5
+    assert abs(f64[3].sum() - ref[3].sum()) < 1e-5 * np.abs(ref[3]).sum()
+    assert rel_err(mix[:3], f64[:3]) < 2e-6
+    assert rel_err(mix[:3], ref[:3]) < 1e-5
+    assert _net_force(f64) < 1e-12
+    assert _net_force(mix) < 1e-7
+    # full-size binning is bit-exact
+    ac, si, cp, cc = cells
+    oac, osi, ocp, occ = o.bin_atoms(0, *h[:3], (15, 15, 15), 10.0)
+    np.testing.assert_array_equal(ac, oac)
+    np.testing.assert_array_equal(si, osi)
+    np.testing.assert_array_equal(cc, occ)
+
+
+def test_ionic_1m_mixed_vs_fp64(cuda):
+    """333,000 waters plus 898 Na+/Cl- pairs in a 215 A box, rcut 12 A: uneven
+    cell occupancy; tighter tol."""
+    import torch
+    import glst_amd as g
+    h = g.make_water_box(333000, 215.0, seed=1, nion=898)
+    assert h[0].size == 1_000_796
+    assert abs(h[3].sum()) < 1e-6
+    tol = 1e-5
+    f64, info, _ = _run(h, tol, 215.0, 12.0, "double", torch)
+    mix, _, _ = _run(h, tol, 215.0, 12.0, "float", torch)
+    assert tuple(info.ncell) == (18, 18, 18)
+    assert rel_err(mix[:3], f64[:3]) < 2e-6
+    assert rel_err(mix[3:], f64[3:]) < 2e-6
+    assert _net_force(f64) < 1e-12
+    assert _net_force(mix) < 1e-7
