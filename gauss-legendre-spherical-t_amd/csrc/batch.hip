@@ -301,6 +301,16 @@ dev::LrTaskArgs glst_batch::task_args() const {
           kap32.as<float>(),      t_split.as<uint32_t>()};
 }
 
+void glst_batch::release_arenas() {
+  GLST_HIP(hipStreamSynchronize(stream));
+  for (DeviceBuffer *b :
+       {&d_sys, &d_seg, &atom_cell, &rank, &hist, &cell_count, &cell_start,
+        &maxc, &sorted_idx, &cell_sorted, &atoms64, &atoms32, &near4, &fx, &fy,
+        &fz, &en, &k64, &group, &r_in, &r_out, &t_node0, &t_nr, &t_koff,
+        &t_dir64, &t_dir32, &kap64, &kap32, &t_split, &S, &R, &partial})
+    b->release();
+}
+
 void glst_batch::zero_ef() {
   for (DeviceBuffer *b : {&fx, &fy, &fz, &en})
     dev::fill_f64(b->as<double>(), 0.0, B.natom, stream);

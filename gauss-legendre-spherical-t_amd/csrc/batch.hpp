@@ -119,6 +119,8 @@ struct glst_batch {
   void configure(int ct,
                  std::vector<std::shared_ptr<const glst::BatchSystem>> cs);
   void zero_ef();
+  // frees every arena (the next configure allocates them for its batch)
+  void release_arenas();
   void step_async(const double *rx, const double *ry, const double *rz,
                   const double *qc);
   // unpermuted outputs in packed original order (stream-ordered, no sync)

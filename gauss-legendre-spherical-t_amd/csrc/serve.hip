@@ -1,6 +1,6 @@
 // Serving loop behind include/glst_serve.h: continuous batching of force
-// requests over the batched engine (batch.hpp), admission by device budget,
-// per-shape setup cache.
+// requests over the batched engine (batch.hpp) with two batches in flight,
+// admission by device budget, grid-setup cache.
 #include "glst_serve.h"
 #include "batch.hpp"
 
@@ -65,26 +65,64 @@ struct Request {
   double *out[4];
 };
 
+// Batches in flight at once.  Each slot owns a batch (arenas and its own
+// stream), the packed request inputs/outputs and the copy-run table; while
+// one slot's batch runs on the device, the next is admitted, packed and
+// launched on the other.
+constexpr size_t kSlots = 2;
+
+struct Slot {
+  glst_batch batch;
+  DeviceBuffer in[4], out[4], runs;
+  std::vector<dev::CopyRun> host_runs;
+  std::vector<const BatchSystem *> composition;  // of the configured batch
+  std::vector<uint64_t> tickets;                 // of the batch in flight
+  uint64_t atoms = 0;
+  hipEvent_t done = nullptr;
+  bool busy = false;
+  ~Slot() {
+    if (done) (void)hipEventDestroy(done);
+  }
+};
+
 }  // namespace
 
 struct glst_server {
   std::string last_error;
   int ct = GLST_CT_DOUBLE;
   double tol = 0.0;
-  size_t budget = 0;
+  size_t budget = 0;  // whole server; each slot's batch gets budget / kSlots
   uint32_t max_batch = 0;
-  glst_batch batch;
-  bool batch_open = false;
-  std::vector<const BatchSystem *> composition;  // of the configured batch
+  size_t lds_limit = 65536;
+  std::vector<std::unique_ptr<Slot>> slots;
+  std::deque<size_t> in_flight;  // slot indices, oldest first
   Lru<GridKey, std::shared_ptr<const ShapeSetup>> grids{kMaxCachedGrids};
   Lru<SysKey, SysEntry> systems{kMaxCachedSystems};
   std::deque<Request> queue;
   std::vector<uint64_t> done;
   uint64_t next_ticket = 1;
   glst_server_stats_t stats{};
-  // packed inputs/outputs of the current batch and the copy-run table
-  DeviceBuffer in[4], out[4], runs;
-  std::vector<dev::CopyRun> host_runs;
+
+  size_t share() const { return budget / kSlots; }
+
+  ~glst_server() {
+    // batches still in flight finish before their arenas go away
+    for (auto &sl : slots) {
+      (void)hipStreamSynchronize(sl->batch.stream);
+      (void)hipStreamSynchronize(sl->batch.side);
+    }
+  }
+
+  void open(void *stream) {
+    for (size_t i = 0; i < kSlots; ++i) {
+      auto sl = std::make_unique<Slot>();
+      // the caller's stream serves the first slot; the others get their own
+      sl->batch.open(i == 0 ? stream : nullptr);
+      GLST_HIP(hipEventCreateWithFlags(&sl->done, hipEventDisableTiming));
+      slots.push_back(std::move(sl));
+    }
+    lds_limit = slots[0]->batch.lds_limit;
+  }
 
   SysEntry &lookup(uint32_t natom, const double *box, double rcut) {
     uint32_t n[3];
@@ -108,7 +146,7 @@ struct glst_server {
     }
     SysEntry e;
     e.sys = make_batch_system(natom, shape);
-    e.bytes = pack_batch(ct, {e.sys}, batch.lds_limit).device_bytes() +
+    e.bytes = pack_batch(ct, {e.sys}, lds_limit).device_bytes() +
               8 * sizeof(double) * static_cast<size_t>(natom);
     return systems.insert(skey, std::move(e));
   }
@@ -122,11 +160,13 @@ struct glst_server {
       if (!inp[k] || !outp[k]) throw ArgError("null input or output array");
     check_system_args(ct, box[0], box[1], box[2], rcut, tol);
     SysEntry &e = lookup(natom, box, rcut);
-    if (e.bytes > budget) {
+    if (e.bytes > share()) {
       ++stats.rejected;
       throw AllocError("request needs " + std::to_string(e.bytes) +
-                       " device bytes; the server's budget is " +
-                       std::to_string(budget));
+                       " device bytes; a batch of this server may hold " +
+                       std::to_string(share()) + " (budget " +
+                       std::to_string(budget) + " over " +
+                       std::to_string(kSlots) + " batches in flight)");
     }
     Request r;
     r.ticket = next_ticket++;
@@ -141,13 +181,12 @@ struct glst_server {
     *ticket = r.ticket;
   }
 
-  uint32_t step() {
-    if (queue.empty()) return 0;
-    // admission: oldest first while the budget and max_batch allow
+  // Admit the oldest queued requests into free slot `sl` and launch them.
+  void launch(Slot &sl) {
     size_t take = 0, bytes = 0;
     while (take < queue.size() && (max_batch == 0 || take < max_batch)) {
       const size_t b = queue[take].bytes;
-      if (take > 0 && bytes + b > budget) break;
+      if (take > 0 && bytes + b > share()) break;
       bytes += b;
       ++take;
     }
@@ -159,54 +198,111 @@ struct glst_server {
       comp.push_back(queue[i].sys.get());
       natom += queue[i].sys->natom;
     }
-    if (!batch_open) {
-      batch.open(nullptr);
-      batch_open = true;
-    }
-    if (comp != composition) {
-      batch.configure(ct, std::move(cs));
-      composition = comp;
+    glst_batch &b = sl.batch;
+    if (comp != sl.composition) {
+      b.configure(ct, cs);
+      if (b.device_bytes() > share()) {
+        // arenas grown by earlier batches: size them for this one alone
+        b.release_arenas();
+        b.configure(ct, cs);
+      }
+      sl.composition = comp;
       ++stats.reconfigures;
     } else {
-      batch.zero_ef();
+      b.zero_ef();
     }
-    hipStream_t s = batch.stream;
+    hipStream_t s = b.stream;
     for (int k = 0; k < 4; ++k) {
-      in[k].reserve(natom * sizeof(double));
-      out[k].reserve(natom * sizeof(double));
+      sl.in[k].reserve(natom * sizeof(double));
+      sl.out[k].reserve(natom * sizeof(double));
     }
-    // gather the requests' inputs into the packed arrays
-    host_runs.clear();
+    // gather the requests' inputs into the packed arrays, scatter back after
+    sl.host_runs.clear();
     for (size_t i = 0; i < take; ++i) {
-      const uint32_t off = batch.B.sys[i].atom0, n = batch.B.sys[i].natom;
+      const uint32_t off = b.B.sys[i].atom0, n = b.B.sys[i].natom;
       for (int k = 0; k < 4; ++k)
-        host_runs.push_back({queue[i].in[k], in[k].as<double>() + off, n});
+        sl.host_runs.push_back({queue[i].in[k], sl.in[k].as<double>() + off, n});
     }
     for (size_t i = 0; i < take; ++i) {
-      const uint32_t off = batch.B.sys[i].atom0, n = batch.B.sys[i].natom;
+      const uint32_t off = b.B.sys[i].atom0, n = b.B.sys[i].natom;
       for (int k = 0; k < 4; ++k)
-        host_runs.push_back({out[k].as<double>() + off, queue[i].out[k], n});
+        sl.host_runs.push_back(
+            {sl.out[k].as<double>() + off, queue[i].out[k], n});
     }
-    runs.reserve(host_runs.size() * sizeof(dev::CopyRun));
-    GLST_HIP(hipMemcpyAsync(runs.as<void>(), host_runs.data(),
-                            host_runs.size() * sizeof(dev::CopyRun),
+    sl.runs.reserve(sl.host_runs.size() * sizeof(dev::CopyRun));
+    GLST_HIP(hipMemcpyAsync(sl.runs.as<void>(), sl.host_runs.data(),
+                            sl.host_runs.size() * sizeof(dev::CopyRun),
                             hipMemcpyHostToDevice, s));
     const uint32_t nrun = static_cast<uint32_t>(4 * take);
-    dev::copy_runs(runs.as<dev::CopyRun>(), nrun, s);
+    dev::copy_runs(sl.runs.as<dev::CopyRun>(), nrun, s);
     check_launch("gather requests");
-    batch.step_async(in[0].as<double>(), in[1].as<double>(),
-                     in[2].as<double>(), in[3].as<double>());
-    batch.get_ef_async(out[0].as<double>(), out[1].as<double>(),
-                       out[2].as<double>(), out[3].as<double>());
-    dev::copy_runs(runs.as<dev::CopyRun>() + nrun, nrun, s);
+    b.step_async(sl.in[0].as<double>(), sl.in[1].as<double>(),
+                 sl.in[2].as<double>(), sl.in[3].as<double>());
+    b.get_ef_async(sl.out[0].as<double>(), sl.out[1].as<double>(),
+                   sl.out[2].as<double>(), sl.out[3].as<double>());
+    dev::copy_runs(sl.runs.as<dev::CopyRun>() + nrun, nrun, s);
     check_launch("scatter results");
-    GLST_HIP(hipStreamSynchronize(s));
-    for (size_t i = 0; i < take; ++i) done.push_back(queue[i].ticket);
+    GLST_HIP(hipEventRecord(sl.done, s));
+    sl.tickets.clear();
+    for (size_t i = 0; i < take; ++i) sl.tickets.push_back(queue[i].ticket);
+    sl.atoms = natom;
+    sl.busy = true;
     queue.erase(queue.begin(), queue.begin() + take);
     ++stats.batches;
-    stats.completed += take;
-    stats.atoms += natom;
-    return static_cast<uint32_t>(take);
+  }
+
+  uint32_t retire(Slot &sl) {
+    for (uint64_t t : sl.tickets) done.push_back(t);
+    const uint32_t n = static_cast<uint32_t>(sl.tickets.size());
+    stats.completed += n;
+    stats.atoms += sl.atoms;
+    sl.tickets.clear();
+    sl.busy = false;
+    return n;
+  }
+
+  // Launch while a slot is free and requests are queued, retire finished
+  // batches; with `block`, wait for the oldest batch in flight when nothing
+  // else could be done.
+  uint32_t advance(bool block) {
+    uint32_t completed = 0;
+    bool launched = false;
+    for (;;) {
+      // retire finished batches, oldest first
+      while (!in_flight.empty()) {
+        Slot &sl = *slots[in_flight.front()];
+        const hipError_t q = hipEventQuery(sl.done);
+        if (q == hipErrorNotReady) break;
+        GLST_HIP(q);
+        completed += retire(sl);
+        in_flight.pop_front();
+      }
+      if (queue.empty()) break;
+      size_t free_slot = kSlots;
+      for (size_t i = 0; i < kSlots; ++i)
+        if (!slots[i]->busy) {
+          free_slot = i;
+          break;
+        }
+      if (free_slot == kSlots) break;
+      launch(*slots[free_slot]);
+      in_flight.push_back(free_slot);
+      launched = true;
+    }
+    if (block && completed == 0 && !in_flight.empty()) {
+      Slot &sl = *slots[in_flight.front()];
+      GLST_HIP(hipEventSynchronize(sl.done));
+      completed += retire(sl);
+      in_flight.pop_front();
+      (void)launched;
+    }
+    return completed;
+  }
+
+  uint32_t unfinished() const {
+    size_t n = queue.size();
+    for (size_t i : in_flight) n += slots[i]->tickets.size();
+    return static_cast<uint32_t>(n);
   }
 };
 
@@ -237,8 +333,7 @@ glst_status_t glst_server_create(int ct, double tol, size_t budget_bytes,
     v->tol = tol;
     v->budget = budget_bytes;
     v->max_batch = max_batch;
-    v->batch.open(stream);
-    v->batch_open = true;
+    v->open(stream);
   });
   if (st != GLST_OK) {
     delete v;
@@ -270,7 +365,15 @@ glst_status_t glst_server_submit(glst_server_t srv, uint32_t natom,
 glst_status_t glst_server_step(glst_server_t srv, uint32_t *ncompleted) {
   NEED_SERVER(srv);
   return run_guarded(&srv->last_error, [&] {
-    const uint32_t n = srv->step();
+    const uint32_t n = srv->advance(true);
+    if (ncompleted) *ncompleted = n;
+  });
+}
+
+glst_status_t glst_server_poll(glst_server_t srv, uint32_t *ncompleted) {
+  NEED_SERVER(srv);
+  return run_guarded(&srv->last_error, [&] {
+    const uint32_t n = srv->advance(false);
     if (ncompleted) *ncompleted = n;
   });
 }
@@ -291,7 +394,7 @@ glst_status_t glst_server_completed(glst_server_t srv, uint64_t *tickets,
 glst_status_t glst_server_pending(glst_server_t srv, uint32_t *n) {
   NEED_SERVER(srv);
   if (!n) return GLST_ERR_INVALID;
-  *n = static_cast<uint32_t>(srv->queue.size());
+  *n = srv->unfinished();
   return GLST_OK;
 }
 

@@ -155,6 +155,7 @@ def _load():
         "glst_server_submit": [P, u, P, d, P, P, P, P, P, P, P, P,
                                POINTER(c_uint64)],
         "glst_server_step": [P, POINTER(u)],
+        "glst_server_poll": [P, POINTER(u)],
         "glst_server_completed": [P, P, u, POINTER(u)],
         "glst_server_pending": [P, POINTER(u)],
         "glst_server_get_stats": [P, P],
@@ -621,8 +622,16 @@ class GLSTServer:
         return t.value
 
     def step(self) -> int:
+        """Launch what fits, retire finished batches; wait for the oldest
+        batch in flight if none had finished.  Returns requests answered."""
         n = c_uint32()
         self._c(_lib.glst_server_step(self._h, byref(n)))
+        return n.value
+
+    def poll(self) -> int:
+        """As step, but never waits for the device."""
+        n = c_uint32()
+        self._c(_lib.glst_server_poll(self._h, byref(n)))
         return n.value
 
     def completed(self, cap: int = 4096):
@@ -633,6 +642,7 @@ class GLSTServer:
         return [int(v) for v in out[: n.value]]
 
     def pending(self) -> int:
+        """Requests not yet answered (queued or in flight)."""
         n = c_uint32()
         self._c(_lib.glst_server_pending(self._h, byref(n)))
         return n.value

@@ -71,11 +71,10 @@ def test_requests_alone_and_in_a_mix(cuda, ct, bar):
     outs1 = [_outs(h[0].size, torch) for _, h in systems]
     for (L, _), i, o in zip(systems, ins, outs1):
         one.submit(L, RCUT, *i, *o)
-    steps = 0
+    answered = 0
     while one.pending():
-        assert one.step() == 1
-        steps += 1
-    assert steps == len(systems)
+        answered += one.step()
+    assert answered == len(systems)
     for o, r in zip(outs1, refs):
         _check(o, r, bar)
     st = one.stats()
@@ -91,19 +90,21 @@ def test_budget_admission(cuda):
     big = _system(4000, 70)
     need = max(g.batch_device_bytes([h[0].size], [L], RCUT, TOL)
                for L, h in small)
-    # room for about two small requests per batch
-    srv = g.GLSTServer(TOL, int(2.5 * need), ct="double")
+    # two batches may be in flight, each with half the budget: room for
+    # two small requests per batch
+    srv = g.GLSTServer(TOL, int(5.0 * need), ct="double")
     ins = [_dev(h, torch) for _, h in small]
     outs = [_outs(h[0].size, torch) for _, h in small]
     for (L, _), i, o in zip(small, ins, outs):
         srv.submit(L, RCUT, *i, *o)
-    sizes = []
+    answered = 0
     while srv.pending():
-        sizes.append(srv.step())
-    assert len(sizes) >= 2 and max(sizes) <= 3 and sum(sizes) == 4
+        answered += srv.step()
+    assert answered == 4
+    assert srv.stats()["batches"] == 2
     for (L, h), o in zip(small, outs):
         _check(o, _single(L, h, "double", torch), 1e-12)
-    # a request larger than the whole budget is refused at submission
+    # a request larger than a batch's share is refused at submission
     Lb, hb = big
     with pytest.raises(g.GLSTError) as e:
         srv.submit(Lb, RCUT, *_dev(hb, torch), *_outs(hb[0].size, torch))
@@ -126,10 +127,34 @@ def test_resident_system_reuses_setup_and_batch(cuda):
                for a in h[:3]]
         hs = (pos[0], pos[1], pos[2], h[3])
         o = _outs(h[0].size, torch)
-        srv.submit(L, RCUT, *_dev(hs, torch), *o)
+        i = _dev(hs, torch)  # borrowed until the request completes
+        srv.submit(L, RCUT, *i, *o)
         assert srv.step() == 1
         _check(o, _single(L, hs, "float", torch), 1e-6)
     st = srv.stats()
     assert st["setups"] == 1 and st["setup_hits"] == 2
     assert st["reconfigures"] == 1 and st["batches"] == 3
+    srv.close()
+
+
+def test_batches_overlap_in_flight(cuda):
+    """poll never blocks: with two slots, a second batch is launched while
+    the first is still in flight, and every answer is still exact."""
+    import torch
+    import glst_amd as g
+    systems = [_system(nw, 90 + k) for k, nw in enumerate([3000, 2500, 800])]
+    srv = g.GLSTServer(TOL, 1 << 34, max_batch=1, ct="double")
+    # inputs are borrowed until the request completes: keep them alive
+    ins = [_dev(h, torch) for _, h in systems]
+    outs = [_outs(h[0].size, torch) for _, h in systems]
+    for (L, h), i, o in zip(systems, ins, outs):
+        srv.submit(L, RCUT, *i, *o)
+    srv.poll()  # launches two batches, returns at once
+    assert srv.stats()["batches"] == 2
+    answered = 0
+    while srv.pending():
+        answered += srv.step()
+    assert answered == 3 and srv.stats()["batches"] == 3
+    for (L, h), o in zip(systems, outs):
+        _check(o, _single(L, h, "double", torch), 1e-12)
     srv.close()
