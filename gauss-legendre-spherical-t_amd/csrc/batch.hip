@@ -60,7 +60,7 @@ size_t BatchLayout::device_bytes() const {
 
 BatchLayout pack_batch(int ct,
                        const std::vector<std::shared_ptr<const BatchSystem>> &cs,
-                       size_t lds_limit) {
+                       size_t lds_limit, uint32_t lr_blocks) {
   if (cs.empty()) throw ArgError("a batch needs at least one system");
   const uint32_t nsys = static_cast<uint32_t>(cs.size());
   BatchLayout B;
@@ -71,7 +71,8 @@ BatchLayout pack_batch(int ct,
   for (const auto &c : cs)
     if (c->su().has_lr && c->su().K > 0) tiles += div_up(c->natom, 256);
   const uint32_t target =
-      tiles ? static_cast<uint32_t>(std::max<uint64_t>(1, (4096 + tiles - 1) / tiles))
+      tiles ? static_cast<uint32_t>(
+                  std::max<uint64_t>(1, (lr_blocks + tiles - 1) / tiles))
             : 1;
   std::vector<SplitPlan> splits(nsys);
   uint32_t task_base = 0, split_base = 0;
@@ -180,6 +181,7 @@ BatchLayout pack_batch(int ct,
 }  // namespace glst
 
 glst_batch::~glst_batch() {
+  drop_graph();
   if (side) (void)hipStreamDestroy(side);
   if (ev_assigned) (void)hipEventDestroy(ev_assigned);
   if (ev_near) (void)hipEventDestroy(ev_near);
@@ -193,6 +195,7 @@ void glst_batch::open(void *stream_) {
                             device) == hipSuccess &&
       lds > 0)
     lds_limit = static_cast<size_t>(lds);
+  dev::prepare_kernels(lds_limit);
   if (stream_) {
     stream = static_cast<hipStream_t>(stream_);
   } else {
@@ -220,7 +223,9 @@ void glst_batch::configure(
     int ct, std::vector<std::shared_ptr<const glst::BatchSystem>> cs) {
   // the previous batch's work must be done before its tables change
   GLST_HIP(hipStreamSynchronize(stream));
-  B = pack_batch(ct, cs, lds_limit);
+  drop_graph();
+  std::fill(last_in, last_in + 4, nullptr);
+  B = pack_batch(ct, cs, lds_limit, lr_blocks);
   const size_t N = B.natom, C = B.ncell;
   atom_cell.reserve(N * 4);
   rank.reserve(N * 4);
@@ -374,6 +379,110 @@ void glst_batch::step_async(const double *rx, const double *ry,
                        near4.as<double4>(), fx.as<double>(), fy.as<double>(),
                        fz.as<double>(), en.as<double>(), stream);
   check_launch("reduce");
+}
+
+void glst_batch::drop_graph() {
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  graph_exec = nullptr;
+}
+
+void glst_batch::step(const double *rx, const double *ry, const double *rz,
+                      const double *qc) {
+  const double *in[4] = {rx, ry, rz, qc};
+  if (graph_mode == 0 || !rx || !ry || !rz || !qc) {
+    step_async(rx, ry, rz, qc);
+    return;
+  }
+  if (graph_exec && std::equal(in, in + 4, graph_in)) {
+    GLST_HIP(hipGraphLaunch(graph_exec, stream));
+    assigned = true;
+    return;
+  }
+  if (graph_mode == 2 && !std::equal(in, in + 4, last_in)) {
+    std::copy(in, in + 4, last_in);
+    step_async(rx, ry, rz, qc);
+    return;
+  }
+  drop_graph();
+  hipGraph_t g = nullptr;
+  GLST_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  try {
+    step_async(rx, ry, rz, qc);
+  } catch (...) {
+    (void)hipStreamEndCapture(stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  GLST_HIP(hipStreamEndCapture(stream, &g));
+  const hipError_t ei = hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) graph_exec = nullptr;
+  GLST_HIP(ei);
+  std::copy(in, in + 4, graph_in);
+  GLST_HIP(hipGraphLaunch(graph_exec, stream));
+}
+
+double glst_batch::autotune(const double *rx, const double *ry,
+                            const double *rz, const double *qc,
+                            uint32_t reps) {
+  if (reps == 0) throw ArgError("autotune needs at least one repetition");
+  hipEvent_t a = nullptr, b = nullptr;
+  GLST_HIP(hipEventCreate(&a));
+  GLST_HIP(hipEventCreate(&b));
+  auto time = [&] {
+    step(rx, ry, rz, qc);  // warm-up (and capture in graph mode)
+    step(rx, ry, rz, qc);
+    GLST_HIP(hipEventRecord(a, stream));
+    for (uint32_t r = 0; r < reps; ++r) step(rx, ry, rz, qc);
+    GLST_HIP(hipEventRecord(b, stream));
+    GLST_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    GLST_HIP(hipEventElapsedTime(&ms, a, b));
+    return static_cast<double>(ms) / reps;
+  };
+  double best;
+  try {
+    best = time();
+    const int keep_overlap = overlap_near;
+    int pick_overlap = keep_overlap;
+    for (int v : {0, 1}) {
+      if (v == keep_overlap) continue;
+      overlap_near = v;
+      drop_graph();
+      const double t = time();
+      if (t < best) {
+        best = t;
+        pick_overlap = v;
+      }
+    }
+    overlap_near = pick_overlap;
+    drop_graph();
+    if (B.partial > 0) {
+      const uint32_t keep = lr_blocks;
+      uint32_t pick = keep;
+      for (uint32_t v : {1024u, 2048u, 4096u, 8192u, 16384u}) {
+        if (v == keep) continue;
+        lr_blocks = v;
+        configure(B.ct, B.cs);
+        const double t = time();
+        if (t < best) {
+          best = t;
+          pick = v;
+        }
+      }
+      lr_blocks = pick;
+      configure(B.ct, B.cs);
+    }
+  } catch (...) {
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    throw;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  zero_ef();
+  GLST_HIP(hipStreamSynchronize(stream));
+  return best;
 }
 
 void glst_batch::get_ef_async(double *ofx, double *ofy, double *ofz,
@@ -530,7 +639,7 @@ glst_status_t glst_batch_calc_ener_force_async(glst_batch_t b,
                                                const double *d_qc) {
   NEED_BATCH(b);
   return run_guarded(&b->last_error,
-                     [&] { b->step_async(d_rx, d_ry, d_rz, d_qc); });
+                     [&] { b->step(d_rx, d_ry, d_rz, d_qc); });
 }
 
 glst_status_t glst_batch_calc_ener_force(glst_batch_t b, const double *d_rx,
@@ -539,7 +648,7 @@ glst_status_t glst_batch_calc_ener_force(glst_batch_t b, const double *d_rx,
                                          const double *d_qc) {
   NEED_BATCH(b);
   return run_guarded(&b->last_error, [&] {
-    b->step_async(d_rx, d_ry, d_rz, d_qc);
+    b->step(d_rx, d_ry, d_rz, d_qc);
     GLST_HIP(hipStreamSynchronize(b->stream));
   });
 }
@@ -564,13 +673,33 @@ glst_status_t glst_batch_set_option(glst_batch_t b, const char *name,
   return run_guarded(&b->last_error, [&] {
     if (!name) throw ArgError("null option name");
     const std::string n = name;
+    b->drop_graph();  // every option changes the captured step
     if (n == "overlap_near") {
       if (value != 0 && value != 1)
         throw ArgError("overlap_near must be 0 or 1 for a batch");
       b->overlap_near = static_cast<int>(value);
+    } else if (n == "lr_blocks") {
+      if (value < 1 || value > (1 << 24))
+        throw ArgError("lr_blocks must be in 1 .. 2^24");
+      b->lr_blocks = static_cast<uint32_t>(value);
+      b->configure(b->B.ct, b->B.cs);
+    } else if (n == "graph") {
+      if (value < 0 || value > 2) throw ArgError("graph must be 0, 1 or 2");
+      b->graph_mode = static_cast<int>(value);
     } else {
       throw ArgError("unknown option \"" + n + "\"");
     }
+  });
+}
+
+glst_status_t glst_batch_autotune(glst_batch_t b, const double *d_rx,
+                                  const double *d_ry, const double *d_rz,
+                                  const double *d_qc, uint32_t reps,
+                                  double *best_ms) {
+  NEED_BATCH(b);
+  return run_guarded(&b->last_error, [&] {
+    const double t = b->autotune(d_rx, d_ry, d_rz, d_qc, reps);
+    if (best_ms) *best_ms = t;
   });
 }
 
@@ -578,6 +707,7 @@ glst_status_t glst_batch_set_stream(glst_batch_t b, void *stream) {
   NEED_BATCH(b);
   return run_guarded(&b->last_error, [&] {
     GLST_HIP(hipStreamSynchronize(b->stream));
+    b->drop_graph();
     if (b->own_stream) GLST_HIP(hipStreamDestroy(b->stream));
     b->own_stream = false;
     b->stream = static_cast<hipStream_t>(stream);

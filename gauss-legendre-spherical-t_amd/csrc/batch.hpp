@@ -77,11 +77,12 @@ struct BatchLayout {
 };
 
 // Packs systems (all of precision ct) into one batch layout.  lds_limit:
-// the device's LDS per workgroup; a system whose one-node cell table does not
-// fit is rejected.
+// the device's LDS per workgroup (a system whose one-node cell table does not
+// fit is rejected); lr_blocks: back-projection workgroups aimed for over the
+// whole batch.
 BatchLayout pack_batch(int ct,
                        const std::vector<std::shared_ptr<const BatchSystem>> &cs,
-                       size_t lds_limit);
+                       size_t lds_limit, uint32_t lr_blocks = 4096);
 
 }  // namespace glst
 
@@ -94,7 +95,15 @@ struct glst_batch {
   bool own_stream = false;
   hipEvent_t ev_assigned = nullptr, ev_near = nullptr;
   int overlap_near = 1;
+  uint32_t lr_blocks = 4096;
   bool assigned = false;
+  // HIP graph of the step (graph_mode 1: captured on the first step with
+  // given inputs; 2: on the second consecutive step with the same inputs,
+  // so batches used once are never captured)
+  int graph_mode = 0;
+  hipGraphExec_t graph_exec = nullptr;
+  const double *graph_in[4] = {nullptr, nullptr, nullptr, nullptr};
+  const double *last_in[4] = {nullptr, nullptr, nullptr, nullptr};
   // arenas (grow-only)
   glst::DeviceBuffer d_sys, d_seg;
   glst::DeviceBuffer atom_cell, rank, hist, cell_count, cell_start, maxc;
@@ -123,6 +132,14 @@ struct glst_batch {
   void release_arenas();
   void step_async(const double *rx, const double *ry, const double *rz,
                   const double *qc);
+  // step_async, or a replay of the step captured for these inputs
+  void step(const double *rx, const double *ry, const double *rz,
+            const double *qc);
+  void drop_graph();
+  // fastest of the candidate schedules (overlap_near, lr_blocks) for these
+  // inputs, `reps` steps each; returns its ms per step
+  double autotune(const double *rx, const double *ry, const double *rz,
+                  const double *qc, uint32_t reps);
   // unpermuted outputs in packed original order (stream-ordered, no sync)
   void get_ef_async(double *ofx, double *ofy, double *ofz, double *oen);
   size_t device_bytes() const;

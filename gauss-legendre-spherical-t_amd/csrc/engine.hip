@@ -69,7 +69,12 @@ struct glst_engine {
   // 2 alongside the shell sums onwards
   int overlap_near = 1;
   int rmt_shape = 0;  // shell sums: 0 auto, 1 one node per workgroup, 2 global
+  uint32_t lr_blocks = 4096;  // back-projection workgroups aimed for
   KernelTimer timer;
+  // the whole step as a HIP graph for fixed input arrays (option "graph")
+  bool use_graph = false;
+  hipGraphExec_t graph_exec = nullptr;
+  const double *graph_in[4] = {nullptr, nullptr, nullptr, nullptr};
 
   // atoms
   DeviceBuffer atom_cell, rank, hist, cell_count, cell_start, maxc;
@@ -90,6 +95,7 @@ struct glst_engine {
   bool rmt_done = false;
 
   ~glst_engine() {
+    drop_graph();
     if (side) (void)hipStreamDestroy(side);
     if (ev_assigned) (void)hipEventDestroy(ev_assigned);
     if (ev_near) (void)hipEventDestroy(ev_near);
@@ -125,6 +131,7 @@ struct glst_engine {
                               device) == hipSuccess &&
         lds > 0)
       lds_limit = static_cast<size_t>(lds);
+    dev::prepare_kernels(lds_limit);
     if (stream_) {
       stream = static_cast<hipStream_t>(stream_);
       own_stream = false;
@@ -185,7 +192,7 @@ struct glst_engine {
         R.resize(field * (mixed() ? sizeof(float2) : sizeof(double2)));
         const uint32_t ntile = (natom + 255) / 256;
         // enough workgroups to fill the chip several times over
-        set_splits(ntile > 0 ? (4096 + ntile - 1) / ntile : 1);
+        set_splits(ntile > 0 ? (lr_blocks + ntile - 1) / ntile : 1);
         partial.resize(static_cast<size_t>(nsplit) * n * sizeof(double4));
       }
     }
@@ -382,6 +389,145 @@ struct glst_engine {
     reduce(true);
   }
 
+  void drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    graph_exec = nullptr;
+  }
+
+  // shell sums through the global-memory tables (one node's table does not
+  // fit in LDS, or forced)
+  bool rmt_global_path() const {
+    const size_t per_node =
+        (static_cast<size_t>(grid.ncell) + grid.nx + grid.ny + grid.nz) *
+        sizeof(double2);
+    return rmt_shape == 2 || per_node > lds_limit;
+  }
+
+  // New back-projection split target: splits, their table and the partials.
+  void apply_lr_blocks() {
+    drop_graph();
+    if (!(has_lr && Kp > 0)) return;
+    const uint32_t ntile = (natom + 255) / 256;
+    set_splits(ntile > 0 ? (lr_blocks + ntile - 1) / ntile : 1);
+    partial.resize(static_cast<size_t>(nsplit) * natom * sizeof(double4));
+    t_split.resize(split_host.size() * 4);
+    GLST_HIP(hipMemcpy(t_split.as<void>(), split_host.data(),
+                       split_host.size() * 4, hipMemcpyHostToDevice));
+  }
+
+  // calc_ener_force: eagerly, or (graph mode) by replaying the step captured
+  // for these input arrays.
+  void step(const double *rx, const double *ry, const double *rz,
+            const double *qc) {
+    if (!use_graph || natom == 0) {
+      step_async(rx, ry, rz, qc);
+      return;
+    }
+    if (!rx || !ry || !rz || !qc)
+      throw ArgError("assign_atoms: null coordinate or charge pointer");
+    const double *in[4] = {rx, ry, rz, qc};
+    if (!graph_exec || !std::equal(in, in + 4, graph_in)) {
+      drop_graph();
+      // nothing may allocate inside the capture
+      if (has_lr && Kp > 0 && rmt_global_path())
+        rmt_work.resize(static_cast<size_t>(grid.ncell) * Kp * sizeof(double2));
+      const bool timing = timer.enabled;
+      timer.enabled = false;
+      hipGraph_t g = nullptr;
+      GLST_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      try {
+        step_async(rx, ry, rz, qc);
+      } catch (...) {
+        (void)hipStreamEndCapture(stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        timer.enabled = timing;
+        throw;
+      }
+      const hipError_t ec = hipStreamEndCapture(stream, &g);
+      timer.enabled = timing;
+      GLST_HIP(ec);
+      const hipError_t ei = hipGraphInstantiate(&graph_exec, g, nullptr,
+                                                nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ei != hipSuccess) graph_exec = nullptr;
+      GLST_HIP(ei);
+      std::copy(in, in + 4, graph_in);
+    }
+    GLST_HIP(hipGraphLaunch(graph_exec, stream));
+  }
+
+  // Times `reps` steps on these inputs (HIP events on the handle's stream).
+  double time_steps(const double *rx, const double *ry, const double *rz,
+                    const double *qc, uint32_t reps) {
+    step(rx, ry, rz, qc);  // warm-up (and capture in graph mode)
+    hipEvent_t a = nullptr, b = nullptr;
+    GLST_HIP(hipEventCreate(&a));
+    GLST_HIP(hipEventCreate(&b));
+    float ms = 0.f;
+    try {
+      GLST_HIP(hipEventRecord(a, stream));
+      for (uint32_t r = 0; r < reps; ++r) step(rx, ry, rz, qc);
+      GLST_HIP(hipEventRecord(b, stream));
+      GLST_HIP(hipEventSynchronize(b));
+      GLST_HIP(hipEventElapsedTime(&ms, a, b));
+    } catch (...) {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+      throw;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return static_cast<double>(ms) / reps;
+  }
+
+  // Chooses the fastest schedule for these inputs, one knob at a time: the
+  // near-field overlap, the back-projection split target, the shell-sum
+  // workgroup shape.  Results are identical under every schedule up to the
+  // split boundaries of the back-projection; the accumulators are cleared.
+  double autotune(const double *rx, const double *ry, const double *rz,
+                  const double *qc, uint32_t reps) {
+    if (reps == 0) throw ArgError("autotune needs at least one repetition");
+    double best = time_steps(rx, ry, rz, qc, reps);
+    auto knob = [&](auto get, auto set, std::vector<int64_t> values) {
+      const int64_t keep = get();
+      int64_t pick = keep;
+      for (int64_t v : values) {
+        if (v == keep) continue;
+        set(v);
+        const double t = time_steps(rx, ry, rz, qc, reps);
+        if (t < best) {
+          best = t;
+          pick = v;
+        }
+      }
+      if (get() != pick) set(pick);
+    };
+    knob([&] { return static_cast<int64_t>(overlap_near); },
+         [&](int64_t v) {
+           overlap_near = static_cast<int>(v);
+           drop_graph();
+         },
+         {0, 1, 2});
+    if (has_lr && Kp > 0) {
+      knob([&] { return static_cast<int64_t>(lr_blocks); },
+           [&](int64_t v) {
+             lr_blocks = static_cast<uint32_t>(v);
+             apply_lr_blocks();
+           },
+           {1024, 2048, 4096, 8192, 16384});
+      if (!rmt_global_path())
+        knob([&] { return static_cast<int64_t>(rmt_shape); },
+             [&](int64_t v) {
+               rmt_shape = static_cast<int>(v);
+               drop_graph();
+             },
+             {0, 1});
+    }
+    zero_ef();
+    GLST_HIP(hipStreamSynchronize(stream));
+    return best;
+  }
+
   void get_ef(double *ofx, double *ofy, double *ofz, double *oen) {
     if (natom > 0 && (!ofx || !ofy || !ofz || !oen))
       throw ArgError("get_ef: null output pointer");
@@ -559,6 +705,7 @@ glst_status_t glst_set_stream(glst_handle_t h, void *stream) {
   NEED_HANDLE(h);
   return guarded(h, [&] {
     GLST_HIP(hipStreamSynchronize(h->stream));
+    h->drop_graph();
     if (h->own_stream) GLST_HIP(hipStreamDestroy(h->stream));
     if (stream) {
       h->stream = static_cast<hipStream_t>(stream);
@@ -607,7 +754,7 @@ glst_status_t glst_calc_ener_force(glst_handle_t h, const double *d_rx,
                                    const double *d_qc) {
   NEED_HANDLE(h);
   return guarded(h, [&] {
-    h->step_async(d_rx, d_ry, d_rz, d_qc);
+    h->step(d_rx, d_ry, d_rz, d_qc);
     h->comm_ef();
   });
 }
@@ -617,7 +764,7 @@ glst_status_t glst_calc_ener_force_async(glst_handle_t h, const double *d_rx,
                                          const double *d_rz,
                                          const double *d_qc) {
   NEED_HANDLE(h);
-  return guarded(h, [&] { h->step_async(d_rx, d_ry, d_rz, d_qc); });
+  return guarded(h, [&] { h->step(d_rx, d_ry, d_rz, d_qc); });
 }
 
 glst_status_t glst_get_ef(glst_handle_t h, double *d_fx, double *d_fy,
@@ -799,6 +946,7 @@ glst_status_t glst_set_option(glst_handle_t h, const char *name,
   if (!name) return GLST_ERR_INVALID;
   return guarded(h, [&] {
     const std::string n = name;
+    h->drop_graph();  // every option changes the captured step
     if (n == "overlap_near") {
       if (value < 0 || value > 2) throw ArgError("overlap_near must be 0, 1 or 2");
       h->overlap_near = static_cast<int>(value);
@@ -806,8 +954,46 @@ glst_status_t glst_set_option(glst_handle_t h, const char *name,
     else if (n == "rmt_shape") {
       if (value < 0 || value > 2) throw ArgError("rmt_shape must be 0, 1 or 2");
       h->rmt_shape = static_cast<int>(value);
+    } else if (n == "lr_blocks") {
+      if (value < 1 || value > (1 << 24))
+        throw ArgError("lr_blocks must be in 1 .. 2^24");
+      h->lr_blocks = static_cast<uint32_t>(value);
+      h->apply_lr_blocks();
+    } else if (n == "graph") {
+      if (value != 0 && value != 1) throw ArgError("graph must be 0 or 1");
+      h->use_graph = value != 0;
     } else
       throw ArgError("unknown option \"" + n + "\"");
+  });
+}
+
+glst_status_t glst_get_option(glst_handle_t h, const char *name,
+                              int64_t *value) {
+  NEED_HANDLE(h);
+  if (!name || !value) return GLST_ERR_INVALID;
+  return guarded(h, [&] {
+    const std::string n = name;
+    if (n == "overlap_near")
+      *value = h->overlap_near;
+    else if (n == "rmt_shape")
+      *value = h->rmt_shape;
+    else if (n == "lr_blocks")
+      *value = h->lr_blocks;
+    else if (n == "graph")
+      *value = h->use_graph ? 1 : 0;
+    else
+      throw ArgError("unknown option \"" + n + "\"");
+  });
+}
+
+glst_status_t glst_autotune(glst_handle_t h, const double *d_rx,
+                            const double *d_ry, const double *d_rz,
+                            const double *d_qc, uint32_t reps,
+                            double *best_ms) {
+  NEED_HANDLE(h);
+  return guarded(h, [&] {
+    const double t = h->autotune(d_rx, d_ry, d_rz, d_qc, reps);
+    if (best_ms) *best_ms = t;
   });
 }
 

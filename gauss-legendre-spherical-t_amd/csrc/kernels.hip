@@ -1608,6 +1608,21 @@ __global__ void copy_runs_kernel(const CopyRun *__restrict__ runs) {
 // ---------------------------------------------------------------------------
 // launchers
 
+void prepare_kernels(size_t lds_limit) {
+  const int lds = static_cast<int>(lds_limit);
+  const void *fns[] = {reinterpret_cast<const void *>(&rmt_lds_kernel<true>),
+                       reinterpret_cast<const void *>(&rmt_lds_kernel<false>),
+                       reinterpret_cast<const void *>(&rmt_lds_batch_kernel<true>),
+                       reinterpret_cast<const void *>(&rmt_lds_batch_kernel<false>)};
+  for (const void *f : fns) {
+    const hipError_t e = hipFuncSetAttribute(
+        f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+    }
+  }
+}
+
 void fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s) {
   if (n == 0) return;
   fill_u32_kernel<<<blocks_for(n, 256), 256, 0, s>>>(p, value, n);
@@ -1697,15 +1712,9 @@ bool rmt_lds(bool mixed, uint32_t tmax, const void *S, const double4 *k64,
   const uint32_t ntile = Kp / T;  // Kp is even
   const size_t lds = T * per_node;
   if (mixed) {
-    (void)hipFuncSetAttribute(
-        reinterpret_cast<const void *>(&rmt_lds_kernel<true>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
     rmt_lds_kernel<true><<<ntile, kRmtBlock, lds, s>>>(
         S, k64, group, r_in, r_out, Kp, T, ntile, g.nx, g.ny, g.nz, g.h, R);
   } else {
-    (void)hipFuncSetAttribute(
-        reinterpret_cast<const void *>(&rmt_lds_kernel<false>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
     rmt_lds_kernel<false><<<ntile, kRmtBlock, lds, s>>>(
         S, k64, group, r_in, r_out, Kp, T, ntile, g.nx, g.ny, g.nz, g.h, R);
   }
@@ -1893,15 +1902,9 @@ void rmt_batch(bool mixed, const void *S, const double4 *k64,
                size_t lds_bytes, void *R, hipStream_t s) {
   if (L.nblk == 0) return;
   if (mixed) {
-    (void)hipFuncSetAttribute(
-        reinterpret_cast<const void *>(&rmt_lds_batch_kernel<true>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_bytes));
     rmt_lds_batch_kernel<true><<<L.nblk, kRmtBlock, lds_bytes, s>>>(
         S, k64, group, r_in, r_out, sys, L.seg, L.nseg, R);
   } else {
-    (void)hipFuncSetAttribute(
-        reinterpret_cast<const void *>(&rmt_lds_batch_kernel<false>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_bytes));
     rmt_lds_batch_kernel<false><<<L.nblk, kRmtBlock, lds_bytes, s>>>(
         S, k64, group, r_in, r_out, sys, L.seg, L.nseg, R);
   }

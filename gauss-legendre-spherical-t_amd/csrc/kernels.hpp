@@ -18,6 +18,11 @@ struct GridShape {
   double h;  // cell edge (double of the CT value)
 };
 
+// Once per process and device, before any launch: lets the LDS-resident
+// shell-sum kernels use up to lds_limit bytes of dynamic LDS (set outside
+// the launches so that a step can be captured in a HIP graph).
+void prepare_kernels(size_t lds_limit);
+
 // ---- binning and stable counting sort (assign_atoms) ----------------------
 void fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s);
 void fill_f64(double *p, double value, size_t n, hipStream_t s);

@@ -123,6 +123,8 @@ def _load():
         "glst_get_rmt": [P, P, P],
         "glst_set_timing": [P, c_int],
         "glst_set_option": [P, c_char_p, ctypes.c_int64],
+        "glst_get_option": [P, c_char_p, POINTER(ctypes.c_int64)],
+        "glst_autotune": [P, P, P, P, P, u, POINTER(d)],
         "glst_kernel_time": [P, c_char_p, POINTER(d), POINTER(u)],
         "glst_reset_timing": [P],
         "glst_coulomb_direct": [P, P, P, P, u, P, P, P, P, P],
@@ -148,6 +150,7 @@ def _load():
         "glst_batch_get_ef": [P, P, P, P, P],
         "glst_batch_zero_ef": [P],
         "glst_batch_set_option": [P, c_char_p, ctypes.c_int64],
+        "glst_batch_autotune": [P, P, P, P, P, u, POINTER(d)],
         "glst_batch_set_stream": [P, P],
         # serving loop (include/glst_serve.h)
         "glst_server_create": [c_int, d, c_size_t, u, P, POINTER(P)],
@@ -441,6 +444,18 @@ class GLSTForce:
     def set_option(self, name: str, value: int):
         self._c(_lib.glst_set_option(self._h, name.encode(), int(value)))
 
+    def get_option(self, name: str) -> int:
+        v = ctypes.c_int64()
+        self._c(_lib.glst_get_option(self._h, name.encode(), byref(v)))
+        return v.value
+
+    def autotune(self, rx, ry, rz, qc, reps: int = 3) -> float:
+        """Keep the fastest schedule for these inputs; returns ms/step."""
+        t = c_double()
+        self._c(_lib.glst_autotune(self._h, _ptr(rx), _ptr(ry), _ptr(rz),
+                                   _ptr(qc), reps, byref(t)))
+        return t.value
+
     # timing
     def set_timing(self, on: bool):
         self._c(_lib.glst_set_timing(self._h, 1 if on else 0))
@@ -563,6 +578,13 @@ class GLSTBatch:
 
     def set_option(self, name: str, value: int):
         self._c(_lib.glst_batch_set_option(self._h, name.encode(), int(value)))
+
+    def autotune(self, rx, ry, rz, qc, reps: int = 3) -> float:
+        """Keep the fastest schedule for these inputs; returns ms/step."""
+        t = c_double()
+        self._c(_lib.glst_batch_autotune(self._h, _ptr(rx), _ptr(ry), _ptr(rz),
+                                         _ptr(qc), reps, byref(t)))
+        return t.value
 
 
 # ---- the serving loop -------------------------------------------------------

@@ -71,9 +71,22 @@ glst_status_t glst_batch_get_ef(glst_batch_t b, double *d_fx, double *d_fy,
                                 double *d_fz, double *d_en);
 glst_status_t glst_batch_zero_ef(glst_batch_t b);
 
-/* Options: "overlap_near" (default 1) as for glst_set_option. */
+/* Options: "overlap_near" (0 or 1, default 1) as for glst_set_option;
+ * "lr_blocks" (default 4096): back-projection workgroups aimed for over the
+ * whole batch (re-packs the batch); "graph" (default 0): 1 captures the
+ * whole step as a HIP graph for the input arrays of the first step and
+ * replays it while they are passed again, 2 captures only on the second
+ * consecutive step with the same inputs. */
 glst_status_t glst_batch_set_option(glst_batch_t b, const char *name,
                                     int64_t value);
+/* Autotuner: times the step on these inputs under each "overlap_near" and
+ * "lr_blocks" candidate (`reps` steps each, one knob at a time), keeps the
+ * fastest and clears the accumulators; *best_ms (may be NULL) receives its
+ * time per step. */
+glst_status_t glst_batch_autotune(glst_batch_t b, const double *d_rx,
+                                  const double *d_ry, const double *d_rz,
+                                  const double *d_qc, uint32_t reps,
+                                  double *best_ms);
 glst_status_t glst_batch_set_stream(glst_batch_t b, void *stream);
 const char *glst_batch_last_error(glst_batch_t b);
 

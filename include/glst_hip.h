@@ -182,9 +182,27 @@ glst_status_t glst_get_rmt(glst_handle_t h, double *re, double *im);
  *   "rmt_shape" (default 0): shell sums with LDS-resident tables of two
  *   nodes per workgroup where they fit, else one (0); always one node per
  *   workgroup (1); or the global-memory tables otherwise used when one
- *   node's table does not fit in LDS (2).  Results agree to FP64 rounding. */
+ *   node's table does not fit in LDS (2).  Results agree to FP64 rounding.
+ *   "lr_blocks" (default 4096): back-projection workgroups aimed for; the
+ *   direction ranges of each alpha group are split accordingly.
+ *   "graph" (default 0): calc_ener_force captures the whole step as a HIP
+ *   graph for the input arrays of its first call and replays it while the
+ *   same arrays are passed (per-kernel timing is then not recorded).
+ * Every option leaves the results unchanged up to the FP rounding noted. */
 glst_status_t glst_set_option(glst_handle_t h, const char *name,
                               int64_t value);
+glst_status_t glst_get_option(glst_handle_t h, const char *name,
+                              int64_t *value);
+
+/* Autotuner (no counterpart in the reference): times calc_ener_force on
+ * these inputs under candidate schedules, `reps` steps each, and keeps the
+ * fastest, one knob at a time: "overlap_near" (0, 1, 2), "lr_blocks"
+ * (1024 .. 16384), "rmt_shape" (0, 1).  Clears the accumulators afterwards;
+ * *best_ms (may be NULL) receives the chosen schedule's time per step. */
+glst_status_t glst_autotune(glst_handle_t h, const double *d_rx,
+                            const double *d_ry, const double *d_rz,
+                            const double *d_qc, uint32_t reps,
+                            double *best_ms);
 
 /* Per-kernel timing with HIP events on the handle's stream.  When enabled,
  * every launch of the named kernel is bracketed by events; glst_kernel_time
