@@ -51,8 +51,18 @@ def ref_style_errors(got, ref):
     return out
 
 
-def time_path(g, torch, d, natom, ct, steps, warmup, timing):
+def time_path(g, torch, d, natom, ct, steps, warmup, timing, tune=False,
+              graph=False):
+    """Times `steps` steps after `warmup`; with `tune`, the engine's
+    autotuner first picks the schedule for this workload; with `graph`, the
+    same steps are then timed again replayed as a HIP graph (which records
+    no per-kernel events)."""
     f = g.GLSTForce(natom, TOL, BOX, rcut=RCUT, ct=ct)
+    tuned = None
+    if tune:
+        f.autotune(*d, reps=3)
+        tuned = {k: f.get_option(k)
+                 for k in ("overlap_near", "lr_blocks", "rmt_shape")}
     for _ in range(warmup):
         f.zero_ef()
         f.calc_ener_force(*d)
@@ -73,8 +83,23 @@ def time_path(g, torch, d, natom, ct, steps, warmup, timing):
             stages[s] = ms / n
     got = [t.cpu().numpy() for t in f.get_ef()]
     info = f.info()
+    graph_dt = None
+    if graph:
+        f.set_timing(False)
+        f.set_option("graph", 1)
+        for _ in range(warmup):
+            f.zero_ef()
+            f.calc_ener_force(*d)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            f.zero_ef()
+            f.calc_ener_force(*d)
+        torch.cuda.synchronize()
+        graph_dt = (time.perf_counter() - t0) / steps
+        got = [t.cpu().numpy() for t in f.get_ef()]
     f.close()
-    return dt, stages, got, info
+    return dt, stages, got, info, tuned, graph_dt
 
 
 def cpu_baseline(h, natom):
@@ -228,8 +253,12 @@ def main():
     d = [torch.as_tensor(a, dtype=torch.float64, device="cuda") for a in h]
     log(f"bench: {natom} atoms, box {BOX}, rcut {RCUT}, tol {TOL}")
 
-    dt_mixed, st_mixed, got_mixed, info = time_path(
-        g, torch, d, natom, "float", args.steps, args.warmup, True)
+    dt_ev, st_mixed, got_mixed, info, tuned, dt_graph = time_path(
+        g, torch, d, natom, "float", args.steps, args.warmup, True, tune=True,
+        graph=not args.profile)
+    # the step as run for the result: autotuned schedule, replayed as a HIP
+    # graph (per-stage times come from the eager steps with events)
+    dt_mixed = dt_graph if dt_graph else dt_ev
     log(f"mixed: {dt_mixed * 1e3:.3f} ms/step  stages {st_mixed}")
     if args.profile:
         # calibration: reads 1 GiB and writes 1 GiB
@@ -244,8 +273,10 @@ def main():
                                                 "write": 1 << 30}}),
               flush=True)
         return
-    dt_fp64, st_fp64, got_fp64, _ = time_path(
-        g, torch, d, natom, "double", args.fp64_steps, 1, True)
+    # the FP64 path gets the same autotuning (its step is too long for
+    # launch overheads to matter, so it runs eagerly)
+    dt_fp64, st_fp64, got_fp64, _, tuned64, _ = time_path(
+        g, torch, d, natom, "double", args.fp64_steps, 1, True, tune=True)
     log(f"fp64 : {dt_fp64 * 1e3:.3f} ms/step  stages {st_fp64}")
 
     ref = [t.cpu().numpy() for t in g.coulomb_direct(*d)]
@@ -318,8 +349,10 @@ def main():
         "rel_force_err_fp64": err_fp64,
         "ref_style_err": ref_style_errors(got_mixed, ref),
         "fp64_path": {"value": natom / dt_fp64, "ms_per_step": dt_fp64 * 1e3,
-                      "steps": args.fp64_steps},
+                      "steps": args.fp64_steps, "schedule": tuned64},
         "speedup_vs_fp64": dt_fp64 / dt_mixed,
+        "schedule": dict(tuned or {}, graph=True,
+                         eager_ms_per_step=dt_ev * 1e3),
         "stages_ms": st_mixed,
         "stages_ms_fp64": st_fp64,
         "roofline": roof,

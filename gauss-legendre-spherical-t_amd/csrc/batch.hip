@@ -460,7 +460,7 @@ double glst_batch::autotune(const double *rx, const double *ry,
     if (B.partial > 0) {
       const uint32_t keep = lr_blocks;
       uint32_t pick = keep;
-      for (uint32_t v : {1024u, 2048u, 4096u, 8192u, 16384u}) {
+      for (uint32_t v : {1024u, 2048u, 4096u, 8192u, 16384u, 32768u, 65536u}) {
         if (v == keep) continue;
         lr_blocks = v;
         configure(B.ct, B.cs);

@@ -514,7 +514,7 @@ struct glst_engine {
              lr_blocks = static_cast<uint32_t>(v);
              apply_lr_blocks();
            },
-           {1024, 2048, 4096, 8192, 16384});
+           {1024, 2048, 4096, 8192, 16384, 32768, 65536});
       if (!rmt_global_path())
         knob([&] { return static_cast<int64_t>(rmt_shape); },
              [&](int64_t v) {

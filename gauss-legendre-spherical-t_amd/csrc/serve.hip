@@ -118,6 +118,9 @@ struct glst_server {
       auto sl = std::make_unique<Slot>();
       // the caller's stream serves the first slot; the others get their own
       sl->batch.open(i == 0 ? stream : nullptr);
+      // a composition run twice in a row (resident systems) is captured as
+      // a HIP graph and replayed from then on
+      sl->batch.graph_mode = 2;
       GLST_HIP(hipEventCreateWithFlags(&sl->done, hipEventDisableTiming));
       slots.push_back(std::move(sl));
     }

@@ -197,7 +197,7 @@ glst_status_t glst_get_option(glst_handle_t h, const char *name,
 /* Autotuner (no counterpart in the reference): times calc_ener_force on
  * these inputs under candidate schedules, `reps` steps each, and keeps the
  * fastest, one knob at a time: "overlap_near" (0, 1, 2), "lr_blocks"
- * (1024 .. 16384), "rmt_shape" (0, 1).  Clears the accumulators afterwards;
+ * (1024 .. 65536, powers of two), "rmt_shape" (0, 1).  Clears the accumulators afterwards;
  * *best_ms (may be NULL) receives the chosen schedule's time per step. */
 glst_status_t glst_autotune(glst_handle_t h, const double *d_rx,
                             const double *d_ry, const double *d_rz,
