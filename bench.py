@@ -52,17 +52,21 @@ def ref_style_errors(got, ref):
 
 
 def time_path(g, torch, d, natom, ct, steps, warmup, timing, tune=False,
-              graph=False):
+              graph=False, schedule=None):
     """Times `steps` steps after `warmup`; with `tune`, the engine's
     autotuner first picks the schedule for this workload; with `graph`, the
     same steps are then timed again replayed as a HIP graph (which records
     no per-kernel events)."""
     f = g.GLSTForce(natom, TOL, BOX, rcut=RCUT, ct=ct)
     tuned = None
-    if tune:
+    keys = ("overlap_near", "lr_blocks", "rmt_shape")
+    if schedule is not None:  # a pinned schedule (profiling runs)
+        for k, v in zip(keys, schedule):
+            f.set_option(k, v)
+        tuned = {k: f.get_option(k) for k in keys}
+    elif tune:
         f.autotune(*d, reps=3)
-        tuned = {k: f.get_option(k)
-                 for k in ("overlap_near", "lr_blocks", "rmt_shape")}
+        tuned = {k: f.get_option(k) for k in keys}
     for _ in range(warmup):
         f.zero_ef()
         f.calc_ener_force(*d)
@@ -233,6 +237,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the served-latency measurement")
+    ap.add_argument("--schedule", default=None,
+                    help="pin the mixed path's schedule instead of "
+                         "autotuning: overlap_near,lr_blocks,rmt_shape")
     ap.add_argument("--profile", action="store_true",
                     help="mixed path only (for rocprofv3 runs), plus one "
                          "1 GiB elementwise copy used to calibrate counters")
@@ -253,9 +260,11 @@ def main():
     d = [torch.as_tensor(a, dtype=torch.float64, device="cuda") for a in h]
     log(f"bench: {natom} atoms, box {BOX}, rcut {RCUT}, tol {TOL}")
 
+    pinned = ([int(v) for v in args.schedule.split(",")]
+              if args.schedule else None)
     dt_ev, st_mixed, got_mixed, info, tuned, dt_graph = time_path(
         g, torch, d, natom, "float", args.steps, args.warmup, True, tune=True,
-        graph=not args.profile)
+        graph=not args.profile, schedule=pinned)
     # the step as run for the result: autotuned schedule, replayed as a HIP
     # graph (per-stage times come from the eager steps with events)
     dt_mixed = dt_graph if dt_graph else dt_ev
@@ -268,6 +277,7 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"profile": True, "value": natom / dt_mixed,
                           "ms_per_step": dt_mixed * 1e3,
+                          "schedule": tuned,
                           "stages_ms": st_mixed,
                           "calibration_bytes": {"read": 1 << 30,
                                                 "write": 1 << 30}}),

@@ -69,7 +69,9 @@ struct glst_engine {
   // 2 alongside the shell sums onwards
   int overlap_near = 1;
   int rmt_shape = 0;  // shell sums: 0 auto, 1 one node per workgroup, 2 global
-  uint32_t lr_blocks = 4096;  // back-projection workgroups aimed for
+  // back-projection workgroups aimed for (many short workgroups balance the
+  // per-group launches better; the autotuner may pick another value)
+  uint32_t lr_blocks = 16384;
   KernelTimer timer;
   // the whole step as a HIP graph for fixed input arrays (option "graph")
   bool use_graph = false;

@@ -183,7 +183,7 @@ glst_status_t glst_get_rmt(glst_handle_t h, double *re, double *im);
  *   nodes per workgroup where they fit, else one (0); always one node per
  *   workgroup (1); or the global-memory tables otherwise used when one
  *   node's table does not fit in LDS (2).  Results agree to FP64 rounding.
- *   "lr_blocks" (default 4096): back-projection workgroups aimed for; the
+ *   "lr_blocks" (default 16384): back-projection workgroups aimed for; the
  *   direction ranges of each alpha group are split accordingly.
  *   "graph" (default 0): calc_ener_force captures the whole step as a HIP
  *   graph for the input arrays of its first call and replays it while the
@@ -197,8 +197,9 @@ glst_status_t glst_get_option(glst_handle_t h, const char *name,
 /* Autotuner (no counterpart in the reference): times calc_ener_force on
  * these inputs under candidate schedules, `reps` steps each, and keeps the
  * fastest, one knob at a time: "overlap_near" (0, 1, 2), "lr_blocks"
- * (1024 .. 65536, powers of two), "rmt_shape" (0, 1).  Clears the accumulators afterwards;
- * *best_ms (may be NULL) receives the chosen schedule's time per step. */
+ * (powers of two from 1024 to 65536), "rmt_shape" (0, 1).  Clears the
+ * accumulators afterwards; *best_ms (may be NULL) receives the chosen
+ * schedule's time per step. */
 glst_status_t glst_autotune(glst_handle_t h, const double *d_rx,
                             const double *d_ry, const double *d_rz,
                             const double *d_qc, uint32_t reps,
