@@ -1181,9 +1181,10 @@ __global__ void lr_reduce_kernel(const double4 *__restrict__ partial,
 // floor(256/n) times: lane (i, s) takes atom i against every S-th atom
 // (phase s) of each neighbour cell, so a cell of 85 atoms keeps 255 lanes
 // busy instead of 128 lanes in two 64-atom passes (denser cells run in
-// 256-atom chunks with S = 1).  Neighbour atoms are read with vector loads,
-// several in flight per lane.  The S partial sums of an atom are added in
-// LDS in a fixed order.
+// 256-atom chunks with S = 1).  Each neighbour cell is staged through an
+// LDS tile of up to 256 atom records, loaded once by the workgroup and read
+// by every lane (the lanes of one phase read the same record); the S partial
+// sums of an atom are then added in LDS in a fixed order.
 constexpr uint32_t kSrBlock = 256;
 constexpr uint32_t kSrMaxPhase = 16;
 
@@ -1241,6 +1242,7 @@ __device__ __forceinline__ void sr_body(
     double *__restrict__ fy, double *__restrict__ fz,
     double *__restrict__ en, double4 *__restrict__ out4, uint32_t cell) {
   __shared__ double4 part[kSrBlock];
+  __shared__ double4 tile[kSrBlock];
   const uint32_t ix = cell / (ny * nz), iy = (cell / nz) % ny, iz = cell % nz;
   const uint32_t a0 = start[cell], a1 = start[cell + 1];
   const uint32_t n = a1 - a0;
@@ -1250,42 +1252,48 @@ __device__ __forceinline__ void sr_body(
   const uint32_t chunk = kSrBlock / S;  // atoms per pass
   const uint32_t tid = threadIdx.x;
   const uint32_t li = tid / S, s = tid - li * S;
-  // waves whose lanes all lie past chunk * S have nothing to do
+  // waves whose lanes all lie past chunk * S only help load the tiles
   const bool wave_busy = (tid & ~63u) < chunk * S;
   for (uint32_t base = a0; base < a1; base += chunk) {
     const uint32_t i = base + li;
     const bool valid = li < chunk && i < a1;
     const uint32_t ii = valid ? i : a0;
     double FX = 0.0, FY = 0.0, FZ = 0.0, EN = 0.0;
-    if (wave_busy) {
-      const double4 ai = atoms[ii];
-      using Q = typename std::conditional<MIXED, float, double>::type;
-      const Q qi = static_cast<Q>(ai.w);
-      for (int dx = -1; dx <= 1; ++dx) {
-        if ((ix == 0 && dx < 0) || (ix == nx - 1 && dx > 0)) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-          if ((iy == 0 && dy < 0) || (iy == ny - 1 && dy > 0)) continue;
-          for (int dz = -1; dz <= 1; ++dz) {
-            if ((iz == 0 && dz < 0) || (iz == nz - 1 && dz > 0)) continue;
-            const uint32_t nb = ((ix + dx) * ny + (iy + dy)) * nz + (iz + dz);
-            const uint32_t b1 = start[nb + 1];
-            uint32_t j = start[nb] + s;
-            // Mixed: FP32 sums over one neighbour cell, flushed to FP64.
-            PairAcc<MIXED> acc;
-            for (; j + 3 * S < b1; j += 4 * S) {
-              const double4 q0 = atoms[j], q1 = atoms[j + S],
-                            q2 = atoms[j + 2 * S], q3 = atoms[j + 3 * S];
-              acc.add(ai, qi, q0, j == ii);
-              acc.add(ai, qi, q1, j + S == ii);
-              acc.add(ai, qi, q2, j + 2 * S == ii);
-              acc.add(ai, qi, q3, j + 3 * S == ii);
+    const double4 ai = atoms[ii];
+    using Q = typename std::conditional<MIXED, float, double>::type;
+    const Q qi = static_cast<Q>(ai.w);
+    for (int dx = -1; dx <= 1; ++dx) {
+      if ((ix == 0 && dx < 0) || (ix == nx - 1 && dx > 0)) continue;
+      for (int dy = -1; dy <= 1; ++dy) {
+        if ((iy == 0 && dy < 0) || (iy == ny - 1 && dy > 0)) continue;
+        for (int dz = -1; dz <= 1; ++dz) {
+          if ((iz == 0 && dz < 0) || (iz == nz - 1 && dz > 0)) continue;
+          const uint32_t nb = ((ix + dx) * ny + (iy + dy)) * nz + (iz + dz);
+          const uint32_t b0 = start[nb], b1 = start[nb + 1];
+          // Mixed: FP32 sums over one neighbour cell, flushed to FP64.
+          PairAcc<MIXED> acc;
+          for (uint32_t t0 = b0; t0 < b1; t0 += kSrBlock) {
+            const uint32_t tn = (b1 - t0 < kSrBlock) ? b1 - t0 : kSrBlock;
+            __syncthreads();  // the previous tile has been consumed
+            if (tid < tn) tile[tid] = atoms[t0 + tid];
+            __syncthreads();
+            if (wave_busy) {
+              uint32_t j = s;
+              for (; j + 3 * S < tn; j += 4 * S) {
+                const double4 q0 = tile[j], q1 = tile[j + S],
+                              q2 = tile[j + 2 * S], q3 = tile[j + 3 * S];
+                acc.add(ai, qi, q0, t0 + j == ii);
+                acc.add(ai, qi, q1, t0 + j + S == ii);
+                acc.add(ai, qi, q2, t0 + j + 2 * S == ii);
+                acc.add(ai, qi, q3, t0 + j + 3 * S == ii);
+              }
+              for (; j < tn; j += S) acc.add(ai, qi, tile[j], t0 + j == ii);
             }
-            for (; j < b1; j += S) acc.add(ai, qi, atoms[j], j == ii);
-            FX += acc.fx;
-            FY += acc.fy;
-            FZ += acc.fz;
-            EN += acc.en;
           }
+          FX += acc.fx;
+          FY += acc.fy;
+          FZ += acc.fz;
+          EN += acc.en;
         }
       }
     }
