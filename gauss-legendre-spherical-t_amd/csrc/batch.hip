@@ -107,8 +107,8 @@ BatchLayout pack_batch(int ct,
       throw ArgError("batch exceeds 2^32 atoms or 2^31 cells");
     if (lr) {
       const size_t per_node =
-          (static_cast<size_t>(u.grid.ncell) + u.grid.nx + u.grid.ny +
-           u.grid.nz) * sizeof(double2);
+          (static_cast<size_t>(dev::rmt_table_stride(u.grid.ncell)) +
+           u.grid.nx + u.grid.ny + u.grid.nz) * sizeof(double2);
       if (per_node > lds_limit)
         throw ArgError("system " + std::to_string(s) + ": its " +
                        std::to_string(u.grid.ncell) +

@@ -400,7 +400,8 @@ struct glst_engine {
   // fit in LDS, or forced)
   bool rmt_global_path() const {
     const size_t per_node =
-        (static_cast<size_t>(grid.ncell) + grid.nx + grid.ny + grid.nz) *
+        (static_cast<size_t>(dev::rmt_table_stride(grid.ncell)) + grid.nx +
+         grid.ny + grid.nz) *
         sizeof(double2);
     return rmt_shape == 2 || per_node > lds_limit;
   }

@@ -531,7 +531,8 @@ __device__ __forceinline__ void rmt_lds_body(
   extern __shared__ double2 P[];  // [T][ncell], then [T][nx+ny+nz] phases
   const uint32_t ncell = nx * ny * nz;
   const uint32_t nax = nx + ny + nz;
-  double2 *phase = P + T * ncell;
+  const uint32_t ts = rmt_table_stride(ncell);
+  double2 *phase = P + T * ts;
   const uint32_t node0 = tile * T;
   const uint32_t tid = threadIdx.x;
   // (1) slot pair (2m, 2m+1) of cell c sits at index 2 (m ncell + c)
@@ -558,7 +559,7 @@ __device__ __forceinline__ void rmt_lds_body(
         const uint32_t c = c0 + u * kRmtBlock;
         if (c < ncell) {
           P[c] = lo[u];
-          P[ncell + c] = hi[u];
+          P[ts + c] = hi[u];
         }
       }
     }
@@ -606,7 +607,7 @@ __device__ __forceinline__ void rmt_lds_body(
       const double2 a = ph[x], b = ph[nx + y];
       const double pr = a.x * b.x - a.y * b.y, pi = a.x * b.y + a.y * b.x;
       const double2 *pz = ph + nx + ny;
-      double2 *p = P + t * ncell + xy * nz;
+      double2 *p = P + t * ts + xy * nz;
       double2 run = make_double2(0.0, 0.0);
       for (uint32_t z0 = 0; z0 < nz; z0 += kRmtBatch) {
         const uint32_t nb = (nz - z0 < kRmtBatch) ? nz - z0 : kRmtBatch;
@@ -631,12 +632,12 @@ __device__ __forceinline__ void rmt_lds_body(
   // (3) inclusive prefix along y, then x (src/glst_force.cu:726-772)
   for (uint32_t l = tid; l < T * nx * nz; l += kRmtBlock) {
     const uint32_t t = l / (nx * nz), xz = l % (nx * nz);
-    lds_scan(P + t * ncell + (xz / nz) * ny * nz + (xz % nz), ny, nz);
+    lds_scan(P + t * ts + (xz / nz) * ny * nz + (xz % nz), ny, nz);
   }
   __syncthreads();
   for (uint32_t l = tid; l < T * ny * nz; l += kRmtBlock) {
     const uint32_t t = l / (ny * nz), yz = l % (ny * nz);
-    lds_scan(P + t * ncell + yz, nx, ny * nz);
+    lds_scan(P + t * ts + yz, nx, ny * nz);
   }
   __syncthreads();
   // (4) shells; y is cut into segments so that the workgroup stays busy
@@ -654,7 +655,7 @@ __device__ __forceinline__ void rmt_lds_body(
     const uint32_t g = group[node];
     const uint32_t rin = r_in[g], rout = r_out[g];
     const double w = k64[node].w;
-    const double2 *Pt = P + t * ncell;
+    const double2 *Pt = P + t * ts;
     const CubeXZ co = cube_xz(x, z, rout, nx, ny, nz);
     const CubeXZ ci = cube_xz(x, z, rin, nx, ny, nz);
     const double2 *ph = phase + t * nax;
@@ -1713,8 +1714,9 @@ bool rmt_lds(bool mixed, uint32_t tmax, const void *S, const double4 *k64,
              const uint32_t *group, const uint32_t *r_in,
              const uint32_t *r_out, uint32_t Kp, const GridShape &g,
              size_t lds_limit, void *R, hipStream_t s) {
-  const size_t per_node = (static_cast<size_t>(g.ncell) + g.nx + g.ny + g.nz) *
-                          sizeof(double2);
+  const size_t per_node =
+      (static_cast<size_t>(rmt_table_stride(g.ncell)) + g.nx + g.ny + g.nz) *
+      sizeof(double2);
   const uint32_t T = (tmax >= 2 && 2 * per_node <= lds_limit) ? 2 : 1;
   if (T * per_node > lds_limit) return false;
   const uint32_t ntile = Kp / T;  // Kp is even

@@ -76,6 +76,15 @@ void sf(bool mixed, const void *atoms, const uint32_t *start,
         void *S, hipStream_t s);
 
 // ---- remote (shell) sums (sum_rmt_sf) --------------------------------------
+// LDS elements (double2) between the summed-volume tables of consecutive
+// nodes: ncell rounded up so that the stride is 8 elements (32 banks) off a
+// multiple of 16 elements (64 banks), so that the lanes of the two nodes of
+// a pair read different banks.  One node needs (stride + nx + ny + nz)
+// elements of LDS.
+__host__ __device__ inline uint32_t rmt_table_stride(uint32_t ncell) {
+  return ncell + ((8u - (ncell & 15u)) & 15u);
+}
+
 // R_c(k) = W_k * sum_{c' : r_in < cheb(c,c') <= r_out} S_c'(k); the mixed
 // path stores exp(+i k.center_c) R_c(k) as float2, the FP64 path R_c(k) as
 // double2, both at R[c*Kp + k].  LDS-resident tables of tmax (1 or 2) nodes
