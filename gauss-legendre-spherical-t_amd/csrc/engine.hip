@@ -292,7 +292,8 @@ struct glst_engine {
     if (!sf_done) throw StateError("sum_rmt_sf called before calc_sf");
     Scope sc(timer, "rmt", stream);
     if (rmt_shape == 2 ||
-        !dev::rmt_lds(mixed(), rmt_shape == 1 ? 1 : 2, S.as<void>(),
+        !dev::rmt_lds(mixed(), rmt_shape == 1 ? 1 : dev::kRmtMaxNodes,
+                      S.as<void>(),
                       k64.as<double4>(), group.as<uint32_t>(),
                       r_in.as<uint32_t>(), r_out.as<uint32_t>(), Kp, grid,
                       lds_limit, R.as<void>(), stream)) {
@@ -399,11 +400,9 @@ struct glst_engine {
   // shell sums through the global-memory tables (one node's table does not
   // fit in LDS, or forced)
   bool rmt_global_path() const {
-    const size_t per_node =
-        (static_cast<size_t>(dev::rmt_table_stride(grid.ncell)) + grid.nx +
-         grid.ny + grid.nz) *
-        sizeof(double2);
-    return rmt_shape == 2 || per_node > lds_limit;
+    return rmt_shape == 2 ||
+           dev::rmt_nodes_per_group(grid.ncell, grid.nx + grid.ny + grid.nz,
+                                    lds_limit, 1) == 0;
   }
 
   // New back-projection split target: splits, their table and the partials.

@@ -514,8 +514,9 @@ __device__ inline double2 cube_y(const double2 *P, const CubeXZ &c,
                       r0.y - g1.y - g2.y - g3.y + g4.y + g5.y + g6.y - g7.y);
 }
 
-// One workgroup holds the summed-volume tables of T (1 or 2) consecutive
-// nodes in LDS.  (1) It reads the nodes' S slots, one contiguous run of the
+// One workgroup holds the summed-volume tables of T consecutive nodes in LDS
+// (T = 1, or an even count: two for a 15^3 grid, up to 64 for small grids of
+// a batch).  (1) It reads the nodes' S slots, one contiguous run of the
 // node-pair tiles, converting to FP64; (2) mixed path: turns the cell-local
 // sums into S_c = exp(-i k.c) local with the FP64 axis factors and takes the
 // running sum along z (the FP64 kernel stored z-running sums already);
@@ -531,35 +532,38 @@ __device__ __forceinline__ void rmt_lds_body(
   extern __shared__ double2 P[];  // [T][ncell], then [T][nx+ny+nz] phases
   const uint32_t ncell = nx * ny * nz;
   const uint32_t nax = nx + ny + nz;
-  const uint32_t ts = rmt_table_stride(ncell);
+  const uint32_t ts = rmt_table_stride(ncell, T);
   double2 *phase = P + T * ts;
   const uint32_t node0 = tile * T;
   const uint32_t tid = threadIdx.x;
-  // (1) slot pair (2m, 2m+1) of cell c sits at index 2 (m ncell + c)
+  // (1) slot pair (2m, 2m+1) of cell c sits at index 2 (m ncell + c), so the
+  // T/2 pairs of an even T are one contiguous run
   const size_t run0 = static_cast<size_t>(node0 >> 1) * ncell;
-  if (T == 2) {
-    for (uint32_t c0 = tid; c0 < ncell; c0 += kRmtBlock * kRmtLoadBatch) {
+  if (T >= 2) {
+    const uint32_t total = (T >> 1) * ncell;
+    for (uint32_t e0 = tid; e0 < total; e0 += kRmtBlock * kRmtLoadBatch) {
       double2 lo[kRmtLoadBatch], hi[kRmtLoadBatch];
 #pragma unroll
       for (uint32_t u = 0; u < kRmtLoadBatch; ++u) {
-        const uint32_t c0u = c0 + u * kRmtBlock;
-        const uint32_t c = c0u < ncell ? c0u : ncell - 1;
+        const uint32_t eu = e0 + u * kRmtBlock;
+        const uint32_t e = eu < total ? eu : total - 1;
         if (MIXED) {
-          const float4 v = static_cast<const float4 *>(Sv)[run0 + c];
+          const float4 v = static_cast<const float4 *>(Sv)[run0 + e];
           lo[u] = make_double2(v.x, v.y);
           hi[u] = make_double2(v.z, v.w);
         } else {
-          const double2 *p = static_cast<const double2 *>(Sv) + 2 * (run0 + c);
+          const double2 *p = static_cast<const double2 *>(Sv) + 2 * (run0 + e);
           lo[u] = p[0];
           hi[u] = p[1];
         }
       }
 #pragma unroll
       for (uint32_t u = 0; u < kRmtLoadBatch; ++u) {
-        const uint32_t c = c0 + u * kRmtBlock;
-        if (c < ncell) {
-          P[c] = lo[u];
-          P[ts + c] = hi[u];
+        const uint32_t e = e0 + u * kRmtBlock;
+        if (e < total) {
+          const uint32_t m = e / ncell, c = e - m * ncell;
+          P[(2 * m) * ts + c] = lo[u];
+          P[(2 * m + 1) * ts + c] = hi[u];
         }
       }
     }
@@ -1714,13 +1718,11 @@ bool rmt_lds(bool mixed, uint32_t tmax, const void *S, const double4 *k64,
              const uint32_t *group, const uint32_t *r_in,
              const uint32_t *r_out, uint32_t Kp, const GridShape &g,
              size_t lds_limit, void *R, hipStream_t s) {
-  const size_t per_node =
-      (static_cast<size_t>(rmt_table_stride(g.ncell)) + g.nx + g.ny + g.nz) *
-      sizeof(double2);
-  const uint32_t T = (tmax >= 2 && 2 * per_node <= lds_limit) ? 2 : 1;
-  if (T * per_node > lds_limit) return false;
-  const uint32_t ntile = Kp / T;  // Kp is even
-  const size_t lds = T * per_node;
+  const uint32_t nax = g.nx + g.ny + g.nz;
+  const uint32_t T = rmt_nodes_per_group(g.ncell, nax, lds_limit, tmax);
+  if (T == 0) return false;
+  const uint32_t ntile = Kp / T;  // Kp is a multiple of 256
+  const size_t lds = rmt_lds_bytes(g.ncell, nax, T);
   if (mixed) {
     rmt_lds_kernel<true><<<ntile, kRmtBlock, lds, s>>>(
         S, k64, group, r_in, r_out, Kp, T, ntile, g.nx, g.ny, g.nz, g.h, R);

@@ -76,20 +76,31 @@ void sf(bool mixed, const void *atoms, const uint32_t *start,
         void *S, hipStream_t s);
 
 // ---- remote (shell) sums (sum_rmt_sf) --------------------------------------
-// LDS elements (double2) between the summed-volume tables of consecutive
-// nodes: ncell rounded up so that the stride is 8 elements (32 banks) off a
-// multiple of 16 elements (64 banks), so that the lanes of the two nodes of
-// a pair read different banks.  One node needs (stride + nx + ny + nz)
-// elements of LDS.
-__host__ __device__ inline uint32_t rmt_table_stride(uint32_t ncell) {
-  return ncell + ((8u - (ncell & 15u)) & 15u);
+// LDS layout of the shell-sum workgroups.  A workgroup holds the summed-
+// volume tables of T nodes (T = 1 or a power of two), T tables of
+// rmt_table_stride elements (double2) plus T * (nx+ny+nz) phase factors.
+// The stride is ncell rounded up to a residue mod 16 elements chosen for T,
+// so that the 8 lanes an LDS cycle serves (consecutive nodes, then
+// consecutive z) read 8 different 4-bank groups.
+__host__ __device__ inline uint32_t rmt_table_stride(uint32_t ncell,
+                                                     uint32_t T) {
+  const uint32_t r =
+      T >= 8 ? 1u : (T == 4 ? 2u : (T == 2 ? 8u : (ncell & 15u)));
+  return ncell + ((r - (ncell & 15u)) & 15u);
 }
+inline size_t rmt_lds_bytes(uint32_t ncell, uint32_t nax, uint32_t T) {
+  return static_cast<size_t>(T) * (rmt_table_stride(ncell, T) + nax) * 16;
+}
+// Largest T <= tmax (a power of two) whose tables fit lds_limit bytes; 0 when
+// not even one node's table fits.
+inline uint32_t rmt_nodes_per_group(uint32_t ncell, uint32_t nax,
+                                    size_t lds_limit, uint32_t tmax) {
+  for (uint32_t T = tmax; T >= 2; T >>= 1)
+    if (rmt_lds_bytes(ncell, nax, T) <= lds_limit) return T;
+  return rmt_lds_bytes(ncell, nax, 1) <= lds_limit ? 1u : 0u;
+}
+constexpr uint32_t kRmtMaxNodes = 64;
 
-// R_c(k) = W_k * sum_{c' : r_in < cheb(c,c') <= r_out} S_c'(k); the mixed
-// path stores exp(+i k.center_c) R_c(k) as float2, the FP64 path R_c(k) as
-// double2, both at R[c*Kp + k].  LDS-resident tables of tmax (1 or 2) nodes
-// per workgroup; returns false when the table of one node does not fit in
-// lds_limit bytes (the caller then uses rmt_global).
 bool rmt_lds(bool mixed, uint32_t tmax, const void *S, const double4 *k64,
              const uint32_t *group, const uint32_t *r_in,
              const uint32_t *r_out, uint32_t Kp, const GridShape &g,
