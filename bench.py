@@ -59,7 +59,7 @@ def time_path(g, torch, d, natom, ct, steps, warmup, timing, tune=False,
     no per-kernel events)."""
     f = g.GLSTForce(natom, TOL, BOX, rcut=RCUT, ct=ct)
     tuned = None
-    keys = ("overlap_near", "lr_blocks", "rmt_shape")
+    keys = ("overlap_near", "lr_blocks", "rmt_shape", "pipeline")
     if schedule is not None:  # a pinned schedule (profiling runs)
         for k, v in zip(keys, schedule):
             f.set_option(k, v)
@@ -239,7 +239,8 @@ def main():
                     help="skip the served-latency measurement")
     ap.add_argument("--schedule", default=None,
                     help="pin the mixed path's schedule instead of "
-                         "autotuning: overlap_near,lr_blocks,rmt_shape")
+                         "autotuning: overlap_near,lr_blocks,rmt_shape"
+                         "[,pipeline]")
     ap.add_argument("--profile", action="store_true",
                     help="mixed path only (for rocprofv3 runs), plus one "
                          "1 GiB elementwise copy used to calibrate counters")

@@ -106,15 +106,15 @@ BatchLayout pack_batch(int ct,
     if (B.natom > 0xFFFFFFFFull || B.ncell > 0x7FFFFFFFull)
       throw ArgError("batch exceeds 2^32 atoms or 2^31 cells");
     if (lr) {
-      const uint32_t nax = u.grid.nx + u.grid.ny + u.grid.nz;
-      d.T = dev::rmt_nodes_per_group(u.grid.ncell, nax, lds_limit,
-                                     dev::kRmtMaxNodes);
+      d.T = dev::rmt_nodes_per_group(u.grid.nx, u.grid.ny, u.grid.nz,
+                                     lds_limit, dev::kRmtMaxNodes);
       if (d.T == 0)
         throw ArgError("system " + std::to_string(s) + ": its " +
                        std::to_string(u.grid.ncell) +
                        "-cell grid is too large for the batched engine; "
                        "use a single-system handle");
-      B.lds = std::max(B.lds, dev::rmt_lds_bytes(u.grid.ncell, nax, d.T));
+      B.lds = std::max(B.lds, dev::rmt_lds_bytes(u.grid.nx, u.grid.ny,
+                                                 u.grid.nz, d.T));
       const NodeLayout &L = cs[s]->nl();
       SplitPlan &P = splits[s];
       P = make_splits(u.nodes, u.K, target);

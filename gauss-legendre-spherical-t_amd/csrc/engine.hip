@@ -69,6 +69,11 @@ struct glst_engine {
   // 2 alongside the shell sums onwards
   int overlap_near = 1;
   int rmt_shape = 0;  // shell sums: 0 auto, 1 one node per workgroup, 2 global
+  // mixed path, tables in LDS: pipeline S(k) -> shell sums ->
+  // back-projection across alpha groups, the shell sums on a third stream
+  int pipeline = 1;
+  hipStream_t rmt_stream = nullptr;
+  std::vector<hipEvent_t> ev_sf, ev_rmt;  // per alpha group
   // back-projection workgroups aimed for (many short workgroups balance the
   // per-group launches better; the autotuner may pick another value)
   uint32_t lr_blocks = 16384;
@@ -99,6 +104,9 @@ struct glst_engine {
   ~glst_engine() {
     drop_graph();
     if (side) (void)hipStreamDestroy(side);
+    if (rmt_stream) (void)hipStreamDestroy(rmt_stream);
+    for (hipEvent_t e : ev_sf) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ev_rmt) (void)hipEventDestroy(e);
     if (ev_assigned) (void)hipEventDestroy(ev_assigned);
     if (ev_near) (void)hipEventDestroy(ev_near);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -157,6 +165,13 @@ struct glst_engine {
     GLST_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
     GLST_HIP(hipEventCreateWithFlags(&ev_assigned, hipEventDisableTiming));
     GLST_HIP(hipEventCreateWithFlags(&ev_near, hipEventDisableTiming));
+    GLST_HIP(hipStreamCreateWithFlags(&rmt_stream, hipStreamNonBlocking));
+    for (size_t g = 0; g < nodes.dirs.size(); ++g) {
+      ev_sf.push_back(nullptr);
+      GLST_HIP(hipEventCreateWithFlags(&ev_sf.back(), hipEventDisableTiming));
+      ev_rmt.push_back(nullptr);
+      GLST_HIP(hipEventCreateWithFlags(&ev_rmt.back(), hipEventDisableTiming));
+    }
     allocate();
     upload_nodes();
     dev::fill_f64(fx.as<double>(), 0.0, natom, stream);
@@ -354,6 +369,102 @@ struct glst_engine {
 
   void comm_ef() { GLST_HIP(hipStreamSynchronize(stream)); }
 
+  // ---- the far field of the fused step, pipelined across alpha groups ----
+  //
+  // Alpha group g owns device nodes [first_g, first_g + count_g).  Shell-sum
+  // chunk g is the workgroup tiles [tile_end[g-1], tile_end[g]) of T nodes:
+  // tile_end[g] = floor((first_g + count_g) / T), so chunk g holds nodes of
+  // groups <= g only and may start once S(k) of group g is complete; a tile
+  // that straddles groups g and g+1 belongs to chunk g+1, and the
+  // back-projection of group g then waits for chunk rmt_need[g] = g+1.  The
+  // LDS-bound shell sums of one group so run alongside the VALU-bound S(k)
+  // of later groups and back-projection of earlier ones.
+  uint32_t pipe_tmax = 0;
+  std::vector<uint32_t> tile_end, rmt_need;
+  bool pipelined() const {
+    return pipeline && mixed() && has_lr && Kp > 0 && !sf_groups.empty() &&
+           ev_sf.size() == sf_groups.size() &&
+           lr_groups.size() == sf_groups.size() && !rmt_global_path();
+  }
+  void plan_pipeline() {
+    const size_t G = sf_groups.size();
+    pipe_tmax = rmt_shape == 1 ? 1 : dev::kRmtMaxNodes;
+    const uint32_t T = dev::rmt_nodes_per_group(grid.nx, grid.ny, grid.nz,
+                                                lds_limit, pipe_tmax);
+    if (T == 0) throw StateError("shell-sum tables do not fit in LDS");
+    const uint32_t ntile = Kp / T;
+    tile_end.assign(G, ntile);
+    rmt_need.assign(G, static_cast<uint32_t>(G - 1));
+    for (size_t g = 0; g + 1 < G; ++g) {
+      const uint32_t end = nodes.first[g] + nodes.count[g];
+      tile_end[g] = std::min(end / T, ntile);
+      if (g > 0) tile_end[g] = std::max(tile_end[g], tile_end[g - 1]);
+    }
+    for (size_t g = 0; g < G; ++g) {
+      const uint64_t end = nodes.first[g] + nodes.count[g];
+      size_t j = g;
+      while (j + 1 < G && static_cast<uint64_t>(tile_end[j]) * T < end) ++j;
+      rmt_need[g] = static_cast<uint32_t>(j);
+    }
+  }
+  void sf_stage() {
+    if (!pipelined()) {
+      calc_sf();
+      return;
+    }
+    plan_pipeline();
+    {
+      Scope sc(timer, "sf", stream);
+      const dev::LrTaskArgs ta = task_args();
+      for (size_t g = 0; g < sf_groups.size(); ++g) {
+        dev::sf(true, lr_atoms(), cell_start.as<uint32_t>(), k64.as<double4>(),
+                ta, &sf_groups[g], 1, Kp, grid, S.as<void>(), stream);
+        GLST_HIP(hipEventRecord(ev_sf[g], stream));
+      }
+      check_launch("sf");
+    }
+    sf_done = true;
+    rmt_done = false;
+  }
+  void rmt_stage() {
+    if (!pipelined()) {
+      sum_rmt_sf();
+      return;
+    }
+    GLST_HIP(hipStreamWaitEvent(rmt_stream, ev_sf[0], 0));
+    {
+      Scope sc(timer, "rmt", rmt_stream);
+      for (size_t g = 0; g < sf_groups.size(); ++g) {
+        if (g > 0) GLST_HIP(hipStreamWaitEvent(rmt_stream, ev_sf[g], 0));
+        if (!dev::rmt_lds(true, pipe_tmax, S.as<void>(), k64.as<double4>(),
+                          group.as<uint32_t>(), r_in.as<uint32_t>(),
+                          r_out.as<uint32_t>(), Kp, grid, lds_limit,
+                          R.as<void>(), rmt_stream, g > 0 ? tile_end[g - 1] : 0,
+                          tile_end[g]))
+          throw StateError("shell-sum tables do not fit in LDS");
+        GLST_HIP(hipEventRecord(ev_rmt[g], rmt_stream));
+      }
+      check_launch("rmt");
+    }
+    rmt_done = true;
+  }
+  void lr_stage() {
+    if (!(has_lr && Kp > 0)) return;
+    if (!pipelined()) {
+      lr_partials();
+      return;
+    }
+    Scope sc(timer, "lr", stream);
+    const dev::LrTaskArgs ta = task_args();
+    for (size_t g = 0; g < lr_groups.size(); ++g) {
+      GLST_HIP(hipStreamWaitEvent(stream, ev_rmt[rmt_need[g]], 0));
+      dev::lr(true, lr_atoms(), cell_sorted.as<uint32_t>(), ta, &lr_groups[g],
+              1, R.as<void>(), Kp, natom, nsplit, partial.as<double4>(),
+              stream);
+    }
+    check_launch("lr");
+  }
+
   // The fused step: the near field (which needs only the sorted atoms) runs
   // on the side stream while the main stream does S(k), the shell sums and
   // the back-projection; one fixed-order reduction adds both parts.
@@ -362,9 +473,12 @@ struct glst_engine {
     assign_atoms(rx, ry, rz, qc);
     if (natom == 0) return;
     if (!overlap_near) {
-      calc_sf();
-      sum_rmt_sf();
-      calc_lr_ef();
+      sf_stage();
+      rmt_stage();
+      if (has_lr && Kp > 0) {
+        lr_stage();
+        reduce(false);
+      }
       calc_sr_ef();
       return;
     }
@@ -384,10 +498,10 @@ struct glst_engine {
     // through the whole far field; started after S(k) (mode 2) it delays
     // the LDS-heavy shell-sum workgroups and ends up slower.
     if (overlap_near == 1 || !(has_lr && Kp > 0)) launch_near();
-    calc_sf();
+    sf_stage();
     if (overlap_near == 2 && has_lr && Kp > 0) launch_near();
-    sum_rmt_sf();
-    if (has_lr && Kp > 0) lr_partials();
+    rmt_stage();
+    lr_stage();
     GLST_HIP(hipStreamWaitEvent(stream, ev_near, 0));
     reduce(true);
   }
@@ -401,8 +515,8 @@ struct glst_engine {
   // fit in LDS, or forced)
   bool rmt_global_path() const {
     return rmt_shape == 2 ||
-           dev::rmt_nodes_per_group(grid.ncell, grid.nx + grid.ny + grid.nz,
-                                    lds_limit, 1) == 0;
+           dev::rmt_nodes_per_group(grid.nx, grid.ny, grid.nz, lds_limit,
+                                    1) == 0;
   }
 
   // New back-projection split target: splits, their table and the partials.
@@ -504,6 +618,13 @@ struct glst_engine {
       }
       if (get() != pick) set(pick);
     };
+    if (mixed() && has_lr && Kp > 0 && !rmt_global_path())
+      knob([&] { return static_cast<int64_t>(pipeline); },
+           [&](int64_t v) {
+             pipeline = static_cast<int>(v);
+             drop_graph();
+           },
+           {0, 1});
     knob([&] { return static_cast<int64_t>(overlap_near); },
          [&](int64_t v) {
            overlap_near = static_cast<int>(v);
@@ -956,6 +1077,9 @@ glst_status_t glst_set_option(glst_handle_t h, const char *name,
     else if (n == "rmt_shape") {
       if (value < 0 || value > 2) throw ArgError("rmt_shape must be 0, 1 or 2");
       h->rmt_shape = static_cast<int>(value);
+    } else if (n == "pipeline") {
+      if (value < 0 || value > 1) throw ArgError("pipeline must be 0 or 1");
+      h->pipeline = static_cast<int>(value);
     } else if (n == "lr_blocks") {
       if (value < 1 || value > (1 << 24))
         throw ArgError("lr_blocks must be in 1 .. 2^24");
@@ -979,6 +1103,8 @@ glst_status_t glst_get_option(glst_handle_t h, const char *name,
       *value = h->overlap_near;
     else if (n == "rmt_shape")
       *value = h->rmt_shape;
+    else if (n == "pipeline")
+      *value = h->pipeline;
     else if (n == "lr_blocks")
       *value = h->lr_blocks;
     else if (n == "graph")

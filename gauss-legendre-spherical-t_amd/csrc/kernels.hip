@@ -445,6 +445,15 @@ constexpr uint32_t kRmtBlock = 1024;
 constexpr uint32_t kRmtBatch = 16;
 constexpr uint32_t kRmtLoadBatch = 4;
 
+// n / d for n < 2^22 and 0 < d, with rd = 1.0f / d: the float quotient is
+// within one of the exact one, and one correction step fixes it.
+__device__ inline uint32_t fast_div(uint32_t n, uint32_t d, float rd) {
+  uint32_t q = static_cast<uint32_t>(static_cast<float>(n) * rd);
+  const int32_t r = static_cast<int32_t>(n - q * d);
+  q = r < 0 ? q - 1 : (r >= static_cast<int32_t>(d) ? q + 1 : q);
+  return q;
+}
+
 // Inclusive running sum along one line of the LDS table (n elements, stride
 // elements apart), in batches of kRmtBatch.
 __device__ inline void lds_scan(double2 *p, uint32_t n, uint32_t stride) {
@@ -465,38 +474,37 @@ __device__ inline void lds_scan(double2 *p, uint32_t n, uint32_t stride) {
   }
 }
 
-// The x/z part of a cube of Chebyshev radius r around (x, ., z), clamped to
-// the grid: table offsets of the x-planes x1 and x0-1, the z-rows z1 and
-// z0-1, and whether x0 / z0 touch the grid's low face.
+// Bordered tables.  One node's summed-volume table in LDS carries a zero
+// plane before the first cell of every axis: cell (x, y, z) sits at
+// ((x+1) (ny+1) + (y+1)) Zp + (z+1), Zp = rmt_zpitch(nz) (odd, so that
+// neighbouring z-lines fall on different banks).  The inclusion-exclusion
+// corner before x0 of a box starting at x0 = 0 then reads the zero plane
+// instead of being selected away: every corner is one LDS read, unmasked,
+// and the sum keeps box_sum's term order (subtracting an exact zero).
+
+// Table offsets of a cube of Chebyshev radius r around (x, ., z), clamped to
+// the grid: x planes and z indices of its upper corner (bx1, z1) and of the
+// planes just before it (bxm, zm).
 struct CubeXZ {
   uint32_t bx1, bxm, z1, zm;
-  bool xb, zb;
 };
 
 __device__ inline CubeXZ cube_xz(uint32_t x, uint32_t z, uint32_t r,
-                                 uint32_t nx, uint32_t ny, uint32_t nz) {
-  const uint32_t x0 = x < r ? 0 : x - r, z0 = z < r ? 0 : z - r;
-  const uint32_t x1 = (x + r >= nx) ? nx - 1 : x + r;
+                                 uint32_t nx, uint32_t nz, uint32_t plane) {
   CubeXZ c;
-  c.xb = x0 == 0;
-  c.zb = z0 == 0;
-  c.bx1 = x1 * ny * nz;
-  c.bxm = (c.xb ? 0 : x0 - 1) * ny * nz;
-  c.z1 = (z + r >= nz) ? nz - 1 : z + r;
-  c.zm = c.zb ? 0 : z0 - 1;
+  c.bx1 = ((x + r >= nx) ? nx : x + r + 1) * plane;
+  c.bxm = (x < r ? 0 : x - r) * plane;
+  c.z1 = (z + r >= nz) ? nz : z + r + 1;
+  c.zm = z < r ? 0 : z - r;
   return c;
 }
 
-// Cube sum at row y from the summed-volume table P: the inclusion-exclusion
-// of box_sum with the same term order; every corner is read (all offsets lie
-// inside the table) and those outside the grid are selected away.
+// Cube sum at row y from the bordered table P (box_sum's term order).
 __device__ inline double2 cube_y(const double2 *P, const CubeXZ &c,
                                  uint32_t y, uint32_t r, uint32_t ny,
-                                 uint32_t nz) {
-  const uint32_t y0 = y < r ? 0 : y - r;
-  const uint32_t y1 = (y + r >= ny) ? ny - 1 : y + r;
-  const bool yb = y0 == 0;
-  const uint32_t o1 = y1 * nz, om = (yb ? 0 : y0 - 1) * nz;
+                                 uint32_t zp) {
+  const uint32_t o1 = ((y + r >= ny) ? ny : y + r + 1) * zp;
+  const uint32_t om = (y < r ? 0 : y - r) * zp;
   const double2 r0 = P[c.bx1 + o1 + c.z1];  // (x1, y1, z1)
   const double2 r1 = P[c.bxm + o1 + c.z1];  // (x0-1, y1, z1)
   const double2 r2 = P[c.bx1 + om + c.z1];  // (x1, y0-1, z1)
@@ -505,23 +513,23 @@ __device__ inline double2 cube_y(const double2 *P, const CubeXZ &c,
   const double2 r5 = P[c.bxm + o1 + c.zm];  // (x0-1, y1, z0-1)
   const double2 r6 = P[c.bx1 + om + c.zm];  // (x1, y0-1, z0-1)
   const double2 r7 = P[c.bxm + om + c.zm];  // (x0-1, y0-1, z0-1)
-  const double2 zero = make_double2(0.0, 0.0);
-  const double2 g1 = c.xb ? zero : r1, g2 = yb ? zero : r2,
-                g3 = c.zb ? zero : r3, g4 = (c.xb || yb) ? zero : r4,
-                g5 = (c.xb || c.zb) ? zero : r5, g6 = (yb || c.zb) ? zero : r6,
-                g7 = (c.xb || yb || c.zb) ? zero : r7;
-  return make_double2(r0.x - g1.x - g2.x - g3.x + g4.x + g5.x + g6.x - g7.x,
-                      r0.y - g1.y - g2.y - g3.y + g4.y + g5.y + g6.y - g7.y);
+  return make_double2(r0.x - r1.x - r2.x - r3.x + r4.x + r5.x + r6.x - r7.x,
+                      r0.y - r1.y - r2.y - r3.y + r4.y + r5.y + r6.y - r7.y);
 }
 
-// One workgroup holds the summed-volume tables of T consecutive nodes in LDS
-// (T = 1, or an even count: two for a 15^3 grid, up to 64 for small grids of
-// a batch).  (1) It reads the nodes' S slots, one contiguous run of the
-// node-pair tiles, converting to FP64; (2) mixed path: turns the cell-local
-// sums into S_c = exp(-i k.c) local with the FP64 axis factors and takes the
-// running sum along z (the FP64 kernel stored z-running sums already);
-// (3) completes the table along y and x; (4) gathers the shells, lanes on
-// consecutive (node, z), every thread walking y.
+// One workgroup holds the bordered summed-volume tables of T consecutive
+// nodes in LDS (T = 1, or a power of two up to 64: two for a 15^3 grid, up
+// to 64 for the small grids of a batch).  (1) It reads the nodes' S slots,
+// one contiguous run of the node-pair tiles, converting to FP64, into the
+// bordered tables; writes the zero borders; mixed path: the FP64 axis
+// factors of the cell centres.  (2) Mixed path: turns the cell-local sums
+// into S_c = exp(-i k.c) local and takes the running sum along z (the FP64
+// slots hold z-running sums already).  (3) Running sums along y, then x.
+// (4) Shells: lanes on consecutive (node, z), every thread walking y; the
+// mixed path applies w exp(+i k.c) with the node weight folded into the
+// thread's x and z factors.  In (2)-(4) lanes run over the nodes first,
+// then over a line coordinate padded to a multiple of rmt_line_group(T),
+// which with rmt_table_stride keeps every LDS cycle free of bank conflicts.
 template <bool MIXED>
 __device__ __forceinline__ void rmt_lds_body(
     const void *__restrict__ Sv, const double4 *__restrict__ k64,
@@ -529,18 +537,30 @@ __device__ __forceinline__ void rmt_lds_body(
     const uint32_t *__restrict__ r_out, uint32_t Kp, uint32_t T,
     uint32_t nx, uint32_t ny, uint32_t nz, double h, void *__restrict__ R,
     uint32_t tile) {
-  extern __shared__ double2 P[];  // [T][ncell], then [T][nx+ny+nz] phases
+  extern __shared__ double2 P[];  // [T][ts], then [T][nx+ny+nz] phases
   const uint32_t ncell = nx * ny * nz;
   const uint32_t nax = nx + ny + nz;
-  const uint32_t ts = rmt_table_stride(ncell, T);
+  const uint32_t zp = rmt_zpitch(nz), plane = (ny + 1) * zp;
+  const uint32_t ts = rmt_table_stride(rmt_table_elems(nx, ny, nz), T);
+  const uint32_t lgT = 31u - __builtin_clz(T);
   double2 *phase = P + T * ts;
   const uint32_t node0 = tile * T;
   const uint32_t tid = threadIdx.x;
+  const double2 zero = make_double2(0.0, 0.0);
+  // bordered table index of cell c
+  const float rnz = 1.0f / static_cast<float>(nz);
+  const float rny = 1.0f / static_cast<float>(ny);
+  auto cell_index = [&](uint32_t c) {
+    const uint32_t line = fast_div(c, nz, rnz), z = c - line * nz;
+    const uint32_t x = fast_div(line, ny, rny), y = line - x * ny;
+    return (x + 1) * plane + (y + 1) * zp + z + 1;
+  };
   // (1) slot pair (2m, 2m+1) of cell c sits at index 2 (m ncell + c), so the
-  // T/2 pairs of an even T are one contiguous run
+  // T/2 pairs of an even T are one contiguous run, read coalesced
   const size_t run0 = static_cast<size_t>(node0 >> 1) * ncell;
   if (T >= 2) {
     const uint32_t total = (T >> 1) * ncell;
+    const float rnc = 1.0f / static_cast<float>(ncell);
     for (uint32_t e0 = tid; e0 < total; e0 += kRmtBlock * kRmtLoadBatch) {
       double2 lo[kRmtLoadBatch], hi[kRmtLoadBatch];
 #pragma unroll
@@ -561,9 +581,10 @@ __device__ __forceinline__ void rmt_lds_body(
       for (uint32_t u = 0; u < kRmtLoadBatch; ++u) {
         const uint32_t e = e0 + u * kRmtBlock;
         if (e < total) {
-          const uint32_t m = e / ncell, c = e - m * ncell;
-          P[(2 * m) * ts + c] = lo[u];
-          P[(2 * m + 1) * ts + c] = hi[u];
+          const uint32_t m = fast_div(e, ncell, rnc), c = e - m * ncell;
+          const uint32_t i = cell_index(c);
+          P[(2 * m) * ts + i] = lo[u];
+          P[(2 * m + 1) * ts + i] = hi[u];
         }
       }
     }
@@ -585,7 +606,7 @@ __device__ __forceinline__ void rmt_lds_body(
 #pragma unroll
       for (uint32_t u = 0; u < kRmtLoadBatch; ++u) {
         const uint32_t c = c0 + u * kRmtBlock;
-        if (c < ncell) P[c] = v[u];
+        if (c < ncell) P[cell_index(c)] = v[u];
       }
     }
   }
@@ -601,18 +622,37 @@ __device__ __forceinline__ void rmt_lds_body(
         axis_phase(phase + e, k.z, h, a - nx - ny);
     }
   }
+  const uint32_t nborder = plane + nx * zp + nx * ny;
+  for (uint32_t e = tid; e < T * nborder; e += kRmtBlock) {
+    const uint32_t t = e / nborder;
+    uint32_t b = e - t * nborder, idx;
+    if (b < plane) {
+      idx = b;  // the x plane before the grid
+    } else if ((b -= plane) < nx * zp) {
+      idx = (1 + b / zp) * plane + b % zp;  // the y row before the grid
+    } else {
+      b -= nx * zp;
+      idx = (1 + b / ny) * plane + (1 + b % ny) * zp;  // the z slot before it
+    }
+    P[t * ts + idx] = zero;
+  }
   __syncthreads();
+  const uint32_t lg = rmt_line_group(T);
+  const uint32_t ypad = (ny + lg - 1) / lg * lg, zpad = (nz + lg - 1) / lg * lg;
+  const float rypad = 1.0f / static_cast<float>(ypad);
+  const float rzpad = 1.0f / static_cast<float>(zpad);
   if (MIXED) {
-    // (2) z-lines, neighbouring lanes on neighbouring lines
-    for (uint32_t l = tid; l < T * nx * ny; l += kRmtBlock) {
-      const uint32_t t = l / (nx * ny), xy = l - t * (nx * ny);
-      const uint32_t x = xy / ny, y = xy - x * ny;
+    // (2) z-lines
+    for (uint32_t l = tid; l < T * nx * ypad; l += kRmtBlock) {
+      const uint32_t t = l & (T - 1), q = l >> lgT;
+      const uint32_t x = fast_div(q, ypad, rypad), y = q - x * ypad;
+      if (y >= ny) continue;
       const double2 *ph = phase + t * nax;
       const double2 a = ph[x], b = ph[nx + y];
       const double pr = a.x * b.x - a.y * b.y, pi = a.x * b.y + a.y * b.x;
       const double2 *pz = ph + nx + ny;
-      double2 *p = P + t * ts + xy * nz;
-      double2 run = make_double2(0.0, 0.0);
+      double2 *p = P + t * ts + (x + 1) * plane + (y + 1) * zp + 1;
+      double2 run = zero;
       for (uint32_t z0 = 0; z0 < nz; z0 += kRmtBatch) {
         const uint32_t nb = (nz - z0 < kRmtBatch) ? nz - z0 : kRmtBatch;
         double2 v[kRmtBatch];
@@ -634,50 +674,56 @@ __device__ __forceinline__ void rmt_lds_body(
     __syncthreads();
   }
   // (3) inclusive prefix along y, then x (src/glst_force.cu:726-772)
-  for (uint32_t l = tid; l < T * nx * nz; l += kRmtBlock) {
-    const uint32_t t = l / (nx * nz), xz = l % (nx * nz);
-    lds_scan(P + t * ts + (xz / nz) * ny * nz + (xz % nz), ny, nz);
+  for (uint32_t l = tid; l < T * nx * zpad; l += kRmtBlock) {
+    const uint32_t t = l & (T - 1), q = l >> lgT;
+    const uint32_t x = fast_div(q, zpad, rzpad), z = q - x * zpad;
+    if (z < nz) lds_scan(P + t * ts + (x + 1) * plane + zp + (z + 1), ny, zp);
   }
   __syncthreads();
-  for (uint32_t l = tid; l < T * ny * nz; l += kRmtBlock) {
-    const uint32_t t = l / (ny * nz), yz = l % (ny * nz);
-    lds_scan(P + t * ts + yz, nx, ny * nz);
+  for (uint32_t l = tid; l < T * ny * zpad; l += kRmtBlock) {
+    const uint32_t t = l & (T - 1), q = l >> lgT;
+    const uint32_t y = fast_div(q, zpad, rzpad), z = q - y * zpad;
+    if (z < nz) lds_scan(P + t * ts + plane + (y + 1) * zp + (z + 1), nx, plane);
   }
   __syncthreads();
   // (4) shells; y is cut into segments so that the workgroup stays busy
-  const uint32_t lines = T * nx * nz;
-  uint32_t nseg = kRmtBlock / lines;
+  const uint32_t glines = T * nx * zpad;
+  uint32_t nseg = kRmtBlock / glines;
   nseg = nseg < 1 ? 1 : (nseg > ny ? ny : nseg);
   const uint32_t ylen = (ny + nseg - 1) / nseg;
-  for (uint32_t task = tid; task < lines * nseg; task += kRmtBlock) {
-    const uint32_t t = task % T;
-    uint32_t q = task / T;
-    const uint32_t z = q % nz;
-    q /= nz;
-    const uint32_t x = q % nx, seg = q / nx;
+  for (uint32_t task = tid; task < glines * nseg; task += kRmtBlock) {
+    const uint32_t t = task & (T - 1);
+    const uint32_t q = task >> lgT;
+    const uint32_t xs = fast_div(q, zpad, rzpad), z = q - xs * zpad;
+    if (z >= nz) continue;
+    const uint32_t x = xs % nx, seg = xs / nx;
     const uint32_t node = node0 + t;
     const uint32_t g = group[node];
     const uint32_t rin = r_in[g], rout = r_out[g];
     const double w = k64[node].w;
     const double2 *Pt = P + t * ts;
-    const CubeXZ co = cube_xz(x, z, rout, nx, ny, nz);
-    const CubeXZ ci = cube_xz(x, z, rin, nx, ny, nz);
+    const CubeXZ co = cube_xz(x, z, rout, nx, nz, plane);
+    const CubeXZ ci = cube_xz(x, z, rin, nx, nz, plane);
     const double2 *ph = phase + t * nax;
+    double qr = 0.0, qi = 0.0;
+    if (MIXED) {  // w exp(+i k_x c_x) exp(+i k_z c_z)
+      const double2 a = ph[x], c = ph[nx + ny + z];
+      qr = (a.x * c.x - a.y * c.y) * w;
+      qi = (a.x * c.y + a.y * c.x) * w;
+    }
     const uint32_t yb = seg * ylen;
     const uint32_t ye = (yb + ylen < ny) ? yb + ylen : ny;
     for (uint32_t y = yb; y < ye; ++y) {
-      const double2 o = cube_y(Pt, co, y, rout, ny, nz);
-      const double2 in = cube_y(Pt, ci, y, rin, ny, nz);
-      const double re = (o.x - in.x) * w, im = (o.y - in.y) * w;
+      const double2 o = cube_y(Pt, co, y, rout, ny, zp);
+      const double2 in = cube_y(Pt, ci, y, rin, ny, zp);
+      const double dr = o.x - in.x, di = o.y - in.y;
       const size_t off = static_cast<size_t>((x * ny + y) * nz + z) * Kp + node;
       if (MIXED) {
-        // exp(+i k.c) R_c(k)
-        const double2 a = ph[x], b = ph[nx + y], c = ph[nx + ny + z];
-        const double pr = a.x * b.x - a.y * b.y, pi = a.x * b.y + a.y * b.x;
-        const double er = pr * c.x - pi * c.y, ei = pr * c.y + pi * c.x;
-        store_rmt_mixed(R, off, re * er - im * ei, re * ei + im * er);
+        const double2 b = ph[nx + y];
+        const double er = qr * b.x - qi * b.y, ei = qr * b.y + qi * b.x;
+        store_rmt_mixed(R, off, dr * er - di * ei, dr * ei + di * er);
       } else {
-        static_cast<double2 *>(R)[off] = make_double2(re, im);
+        static_cast<double2 *>(R)[off] = make_double2(dr * w, di * w);
       }
     }
   }
@@ -688,10 +734,10 @@ __global__ __launch_bounds__(kRmtBlock) void rmt_lds_kernel(
     const void *__restrict__ Sv, const double4 *__restrict__ k64,
     const uint32_t *__restrict__ group, const uint32_t *__restrict__ r_in,
     const uint32_t *__restrict__ r_out, uint32_t Kp, uint32_t T,
-    uint32_t ntile, uint32_t nx, uint32_t ny, uint32_t nz, double h,
-    void *__restrict__ R) {
+    uint32_t tile0, uint32_t ntile, uint32_t nx, uint32_t ny, uint32_t nz,
+    double h, void *__restrict__ R) {
   rmt_lds_body<MIXED>(Sv, k64, group, r_in, r_out, Kp, T, nx, ny, nz, h, R,
-                      xcd_remap(blockIdx.x, ntile));
+                      tile0 + xcd_remap(blockIdx.x, ntile));
 }
 
 // ---- global-memory tables (grids too large for LDS) ----
@@ -1692,9 +1738,9 @@ void bin_scatter(bool mixed, const double *rx, const double *ry,
 }
 
 void sf(bool mixed, const void *atoms, const uint32_t *start,
-        const double4 *k64, const LrTaskArgs &ta,
-        const std::vector<SfGroup> &groups, uint32_t Kp, const GridShape &g,
-        void *S, hipStream_t s) {
+        const double4 *k64, const LrTaskArgs &ta, const SfGroup *groups,
+        size_t ngroup, uint32_t Kp, const GridShape &g, void *S,
+        hipStream_t s) {
   if (!mixed) {
     const dim3 grid(g.nx * g.ny, Kp / kSfBlock);
     sf_f64_kernel<<<grid, kSfBlock, 0, s>>>(static_cast<const double4 *>(atoms),
@@ -1704,7 +1750,8 @@ void sf(bool mixed, const void *atoms, const uint32_t *start,
   }
   const LrTasks tk{ta.node0, ta.nr,      ta.koff,   ta.dir64,
                    ta.dir32, ta.kappa64, ta.kappa32, ta.split};
-  for (const SfGroup &gr : groups) {
+  for (size_t q = 0; q < ngroup; ++q) {
+    const SfGroup &gr = groups[q];
     if (gr.ntask == 0) continue;
     const dim3 grid(g.ncell, (gr.ntask + kSfDirs - 1) / kSfDirs);
     kSfNrKernels[gr.nr <= static_cast<uint32_t>(kSfMaxUnrolled) ? gr.nr : 0]
@@ -1717,18 +1764,23 @@ void sf(bool mixed, const void *atoms, const uint32_t *start,
 bool rmt_lds(bool mixed, uint32_t tmax, const void *S, const double4 *k64,
              const uint32_t *group, const uint32_t *r_in,
              const uint32_t *r_out, uint32_t Kp, const GridShape &g,
-             size_t lds_limit, void *R, hipStream_t s) {
-  const uint32_t nax = g.nx + g.ny + g.nz;
-  const uint32_t T = rmt_nodes_per_group(g.ncell, nax, lds_limit, tmax);
+             size_t lds_limit, void *R, hipStream_t s, uint32_t tile0,
+             uint32_t tile1) {
+  const uint32_t T = rmt_nodes_per_group(g.nx, g.ny, g.nz, lds_limit, tmax);
   if (T == 0) return false;
-  const uint32_t ntile = Kp / T;  // Kp is a multiple of 256
-  const size_t lds = rmt_lds_bytes(g.ncell, nax, T);
+  const uint32_t all = Kp / T;  // Kp is a multiple of 256
+  if (tile1 > all) tile1 = all;
+  if (tile0 >= tile1) return true;
+  const uint32_t ntile = tile1 - tile0;
+  const size_t lds = rmt_lds_bytes(g.nx, g.ny, g.nz, T);
   if (mixed) {
     rmt_lds_kernel<true><<<ntile, kRmtBlock, lds, s>>>(
-        S, k64, group, r_in, r_out, Kp, T, ntile, g.nx, g.ny, g.nz, g.h, R);
+        S, k64, group, r_in, r_out, Kp, T, tile0, ntile, g.nx, g.ny, g.nz,
+        g.h, R);
   } else {
     rmt_lds_kernel<false><<<ntile, kRmtBlock, lds, s>>>(
-        S, k64, group, r_in, r_out, Kp, T, ntile, g.nx, g.ny, g.nz, g.h, R);
+        S, k64, group, r_in, r_out, Kp, T, tile0, ntile, g.nx, g.ny, g.nz,
+        g.h, R);
   }
   return true;
 }
@@ -1771,16 +1823,17 @@ void rmt_global(bool mixed, const void *S, double2 *G, const double4 *k64,
 }
 
 void lr(bool mixed, const void *atoms, const uint32_t *cell_sorted,
-        const LrTaskArgs &ta, const std::vector<LrGroup> &groups,
-        const void *R, uint32_t Kp, uint32_t natom,
-        uint32_t nsplit, double4 *partial, hipStream_t s) {
+        const LrTaskArgs &ta, const LrGroup *groups, size_t ngroup,
+        const void *R, uint32_t Kp, uint32_t natom, uint32_t nsplit,
+        double4 *partial, hipStream_t s) {
   if (natom == 0 || nsplit == 0) return;
   const uint32_t ntile = blocks_for(natom, kLrBlock);
   const dim3 grid(ntile, nsplit);
   const LrTasks tk{ta.node0, ta.nr, ta.koff, ta.dir64, ta.dir32,
                    ta.kappa64, ta.kappa32, ta.split};
   if (mixed) {
-    for (const LrGroup &gr : groups) {
+    for (size_t q = 0; q < ngroup; ++q) {
+      const LrGroup &gr = groups[q];
       if (gr.nsplit == 0) continue;
       const LrNrKernel k =
           kLrNrKernels[gr.nr <= static_cast<uint32_t>(kLrMaxUnrolled) ? gr.nr : 0];

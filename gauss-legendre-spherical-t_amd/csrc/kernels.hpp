@@ -70,41 +70,74 @@ struct SfGroup {
 // sum_{j in c} q_j exp(-i k.d_j) with d_j the position relative to the cell
 // centre; one lane per direction of an alpha group, the group's radii
 // unrolled (padded node columns are not written).
+// The mixed path launches once per listed group (groups[0..ngroup)); the
+// FP64 path ignores the list and covers every node in one launch.
 void sf(bool mixed, const void *atoms, const uint32_t *start,
-        const double4 *k64, const LrTaskArgs &tasks,
-        const std::vector<SfGroup> &groups, uint32_t Kp, const GridShape &g,
-        void *S, hipStream_t s);
+        const double4 *k64, const LrTaskArgs &tasks, const SfGroup *groups,
+        size_t ngroup, uint32_t Kp, const GridShape &g, void *S,
+        hipStream_t s);
+inline void sf(bool mixed, const void *atoms, const uint32_t *start,
+               const double4 *k64, const LrTaskArgs &tasks,
+               const std::vector<SfGroup> &groups, uint32_t Kp,
+               const GridShape &g, void *S, hipStream_t s) {
+  sf(mixed, atoms, start, k64, tasks, groups.data(), groups.size(), Kp, g, S,
+     s);
+}
 
 // ---- remote (shell) sums (sum_rmt_sf) --------------------------------------
 // LDS layout of the shell-sum workgroups.  A workgroup holds the summed-
 // volume tables of T nodes (T = 1 or a power of two), T tables of
 // rmt_table_stride elements (double2) plus T * (nx+ny+nz) phase factors.
-// The stride is ncell rounded up to a residue mod 16 elements chosen for T,
-// so that the 8 lanes an LDS cycle serves (consecutive nodes, then
-// consecutive z) read 8 different 4-bank groups.
-__host__ __device__ inline uint32_t rmt_table_stride(uint32_t ncell,
-                                                     uint32_t T) {
-  const uint32_t r =
-      T >= 8 ? 1u : (T == 4 ? 2u : (T == 2 ? 8u : (ncell & 15u)));
-  return ncell + ((r - (ncell & 15u)) & 15u);
+// A table has a zero plane before the first cell of each axis and z-lines
+// of odd pitch: (nx+1) x (ny+1) x rmt_zpitch(nz) elements.  An LDS cycle
+// serves 16 consecutive lanes of a 16-byte access, conflict-free when their
+// elements differ mod 16 (or coincide).  Lanes run over (node, then a line
+// coordinate padded to a multiple of 16/T), so the stride is that size
+// rounded up to 16/T mod 16 (1 from T = 16 on): the T nodes then take
+// disjoint slots and the 16/T line positions fill the rest.
+__host__ __device__ inline uint32_t rmt_zpitch(uint32_t nz) {
+  return (nz + 1) | 1u;
 }
-inline size_t rmt_lds_bytes(uint32_t ncell, uint32_t nax, uint32_t T) {
-  return static_cast<size_t>(T) * (rmt_table_stride(ncell, T) + nax) * 16;
+__host__ __device__ inline uint32_t rmt_table_elems(uint32_t nx, uint32_t ny,
+                                                    uint32_t nz) {
+  return (nx + 1) * (ny + 1) * rmt_zpitch(nz);
+}
+__host__ __device__ inline uint32_t rmt_table_stride(uint32_t elems,
+                                                     uint32_t T) {
+  if (T <= 1) return elems;
+  const uint32_t r = T >= 16 ? 1u : 16u / T;
+  return elems + ((r - (elems & 15u)) & 15u);
+}
+// Lanes per node group of an LDS cycle: line coordinates are padded to a
+// multiple of this.
+__host__ __device__ inline uint32_t rmt_line_group(uint32_t T) {
+  return T >= 16 ? 1u : 16u / T;
+}
+inline size_t rmt_lds_bytes(uint32_t nx, uint32_t ny, uint32_t nz,
+                            uint32_t T) {
+  return static_cast<size_t>(T) *
+         (rmt_table_stride(rmt_table_elems(nx, ny, nz), T) + nx + ny + nz) *
+         16;
 }
 // Largest T <= tmax (a power of two) whose tables fit lds_limit bytes; 0 when
 // not even one node's table fits.
-inline uint32_t rmt_nodes_per_group(uint32_t ncell, uint32_t nax,
+inline uint32_t rmt_nodes_per_group(uint32_t nx, uint32_t ny, uint32_t nz,
                                     size_t lds_limit, uint32_t tmax) {
   for (uint32_t T = tmax; T >= 2; T >>= 1)
-    if (rmt_lds_bytes(ncell, nax, T) <= lds_limit) return T;
-  return rmt_lds_bytes(ncell, nax, 1) <= lds_limit ? 1u : 0u;
+    if (rmt_lds_bytes(nx, ny, nz, T) <= lds_limit) return T;
+  return rmt_lds_bytes(nx, ny, nz, 1) <= lds_limit ? 1u : 0u;
 }
 constexpr uint32_t kRmtMaxNodes = 64;
 
+// Workgroup tiles [tile0, tile1) of T nodes each (T from
+// rmt_nodes_per_group with this tmax; tile1 is clamped to Kp / T), so that
+// the shell sums of a node range can start as soon as its S is complete.
+// False: one node's tables do not fit in LDS (nothing launched).
 bool rmt_lds(bool mixed, uint32_t tmax, const void *S, const double4 *k64,
              const uint32_t *group, const uint32_t *r_in,
              const uint32_t *r_out, uint32_t Kp, const GridShape &g,
-             size_t lds_limit, void *R, hipStream_t s);
+             size_t lds_limit, void *R, hipStream_t s, uint32_t tile0 = 0,
+             uint32_t tile1 = 0xffffffffu);
 // Global-memory tables: G (ncell*Kp double2, scratch) receives S in FP64
 // cell-major order, is prefix-summed in place, and the shells are gathered
 // from it.  S is left unchanged.
@@ -132,10 +165,19 @@ struct LrGroup {
 };
 // partial[split*natom + i] = (fx, fy, fz, en) of sorted atom i over the
 // split's tasks.
+// The mixed path launches once per listed group; the FP64 path covers
+// every split in one launch.
 void lr(bool mixed, const void *atoms, const uint32_t *cell_sorted,
-        const LrTaskArgs &tasks, const std::vector<LrGroup> &groups,
-        const void *R, uint32_t Kp, uint32_t natom,
-        uint32_t nsplit, double4 *partial, hipStream_t s);
+        const LrTaskArgs &tasks, const LrGroup *groups, size_t ngroup,
+        const void *R, uint32_t Kp, uint32_t natom, uint32_t nsplit,
+        double4 *partial, hipStream_t s);
+inline void lr(bool mixed, const void *atoms, const uint32_t *cell_sorted,
+               const LrTaskArgs &tasks, const std::vector<LrGroup> &groups,
+               const void *R, uint32_t Kp, uint32_t natom, uint32_t nsplit,
+               double4 *partial, hipStream_t s) {
+  lr(mixed, atoms, cell_sorted, tasks, groups.data(), groups.size(), R, Kp,
+     natom, nsplit, partial, s);
+}
 // f[i] += sum over splits (fixed order) + near[i] (when near != nullptr).
 void lr_reduce(const double4 *partial, uint32_t nsplit, uint32_t natom,
                const double4 *near, double *fx, double *fy, double *fz,

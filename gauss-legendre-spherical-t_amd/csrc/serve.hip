@@ -290,6 +290,8 @@ struct glst_server {
       if (free_slot == kSlots) break;
       launch(*slots[free_slot]);
       in_flight.push_back(free_slot);
+      stats.max_in_flight =
+          std::max<uint64_t>(stats.max_in_flight, in_flight.size());
       launched = true;
     }
     if (block && completed == 0 && !in_flight.empty()) {

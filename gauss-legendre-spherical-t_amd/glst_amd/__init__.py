@@ -592,7 +592,7 @@ class GLSTBatch:
 class ServerStats(Structure):
     _fields_ = [(n, c_uint64) for n in (
         "submitted", "rejected", "completed", "batches", "reconfigures",
-        "setups", "setup_hits", "atoms")]
+        "setups", "setup_hits", "atoms", "max_in_flight")]
 
 
 class GLSTServer:

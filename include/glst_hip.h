@@ -179,10 +179,16 @@ glst_status_t glst_get_rmt(glst_handle_t h, double *re, double *im);
  *   second stream of the handle, started with the structure factors (1) or
  *   after them, overlapping the shell sums and the back-projection (2); 0
  *   runs the stages one after another.  The results are bitwise identical.
- *   "rmt_shape" (default 0): shell sums with LDS-resident tables of two
- *   nodes per workgroup where they fit, else one (0); always one node per
- *   workgroup (1); or the global-memory tables otherwise used when one
- *   node's table does not fit in LDS (2).  Results agree to FP64 rounding.
+ *   "rmt_shape" (default 0): shell sums with LDS-resident tables of as
+ *   many nodes per workgroup as fit (a power of two up to 64) (0); always
+ *   one node per workgroup (1); or the global-memory tables otherwise used
+ *   when one node's table does not fit in LDS (2).  Results agree to FP64
+ *   rounding.
+ *   "pipeline" (default 1): CT = float with LDS-resident tables only;
+ *   calc_ener_force overlaps the stages of successive alpha groups: the
+ *   shell sums of group g run on a third stream once S(k) of group g is
+ *   complete, alongside S(k) of later groups and the back-projection of
+ *   earlier ones (0: one stage after another).  Bitwise identical.
  *   "lr_blocks" (default 16384): back-projection workgroups aimed for; the
  *   direction ranges of each alpha group are split accordingly.
  *   "graph" (default 0): calc_ener_force captures the whole step as a HIP
@@ -196,8 +202,9 @@ glst_status_t glst_get_option(glst_handle_t h, const char *name,
 
 /* Autotuner (no counterpart in the reference): times calc_ener_force on
  * these inputs under candidate schedules, `reps` steps each, and keeps the
- * fastest, one knob at a time: "overlap_near" (0, 1, 2), "lr_blocks"
- * (powers of two from 1024 to 65536), "rmt_shape" (0, 1).  Clears the
+ * fastest, one knob at a time: "pipeline" (0, 1; CT = float),
+ * "overlap_near" (0, 1, 2), "lr_blocks" (powers of two from 1024 to 65536),
+ * "rmt_shape" (0, 1).  Clears the
  * accumulators afterwards; *best_ms (may be NULL) receives the chosen
  * schedule's time per step. */
 glst_status_t glst_autotune(glst_handle_t h, const double *d_rx,

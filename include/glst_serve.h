@@ -43,6 +43,7 @@ typedef struct glst_server_stats {
   uint64_t setups;        /* per-shape setups built */
   uint64_t setup_hits;    /* submissions served from the setup cache */
   uint64_t atoms;         /* atoms of all completed requests */
+  uint64_t max_in_flight; /* most batches in flight at once (at most 2) */
 } glst_server_stats_t;
 
 /* budget_bytes: device memory the server's batches may hold together;

@@ -206,21 +206,63 @@ def test_mixed_options_agree(cuda, water1k):
     out = {}
     for rs in (0, 1, 2):
         for ov in (0, 1, 2):
-            f = g.GLSTForce(rx.size, 1e-5, 31.04, rcut=9.0, ct="float")
-            f.set_option("rmt_shape", rs)
-            f.set_option("overlap_near", ov)
-            f.calc_ener_force(*d)
-            out[rs, ov] = _host(f.get_ef())
-            f.close()
+            for pl in (0, 1):
+                f = g.GLSTForce(rx.size, 1e-5, 31.04, rcut=9.0, ct="float")
+                f.set_option("rmt_shape", rs)
+                f.set_option("overlap_near", ov)
+                f.set_option("pipeline", pl)
+                assert f.get_option("pipeline") == pl
+                f.calc_ener_force(*d)
+                out[rs, ov, pl] = _host(f.get_ef())
+                f.close()
     for rs in (0, 1, 2):
-        for ov in (1, 2):
-            for a, b in zip(out[rs, 0], out[rs, ov]):
-                np.testing.assert_array_equal(a, b)
-        assert rel_err(out[rs, 0][:3], out[0, 0][:3]) < 1e-7
+        for ov in (0, 1, 2):
+            for pl in (0, 1):
+                for a, b in zip(out[rs, 0, 0], out[rs, ov, pl]):
+                    np.testing.assert_array_equal(a, b)
+        assert rel_err(out[rs, 0, 0][:3], out[0, 0, 0][:3]) < 1e-7
     f = g.GLSTForce(rx.size, 1e-5, 31.04, rcut=9.0, ct="float")
     with pytest.raises(Exception):
         f.set_option("rmt_shape", 3)
     with pytest.raises(Exception):
         f.set_option("overlap_near", 3)
     with pytest.raises(Exception):
+        f.set_option("pipeline", 2)
+    with pytest.raises(Exception):
         f.set_option("no_such_option", 1)
+
+
+@pytest.mark.parametrize("nw,L,rcut", [(4000, 49.0, 9.0), (12000, 71.0, 9.0),
+                                       (20000, 84.0, 10.0)])
+def test_pipelined_far_field_bitwise(cuda, nw, L, rcut):
+    """The far field pipelined across alpha groups (shell sums of group g on a
+    third stream, started per group; workgroup tiles that straddle two groups
+    wait for both) equals the serial schedule bit for bit, eagerly over
+    repeated steps and as a replayed graph, on grids with several groups."""
+    import torch
+    import glst_amd as g
+    h = g.make_water_box(nw, L, seed=7)
+    d = _dev(h, torch)
+    res = {}
+    for pl in (0, 1):
+        for graph in (0, 1):
+            f = g.GLSTForce(h[0].size, 1e-4, L, rcut=rcut, ct="float")
+            f.set_option("pipeline", pl)
+            f.set_option("graph", graph)
+            if pl == 1 and graph == 0:
+                info = f.info()
+                assert info.ngroup >= 2
+            steps = []
+            for _ in range(3):
+                f.zero_ef()
+                f.calc_ener_force(*d)
+                torch.zeros(1 << 20, device="cuda").sum().item()  # eager work
+                steps.append(_host(f.get_ef()))
+            f.close()
+            for st in steps[1:]:
+                for a, b in zip(steps[0], st):
+                    np.testing.assert_array_equal(a, b)
+            res[pl, graph] = steps[0]
+    for key in res:
+        for a, b in zip(res[0, 0], res[key]):
+            np.testing.assert_array_equal(a, b)
