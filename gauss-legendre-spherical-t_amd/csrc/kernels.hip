@@ -444,6 +444,16 @@ constexpr uint32_t kRmtBlock = 1024;
 // and S slots loaded per thread before the first is stored.
 constexpr uint32_t kRmtBatch = 16;
 constexpr uint32_t kRmtLoadBatch = 4;
+// Threads of the persistent single-system workgroups and the S slots per
+// thread they hold across a gather.
+#ifndef GLST_RMT_PBLOCK
+#define GLST_RMT_PBLOCK 512
+#endif
+#ifndef GLST_RMT_PREFETCH
+#define GLST_RMT_PREFETCH 8
+#endif
+constexpr uint32_t kRmtPBlock = GLST_RMT_PBLOCK;
+constexpr uint32_t kRmtPrefetch = GLST_RMT_PREFETCH;
 
 // n / d for n < 2^22 and 0 < d, with rd = 1.0f / d: the float quotient is
 // within one of the exact one, and one correction step fixes it.
@@ -517,142 +527,183 @@ __device__ inline double2 cube_y(const double2 *P, const CubeXZ &c,
                       r0.y - r1.y - r2.y - r3.y + r4.y + r5.y + r6.y - r7.y);
 }
 
-// One workgroup holds the bordered summed-volume tables of T consecutive
-// nodes in LDS (T = 1, or a power of two up to 64: two for a 15^3 grid, up
-// to 64 for the small grids of a batch).  (1) It reads the nodes' S slots,
-// one contiguous run of the node-pair tiles, converting to FP64, into the
-// bordered tables; writes the zero borders; mixed path: the FP64 axis
-// factors of the cell centres.  (2) Mixed path: turns the cell-local sums
-// into S_c = exp(-i k.c) local and takes the running sum along z (the FP64
-// slots hold z-running sums already).  (3) Running sums along y, then x.
-// (4) Shells: lanes on consecutive (node, z), every thread walking y; the
-// mixed path applies w exp(+i k.c) with the node weight folded into the
-// thread's x and z factors.  In (2)-(4) lanes run over the nodes first,
-// then over a line coordinate padded to a multiple of rmt_line_group(T),
-// which with rmt_table_stride keeps every LDS cycle free of bank conflicts.
-template <bool MIXED>
-__device__ __forceinline__ void rmt_lds_body(
-    const void *__restrict__ Sv, const double4 *__restrict__ k64,
-    const uint32_t *__restrict__ group, const uint32_t *__restrict__ r_in,
-    const uint32_t *__restrict__ r_out, uint32_t Kp, uint32_t T,
-    uint32_t nx, uint32_t ny, uint32_t nz, double h, void *__restrict__ R,
-    uint32_t tile) {
-  extern __shared__ double2 P[];  // [T][ts], then [T][nx+ny+nz] phases
-  const uint32_t ncell = nx * ny * nz;
-  const uint32_t nax = nx + ny + nz;
-  const uint32_t zp = rmt_zpitch(nz), plane = (ny + 1) * zp;
-  const uint32_t ts = rmt_table_stride(rmt_table_elems(nx, ny, nz), T);
-  const uint32_t lgT = 31u - __builtin_clz(T);
-  double2 *phase = P + T * ts;
-  const uint32_t node0 = tile * T;
-  const uint32_t tid = threadIdx.x;
-  const double2 zero = make_double2(0.0, 0.0);
-  // bordered table index of cell c
-  const float rnz = 1.0f / static_cast<float>(nz);
-  const float rny = 1.0f / static_cast<float>(ny);
-  auto cell_index = [&](uint32_t c) {
+// Geometry of a workgroup's bordered tables (rmt_table_elems): T tables of
+// ts elements, then T * nax phase factors.  Lanes of the passes run over the
+// nodes first, then over a line coordinate padded to ypad / zpad (multiples
+// of rmt_line_group(T)), which with rmt_table_stride keeps every LDS cycle
+// free of bank conflicts.
+struct RmtGeom {
+  uint32_t T, lgT, nx, ny, nz, ncell, nax, zp, plane, ts, ypad, zpad;
+  float rnz, rny, rnc, rypad, rzpad;
+  __device__ RmtGeom(uint32_t T_, uint32_t nx_, uint32_t ny_, uint32_t nz_)
+      : T(T_), lgT(31u - __builtin_clz(T_)), nx(nx_), ny(ny_), nz(nz_),
+        ncell(nx_ * ny_ * nz_), nax(nx_ + ny_ + nz_), zp(rmt_zpitch(nz_)),
+        plane((ny_ + 1) * rmt_zpitch(nz_)),
+        ts(rmt_table_stride(rmt_table_elems(nx_, ny_, nz_), T_)) {
+    const uint32_t lg = rmt_line_group(T);
+    ypad = (ny + lg - 1) / lg * lg;
+    zpad = (nz + lg - 1) / lg * lg;
+    rnz = 1.0f / static_cast<float>(nz);
+    rny = 1.0f / static_cast<float>(ny);
+    rnc = 1.0f / static_cast<float>(ncell);
+    rypad = 1.0f / static_cast<float>(ypad);
+    rzpad = 1.0f / static_cast<float>(zpad);
+  }
+  // table index of cell c ((x ny + y) nz + z)
+  __device__ uint32_t cell_index(uint32_t c) const {
     const uint32_t line = fast_div(c, nz, rnz), z = c - line * nz;
     const uint32_t x = fast_div(line, ny, rny), y = line - x * ny;
     return (x + 1) * plane + (y + 1) * zp + z + 1;
-  };
-  // (1) slot pair (2m, 2m+1) of cell c sits at index 2 (m ncell + c), so the
-  // T/2 pairs of an even T are one contiguous run, read coalesced
-  const size_t run0 = static_cast<size_t>(node0 >> 1) * ncell;
-  if (T >= 2) {
-    const uint32_t total = (T >> 1) * ncell;
-    const float rnc = 1.0f / static_cast<float>(ncell);
-    for (uint32_t e0 = tid; e0 < total; e0 += kRmtBlock * kRmtLoadBatch) {
-      double2 lo[kRmtLoadBatch], hi[kRmtLoadBatch];
+  }
+  // S elements one tile reads: slot pairs (T >= 2) or slots (T = 1)
+  __device__ uint32_t load_count() const {
+    return T >= 2 ? (T >> 1) * ncell : ncell;
+  }
+};
+
+// One round of a tile's S slots held in registers, N per thread (the mixed
+// path keeps the FP32 pairs).
+template <bool MIXED, uint32_t N>
+struct RmtRound;
+template <uint32_t N>
+struct RmtRound<true, N> {
+  float4 v[N];
+};
+template <uint32_t N>
+struct RmtRound<false, N> {
+  double2 lo[N], hi[N];
+};
+
+// Reads round e0 (elements e0 + u BS, u < N; BS threads) of the S slots of nodes
+// node0 .. node0+T-1.  The pair (2m, 2m+1) of cell c sits at 2 (m ncell + c),
+// so the T/2 pairs of an even T are one contiguous run, read coalesced; T = 1
+// reads its node's half of each pair.
+template <uint32_t BS, bool MIXED, uint32_t N>
+__device__ __forceinline__ void rmt_fetch(const void *__restrict__ Sv,
+                                          const RmtGeom &g, uint32_t node0,
+                                          uint32_t e0, RmtRound<MIXED, N> &rd) {
+  const size_t run0 = static_cast<size_t>(node0 >> 1) * g.ncell;
+  const uint32_t total = g.load_count();
 #pragma unroll
-      for (uint32_t u = 0; u < kRmtLoadBatch; ++u) {
-        const uint32_t eu = e0 + u * kRmtBlock;
-        const uint32_t e = eu < total ? eu : total - 1;
-        if (MIXED) {
-          const float4 v = static_cast<const float4 *>(Sv)[run0 + e];
-          lo[u] = make_double2(v.x, v.y);
-          hi[u] = make_double2(v.z, v.w);
-        } else {
-          const double2 *p = static_cast<const double2 *>(Sv) + 2 * (run0 + e);
-          lo[u] = p[0];
-          hi[u] = p[1];
-        }
+  for (uint32_t u = 0; u < N; ++u) {
+    const uint32_t eu = e0 + u * BS;
+    const uint32_t e = eu < total ? eu : total - 1;
+    if (g.T >= 2) {
+      if constexpr (MIXED) {
+        rd.v[u] = static_cast<const float4 *>(Sv)[run0 + e];
+      } else {
+        const double2 *p = static_cast<const double2 *>(Sv) + 2 * (run0 + e);
+        rd.lo[u] = p[0];
+        rd.hi[u] = p[1];
       }
-#pragma unroll
-      for (uint32_t u = 0; u < kRmtLoadBatch; ++u) {
-        const uint32_t e = e0 + u * kRmtBlock;
-        if (e < total) {
-          const uint32_t m = fast_div(e, ncell, rnc), c = e - m * ncell;
-          const uint32_t i = cell_index(c);
-          P[(2 * m) * ts + i] = lo[u];
-          P[(2 * m + 1) * ts + i] = hi[u];
-        }
-      }
-    }
-  } else {
-    const uint32_t par = node0 & 1;
-    for (uint32_t c0 = tid; c0 < ncell; c0 += kRmtBlock * kRmtLoadBatch) {
-      double2 v[kRmtLoadBatch];
-#pragma unroll
-      for (uint32_t u = 0; u < kRmtLoadBatch; ++u) {
-        const uint32_t c0u = c0 + u * kRmtBlock;
-        const size_t slot = 2 * (run0 + (c0u < ncell ? c0u : ncell - 1)) + par;
-        if (MIXED) {
-          const float2 f = static_cast<const float2 *>(Sv)[slot];
-          v[u] = make_double2(f.x, f.y);
-        } else {
-          v[u] = static_cast<const double2 *>(Sv)[slot];
-        }
-      }
-#pragma unroll
-      for (uint32_t u = 0; u < kRmtLoadBatch; ++u) {
-        const uint32_t c = c0 + u * kRmtBlock;
-        if (c < ncell) P[cell_index(c)] = v[u];
+    } else {
+      const size_t slot = 2 * (run0 + e) + (node0 & 1u);
+      if constexpr (MIXED) {
+        const float2 f = static_cast<const float2 *>(Sv)[slot];
+        rd.v[u] = make_float4(f.x, f.y, 0.f, 0.f);
+      } else {
+        rd.lo[u] = static_cast<const double2 *>(Sv)[slot];
       }
     }
   }
-  if (MIXED) {
-    for (uint32_t e = tid; e < T * nax; e += kRmtBlock) {
-      const uint32_t t = e / nax, a = e % nax;
-      const double4 k = k64[node0 + t];
-      if (a < nx)
-        axis_phase(phase + e, k.x, h, a);
-      else if (a < nx + ny)
-        axis_phase(phase + e, k.y, h, a - nx);
-      else
-        axis_phase(phase + e, k.z, h, a - nx - ny);
+}
+
+// Stores a fetched round into the bordered tables, in FP64.
+template <uint32_t BS, bool MIXED, uint32_t N>
+__device__ __forceinline__ void rmt_put(double2 *P, const RmtGeom &g,
+                                        uint32_t e0,
+                                        const RmtRound<MIXED, N> &rd) {
+  const uint32_t total = g.load_count();
+#pragma unroll
+  for (uint32_t u = 0; u < N; ++u) {
+    const uint32_t e = e0 + u * BS;
+    if (e < total) {
+      double2 lo, hi;
+      if constexpr (MIXED) {
+        lo = make_double2(rd.v[u].x, rd.v[u].y);
+        hi = make_double2(rd.v[u].z, rd.v[u].w);
+      } else {
+        lo = rd.lo[u];
+        hi = rd.hi[u];
+      }
+      if (g.T >= 2) {
+        const uint32_t m = fast_div(e, g.ncell, g.rnc), c = e - m * g.ncell;
+        const uint32_t i = g.cell_index(c);
+        P[(2 * m) * g.ts + i] = lo;
+        P[(2 * m + 1) * g.ts + i] = hi;
+      } else {
+        P[g.cell_index(e)] = lo;
+      }
     }
   }
-  const uint32_t nborder = plane + nx * zp + nx * ny;
-  for (uint32_t e = tid; e < T * nborder; e += kRmtBlock) {
+}
+
+// Axis factor e (< T nax) of nodes node0..: exp(+i k_d (i + 1/2) h).
+__device__ inline double2 rmt_phase(const double4 *__restrict__ k64,
+                                    const RmtGeom &g, uint32_t node0, double h,
+                                    uint32_t e) {
+  const uint32_t t = e / g.nax, a = e - t * g.nax;
+  const double4 k = k64[node0 + t];
+  double2 r;
+  if (a < g.nx)
+    axis_phase(&r, k.x, h, a);
+  else if (a < g.nx + g.ny)
+    axis_phase(&r, k.y, h, a - g.nx);
+  else
+    axis_phase(&r, k.z, h, a - g.nx - g.ny);
+  return r;
+}
+
+// Zero planes before each axis of every table (the passes touch interior
+// cells only, so they stay zero across tiles).
+template <uint32_t BS>
+__device__ inline void rmt_border(double2 *P, const RmtGeom &g) {
+  const uint32_t nborder = g.plane + g.nx * g.zp + g.nx * g.ny;
+  for (uint32_t e = threadIdx.x; e < g.T * nborder; e += BS) {
     const uint32_t t = e / nborder;
     uint32_t b = e - t * nborder, idx;
-    if (b < plane) {
+    if (b < g.plane) {
       idx = b;  // the x plane before the grid
-    } else if ((b -= plane) < nx * zp) {
-      idx = (1 + b / zp) * plane + b % zp;  // the y row before the grid
+    } else if ((b -= g.plane) < g.nx * g.zp) {
+      idx = (1 + b / g.zp) * g.plane + b % g.zp;  // the y row before the grid
     } else {
-      b -= nx * zp;
-      idx = (1 + b / ny) * plane + (1 + b % ny) * zp;  // the z slot before it
+      b -= g.nx * g.zp;
+      idx = (1 + b / g.ny) * g.plane + (1 + b % g.ny) * g.zp;  // z slot
     }
-    P[t * ts + idx] = zero;
+    P[t * g.ts + idx] = make_double2(0.0, 0.0);
   }
-  __syncthreads();
-  const uint32_t lg = rmt_line_group(T);
-  const uint32_t ypad = (ny + lg - 1) / lg * lg, zpad = (nz + lg - 1) / lg * lg;
-  const float rypad = 1.0f / static_cast<float>(ypad);
-  const float rzpad = 1.0f / static_cast<float>(zpad);
+}
+
+// Tables and phase factors of nodes node0.. are in LDS (after a barrier).
+// (2) Mixed path: turns the cell-local sums into S_c = exp(-i k.c) local and
+// takes the running sum along z (the FP64 slots hold z-running sums
+// already).  (3) Running sums along y, then x.  `between` runs next (the
+// persistent kernel fetches its next tile there).  (4) Shells: lanes on
+// consecutive (node, z), every thread walking y; the mixed path applies
+// w exp(+i k.c) with the node weight folded into the thread's x and z
+// factors.
+template <uint32_t BS, bool MIXED, class F>
+__device__ __forceinline__ void rmt_compute(
+    double2 *P, const RmtGeom &g, const double4 *__restrict__ k64,
+    const uint32_t *__restrict__ group, const uint32_t *__restrict__ r_in,
+    const uint32_t *__restrict__ r_out, uint32_t Kp, uint32_t node0,
+    void *__restrict__ R, F between) {
+  const uint32_t T = g.T, nx = g.nx, ny = g.ny, nz = g.nz, nax = g.nax;
+  const uint32_t zp = g.zp, plane = g.plane, ts = g.ts, lgT = g.lgT;
+  const uint32_t tid = threadIdx.x;
+  const double2 *phase = P + T * ts;
   if (MIXED) {
     // (2) z-lines
-    for (uint32_t l = tid; l < T * nx * ypad; l += kRmtBlock) {
+    for (uint32_t l = tid; l < T * nx * g.ypad; l += BS) {
       const uint32_t t = l & (T - 1), q = l >> lgT;
-      const uint32_t x = fast_div(q, ypad, rypad), y = q - x * ypad;
+      const uint32_t x = fast_div(q, g.ypad, g.rypad), y = q - x * g.ypad;
       if (y >= ny) continue;
       const double2 *ph = phase + t * nax;
       const double2 a = ph[x], b = ph[nx + y];
       const double pr = a.x * b.x - a.y * b.y, pi = a.x * b.y + a.y * b.x;
       const double2 *pz = ph + nx + ny;
       double2 *p = P + t * ts + (x + 1) * plane + (y + 1) * zp + 1;
-      double2 run = zero;
+      double2 run = make_double2(0.0, 0.0);
       for (uint32_t z0 = 0; z0 < nz; z0 += kRmtBatch) {
         const uint32_t nb = (nz - z0 < kRmtBatch) ? nz - z0 : kRmtBatch;
         double2 v[kRmtBatch];
@@ -674,32 +725,33 @@ __device__ __forceinline__ void rmt_lds_body(
     __syncthreads();
   }
   // (3) inclusive prefix along y, then x (src/glst_force.cu:726-772)
-  for (uint32_t l = tid; l < T * nx * zpad; l += kRmtBlock) {
+  for (uint32_t l = tid; l < T * nx * g.zpad; l += BS) {
     const uint32_t t = l & (T - 1), q = l >> lgT;
-    const uint32_t x = fast_div(q, zpad, rzpad), z = q - x * zpad;
+    const uint32_t x = fast_div(q, g.zpad, g.rzpad), z = q - x * g.zpad;
     if (z < nz) lds_scan(P + t * ts + (x + 1) * plane + zp + (z + 1), ny, zp);
   }
   __syncthreads();
-  for (uint32_t l = tid; l < T * ny * zpad; l += kRmtBlock) {
+  for (uint32_t l = tid; l < T * ny * g.zpad; l += BS) {
     const uint32_t t = l & (T - 1), q = l >> lgT;
-    const uint32_t y = fast_div(q, zpad, rzpad), z = q - y * zpad;
+    const uint32_t y = fast_div(q, g.zpad, g.rzpad), z = q - y * g.zpad;
     if (z < nz) lds_scan(P + t * ts + plane + (y + 1) * zp + (z + 1), nx, plane);
   }
   __syncthreads();
+  between();
   // (4) shells; y is cut into segments so that the workgroup stays busy
-  const uint32_t glines = T * nx * zpad;
-  uint32_t nseg = kRmtBlock / glines;
+  const uint32_t glines = T * nx * g.zpad;
+  uint32_t nseg = BS / glines;
   nseg = nseg < 1 ? 1 : (nseg > ny ? ny : nseg);
   const uint32_t ylen = (ny + nseg - 1) / nseg;
-  for (uint32_t task = tid; task < glines * nseg; task += kRmtBlock) {
+  for (uint32_t task = tid; task < glines * nseg; task += BS) {
     const uint32_t t = task & (T - 1);
     const uint32_t q = task >> lgT;
-    const uint32_t xs = fast_div(q, zpad, rzpad), z = q - xs * zpad;
+    const uint32_t xs = fast_div(q, g.zpad, g.rzpad), z = q - xs * g.zpad;
     if (z >= nz) continue;
     const uint32_t x = xs % nx, seg = xs / nx;
     const uint32_t node = node0 + t;
-    const uint32_t g = group[node];
-    const uint32_t rin = r_in[g], rout = r_out[g];
+    const uint32_t gr = group[node];
+    const uint32_t rin = r_in[gr], rout = r_out[gr];
     const double w = k64[node].w;
     const double2 *Pt = P + t * ts;
     const CubeXZ co = cube_xz(x, z, rout, nx, nz, plane);
@@ -729,15 +781,95 @@ __device__ __forceinline__ void rmt_lds_body(
   }
 }
 
+// Shell sums of one tile of T consecutive nodes by one workgroup (the
+// batched engine: every workgroup's tile may belong to another grid).
 template <bool MIXED>
-__global__ __launch_bounds__(kRmtBlock) void rmt_lds_kernel(
+__device__ __forceinline__ void rmt_lds_body(
+    const void *__restrict__ Sv, const double4 *__restrict__ k64,
+    const uint32_t *__restrict__ group, const uint32_t *__restrict__ r_in,
+    const uint32_t *__restrict__ r_out, uint32_t Kp, uint32_t T,
+    uint32_t nx, uint32_t ny, uint32_t nz, double h, void *__restrict__ R,
+    uint32_t tile) {
+  extern __shared__ double2 P[];  // [T][ts], then [T][nx+ny+nz] phases
+  const RmtGeom g(T, nx, ny, nz);
+  double2 *phase = P + T * g.ts;
+  const uint32_t node0 = tile * T;
+  const uint32_t total = g.load_count();
+  for (uint32_t e0 = threadIdx.x; e0 < total;
+       e0 += kRmtBlock * kRmtLoadBatch) {
+    RmtRound<MIXED, kRmtLoadBatch> rd;
+    rmt_fetch<kRmtBlock>(Sv, g, node0, e0, rd);
+    rmt_put<kRmtBlock>(P, g, e0, rd);
+  }
+  if (MIXED)
+    for (uint32_t e = threadIdx.x; e < T * g.nax; e += kRmtBlock)
+      phase[e] = rmt_phase(k64, g, node0, h, e);
+  rmt_border<kRmtBlock>(P, g);
+  __syncthreads();
+  rmt_compute<kRmtBlock, MIXED>(P, g, k64, group, r_in, r_out, Kp, node0, R,
+                                [] {});
+}
+
+// Shell sums of tiles tile0 .. tile0+ntile-1 (T nodes each) by persistent
+// workgroups, one or two per CU.  With at least 8 workgroups, XCD x (the
+// workgroups b with b % 8 == x) owns a contiguous eighth of the tiles and
+// its workgroups take every (count on x)-th tile of it, so the tiles in
+// flight on one XCD are consecutive and their R rows meet in its L2.  While
+// a workgroup gathers one tile's shells, the next tile's S slots (the first
+// round) are already on their way into registers.
+template <bool MIXED>
+__global__ __launch_bounds__(kRmtPBlock) void rmt_lds_kernel(
     const void *__restrict__ Sv, const double4 *__restrict__ k64,
     const uint32_t *__restrict__ group, const uint32_t *__restrict__ r_in,
     const uint32_t *__restrict__ r_out, uint32_t Kp, uint32_t T,
     uint32_t tile0, uint32_t ntile, uint32_t nx, uint32_t ny, uint32_t nz,
     double h, void *__restrict__ R) {
-  rmt_lds_body<MIXED>(Sv, k64, group, r_in, r_out, Kp, T, nx, ny, nz, h, R,
-                      tile0 + xcd_remap(blockIdx.x, ntile));
+  extern __shared__ double2 P[];  // [T][ts], then [T][nx+ny+nz] phases
+  const RmtGeom g(T, nx, ny, nz);
+  double2 *phase = P + T * g.ts;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nwg = gridDim.x;
+  uint32_t tile, end, step;
+  if (nwg >= 8) {
+    const uint32_t x = blockIdx.x & 7u, q = ntile >> 3, r = ntile & 7u;
+    const uint32_t begin = x * q + (x < r ? x : r);
+    tile = begin + (blockIdx.x >> 3);
+    end = begin + q + (x < r ? 1u : 0u);
+    step = (nwg - x + 7) >> 3;
+  } else {
+    tile = blockIdx.x;
+    end = ntile;
+    step = nwg;
+  }
+  const uint32_t total = g.load_count();
+  const uint32_t nph = MIXED ? T * g.nax : 0u;
+  rmt_border<kRmtPBlock>(P, g);
+  // the first kRmtPrefetch rounds of the next tile are held in registers
+  // across the gather; the rest (elements from tid + kRmtPrefetch kRmtPBlock
+  // on) are read when the tile starts
+  RmtRound<MIXED, kRmtPrefetch> rd;
+  if (tile < end) rmt_fetch<kRmtPBlock>(Sv, g, (tile0 + tile) * T, tid, rd);
+  while (tile < end) {  // uniform over the workgroup
+    const uint32_t node0 = (tile0 + tile) * T;
+    for (uint32_t e = tid; e < nph; e += kRmtPBlock)
+      phase[e] = rmt_phase(k64, g, node0, h, e);
+    rmt_put<kRmtPBlock>(P, g, tid, rd);
+    for (uint32_t e0 = tid + kRmtPBlock * kRmtPrefetch; e0 < total;
+         e0 += kRmtPBlock * kRmtLoadBatch) {
+      RmtRound<MIXED, kRmtLoadBatch> more;
+      rmt_fetch<kRmtPBlock>(Sv, g, node0, e0, more);
+      rmt_put<kRmtPBlock>(P, g, e0, more);
+    }
+    __syncthreads();
+    const uint32_t next = tile + step;
+    rmt_compute<kRmtPBlock, MIXED>(
+        P, g, k64, group, r_in, r_out, Kp, node0, R, [&] {
+          if (next < end)
+            rmt_fetch<kRmtPBlock>(Sv, g, (tile0 + next) * T, tid, rd);
+        });
+    __syncthreads();  // the tables are free for the next tile
+    tile = next;
+  }
 }
 
 // ---- global-memory tables (grids too large for LDS) ----
@@ -1235,7 +1367,8 @@ __global__ void lr_reduce_kernel(const double4 *__restrict__ partial,
 // 256-atom chunks with S = 1).  Each neighbour cell is staged through an
 // LDS tile of up to 256 atom records, loaded once by the workgroup and read
 // by every lane (the lanes of one phase read the same record); the S partial
-// sums of an atom are then added in LDS in a fixed order.
+// sums of an atom are then added in LDS in a fixed order, in the same
+// buffer.
 constexpr uint32_t kSrBlock = 256;
 constexpr uint32_t kSrMaxPhase = 16;
 
@@ -1292,8 +1425,10 @@ __device__ __forceinline__ void sr_body(
     uint32_t nx, uint32_t ny, uint32_t nz, double *__restrict__ fx,
     double *__restrict__ fy, double *__restrict__ fz,
     double *__restrict__ en, double4 *__restrict__ out4, uint32_t cell) {
-  __shared__ double4 part[kSrBlock];
+  // one buffer: the neighbour tiles, then the per-lane partial sums (8 KB,
+  // so that two workgroups fit beside the shell sums' tables)
   __shared__ double4 tile[kSrBlock];
+  double4 *const part = tile;
   const uint32_t ix = cell / (ny * nz), iy = (cell / nz) % ny, iz = cell % nz;
   const uint32_t a0 = start[cell], a1 = start[cell + 1];
   const uint32_t n = a1 - a0;
@@ -1348,6 +1483,7 @@ __device__ __forceinline__ void sr_body(
         }
       }
     }
+    __syncthreads();  // the last tile has been consumed
     part[tid] = valid ? make_double4(FX, FY, FZ, EN)
                       : make_double4(0.0, 0.0, 0.0, 0.0);
     __syncthreads();
@@ -1667,7 +1803,25 @@ __global__ void copy_runs_kernel(const CopyRun *__restrict__ runs) {
 // ---------------------------------------------------------------------------
 // launchers
 
+// Compute units and LDS bytes per CU of the device (prepare_kernels); the
+// defaults are MI355X's.
+static uint32_t g_cu_count = 256;
+static size_t g_lds_per_cu = 160 * 1024;
+
 void prepare_kernels(size_t lds_limit) {
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount,
+                              dev) == hipSuccess &&
+        v > 0)
+      g_cu_count = static_cast<uint32_t>(v);
+    if (hipDeviceGetAttribute(
+            &v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) ==
+            hipSuccess &&
+        v > 0)
+      g_lds_per_cu = static_cast<size_t>(v);
+  }
+  (void)hipGetLastError();
   const int lds = static_cast<int>(lds_limit);
   const void *fns[] = {reinterpret_cast<const void *>(&rmt_lds_kernel<true>),
                        reinterpret_cast<const void *>(&rmt_lds_kernel<false>),
@@ -1773,12 +1927,19 @@ bool rmt_lds(bool mixed, uint32_t tmax, const void *S, const double4 *k64,
   if (tile0 >= tile1) return true;
   const uint32_t ntile = tile1 - tile0;
   const size_t lds = rmt_lds_bytes(g.nx, g.ny, g.nz, T);
+  // persistent workgroups: as many as are resident at once (LDS allows one
+  // or two 1024-thread workgroups per CU)
+  size_t per_cu = g_lds_per_cu / lds;
+  const size_t wave_cap = 2048 / kRmtPBlock;  // 32 waves per CU
+  per_cu = per_cu < 1 ? 1 : (per_cu > wave_cap ? wave_cap : per_cu);
+  const uint32_t nwg = static_cast<uint32_t>(
+      std::min<size_t>(ntile, static_cast<size_t>(g_cu_count) * per_cu));
   if (mixed) {
-    rmt_lds_kernel<true><<<ntile, kRmtBlock, lds, s>>>(
+    rmt_lds_kernel<true><<<nwg, kRmtPBlock, lds, s>>>(
         S, k64, group, r_in, r_out, Kp, T, tile0, ntile, g.nx, g.ny, g.nz,
         g.h, R);
   } else {
-    rmt_lds_kernel<false><<<ntile, kRmtBlock, lds, s>>>(
+    rmt_lds_kernel<false><<<nwg, kRmtPBlock, lds, s>>>(
         S, k64, group, r_in, r_out, Kp, T, tile0, ntile, g.nx, g.ny, g.nz,
         g.h, R);
   }
