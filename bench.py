@@ -51,59 +51,52 @@ def ref_style_errors(got, ref):
     return out
 
 
-def time_path(g, torch, d, natom, ct, steps, warmup, timing, tune=False,
-              graph=False, schedule=None):
-    """Times `steps` steps after `warmup`; with `tune`, the engine's
-    autotuner first picks the schedule for this workload; with `graph`, the
-    same steps are then timed again replayed as a HIP graph (which records
-    no per-kernel events)."""
+def time_path(g, torch, d, natom, ct, steps, warmup, tune=False,
+              schedule=None):
+    """Picks the schedule (the engine's autotuner, which also decides between
+    eager steps and graph replay, or `schedule` pinned), then times `steps`
+    steps after `warmup`: as tuned (the result), eagerly and as a replayed
+    graph (both reported), and eagerly with per-stage events (the stage
+    breakdown; graph replay records no per-kernel events)."""
     f = g.GLSTForce(natom, TOL, BOX, rcut=RCUT, ct=ct)
-    tuned = None
-    keys = ("overlap_near", "lr_blocks", "rmt_shape", "pipeline")
+    keys = ("overlap_near", "lr_blocks", "rmt_shape", "pipeline", "graph")
     if schedule is not None:  # a pinned schedule (profiling runs)
         for k, v in zip(keys, schedule):
             f.set_option(k, v)
-        tuned = {k: f.get_option(k) for k in keys}
     elif tune:
         f.autotune(*d, reps=3)
-        tuned = {k: f.get_option(k) for k in keys}
-    for _ in range(warmup):
-        f.zero_ef()
-        f.calc_ener_force(*d)
-    torch.cuda.synchronize()
-    f.set_timing(timing)
-    f.reset_timing()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        f.zero_ef()
-        f.calc_ener_force(*d)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / steps
-    stages = {}
-    for s in STAGES:
-        ms, n = f.kernel_time(s)
-        if n:
-            stages[s] = ms / n
-    got = [t.cpu().numpy() for t in f.get_ef()]
-    info = f.info()
-    graph_dt = None
-    if graph:
-        f.set_timing(False)
-        f.set_option("graph", 1)
+    tuned = {k: f.get_option(k) for k in keys}
+
+    def run(graph, timing):
+        f.set_option("graph", graph)
+        f.set_timing(timing)
         for _ in range(warmup):
             f.zero_ef()
             f.calc_ener_force(*d)
+        torch.cuda.synchronize()
+        f.reset_timing()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
             f.zero_ef()
             f.calc_ener_force(*d)
         torch.cuda.synchronize()
-        graph_dt = (time.perf_counter() - t0) / steps
-        got = [t.cpu().numpy() for t in f.get_ef()]
+        return (time.perf_counter() - t0) / steps
+
+    dt = run(tuned["graph"], False)
+    got = [t.cpu().numpy() for t in f.get_ef()]
+    modes = {tuned["graph"]: dt}
+    modes[1 - tuned["graph"]] = run(1 - tuned["graph"], False)
+    run(0, True)
+    stages = {}
+    for s in STAGES:
+        ms, n = f.kernel_time(s)
+        if n:
+            stages[s] = ms / n
+    info = f.info()
     f.close()
-    return dt, stages, got, info, tuned, graph_dt
+    return {"dt": dt, "eager": modes[0], "graph": modes[1], "stages": stages,
+            "got": got, "info": info, "tuned": tuned}
 
 
 def cpu_baseline(h, natom):
@@ -240,7 +233,7 @@ def main():
     ap.add_argument("--schedule", default=None,
                     help="pin the mixed path's schedule instead of "
                          "autotuning: overlap_near,lr_blocks,rmt_shape"
-                         "[,pipeline]")
+                         "[,pipeline[,graph]]")
     ap.add_argument("--profile", action="store_true",
                     help="mixed path only (for rocprofv3 runs), plus one "
                          "1 GiB elementwise copy used to calibrate counters")
@@ -263,12 +256,13 @@ def main():
 
     pinned = ([int(v) for v in args.schedule.split(",")]
               if args.schedule else None)
-    dt_ev, st_mixed, got_mixed, info, tuned, dt_graph = time_path(
-        g, torch, d, natom, "float", args.steps, args.warmup, True, tune=True,
-        graph=not args.profile, schedule=pinned)
-    # the step as run for the result: autotuned schedule, replayed as a HIP
-    # graph (per-stage times come from the eager steps with events)
-    dt_mixed = dt_graph if dt_graph else dt_ev
+    tm = time_path(g, torch, d, natom, "float", args.steps, args.warmup,
+                   tune=True, schedule=pinned)
+    # the step as run for the result: the autotuned schedule, eager or
+    # replayed as a HIP graph as the tuner chose (per-stage times come from
+    # eager steps with events)
+    dt_mixed, st_mixed, got_mixed = tm["dt"], tm["stages"], tm["got"]
+    info, tuned = tm["info"], tm["tuned"]
     log(f"mixed: {dt_mixed * 1e3:.3f} ms/step  stages {st_mixed}")
     if args.profile:
         # calibration: reads 1 GiB and writes 1 GiB
@@ -284,10 +278,12 @@ def main():
                                                 "write": 1 << 30}}),
               flush=True)
         return
-    # the FP64 path gets the same autotuning (its step is too long for
-    # launch overheads to matter, so it runs eagerly)
-    dt_fp64, st_fp64, got_fp64, _, tuned64, _ = time_path(
-        g, torch, d, natom, "double", args.fp64_steps, 1, True, tune=True)
+    # the FP64 path gets the same autotuning
+    t64 = time_path(g, torch, d, natom, "double", args.fp64_steps, 1,
+                    tune=True)
+    dt_fp64, st_fp64, got_fp64 = t64["dt"], t64["stages"], t64["got"]
+    tuned64 = dict(t64["tuned"], eager_ms_per_step=t64["eager"] * 1e3,
+                   graph_ms_per_step=t64["graph"] * 1e3)
     log(f"fp64 : {dt_fp64 * 1e3:.3f} ms/step  stages {st_fp64}")
 
     ref = [t.cpu().numpy() for t in g.coulomb_direct(*d)]
@@ -362,8 +358,8 @@ def main():
         "fp64_path": {"value": natom / dt_fp64, "ms_per_step": dt_fp64 * 1e3,
                       "steps": args.fp64_steps, "schedule": tuned64},
         "speedup_vs_fp64": dt_fp64 / dt_mixed,
-        "schedule": dict(tuned or {}, graph=True,
-                         eager_ms_per_step=dt_ev * 1e3),
+        "schedule": dict(tuned, eager_ms_per_step=tm["eager"] * 1e3,
+                         graph_ms_per_step=tm["graph"] * 1e3),
         "stages_ms": st_mixed,
         "stages_ms_fp64": st_fp64,
         "roofline": roof,

@@ -596,10 +596,11 @@ struct glst_engine {
     return static_cast<double>(ms) / reps;
   }
 
-  // Chooses the fastest schedule for these inputs, one knob at a time: the
-  // near-field overlap, the back-projection split target, the shell-sum
-  // workgroup shape.  Results are identical under every schedule up to the
-  // split boundaries of the back-projection; the accumulators are cleared.
+  // Chooses the fastest schedule for these inputs, one knob at a time: graph
+  // replay, the far-field pipelining, the near-field overlap, the
+  // back-projection split target, the shell-sum workgroup shape.  Results
+  // are identical under every schedule up to the split boundaries of the
+  // back-projection; the accumulators are cleared.
   double autotune(const double *rx, const double *ry, const double *rz,
                   const double *qc, uint32_t reps) {
     if (reps == 0) throw ArgError("autotune needs at least one repetition");
@@ -618,6 +619,17 @@ struct glst_engine {
       }
       if (get() != pick) set(pick);
     };
+    // graph replay changes how the two streams' kernels are dispatched, so
+    // it is tuned first and again last, with the other knobs in between
+    auto graph_knob = [&] {
+      knob([&] { return static_cast<int64_t>(use_graph ? 1 : 0); },
+           [&](int64_t v) {
+             use_graph = v != 0;
+             drop_graph();
+           },
+           {0, 1});
+    };
+    graph_knob();
     if (mixed() && has_lr && Kp > 0 && !rmt_global_path())
       knob([&] { return static_cast<int64_t>(pipeline); },
            [&](int64_t v) {
@@ -646,6 +658,7 @@ struct glst_engine {
              },
              {0, 1});
     }
+    graph_knob();
     zero_ef();
     GLST_HIP(hipStreamSynchronize(stream));
     return best;

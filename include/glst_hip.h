@@ -202,9 +202,9 @@ glst_status_t glst_get_option(glst_handle_t h, const char *name,
 
 /* Autotuner (no counterpart in the reference): times calc_ener_force on
  * these inputs under candidate schedules, `reps` steps each, and keeps the
- * fastest, one knob at a time: "pipeline" (0, 1; CT = float),
- * "overlap_near" (0, 1, 2), "lr_blocks" (powers of two from 1024 to 65536),
- * "rmt_shape" (0, 1).  Clears the
+ * fastest, one knob at a time: "graph" (0, 1; tuned first and again last),
+ * "pipeline" (0, 1; CT = float), "overlap_near" (0, 1, 2), "lr_blocks"
+ * (powers of two from 1024 to 65536), "rmt_shape" (0, 1).  Clears the
  * accumulators afterwards; *best_ms (may be NULL) receives the chosen
  * schedule's time per step. */
 glst_status_t glst_autotune(glst_handle_t h, const double *d_rx,
