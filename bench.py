@@ -52,12 +52,15 @@ def ref_style_errors(got, ref):
 
 
 def time_path(g, torch, d, natom, ct, steps, warmup, tune=False,
-              schedule=None):
+              schedule=None, profile=False):
     """Picks the schedule (the engine's autotuner, which also decides between
     eager steps and graph replay, or `schedule` pinned), then times `steps`
     steps after `warmup`: as tuned (the result), eagerly and as a replayed
-    graph (both reported), and eagerly with per-stage events (the stage
-    breakdown; graph replay records no per-kernel events)."""
+    graph (both reported), eagerly with per-stage events (the stage
+    breakdown; graph replay records no per-kernel events), and eagerly with
+    the near field and the far-field stages one after another on one stream
+    ("kernel_ms": each stage's events then bracket only its own kernels).
+    `profile`: only the tuned and the stage runs (a pinned schedule)."""
     f = g.GLSTForce(natom, TOL, BOX, rcut=RCUT, ct=ct)
     keys = ("overlap_near", "lr_blocks", "rmt_shape", "pipeline", "graph")
     if schedule is not None:  # a pinned schedule (profiling runs)
@@ -83,20 +86,32 @@ def time_path(g, torch, d, natom, ct, steps, warmup, tune=False,
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps
 
+    def stage_times():
+        out = {}
+        for s in STAGES:
+            ms, n = f.kernel_time(s)
+            if n:
+                out[s] = ms / n
+        return out
+
     dt = run(tuned["graph"], False)
     got = [t.cpu().numpy() for t in f.get_ef()]
     modes = {tuned["graph"]: dt}
-    modes[1 - tuned["graph"]] = run(1 - tuned["graph"], False)
+    if not profile:
+        modes[1 - tuned["graph"]] = run(1 - tuned["graph"], False)
     run(0, True)
-    stages = {}
-    for s in STAGES:
-        ms, n = f.kernel_time(s)
-        if n:
-            stages[s] = ms / n
+    stages = stage_times()
+    kernel = stages
+    if not profile:
+        f.set_option("overlap_near", 0)
+        f.set_option("pipeline", 0)
+        run(0, True)
+        kernel = stage_times()
     info = f.info()
     f.close()
-    return {"dt": dt, "eager": modes[0], "graph": modes[1], "stages": stages,
-            "got": got, "info": info, "tuned": tuned}
+    return {"dt": dt, "eager": modes.get(0), "graph": modes.get(1),
+            "stages": stages, "kernel": kernel, "got": got, "info": info,
+            "tuned": tuned}
 
 
 def cpu_baseline(h, natom):
@@ -257,7 +272,7 @@ def main():
     pinned = ([int(v) for v in args.schedule.split(",")]
               if args.schedule else None)
     tm = time_path(g, torch, d, natom, "float", args.steps, args.warmup,
-                   tune=True, schedule=pinned)
+                   tune=True, schedule=pinned, profile=args.profile)
     # the step as run for the result: the autotuned schedule, eager or
     # replayed as a HIP graph as the tuner chose (per-stage times come from
     # eager steps with events)
@@ -294,14 +309,15 @@ def main():
 
     # Roofline of the dominant kernel, the back-projection (calc_lr_ef:
     # lr_mixed_nr_kernel<NR>, one launch per alpha group; avg_ms is the sum
-    # of the group launches of one step, HIP events on the handle's stream).
+    # of the group launches of one step, HIP events on the handle's stream
+    # around them, from the steps run one stage after another).
     # Algorithmic work: 20 flops (sin and cos counted as one each) per
     # (atom, node) pair over the padded node table, the count of the
     # reference's per-pair formulation.
     K = info.num_nodes
     Kp = (K + 255) // 256 * 256
     pairs = natom * Kp
-    lr_ms = st_mixed.get("lr", float("nan"))
+    lr_ms = tm["kernel"].get("lr", float("nan"))
     achieved = 20.0 * pairs / (lr_ms * 1e-3) / 1e12
     traffic = None
     tf = os.path.join(REPO, "profiles", "lr_mixed_288k_traffic.json")
@@ -361,6 +377,7 @@ def main():
         "schedule": dict(tuned, eager_ms_per_step=tm["eager"] * 1e3,
                          graph_ms_per_step=tm["graph"] * 1e3),
         "stages_ms": st_mixed,
+        "kernel_ms": tm["kernel"],
         "stages_ms_fp64": st_fp64,
         "roofline": roof,
         "cpu_baseline": cpu,
