@@ -266,3 +266,79 @@ def test_pipelined_far_field_bitwise(cuda, nw, L, rcut):
     for key in res:
         for a, b in zip(res[0, 0], res[key]):
             np.testing.assert_array_equal(a, b)
+
+
+def _lattice_charges(box, spacing, seed):
+    """Neutral +/-1 charges on a jittered lattice filling a rectangular box
+    (no near-coincident pairs)."""
+    rng = np.random.default_rng(seed)
+    axes = [np.arange(spacing / 2, b, spacing) for b in box]
+    p = np.stack(np.meshgrid(*axes, indexing="ij"), -1).reshape(-1, 3)
+    p = p + rng.uniform(-0.3, 0.3, p.shape)
+    p = np.clip(p, 0.0, None)
+    q = np.where(np.arange(len(p)) % 2 == 0, 1.0, -1.0)
+    if len(q) % 2:
+        q[-1] = 0.0
+    return p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), q
+
+
+@pytest.mark.parametrize("box", [(31.0, 40.0, 50.0), (52.0, 28.0, 70.0)])
+@pytest.mark.parametrize("ct", ["double", "float"])
+def test_rectangular_grids_match_oracle(cuda, box, ct):
+    """Grids with three different dimensions (odd and even z, the shell-sum
+    tables' padding and border differ per axis): S(k), R(k) for every
+    shell-sum shape and the forces against the FP64 oracle."""
+    import torch
+    import glst_amd as g
+    from oracle import oracle as o
+    rx, ry, rz, qc = _lattice_charges(box, 2.6, 3)
+    tol, rcut = 1e-4, 9.0
+    ict = 1 if ct == "float" else 0
+    (ref, (osre, osim, orre, orim)) = o.glst(rx, ry, rz, qc, tol, box, rcut,
+                                           ct=ict, fields=True)
+    sbar, rbar, fbar = ((1e-12, 1e-11, 1e-10) if ct == "double"
+                        else (1e-5, 1e-5, 2e-6))
+    d = _dev([rx, ry, rz, qc], torch)
+    for rs in (0, 1, 2):
+        f = g.GLSTForce(rx.size, tol, box, rcut=rcut, ct=ct)
+        f.set_option("rmt_shape", rs)
+        assert tuple(f.info().ncell) == o.grid_dims(ict, *box, rcut)
+        f.assign_atoms(*d)
+        f.calc_sf()
+        sre, sim = f.sf()
+        f.sum_rmt_sf()
+        rre, rim = f.rmt()
+        scale = np.abs(osre).max()
+        assert np.abs(sre - osre).max() <= sbar * scale
+        assert np.abs(sim - osim).max() <= sbar * scale
+        rscale = np.abs(orre).max()
+        assert np.abs(rre - orre).max() <= rbar * rscale
+        assert np.abs(rim - orim).max() <= rbar * rscale
+        f.close()
+    f = g.GLSTForce(rx.size, tol, box, rcut=rcut, ct=ct)
+    f.calc_ener_force(*d)
+    got = _host(f.get_ef())
+    f.close()
+    assert rel_err(got[:3], ref[:3]) < fbar
+    assert rel_err(got[3:], ref[3:]) < fbar
+    # batched with a second rectangular system: each equals its own handle
+    # (FP64 1e-12, mixed 1e-6: only the back-projection splits differ)
+    h2 = _lattice_charges(box[::-1], 2.6, 4)
+    systems = [((rx, ry, rz, qc), box), (h2, box[::-1])]
+    b = g.GLSTBatch([h[0].size for h, _ in systems],
+                    [bx for _, bx in systems], rcut, tol, ct=ct)
+    cols = _dev([np.concatenate([h[i] for h, _ in systems]) for i in range(4)],
+                torch)
+    b.calc_ener_force(*cols)
+    out = _host(b.get_ef())
+    bbar = 1e-12 if ct == "double" else 1e-6
+    for s, (h, bx) in enumerate(systems):
+        f = g.GLSTForce(h[0].size, tol, bx, rcut=rcut, ct=ct)
+        f.calc_ener_force(*_dev(h, torch))
+        one = _host(f.get_ef())
+        f.close()
+        a0 = b.offset(s)
+        mine = [o[a0:a0 + h[0].size] for o in out]
+        assert rel_err(mine[:3], one[:3]) < bbar
+        assert rel_err(mine[3:], one[3:]) < bbar
+    b.close()
