@@ -42,10 +42,7 @@ size_t BatchLayout::device_bytes() const {
   b += mixed() ? n * sizeof(float4) : 0;  // atoms32
   b += n * 8 * 4;                         // fx..en
   b += hist * 4 + (c + 1) * 4 + c * 4 + sys.size() * 4;
-  b += nodes * (sizeof(double4) + 4);
-  b += (r_in.size() + r_out.size()) * 4;
-  b += t_node0.size() * (12 + sizeof(double4) + sizeof(float4));
-  b += kap64.size() * 12 + t_split.size() * 4;
+  b += table_bytes + t_split.size() * 4;
   b += field * (mixed() ? 8 : 16) * 2;  // S and R
   b += partial * sizeof(double4);
   b += sys.size() * sizeof(dev::SysDev);
@@ -58,9 +55,134 @@ size_t BatchLayout::device_bytes() const {
   return b;
 }
 
+size_t ShapeTables::shape_bytes(const ShapeSetup &shape) {
+  const NodeLayout &L = shape.nl;
+  return L.k64.size() * (sizeof(double4) + 4) +
+         L.node0.size() * (12 + sizeof(double4) + sizeof(float4)) +
+         L.kappa64.size() * 12 + shape.su.groups.r_in.size() * 8;
+}
+
+const ShapeTables::Offsets &ShapeTables::add(
+    const std::shared_ptr<const ShapeSetup> &shape) {
+  auto it = reg.find(shape.get());
+  if (it != reg.end()) return it->second.second;
+  const NodeLayout &L = shape->nl;
+  Offsets o;
+  o.node0 = static_cast<uint32_t>(k64.size());
+  o.task0 = static_cast<uint32_t>(t_node0.size());
+  o.kap0 = static_cast<uint32_t>(kap64.size());
+  o.group0 = static_cast<uint32_t>(r_in.size());
+  if (k64.size() + L.k64.size() > 0xFFFFFFFFull ||
+      t_node0.size() + L.node0.size() > 0xFFFFFFFFull)
+    throw ArgError("shape tables exceed 2^32 entries");
+  k64.insert(k64.end(), L.k64.begin(), L.k64.end());
+  group.insert(group.end(), L.group.begin(), L.group.end());
+  for (size_t t = 0; t < L.node0.size(); ++t) {
+    t_node0.push_back(L.node0[t]);  // local to the system
+    t_nr.push_back(L.nr[t]);
+    t_koff.push_back(o.kap0 + L.koff[t]);
+    t_dir64.push_back(L.dir64[t]);
+    t_dir32.push_back(L.dir32[t]);
+  }
+  kap64.insert(kap64.end(), L.kappa64.begin(), L.kappa64.end());
+  kap32.insert(kap32.end(), L.kappa32.begin(), L.kappa32.end());
+  r_in.insert(r_in.end(), shape->su.groups.r_in.begin(),
+              shape->su.groups.r_in.end());
+  r_out.insert(r_out.end(), shape->su.groups.r_out.begin(),
+               shape->su.groups.r_out.end());
+  order.push_back(shape.get());
+  return reg.emplace(shape.get(), std::make_pair(shape, o)).first->second.second;
+}
+
+void ShapeTables::upload(hipStream_t s) {
+  if (on_dev == order.size()) return;
+  struct Arr {
+    DeviceBuffer *d;
+    const void *h;
+    size_t n, size;
+  };
+  const Arr arrs[] = {
+      {&d_k64, k64.data(), k64.size(), sizeof(double4)},
+      {&d_group, group.data(), group.size(), 4},
+      {&d_r_in, r_in.data(), r_in.size(), 4},
+      {&d_r_out, r_out.data(), r_out.size(), 4},
+      {&d_node0, t_node0.data(), t_node0.size(), 4},
+      {&d_nr, t_nr.data(), t_nr.size(), 4},
+      {&d_koff, t_koff.data(), t_koff.size(), 4},
+      {&d_dir64, t_dir64.data(), t_dir64.size(), sizeof(double4)},
+      {&d_dir32, t_dir32.data(), t_dir32.size(), sizeof(float4)},
+      {&d_kap64, kap64.data(), kap64.size(), 8},
+      {&d_kap32, kap32.data(), kap32.size(), 4}};
+  // entries of the shapes already on the device
+  size_t done[11] = {};
+  if (on_dev > 0) {
+    const Offsets &o = reg.at(order[on_dev]).second;
+    const size_t groups = o.group0;
+    const size_t tasks = o.task0, nodes = o.node0, kaps = o.kap0;
+    const size_t v[11] = {nodes, nodes, groups, groups, tasks, tasks,
+                          tasks, tasks, tasks, kaps, kaps};
+    std::copy(v, v + 11, done);
+  }
+  bool grow = false;
+  for (const Arr &a : arrs) grow = grow || a.n * a.size > a.d->bytes();
+  if (grow) {
+    // batches in flight read the old copies: wait for them, then
+    // reallocate with headroom and upload every shape again
+    GLST_HIP(hipDeviceSynchronize());
+    for (const Arr &a : arrs)
+      if (a.n * a.size > a.d->bytes()) {
+        const size_t want = std::max(a.n * a.size, 2 * a.d->bytes());
+        a.d->release();
+        a.d->allocate(want);
+      }
+    std::fill(done, done + 11, size_t{0});
+    ++gen;
+  }
+  for (size_t i = 0; i < 11; ++i) {
+    const Arr &a = arrs[i];
+    if (a.n > done[i])
+      GLST_HIP(hipMemcpyAsync(a.d->as<char>() + done[i] * a.size,
+                              static_cast<const char *>(a.h) + done[i] * a.size,
+                              (a.n - done[i]) * a.size, hipMemcpyHostToDevice,
+                              s));
+  }
+  on_dev = order.size();
+  // the host vectors may grow (and move) with the next shape
+  GLST_HIP(hipStreamSynchronize(s));
+}
+
+void ShapeTables::clear() {
+  for (auto *v : {&group, &r_in, &r_out, &t_node0, &t_nr, &t_koff}) v->clear();
+  k64.clear();
+  t_dir64.clear();
+  t_dir32.clear();
+  kap64.clear();
+  kap32.clear();
+  reg.clear();
+  order.clear();
+  on_dev = 0;
+  ++gen;
+}
+
+size_t ShapeTables::host_bytes() const {
+  return k64.size() * (sizeof(double4) + 4) +
+         t_node0.size() * (12 + sizeof(double4) + sizeof(float4)) +
+         kap64.size() * 12 + r_in.size() * 8;
+}
+
+size_t ShapeTables::device_bytes() const {
+  size_t b = 0;
+  for (const DeviceBuffer *d : {&d_k64, &d_group, &d_r_in, &d_r_out, &d_node0,
+                                &d_nr, &d_koff, &d_dir64, &d_dir32, &d_kap64,
+                                &d_kap32})
+    b += d->bytes();
+  return b;
+}
+
 BatchLayout pack_batch(int ct,
                        const std::vector<std::shared_ptr<const BatchSystem>> &cs,
-                       size_t lds_limit, uint32_t lr_blocks) {
+                       ShapeTables &tables, size_t lds_limit,
+                       uint32_t lr_blocks) {
   if (cs.empty()) throw ArgError("a batch needs at least one system");
   const uint32_t nsys = static_cast<uint32_t>(cs.size());
   BatchLayout B;
@@ -75,11 +197,19 @@ BatchLayout pack_batch(int ct,
                   std::max<uint64_t>(1, (lr_blocks + tiles - 1) / tiles))
             : 1;
   std::vector<SplitPlan> splits(nsys);
-  uint32_t task_base = 0, split_base = 0;
+  std::vector<uint32_t> task_first(nsys, 0);
+  uint32_t split_base = 0;
+  std::map<const ShapeSetup *, bool> seen;
   for (uint32_t s = 0; s < nsys; ++s) {
     const SystemSetup &u = cs[s]->su();
     const uint32_t na = cs[s]->natom;
     const bool lr = u.has_lr && u.K > 0;
+    const ShapeTables::Offsets off = tables.add(cs[s]->shape);
+    if (!seen[cs[s]->shape.get()]) {
+      seen[cs[s]->shape.get()] = true;
+      B.table_bytes += ShapeTables::shape_bytes(*cs[s]->shape);
+    }
+    task_first[s] = off.task0;
     dev::SysDev d{};
     d.natom = na;
     d.atom0 = static_cast<uint32_t>(B.natom);
@@ -93,9 +223,9 @@ BatchLayout pack_batch(int ct,
     d.h = u.grid.h;
     d.inv_cell = u.inv_cell;
     d.ntile = div_up(na, 256);
-    d.group0 = B.ngroup;
+    d.group0 = off.group0;
     d.Kp = lr ? u.Kp : 0;
-    d.node0 = static_cast<uint32_t>(B.nodes);
+    d.node0 = off.node0;
     d.field0 = B.field;
     d.T = 1;
     d.split0 = split_base;
@@ -115,38 +245,21 @@ BatchLayout pack_batch(int ct,
                        "use a single-system handle");
       B.lds = std::max(B.lds, dev::rmt_lds_bytes(u.grid.nx, u.grid.ny,
                                                  u.grid.nz, d.T));
-      const NodeLayout &L = cs[s]->nl();
       SplitPlan &P = splits[s];
       P = make_splits(u.nodes, u.K, target);
-      const uint32_t koff = static_cast<uint32_t>(B.kap64.size());
-      B.k64.insert(B.k64.end(), L.k64.begin(), L.k64.end());
-      B.group.insert(B.group.end(), L.group.begin(), L.group.end());
-      for (size_t t = 0; t < L.node0.size(); ++t) {
-        B.t_node0.push_back(L.node0[t]);  // local to the system
-        B.t_nr.push_back(L.nr[t]);
-        B.t_koff.push_back(koff + L.koff[t]);
-        B.t_dir64.push_back(L.dir64[t]);
-        B.t_dir32.push_back(L.dir32[t]);
-      }
-      B.kap64.insert(B.kap64.end(), L.kappa64.begin(), L.kappa64.end());
-      B.kap32.insert(B.kap32.end(), L.kappa32.begin(), L.kappa32.end());
       d.nsplit = P.nsplit;
-      // split boundaries as task indices of the batch
-      for (uint32_t q = 0; q < P.nsplit; ++q)
-        B.t_split.push_back(task_base + P.split[q]);
-      B.nodes += u.Kp;
+      // split boundaries as task indices of the shape tables; a split ends
+      // where the next one begins, so every system's splits are followed by
+      // the end of its task range
+      for (uint32_t q = 0; q <= P.nsplit; ++q)
+        B.t_split.push_back(off.task0 + P.split[q]);
       B.field += static_cast<uint64_t>(u.grid.ncell) * u.Kp;
       B.partial += static_cast<uint64_t>(P.nsplit) * na;
-      task_base += static_cast<uint32_t>(L.node0.size());
-      split_base += P.nsplit;
+      split_base += P.nsplit + 1;
     }
-    B.r_in.insert(B.r_in.end(), u.groups.r_in.begin(), u.groups.r_in.end());
-    B.r_out.insert(B.r_out.end(), u.groups.r_out.begin(),
-                   u.groups.r_out.end());
-    B.ngroup += static_cast<uint32_t>(u.groups.r_in.size());
     B.sys.push_back(d);
   }
-  B.t_split.push_back(task_base);  // end of the last split
+  if (B.t_split.empty()) B.t_split.push_back(0);
   // run tables
   for (uint32_t s = 0; s < nsys; ++s) {
     const dev::SysDev &d = B.sys[s];
@@ -160,7 +273,7 @@ BatchLayout pack_batch(int ct,
     B.sf_f64.add(s, d.nx * d.ny * (d.Kp / 256));
     B.lr_f64.add(s, d.ntile * d.nsplit);
     const SplitPlan &P = splits[s];
-    const uint32_t task0 = B.t_split[d.split0];  // first task of the system
+    const uint32_t task0 = task_first[s];  // first task of the system
     for (size_t g = 0; g < P.sf_groups.size(); ++g) {
       const dev::SfGroup &sg = P.sf_groups[g];
       const dev::LrGroup &lg = P.lr_groups[g];
@@ -204,6 +317,7 @@ void glst_batch::open(void *stream_) {
   GLST_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
   GLST_HIP(hipEventCreateWithFlags(&ev_assigned, hipEventDisableTiming));
   GLST_HIP(hipEventCreateWithFlags(&ev_near, hipEventDisableTiming));
+  if (!tables) tables = std::make_shared<ShapeTables>();
 }
 
 namespace {
@@ -224,7 +338,15 @@ void glst_batch::configure(
   GLST_HIP(hipStreamSynchronize(stream));
   drop_graph();
   std::fill(last_in, last_in + 4, nullptr);
-  B = pack_batch(ct, cs, lds_limit, lr_blocks);
+  if (tables->order.size() > kMaxShapeTables) {
+    // bounded: start over (other batches sharing the tables see the new
+    // generation and configure again)
+    GLST_HIP(hipDeviceSynchronize());
+    tables->clear();
+  }
+  B = pack_batch(ct, cs, *tables, lds_limit, lr_blocks);
+  tables->upload(stream);
+  tables_gen = tables->gen;
   const size_t N = B.natom, C = B.ncell;
   atom_cell.reserve(N * 4);
   rank.reserve(N * 4);
@@ -242,21 +364,13 @@ void glst_batch::configure(
   S.reserve(B.field * fbytes);
   R.reserve(B.field * fbytes);
   partial.reserve(B.partial * sizeof(double4));
+  // B and host_seg stay unchanged until the next configure, which first
+  // waits for this batch's stream: the copies may still be in flight
   upload(d_sys, B.sys, stream);
-  upload(k64, B.k64, stream);
-  upload(group, B.group, stream);
-  upload(r_in, B.r_in, stream);
-  upload(r_out, B.r_out, stream);
-  upload(t_node0, B.t_node0, stream);
-  upload(t_nr, B.t_nr, stream);
-  upload(t_koff, B.t_koff, stream);
-  upload(t_dir64, B.t_dir64, stream);
-  upload(t_dir32, B.t_dir32, stream);
-  upload(kap64, B.kap64, stream);
-  upload(kap32, B.kap32, stream);
   upload(t_split, B.t_split, stream);
   // all run tables in one buffer; launches hold offsets until it exists
-  std::vector<dev::Seg> all;
+  std::vector<dev::Seg> &all = host_seg;
+  all.clear();
   std::vector<std::pair<dev::BatchLaunch *, size_t>> where;
   auto stage = [&](const Plan &p, dev::BatchLaunch &L) {
     L.nseg = static_cast<uint32_t>(p.seg.size());
@@ -294,15 +408,14 @@ void glst_batch::configure(
   zero_ef();
   assigned = false;
   check_launch("batch configure");
-  // host vectors uploaded above must outlive the copies
-  GLST_HIP(hipStreamSynchronize(stream));
 }
 
 dev::LrTaskArgs glst_batch::task_args() const {
-  return {t_node0.as<uint32_t>(), t_nr.as<uint32_t>(),
-          t_koff.as<uint32_t>(),  t_dir64.as<double4>(),
-          t_dir32.as<float4>(),   kap64.as<double>(),
-          kap32.as<float>(),      t_split.as<uint32_t>()};
+  const ShapeTables &t = *tables;
+  return {t.d_node0.as<uint32_t>(), t.d_nr.as<uint32_t>(),
+          t.d_koff.as<uint32_t>(),  t.d_dir64.as<double4>(),
+          t.d_dir32.as<float4>(),   t.d_kap64.as<double>(),
+          t.d_kap32.as<float>(),    t_split.as<uint32_t>()};
 }
 
 void glst_batch::release_arenas() {
@@ -310,8 +423,7 @@ void glst_batch::release_arenas() {
   for (DeviceBuffer *b :
        {&d_sys, &d_seg, &atom_cell, &rank, &hist, &cell_count, &cell_start,
         &maxc, &sorted_idx, &cell_sorted, &atoms64, &atoms32, &near4, &fx, &fy,
-        &fz, &en, &k64, &group, &r_in, &r_out, &t_node0, &t_nr, &t_koff,
-        &t_dir64, &t_dir32, &kap64, &kap32, &t_split, &S, &R, &partial})
+        &fz, &en, &t_split, &S, &R, &partial})
     b->release();
 }
 
@@ -360,12 +472,14 @@ void glst_batch::step_async(const double *rx, const double *ry,
   const void *atoms = mixed() ? atoms32.as<void>()
                               : static_cast<const void *>(atoms64.as<void>());
   const dev::LrTaskArgs ta = task_args();
-  dev::sf_batch(mixed(), atoms, cell_start.as<uint32_t>(), k64.as<double4>(),
-                ta, sys, L_sf, S.as<void>(), stream);
+  const ShapeTables &tb = *tables;
+  dev::sf_batch(mixed(), atoms, cell_start.as<uint32_t>(),
+                tb.d_k64.as<double4>(), ta, sys, L_sf, S.as<void>(), stream);
   check_launch("calc_sf");
-  dev::rmt_batch(mixed(), S.as<void>(), k64.as<double4>(), group.as<uint32_t>(),
-                 r_in.as<uint32_t>(), r_out.as<uint32_t>(), sys, L_rmt, B.lds,
-                 R.as<void>(), stream);
+  dev::rmt_batch(mixed(), S.as<void>(), tb.d_k64.as<double4>(),
+                 tb.d_group.as<uint32_t>(), tb.d_r_in.as<uint32_t>(),
+                 tb.d_r_out.as<uint32_t>(), sys, L_rmt, B.lds, R.as<void>(),
+                 stream);
   check_launch("sum_rmt_sf");
   dev::lr_batch(mixed(), atoms, cell_sorted.as<uint32_t>(), ta, sys, L_lr,
                 R.as<void>(), partial.as<double4>(), stream);
@@ -499,11 +613,9 @@ size_t glst_batch::device_bytes() const {
   const DeviceBuffer *bufs[] = {
       &d_sys,   &d_seg,   &atom_cell, &rank,    &hist,    &cell_count,
       &cell_start, &maxc, &sorted_idx, &cell_sorted, &atoms64, &atoms32,
-      &near4,   &fx,      &fy,        &fz,      &en,      &k64,
-      &group,   &r_in,    &r_out,     &t_node0, &t_nr,    &t_koff,
-      &t_dir64, &t_dir32, &kap64,     &kap32,   &t_split, &S,
-      &R,       &partial};
-  size_t b = 0;
+      &near4,   &fx,      &fy,        &fz,      &en,      &t_split,
+      &S,       &R,       &partial};
+  size_t b = tables ? tables->device_bytes() : 0;
   for (auto p : bufs) b += p->bytes();
   return b;
 }
@@ -577,9 +689,11 @@ glst_status_t glst_batch_device_bytes(int ct, uint32_t nsys,
     return GLST_ERR_INVALID;
   }
   return run_guarded(nullptr, [&] {
-    // LDS capacity of gfx950 (160 KiB per workgroup)
+    // LDS capacity of gfx950 (160 KiB per workgroup); the tables are only
+    // sized here, on the host
+    ShapeTables sizing;
     *bytes = pack_batch(ct, make_systems(ct, nsys, natom, box, rcut, tol),
-                        160 * 1024)
+                        sizing, 160 * 1024)
                  .device_bytes();
   });
 }

@@ -4,12 +4,14 @@
 // Every system is set up exactly as a single glst_force (runtime.hpp: grid,
 // alpha groups, cubature, device node order) and cached as a BatchSystem; a
 // batch packs any list of such systems into shared arenas (atoms, cells,
-// nodes, tasks, splits, S/R fields, histogram, partial rows) and runs each
+// splits, S/R fields, histogram, partial rows), refers to the node and task
+// tables of their grid shapes in device-resident ShapeTables, and runs each
 // stage of the force step as one launch over all of them (S(k) and the
 // back-projection: one launch per radial node count).  A workgroup finds its
 // system through a run table of (first workgroup, system, group data)
 // records.  The arenas only grow, so re-packing a batch for a new list of
-// systems allocates nothing once they are large enough.
+// systems allocates nothing once they are large enough, and copies no table
+// of a shape seen before.
 #pragma once
 
 #include "runtime.hpp"
@@ -41,6 +43,50 @@ struct BatchSystem {
 std::shared_ptr<const BatchSystem> make_batch_system(
     uint32_t natom, std::shared_ptr<const ShapeSetup> shape);
 
+// Node, task, kappa and shell-radius tables of grid shapes, resident on the
+// device.  A shape's tables are appended (host) the first time a batch packs
+// a system of that shape and uploaded once; every system of the shape refers
+// to them by offset, so packing a batch copies no table.  A server's batches
+// share one set.  Device capacity grows with headroom; a growth waits for
+// the device, reallocates and uploads everything again, and clear() starts
+// over (both bump `gen`: a batch configured under an older generation must
+// be configured again, which also drops its captured graph).
+struct ShapeTables {
+  struct Offsets {
+    uint32_t node0 = 0, task0 = 0, kap0 = 0, group0 = 0;
+  };
+  std::vector<double4> k64;  // per node (kx, ky, kz, W)
+  std::vector<uint32_t> group;
+  std::vector<uint32_t> r_in, r_out;  // per alpha group
+  std::vector<uint32_t> t_node0, t_nr, t_koff;  // per task; koff global
+  std::vector<double4> t_dir64;
+  std::vector<float4> t_dir32;
+  std::vector<double> kap64;
+  std::vector<float> kap32;
+  DeviceBuffer d_k64, d_group, d_r_in, d_r_out, d_node0, d_nr, d_koff,
+      d_dir64, d_dir32, d_kap64, d_kap32;
+  std::map<const ShapeSetup *,
+           std::pair<std::shared_ptr<const ShapeSetup>, Offsets>>
+      reg;
+  size_t on_dev = 0;  // shapes (in order of addition) already uploaded
+  std::vector<const ShapeSetup *> order;
+  uint64_t gen = 1;
+
+  // offsets of the shape's tables, appending them (host) if new
+  const Offsets &add(const std::shared_ptr<const ShapeSetup> &shape);
+  // uploads the shapes added since the last call (on stream s; returns when
+  // the host copies may change again)
+  void upload(hipStream_t s);
+  void clear();
+  size_t host_bytes() const;
+  size_t device_bytes() const;
+  // table bytes of one shape
+  static size_t shape_bytes(const ShapeSetup &shape);
+};
+
+// Shapes a table set holds before a configure starts it over.
+constexpr size_t kMaxShapeTables = 1024;
+
 // A launch of a batched stage under construction: run table and workgroups.
 struct Plan {
   std::vector<dev::Seg> seg;
@@ -53,36 +99,32 @@ struct Plan {
   }
 };
 
-// Host layout of a whole batch: SysDev records, arena sizes, packed
-// node/task tables and the run tables of every launch.
+// Host layout of a whole batch: SysDev records (offsets into the shape
+// tables and the batch's arenas), arena sizes, the split table and the run
+// tables of every launch.
 struct BatchLayout {
   int ct = GLST_CT_DOUBLE;
   std::vector<std::shared_ptr<const BatchSystem>> cs;
   std::vector<dev::SysDev> sys;
-  uint64_t natom = 0, ncell = 0, hist = 0, nodes = 0, field = 0, partial = 0;
-  uint32_t ngroup = 0;
-  size_t lds = 0;  // dynamic LDS of the shell-sum launch
-  std::vector<double4> k64;
-  std::vector<uint32_t> group, r_in, r_out;
-  std::vector<uint32_t> t_node0, t_nr, t_koff, t_split;
-  std::vector<double4> t_dir64;
-  std::vector<float4> t_dir32;
-  std::vector<double> kap64;
-  std::vector<float> kap32;
+  uint64_t natom = 0, ncell = 0, hist = 0, field = 0, partial = 0;
+  size_t lds = 0;          // dynamic LDS of the shell-sum launch
+  size_t table_bytes = 0;  // shape tables of the batch's distinct shapes
+  std::vector<uint32_t> t_split;
   Plan bin, hist_p, scatter, sr, reduce, rmt, sf_f64, lr_f64;
   std::map<uint32_t, Plan> sf_nr, lr_nr;  // radial count (0: any) -> plan
 
   bool mixed() const { return ct == GLST_CT_FLOAT; }
-  size_t device_bytes() const;  // arena bytes this layout needs
+  size_t device_bytes() const;  // arena and table bytes this layout needs
 };
 
-// Packs systems (all of precision ct) into one batch layout.  lds_limit:
-// the device's LDS per workgroup (a system whose one-node cell table does not
-// fit is rejected); lr_blocks: back-projection workgroups aimed for over the
-// whole batch.
+// Packs systems (all of precision ct) into one batch layout, adding their
+// shapes to `tables` (host side).  lds_limit: the device's LDS per workgroup
+// (a system whose one-node cell table does not fit is rejected); lr_blocks:
+// back-projection workgroups aimed for over the whole batch.
 BatchLayout pack_batch(int ct,
                        const std::vector<std::shared_ptr<const BatchSystem>> &cs,
-                       size_t lds_limit, uint32_t lr_blocks = 4096);
+                       ShapeTables &tables, size_t lds_limit,
+                       uint32_t lr_blocks = 4096);
 
 }  // namespace glst
 
@@ -104,14 +146,18 @@ struct glst_batch {
   hipGraphExec_t graph_exec = nullptr;
   const double *graph_in[4] = {nullptr, nullptr, nullptr, nullptr};
   const double *last_in[4] = {nullptr, nullptr, nullptr, nullptr};
+  // shape tables (shared with the other batches of a server) and the
+  // generation they were packed under
+  std::shared_ptr<glst::ShapeTables> tables;
+  uint64_t tables_gen = 0;
   // arenas (grow-only)
   glst::DeviceBuffer d_sys, d_seg;
   glst::DeviceBuffer atom_cell, rank, hist, cell_count, cell_start, maxc;
   glst::DeviceBuffer sorted_idx, cell_sorted, atoms64, atoms32, near4;
   glst::DeviceBuffer fx, fy, fz, en;
-  glst::DeviceBuffer k64, group, r_in, r_out, t_node0, t_nr, t_koff, t_dir64,
-      t_dir32, kap64, kap32, t_split;
+  glst::DeviceBuffer t_split;
   glst::DeviceBuffer S, R, partial;
+  std::vector<glst::dev::Seg> host_seg;  // run tables (kept for the copy)
   // launches (run tables inside d_seg)
   glst::dev::BatchLaunch L_bin{}, L_hist{}, L_scatter{}, L_sr{}, L_reduce{},
       L_rmt{};
@@ -119,6 +165,8 @@ struct glst_batch {
 
   bool mixed() const { return B.mixed(); }
   uint32_t nsys() const { return static_cast<uint32_t>(B.sys.size()); }
+  // packed under shape tables that have since been reallocated or cleared
+  bool stale() const { return tables_gen != tables->gen; }
 
   ~glst_batch();
   // device, streams and events (once)

@@ -1,6 +1,7 @@
 // Serving loop behind include/glst_serve.h: continuous batching of force
 // requests over the batched engine (batch.hpp) with two batches in flight,
-// admission by device budget, grid-setup cache.
+// admission by device budget, grid-setup cache, and one set of device-
+// resident shape tables shared by both slots.
 #include "glst_serve.h"
 #include "batch.hpp"
 
@@ -114,8 +115,11 @@ struct glst_server {
   }
 
   void open(void *stream) {
+    // the slots' batches share one set of shape tables
+    auto tables = std::make_shared<ShapeTables>();
     for (size_t i = 0; i < kSlots; ++i) {
       auto sl = std::make_unique<Slot>();
+      sl->batch.tables = tables;
       // the caller's stream serves the first slot; the others get their own
       sl->batch.open(i == 0 ? stream : nullptr);
       // a composition run twice in a row (resident systems) is captured as
@@ -149,7 +153,8 @@ struct glst_server {
     }
     SysEntry e;
     e.sys = make_batch_system(natom, shape);
-    e.bytes = pack_batch(ct, {e.sys}, lds_limit).device_bytes() +
+    ShapeTables sizing;  // host-side sizing only
+    e.bytes = pack_batch(ct, {e.sys}, sizing, lds_limit).device_bytes() +
               8 * sizeof(double) * static_cast<size_t>(natom);
     return systems.insert(skey, std::move(e));
   }
@@ -202,7 +207,7 @@ struct glst_server {
       natom += queue[i].sys->natom;
     }
     glst_batch &b = sl.batch;
-    if (comp != sl.composition) {
+    if (comp != sl.composition || b.stale()) {
       b.configure(ct, cs);
       if (b.device_bytes() > share()) {
         // arenas grown by earlier batches: size them for this one alone

@@ -139,3 +139,31 @@ def test_batch_single_system_equals_handle(cuda):
     ref = _single(systems, 1e-5, "double", torch)[0]
     assert rel_err(got[:3], ref[:3]) < 1e-12
     b.close()
+
+
+@pytest.mark.parametrize("ct,bar", [("double", 1e-12), ("float", 1e-6)])
+def test_batch_shared_shapes_match_single_handles(cuda, ct, bar):
+    """Systems of one grid shape share its device-resident node and task
+    tables (one task range, separate back-projection splits and partial
+    rows): a batch of repeated shapes, interleaved with others, still gives
+    every system what its own handle gives."""
+    import torch
+    import glst_amd as g
+    # 41-44 A boxes all give a 5x5x5 grid at rcut 9; 31 A gives 4x4x4
+    spec = [(2200, 41.0), (700, 31.0), (2300, 42.5), (2100, 44.0),
+            (650, 31.0)]
+    systems = [(L, g.make_water_box(nw, L, seed=200 + k))
+               for k, (nw, L) in enumerate(spec)]
+    b = g.GLSTBatch([h[0].size for _, h in systems], [L for L, _ in systems],
+                    RCUT, 1e-4, ct=ct)
+    assert tuple(b.info(0).ncell) == tuple(b.info(2).ncell) \
+        == tuple(b.info(3).ncell)
+    b.calc_ener_force(*_pack(systems, torch))
+    out = [t.cpu().numpy() for t in b.get_ef()]
+    refs = _single(systems, 1e-4, ct, torch)
+    for s, ((_, h), ref) in enumerate(zip(systems, refs)):
+        a0 = b.offset(s)
+        got = [o[a0:a0 + h[0].size] for o in out]
+        assert rel_err(got[:3], ref[:3]) < bar
+        assert rel_err(got[3:], ref[3:]) < bar
+    b.close()

@@ -162,3 +162,29 @@ def test_batches_overlap_in_flight(cuda):
     for (L, h), o in zip(systems, outs):
         _check(o, _single(L, h, "double", torch), 1e-12)
     srv.close()
+
+
+def test_new_shapes_stream_through_both_slots(cuda):
+    """Requests of six grid shapes (2^3 to 7^3 cells), one per batch, both
+    slots in flight: the shared shape tables grow with each new shape
+    (reallocating while the other slot's batch may still run) and every
+    answer stays exact."""
+    import torch
+    import glst_amd as g
+    waters = [150, 900, 2500, 400, 3200, 1200, 6000, 260]
+    systems = [_system(nw, 300 + k) for k, nw in enumerate(waters)]
+    srv = g.GLSTServer(TOL, 1 << 34, max_batch=1, ct="float")
+    ins = [_dev(h, torch) for _, h in systems]
+    outs = [_outs(h[0].size, torch) for _, h in systems]
+    answered = 0
+    for (L, h), i, o in zip(systems, ins, outs):
+        srv.submit(L, RCUT, *i, *o)
+        answered += srv.poll()
+    while srv.pending():
+        answered += srv.step()
+    assert answered == len(systems)
+    st = srv.stats()
+    assert st["batches"] == len(systems) and st["max_in_flight"] == 2
+    for (L, h), o in zip(systems, outs):
+        _check(o, _single(L, h, "float", torch), 1e-6)
+    srv.close()
