@@ -323,8 +323,9 @@ void glst_batch::open(void *stream_) {
 namespace {
 
 template <typename T>
-void upload(DeviceBuffer &b, const std::vector<T> &v, hipStream_t s) {
-  b.reserve(v.size() * sizeof(T));
+void upload(DeviceBuffer &b, const std::vector<T> &v, hipStream_t s,
+            unsigned headroom) {
+  b.grow(v.size() * sizeof(T), headroom);
   if (!v.empty())
     GLST_HIP(hipMemcpyAsync(b.as<void>(), v.data(), v.size() * sizeof(T),
                             hipMemcpyHostToDevice, s));
@@ -348,26 +349,27 @@ void glst_batch::configure(
   tables->upload(stream);
   tables_gen = tables->gen;
   const size_t N = B.natom, C = B.ncell;
-  atom_cell.reserve(N * 4);
-  rank.reserve(N * 4);
-  sorted_idx.reserve(N * 4);
-  cell_sorted.reserve(N * 4);
-  atoms64.reserve(N * sizeof(double4));
-  if (mixed()) atoms32.reserve(N * sizeof(float4));
-  near4.reserve(N * sizeof(double4));
-  for (DeviceBuffer *b : {&fx, &fy, &fz, &en}) b->reserve(N * 8);
-  hist.reserve(std::max<uint64_t>(B.hist, 1) * 4);
-  cell_count.reserve(C * 4);
-  cell_start.reserve((C + 1) * 4);
-  maxc.reserve(nsys() * 4);
+  const unsigned hr = arena_headroom;
+  atom_cell.grow(N * 4, hr);
+  rank.grow(N * 4, hr);
+  sorted_idx.grow(N * 4, hr);
+  cell_sorted.grow(N * 4, hr);
+  atoms64.grow(N * sizeof(double4), hr);
+  if (mixed()) atoms32.grow(N * sizeof(float4), hr);
+  near4.grow(N * sizeof(double4), hr);
+  for (DeviceBuffer *b : {&fx, &fy, &fz, &en}) b->grow(N * 8, hr);
+  hist.grow(std::max<uint64_t>(B.hist, 1) * 4, hr);
+  cell_count.grow(C * 4, hr);
+  cell_start.grow((C + 1) * 4, hr);
+  maxc.grow(nsys() * 4, hr);
   const size_t fbytes = mixed() ? sizeof(float2) : sizeof(double2);
-  S.reserve(B.field * fbytes);
-  R.reserve(B.field * fbytes);
-  partial.reserve(B.partial * sizeof(double4));
+  S.grow(B.field * fbytes, hr);
+  R.grow(B.field * fbytes, hr);
+  partial.grow(B.partial * sizeof(double4), hr);
   // B and host_seg stay unchanged until the next configure, which first
   // waits for this batch's stream: the copies may still be in flight
-  upload(d_sys, B.sys, stream);
-  upload(t_split, B.t_split, stream);
+  upload(d_sys, B.sys, stream, hr);
+  upload(t_split, B.t_split, stream, hr);
   // all run tables in one buffer; launches hold offsets until it exists
   std::vector<dev::Seg> &all = host_seg;
   all.clear();
@@ -399,7 +401,7 @@ void glst_batch::configure(
     stage(B.sf_f64, L_sf[0]);
     stage(B.lr_f64, L_lr[0]);
   }
-  upload(d_seg, all, stream);
+  upload(d_seg, all, stream, hr);
   for (auto &w : where) w.first->seg = d_seg.as<dev::Seg>() + w.second;
   // the S(k) kernels never write padded node slots
   if (B.field)

@@ -146,6 +146,8 @@ struct glst_batch {
   hipGraphExec_t graph_exec = nullptr;
   const double *graph_in[4] = {nullptr, nullptr, nullptr, nullptr};
   const double *last_in[4] = {nullptr, nullptr, nullptr, nullptr};
+  // per cent added to an arena when it grows (0: exact sizes)
+  unsigned arena_headroom = 25;
   // shape tables (shared with the other batches of a server) and the
   // generation they were packed under
   std::shared_ptr<glst::ShapeTables> tables;

@@ -66,6 +66,13 @@ class DeviceBuffer {
     release();
     allocate(bytes);
   }
+  // reserve with `headroom` percent more on a growth, so that sizes that
+  // vary from batch to batch stop reallocating
+  void grow(size_t bytes, unsigned headroom) {
+    if (bytes <= bytes_) return;
+    release();
+    allocate(bytes + bytes / 100 * headroom);
+  }
   void allocate(size_t bytes) {
     if (bytes == 0) return;
     void *p = nullptr;

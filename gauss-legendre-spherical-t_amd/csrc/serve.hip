@@ -212,7 +212,9 @@ struct glst_server {
       if (b.device_bytes() > share()) {
         // arenas grown by earlier batches: size them for this one alone
         b.release_arenas();
+        b.arena_headroom = 0;
         b.configure(ct, cs);
+        b.arena_headroom = 25;
       }
       sl.composition = comp;
       ++stats.reconfigures;
@@ -221,8 +223,8 @@ struct glst_server {
     }
     hipStream_t s = b.stream;
     for (int k = 0; k < 4; ++k) {
-      sl.in[k].reserve(natom * sizeof(double));
-      sl.out[k].reserve(natom * sizeof(double));
+      sl.in[k].grow(natom * sizeof(double), 25);
+      sl.out[k].grow(natom * sizeof(double), 25);
     }
     // gather the requests' inputs into the packed arrays, scatter back after
     sl.host_runs.clear();
@@ -237,7 +239,7 @@ struct glst_server {
         sl.host_runs.push_back(
             {sl.out[k].as<double>() + off, queue[i].out[k], n});
     }
-    sl.runs.reserve(sl.host_runs.size() * sizeof(dev::CopyRun));
+    sl.runs.grow(sl.host_runs.size() * sizeof(dev::CopyRun), 25);
     GLST_HIP(hipMemcpyAsync(sl.runs.as<void>(), sl.host_runs.data(),
                             sl.host_runs.size() * sizeof(dev::CopyRun),
                             hipMemcpyHostToDevice, s));
