@@ -1372,23 +1372,39 @@ __global__ void lr_reduce_kernel(const double4 *__restrict__ partial,
 constexpr uint32_t kSrBlock = 256;
 constexpr uint32_t kSrMaxPhase = 16;
 
-// (F, e) of one pair; r = r_i - r_j, e = q_i q_j / |r|, F = e r / |r|^2.
-// Mixed: the separation in FP64, the rest in FP32.
+// (F, e) / q_i of one pair; r = r_i - r_j, e = q_i q_j / |r|,
+// F = e r / |r|^2.  q_i multiplies the sums over a neighbour cell once, when
+// they are flushed.  Mixed: the separation in FP64, the rest in FP32.  SELF:
+// the pair may be the atom with itself (only in its own cell), which then
+// adds nothing.
 template <bool MIXED>
 struct PairAcc;
+
+// A near-field LDS tile record.  Mixed tiles carry q_j already rounded to
+// FP32 in the low word of .w (rounded once per staged atom, not per pair).
+template <bool MIXED>
+__device__ __forceinline__ double4 sr_tile_record(const double4 &a) {
+  if constexpr (MIXED) {
+    const float q = static_cast<float>(a.w);
+    return make_double4(a.x, a.y, a.z,
+                        __hiloint2double(0, __float_as_int(q)));
+  } else {
+    return a;
+  }
+}
 
 template <>
 struct PairAcc<true> {
   float fx = 0.f, fy = 0.f, fz = 0.f, en = 0.f;
-  __device__ void add(const double4 &ai, float qi, const double4 &aj,
-                      bool self) {
+  template <bool SELF>
+  __device__ void add(const double4 &ai, const double4 &aj, bool self) {
     const float rx = static_cast<float>(ai.x - aj.x);
     const float ry = static_cast<float>(ai.y - aj.y);
     const float rz = static_cast<float>(ai.z - aj.z);
     float r2 = rx * rx + ry * ry + rz * rz;
-    r2 = self ? INFINITY : r2;  // no self interaction
+    if (SELF) r2 = self ? INFINITY : r2;  // no self interaction
     const float ir = __builtin_amdgcn_rsqf(r2);
-    const float e = qi * static_cast<float>(aj.w) * ir;
+    const float e = __int_as_float(__double2loint(aj.w)) * ir;
     const float f = e * ir * ir;
     fx += f * rx;
     fy += f * ry;
@@ -1400,13 +1416,13 @@ struct PairAcc<true> {
 template <>
 struct PairAcc<false> {
   double fx = 0.0, fy = 0.0, fz = 0.0, en = 0.0;
-  __device__ void add(const double4 &ai, double qi, const double4 &aj,
-                      bool self) {
+  template <bool SELF>
+  __device__ void add(const double4 &ai, const double4 &aj, bool self) {
     const double rx = ai.x - aj.x, ry = ai.y - aj.y, rz = ai.z - aj.z;
     double r2 = rx * rx + ry * ry + rz * rz;
-    r2 = self ? INFINITY : r2;
+    if (SELF) r2 = self ? INFINITY : r2;
     const double ir = 1.0 / sqrt(r2);
-    const double e = qi * aj.w * ir;
+    const double e = aj.w * ir;
     const double f = e * ir * ir;
     fx += f * rx;
     fy += f * ry;
@@ -1458,28 +1474,39 @@ __device__ __forceinline__ void sr_body(
           const uint32_t b0 = start[nb], b1 = start[nb + 1];
           // Mixed: FP32 sums over one neighbour cell, flushed to FP64.
           PairAcc<MIXED> acc;
-          for (uint32_t t0 = b0; t0 < b1; t0 += kSrBlock) {
-            const uint32_t tn = (b1 - t0 < kSrBlock) ? b1 - t0 : kSrBlock;
-            __syncthreads();  // the previous tile has been consumed
-            if (tid < tn) tile[tid] = atoms[t0 + tid];
-            __syncthreads();
-            if (wave_busy) {
-              uint32_t j = s;
-              for (; j + 3 * S < tn; j += 4 * S) {
-                const double4 q0 = tile[j], q1 = tile[j + S],
-                              q2 = tile[j + 2 * S], q3 = tile[j + 3 * S];
-                acc.add(ai, qi, q0, t0 + j == ii);
-                acc.add(ai, qi, q1, t0 + j + S == ii);
-                acc.add(ai, qi, q2, t0 + j + 2 * S == ii);
-                acc.add(ai, qi, q3, t0 + j + 3 * S == ii);
+          // The self pair exists only in the atom's own cell: the other 26
+          // cells run the pair loop without the test (uniform branch).
+          auto sweep = [&](auto self_tag) {
+            constexpr bool SELF = decltype(self_tag)::value;
+            for (uint32_t t0 = b0; t0 < b1; t0 += kSrBlock) {
+              const uint32_t tn = (b1 - t0 < kSrBlock) ? b1 - t0 : kSrBlock;
+              __syncthreads();  // the previous tile has been consumed
+              if (tid < tn) tile[tid] = sr_tile_record<MIXED>(atoms[t0 + tid]);
+              __syncthreads();
+              if (wave_busy) {
+                uint32_t j = s;
+                for (; j + 3 * S < tn; j += 4 * S) {
+                  const double4 q0 = tile[j], q1 = tile[j + S],
+                                q2 = tile[j + 2 * S], q3 = tile[j + 3 * S];
+                  acc.template add<SELF>(ai, q0, t0 + j == ii);
+                  acc.template add<SELF>(ai, q1, t0 + j + S == ii);
+                  acc.template add<SELF>(ai, q2, t0 + j + 2 * S == ii);
+                  acc.template add<SELF>(ai, q3, t0 + j + 3 * S == ii);
+                }
+                for (; j < tn; j += S)
+                  acc.template add<SELF>(ai, tile[j], t0 + j == ii);
               }
-              for (; j < tn; j += S) acc.add(ai, qi, tile[j], t0 + j == ii);
             }
-          }
-          FX += acc.fx;
-          FY += acc.fy;
-          FZ += acc.fz;
-          EN += acc.en;
+          };
+          if (nb == cell)
+            sweep(std::true_type{});
+          else
+            sweep(std::false_type{});
+          // exact products in FP64 for the mixed path (24 x 24 bits)
+          FX += static_cast<double>(qi) * acc.fx;
+          FY += static_cast<double>(qi) * acc.fy;
+          FZ += static_cast<double>(qi) * acc.fz;
+          EN += static_cast<double>(qi) * acc.en;
         }
       }
     }
