@@ -72,6 +72,9 @@ struct glst_engine {
   // mixed path, tables in LDS: pipeline S(k) -> shell sums ->
   // back-projection across alpha groups, the shell sums on a third stream
   int pipeline = 1;
+  // mixed back-projection: 1 over atom groups (eight same-cell atoms share
+  // each remote-sum load), 0 one atom per lane
+  int lr_atom_groups = 1;
   hipStream_t rmt_stream = nullptr;
   std::vector<hipEvent_t> ev_sf, ev_rmt;  // per alpha group
   // back-projection workgroups aimed for (many short workgroups balance the
@@ -86,6 +89,9 @@ struct glst_engine {
   // atoms
   DeviceBuffer atom_cell, rank, hist, cell_count, cell_start, maxc;
   DeviceBuffer sorted_idx, cell_sorted, atoms64, atoms32;
+  // mixed path: atom groups of the back-projection, first group of each cell
+  DeviceBuffer atom_grp, ag_start;
+  uint32_t nag_max = 0;
   DeviceBuffer fx, fy, fz, en;
   // nodes (device order: direction-major inside each group)
   DeviceBuffer k64, group, r_in, r_out;
@@ -120,7 +126,7 @@ struct glst_engine {
                                   &r_in,      &r_out, &S,          &R,
                                   &partial,   &atoms32, &t_node0,   &t_nr,
                                   &t_koff,    &t_dir64, &t_dir32,   &kap64,
-                                  &kap32,     &t_split, &near4};
+                                  &kap32,     &t_split, &near4, &atom_grp, &ag_start};
     size_t s = 0;
     for (auto b : bufs) s += b->bytes();
     return s;
@@ -197,6 +203,10 @@ struct glst_engine {
     cell_sorted.resize(n * 4);
     atoms64.resize(n * sizeof(double4));
     atoms32.resize(mixed() ? n * sizeof(float4) : 0);
+    nag_max = mixed() && has_lr && n > 0 ? dev::atom_groups_max(n, grid.ncell)
+                                         : 0;
+    atom_grp.resize(static_cast<size_t>(nag_max) * sizeof(uint2));
+    ag_start.resize(nag_max ? static_cast<size_t>(grid.ncell) * 4 : 0);
     near4.resize(n * sizeof(double4));
     fx.resize(n * 8);
     fy.resize(n * 8);
@@ -253,6 +263,11 @@ struct glst_engine {
     up(t_split, split.data(), split.size() * 4);
   }
 
+  // atom groups for the mixed back-projection (nullptr: one atom per lane)
+  const uint2 *lr_ag() const {
+    return lr_atom_groups && nag_max ? atom_grp.as<uint2>() : nullptr;
+  }
+
   // atom records the long-range kernels read
   const void *lr_atoms() const {
     return mixed() ? atoms32.as<void>() : atoms64.as<void>();
@@ -278,13 +293,16 @@ struct glst_engine {
       Scope sc(timer, "sort", stream);
       dev::hist_offsets(hist.as<uint32_t>(), nchunk, grid.ncell,
                         cell_count.as<uint32_t>(), stream);
+      uint2 *ag = nag_max ? atom_grp.as<uint2>() : nullptr;
       dev::cell_scan(cell_count.as<uint32_t>(), grid.ncell,
-                     cell_start.as<uint32_t>(), maxc.as<uint32_t>(), stream);
+                     cell_start.as<uint32_t>(), maxc.as<uint32_t>(), stream,
+                     ag_start.as<uint32_t>(), ag, nag_max);
       dev::bin_scatter(mixed(), rx, ry, rz, qc, natom,
                        atom_cell.as<uint32_t>(), rank.as<uint32_t>(),
                        hist.as<uint32_t>(), cell_start.as<uint32_t>(), grid,
                        sorted_idx.as<uint32_t>(), cell_sorted.as<uint32_t>(),
-                       atoms64.as<double4>(), atoms32.as<float4>(), stream);
+                       atoms64.as<double4>(), atoms32.as<float4>(), stream,
+                       ag_start.as<uint32_t>(), ag);
       check_launch("sort");
     }
     assigned = true;
@@ -352,8 +370,8 @@ struct glst_engine {
     Scope sc(timer, "lr", stream);
     const dev::LrTaskArgs ta = task_args();
     dev::lr(mixed(), lr_atoms(), cell_sorted.as<uint32_t>(), ta, lr_groups,
-            R.as<void>(), Kp, natom, nsplit,
-            partial.as<double4>(), stream);
+            R.as<void>(), Kp, natom, nsplit, partial.as<double4>(), stream,
+            lr_ag(), nag_max);
     check_launch("lr");
   }
 
@@ -460,7 +478,7 @@ struct glst_engine {
       GLST_HIP(hipStreamWaitEvent(stream, ev_rmt[rmt_need[g]], 0));
       dev::lr(true, lr_atoms(), cell_sorted.as<uint32_t>(), ta, &lr_groups[g],
               1, R.as<void>(), Kp, natom, nsplit, partial.as<double4>(),
-              stream);
+              stream, lr_ag(), nag_max);
     }
     check_launch("lr");
   }
@@ -1053,8 +1071,15 @@ glst_status_t glst_get_rmt(glst_handle_t h, double *re, double *im) {
     std::vector<double> r(n), i(n);
     h->field_to_host(h->R, h->mixed(), 2, false, r.data(), i.data());
     if (h->mixed()) {
-      // undo the stored exp(+i k.c) factor in FP64
+      // the stored (kappa |V|, arg V / 2 pi) back to V = exp(+i k.c) R_c(k),
+      // then undo the exp(+i k.c) factor in FP64
       const auto &g = h->grid;
+      const double two_pi = 2.0 * M_PI;
+      std::vector<double> inv_kappa(h->K);
+      for (uint32_t k = 0; k < h->K; ++k)
+        inv_kappa[k] = two_pi / std::sqrt(h->nodes.kx[k] * h->nodes.kx[k] +
+                                          h->nodes.ky[k] * h->nodes.ky[k] +
+                                          h->nodes.kz[k] * h->nodes.kz[k]);
       for (uint32_t c = 0; c < g.ncell; ++c) {
         const uint32_t x = c / (g.ny * g.nz), y = (c / g.nz) % g.ny,
                        z = c % g.nz;
@@ -1065,7 +1090,8 @@ glst_status_t glst_get_rmt(glst_handle_t h, double *re, double *im) {
               h->nodes.kx[k] * cx + h->nodes.ky[k] * cy + h->nodes.kz[k] * cz;
           const double cs = std::cos(th), sn = std::sin(th);
           const size_t o = static_cast<size_t>(c) * h->K + k;
-          const double a = r[o], b = i[o];
+          const double mag = r[o] * inv_kappa[k], arg = two_pi * i[o];
+          const double a = mag * std::cos(arg), b = mag * std::sin(arg);
           r[o] = a * cs + b * sn;
           i[o] = b * cs - a * sn;
         }
@@ -1101,6 +1127,10 @@ glst_status_t glst_set_option(glst_handle_t h, const char *name,
     } else if (n == "graph") {
       if (value != 0 && value != 1) throw ArgError("graph must be 0 or 1");
       h->use_graph = value != 0;
+    } else if (n == "lr_atom_groups") {
+      if (value != 0 && value != 1)
+        throw ArgError("lr_atom_groups must be 0 or 1");
+      h->lr_atom_groups = static_cast<int>(value);
     } else
       throw ArgError("unknown option \"" + n + "\"");
   });
@@ -1122,6 +1152,8 @@ glst_status_t glst_get_option(glst_handle_t h, const char *name,
       *value = h->lr_blocks;
     else if (n == "graph")
       *value = h->use_graph ? 1 : 0;
+    else if (n == "lr_atom_groups")
+      *value = h->lr_atom_groups;
     else
       throw ArgError("unknown option \"" + n + "\"");
   });

@@ -36,6 +36,14 @@ namespace {
 
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 
+// Atoms per atom group of the mixed back-projection: consecutive sorted atoms
+// of one cell (a cell of n atoms has ceil(n / kAgAtoms) groups).
+#ifndef GLST_AG_ATOMS
+#define GLST_AG_ATOMS 4
+#endif
+constexpr uint32_t kAgAtoms = GLST_AG_ATOMS;
+static_assert(kAgAtoms == 4 || kAgAtoms == 8, "atoms per group: 4 or 8");
+
 // Blocks are dealt to the 8 XCDs round-robin (linear block b runs on XCD
 // b % 8).  Renumber so that each XCD processes a contiguous range of logical
 // tiles, which keeps neighbouring tiles in the same L2.
@@ -52,6 +60,9 @@ __device__ inline uint32_t xcd_remap(uint32_t b, uint32_t nb) {
 #endif
 #ifndef GLST_LR_FLUSH_TASKS
 #define GLST_LR_FLUSH_TASKS 4
+#endif
+#ifndef GLST_LR_EN_FLUSH_TASKS
+#define GLST_LR_EN_FLUSH_TASKS 16
 #endif
 #ifndef GLST_ACCURATE_SINCOS
 #define GLST_ACCURATE_SINCOS 0
@@ -198,48 +209,68 @@ __global__ void hist_offsets_kernel(uint32_t *__restrict__ hist,
 }
 
 // start[c] = base + exclusive scan of count (start[ncell] too when
-// write_end); *maxc = max count.
+// write_end); *maxc = max count.  With ag_start: ag_start[c] = exclusive scan
+// of the cells' atom-group counts ceil(count / kAgAtoms), and the entries of
+// ag from the total up to nag_max are set to (0, 0) (bin_scatter fills the
+// others).
 __device__ __forceinline__ void cell_scan_body(
     const uint32_t *__restrict__ count, uint32_t ncell,
     uint32_t *__restrict__ start, uint32_t *__restrict__ maxc, uint32_t base,
-    bool write_end) {
+    bool write_end, uint32_t *__restrict__ ag_start = nullptr,
+    uint2 *__restrict__ ag = nullptr, uint32_t nag_max = 0) {
   __shared__ uint32_t sums[1024];
   __shared__ uint32_t maxs[1024];
+  __shared__ uint32_t gsum[1024];
   const uint32_t tid = threadIdx.x;
   const uint32_t seg = (ncell + 1023u) / 1024u;
   const uint32_t b = tid * seg;
   const uint32_t e = (b + seg < ncell) ? b + seg : ncell;
-  uint32_t s = 0, m = 0;
+  uint32_t s = 0, m = 0, gs = 0;
   for (uint32_t c = b; c < e; ++c) {
-    s += count[c];
-    m = count[c] > m ? count[c] : m;
+    const uint32_t n = count[c];
+    s += n;
+    m = n > m ? n : m;
+    gs += (n + kAgAtoms - 1) / kAgAtoms;
   }
   sums[tid] = s;
   maxs[tid] = m;
+  gsum[tid] = gs;
   __syncthreads();
   for (uint32_t off = 1; off < 1024; off <<= 1) {
     const uint32_t add = tid >= off ? sums[tid - off] : 0u;
     const uint32_t mx = tid >= off ? maxs[tid - off] : 0u;
+    const uint32_t gadd = tid >= off ? gsum[tid - off] : 0u;
     __syncthreads();
     sums[tid] += add;
     maxs[tid] = mx > maxs[tid] ? mx : maxs[tid];
+    gsum[tid] += gadd;
     __syncthreads();
   }
   uint32_t run = base + sums[tid] - s;  // exclusive prefix of this segment
+  uint32_t grun = gsum[tid] - gs;
   for (uint32_t c = b; c < e; ++c) {
     start[c] = run;
     run += count[c];
+    if (ag_start) {
+      ag_start[c] = grun;
+      grun += (count[c] + kAgAtoms - 1) / kAgAtoms;
+    }
   }
   if (tid == 1023) {
     if (write_end) start[ncell] = base + sums[1023];
     *maxc = maxs[1023];
   }
+  if (ag)
+    for (uint32_t k = gsum[1023] + tid; k < nag_max; k += 1024)
+      ag[k] = make_uint2(0u, 0u);
 }
 
 __global__ __launch_bounds__(1024) void cell_scan_kernel(
     const uint32_t *__restrict__ count, uint32_t ncell,
-    uint32_t *__restrict__ start, uint32_t *__restrict__ maxc) {
-  cell_scan_body(count, ncell, start, maxc, 0u, true);
+    uint32_t *__restrict__ start, uint32_t *__restrict__ maxc,
+    uint32_t *__restrict__ ag_start, uint2 *__restrict__ ag,
+    uint32_t nag_max) {
+  cell_scan_body(count, ncell, start, maxc, 0u, true, ag_start, ag, nag_max);
 }
 
 // Atom i of one system (inputs, atom_cell, rank, hist and start at the
@@ -254,14 +285,21 @@ __device__ __forceinline__ void bin_scatter_body(
     const uint32_t *__restrict__ start, uint32_t ncell, uint32_t ny,
     uint32_t nz, double h, uint32_t *__restrict__ sorted_idx,
     uint32_t *__restrict__ cell_sorted, double4 *__restrict__ atoms64,
-    float4 *__restrict__ atoms32, uint32_t i, uint32_t atom0) {
+    float4 *__restrict__ atoms32, uint32_t i, uint32_t atom0,
+    const uint32_t *__restrict__ ag_start = nullptr,
+    uint2 *__restrict__ ag = nullptr) {
   if (i >= natom) return;
   const uint32_t c = atom_cell[i];
   const uint32_t ch = i / kBinChunk;
-  const uint32_t dst =
-      start[c] + hist[static_cast<size_t>(ch) * ncell + c] + rank[i];
+  const uint32_t r = hist[static_cast<size_t>(ch) * ncell + c] + rank[i];
+  const uint32_t dst = start[c] + r;
   sorted_idx[dst] = atom0 + i;
   cell_sorted[dst] = c;
+  if (ag && r % kAgAtoms == 0) {  // the first atom of its group
+    const uint32_t n = start[c + 1] - start[c] - r;
+    ag[ag_start[c] + r / kAgAtoms] =
+        make_uint2(dst, n < kAgAtoms ? n : kAgAtoms);
+  }
   const double x = rx[i], y = ry[i], z = rz[i], q = qc[i];
   atoms64[dst] = make_double4(x, y, z, q);
   if (MIXED) {
@@ -285,10 +323,12 @@ __global__ void bin_scatter_kernel(
     const uint32_t *__restrict__ start, uint32_t ncell, uint32_t ny,
     uint32_t nz, double h, uint32_t *__restrict__ sorted_idx,
     uint32_t *__restrict__ cell_sorted, double4 *__restrict__ atoms64,
-    float4 *__restrict__ atoms32) {
+    float4 *__restrict__ atoms32, const uint32_t *__restrict__ ag_start,
+    uint2 *__restrict__ ag) {
   bin_scatter_body<MIXED>(rx, ry, rz, qc, natom, atom_cell, rank, hist, start,
                           ncell, ny, nz, h, sorted_idx, cell_sorted, atoms64,
-                          atoms32, blockIdx.x * blockDim.x + threadIdx.x, 0u);
+                          atoms32, blockIdx.x * blockDim.x + threadIdx.x, 0u,
+                          ag_start, ag);
 }
 
 // ---------------------------------------------------------------------------
@@ -398,12 +438,50 @@ __device__ inline double2 cube_sum(const G &P, uint32_t x, uint32_t y,
   return box_sum(P, x0, y0, z0, x1, y1, z1, ny, nz);
 }
 
-// Mixed-path remote sum of one (cell, node): exp(+i k.c) R_c(k), FP64 in,
-// FP32 out.
+// arg(x + i y) / 2 pi in revolutions, in [-1/2, 1/2]; (0, 0) gives 0.  With
+// t = min(|x|, |y|) / max(|x|, |y|) in [0, 1], atan(t) / 2 pi = t P(t^2) for a
+// degree-7 polynomial P fitted here (least squares on Chebyshev points with
+// Lawson reweighting toward the minimax error); the octant then follows from
+// |y| > |x| and the signs.  Emulated in FP32 with an exact reciprocal, the
+// error stays below 4.3e-8 revolutions (2.7e-7 rad) against atan2 in FP64
+// over 2e6 random (x, y); V_RCP_F32 adds at most one ulp of t.
+__device__ inline float arg_rev(float x, float y) {
+  const float ax = fabsf(x), ay = fabsf(y);
+  const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+  const float t = mx > 0.f ? mn * __builtin_amdgcn_rcpf(mx) : 0.f;
+  const float s = t * t;
+  float p = -0.0006453040987253189f;
+  p = fmaf(p, s, 0.0034795973915606737f);
+  p = fmaf(p, s, -0.008898723870515823f);
+  p = fmaf(p, s, 0.015346036292612553f);
+  p = fmaf(p, s, -0.022136272862553596f);
+  p = fmaf(p, s, 0.031745944172143936f);
+  p = fmaf(p, s, -0.05304612219333649f);
+  p = fmaf(p, s, 0.15915483236312866f);
+  float r = p * t;  // [0, 1/8]
+  r = ay > ax ? 0.25f - r : r;
+  r = x < 0.f ? 0.5f - r : r;
+  return y < 0.f ? -r : r;
+}
+
+// Mixed-path remote sum of one (cell, node), stored in polar form: for
+// V = exp(+i k.c) R_c(k) (FP64 in) the pair (kappa |V|, arg V / 2 pi) in
+// FP32, kappa = |k| / 2 pi in revolutions per Angstrom.  The back-projection
+// then forms Re and kappa Im of exp(i k.d) V as |V| cos(t) and kappa |V|
+// sin(t) with t = kappa p + arg V / 2 pi: one FMA per pair for the phase and
+// none for a complex product.
 __device__ inline void store_rmt_mixed(void *R, size_t off, double re,
-                                       double im) {
+                                       double im, float kappa_rev) {
+  const float fr = static_cast<float>(re), fi = static_cast<float>(im);
+  const float mag = __builtin_amdgcn_sqrtf(fmaf(fr, fr, fi * fi));
   static_cast<float2 *>(R)[off] =
-      make_float2(static_cast<float>(re), static_cast<float>(im));
+      make_float2(kappa_rev * mag, arg_rev(fr, fi));
+}
+
+// |k| / 2 pi of a node (revolutions per Angstrom).
+__device__ inline float kappa_rev(const double4 &k) {
+  constexpr double kInv2Pi = 0.15915494309189533577;
+  return static_cast<float>(sqrt(k.x * k.x + k.y * k.y + k.z * k.z) * kInv2Pi);
 }
 
 // R_c(k) = W_k (cube_out - cube_in); the mixed path multiplies by
@@ -423,7 +501,8 @@ __device__ inline void shell_store(const G &P, uint32_t x, uint32_t y,
   if (MIXED) {
     const double pr = a.x * b.x - a.y * b.y, pi = a.x * b.y + a.y * b.x;
     const double cr = pr * c.x - pi * c.y, ci = pr * c.y + pi * c.x;
-    store_rmt_mixed(R, off, re * cr - im * ci, re * ci + im * cr);
+    store_rmt_mixed(R, off, re * cr - im * ci, re * ci + im * cr,
+                    kappa_rev(k));
   } else {
     static_cast<double2 *>(R)[off] = make_double2(re, im);
   }
@@ -752,7 +831,9 @@ __device__ __forceinline__ void rmt_compute(
     const uint32_t node = node0 + t;
     const uint32_t gr = group[node];
     const uint32_t rin = r_in[gr], rout = r_out[gr];
-    const double w = k64[node].w;
+    const double4 kn = k64[node];
+    const double w = kn.w;
+    const float kr = MIXED ? kappa_rev(kn) : 0.f;
     const double2 *Pt = P + t * ts;
     const CubeXZ co = cube_xz(x, z, rout, nx, nz, plane);
     const CubeXZ ci = cube_xz(x, z, rin, nx, nz, plane);
@@ -773,7 +854,7 @@ __device__ __forceinline__ void rmt_compute(
       if (MIXED) {
         const double2 b = ph[nx + y];
         const double er = qr * b.x - qi * b.y, ei = qr * b.y + qi * b.x;
-        store_rmt_mixed(R, off, dr * er - di * ei, dr * ei + di * er);
+        store_rmt_mixed(R, off, dr * er - di * ei, dr * ei + di * er, kr);
       } else {
         static_cast<double2 *>(R)[off] = make_double2(dr * w, di * w);
       }
@@ -1032,8 +1113,11 @@ __global__ __launch_bounds__(kLrBlock) void lr_f64_kernel(
 // its tasks have the same number NR of radial nodes and share the same
 // kappa_i.  With NR a template parameter the kappa_i sit in registers for the
 // whole split, the NR remote sums of a direction are fetched together with
-// immediate offsets, and the radial loop unrolls completely: per node one
-// multiply for the phase, sin, cos, one multiply and four FMAs.
+// immediate offsets, and the radial loop unrolls completely.  R holds
+// (A, phi) = (kappa |V|, arg V / 2 pi) per (cell, node), so per node:
+// t = kappa_i p + phi (one FMA), sin, cos, b += A sin t, e_i += A cos t.
+// The energy sums e_i are kept per radial node and divided by kappa_i when
+// they are flushed (en = sum_i |V_i| cos t_i).
 template <int NR>
 __device__ __forceinline__ void lr_mixed_dirs(
     const float4 &a, const float2 *__restrict__ Rc, const LrTasks &tk,
@@ -1043,48 +1127,58 @@ __device__ __forceinline__ void lr_mixed_dirs(
   const float *kp = tk.kappa32 + tk.koff[tb];
 #pragma unroll
   for (int i = 0; i < NR; ++i) kap[i] = kp[i];
-  float fx = 0.f, fy = 0.f, fz = 0.f, en = 0.f;
-  uint32_t pending = 0;
-  for (uint32_t t = tb; t < te; ++t) {
-    const float2 *rp = Rc + tk.node0[t];
-    const float4 d = tk.dir32[t];
-    float2 r[NR];
+  float fx = 0.f, fy = 0.f, fz = 0.f, e[NR];
 #pragma unroll
-    for (int i = 0; i < NR; ++i) r[i] = rp[i];
-    const float p = d.x * a.x + d.y * a.y + d.z * a.z;
-    float e0 = 0.f, e1 = 0.f, b0 = 0.f, b1 = 0.f;
+  for (int i = 0; i < NR; ++i) e[i] = 0.f;
+  auto flush_en = [&] {
+    float en = 0.f;
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
-      const float tt = fract_rev(kap[i] * p);
-      const float sn = sin_rev(tt), cs = cos_rev(tt);
-      if (i & 1) {
-        e1 = fmaf(cs, r[i].x, e1);
-        e1 = fmaf(-sn, r[i].y, e1);
-        b1 = fmaf(kap[i], fmaf(cs, r[i].y, sn * r[i].x), b1);
-      } else {
-        e0 = fmaf(cs, r[i].x, e0);
-        e0 = fmaf(-sn, r[i].y, e0);
-        b0 = fmaf(kap[i], fmaf(cs, r[i].y, sn * r[i].x), b0);
-      }
+      en = fmaf(e[i], 1.0f / kap[i], en);
+      e[i] = 0.f;
     }
-    const float b = b0 + b1;
-    en += e0 + e1;
-    fx = fmaf(d.x, b, fx);
-    fy = fmaf(d.y, b, fy);
-    fz = fmaf(d.z, b, fz);
-    if (++pending == GLST_LR_FLUSH_TASKS) {
-      FX += fx;
-      FY += fy;
-      FZ += fz;
-      EN += en;
-      fx = fy = fz = en = 0.f;
-      pending = 0;
+    EN += en;
+  };
+  // Force sums go to FP64 every GLST_LR_FLUSH_TASKS directions, energy
+  // sums every GLST_LR_EN_FLUSH_TASKS; the inner loop has no branch.
+  constexpr uint32_t kEnEvery = GLST_LR_EN_FLUSH_TASKS / GLST_LR_FLUSH_TASKS;
+  uint32_t rounds = 0;
+  for (uint32_t t0 = tb; t0 < te; t0 += GLST_LR_FLUSH_TASKS) {
+    const uint32_t t1 =
+        (te - t0 < GLST_LR_FLUSH_TASKS) ? te : t0 + GLST_LR_FLUSH_TASKS;
+    for (uint32_t t = t0; t < t1; ++t) {
+      const float2 *rp = Rc + tk.node0[t];
+      const float4 d = tk.dir32[t];
+      float2 r[NR];
+#pragma unroll
+      for (int i = 0; i < NR; ++i) r[i] = rp[i];
+      const float p = d.x * a.x + d.y * a.y + d.z * a.z;
+      float b0 = 0.f, b1 = 0.f;
+#pragma unroll
+      for (int i = 0; i < NR; ++i) {
+        const float tt = fmaf(kap[i], p, r[i].y);
+        const float sn = sin_rev(tt), cs = cos_rev(tt);
+        e[i] = fmaf(r[i].x, cs, e[i]);
+        if (i & 1)
+          b1 = fmaf(r[i].x, sn, b1);
+        else
+          b0 = fmaf(r[i].x, sn, b0);
+      }
+      const float b = b0 + b1;
+      fx = fmaf(d.x, b, fx);
+      fy = fmaf(d.y, b, fy);
+      fz = fmaf(d.z, b, fz);
+    }
+    FX += fx;
+    FY += fy;
+    FZ += fz;
+    fx = fy = fz = 0.f;
+    if (++rounds == kEnEvery) {
+      flush_en();
+      rounds = 0;
     }
   }
-  FX += fx;
-  FY += fy;
-  FZ += fz;
-  EN += en;
+  flush_en();
 }
 
 // Any radial count (larger than the unrolled cases).
@@ -1102,12 +1196,11 @@ __device__ __forceinline__ void lr_mixed_dirs_any(
     float e = 0.f, b = 0.f;
     for (uint32_t j = 0; j < nr; ++j) {
       const float kj = kap[j];
-      const float tt = fract_rev(kj * p);
-      const float sn = sin_rev(tt), cs = cos_rev(tt);
       const float2 r = Rc[n0 + j];
-      e = fmaf(cs, r.x, e);
-      e = fmaf(-sn, r.y, e);
-      b = fmaf(kj, fmaf(cs, r.y, sn * r.x), b);
+      const float tt = fmaf(kj, p, r.y);
+      const float sn = sin_rev(tt), cs = cos_rev(tt);
+      e = fmaf(r.x * __builtin_amdgcn_rcpf(kj), cs, e);
+      b = fmaf(r.x, sn, b);
     }
     en += e;
     fx = fmaf(d.x, b, fx);
@@ -1179,6 +1272,164 @@ constexpr auto lr_nr_table(std::integer_sequence<int, N...>) {
 constexpr int kLrMaxUnrolled = 40;
 const auto kLrNrKernels =
     lr_nr_table(std::make_integer_sequence<int, kLrMaxUnrolled + 1>{});
+
+// ---------------------------------------------------------------------------
+// Mixed back-projection over atom groups.  An atom group is up to kAgAtoms
+// consecutive sorted atoms of one cell (a cell of n atoms has ceil(n / 8)
+// groups); ag[g] = (first atom, count), count 0 past the last group.  Eight
+// lanes serve one group: lane m takes directions tb + m, tb + m + 8, ... of
+// the split, loads that direction's NR remote sums (A, phi) of the group's
+// cell once per pass and applies them to all eight atoms, so a pair costs one
+// FMA for the phase, sin, cos and two FMAs, and the vector memory path
+// carries one R value per eight pairs.  A reduce-scatter over the eight lanes
+// then gives lane m the sums of atom m.
+constexpr uint32_t kAgLanes = 8;
+constexpr uint32_t kAgPerBlock = kLrBlock / kAgLanes;
+
+// v[0 .. n) -> v[0 .. n/2): the lane keeps the half of the atoms selected by
+// its `mask` bit and adds the partner lane's values of that half.
+template <class T, int N>
+__device__ __forceinline__ void ag_halve(T (&v)[N], int n, uint32_t lane,
+                                         uint32_t mask) {
+  const bool hi = (lane & mask) != 0;
+#pragma unroll
+  for (int j = 0; j < N / 2; ++j) {
+    if (j < n / 2) {
+      const T keep = hi ? v[j + n / 2] : v[j];
+      const T send = hi ? v[j] : v[j + n / 2];
+      v[j] = keep + __shfl_xor(send, mask, kAgLanes);
+    }
+  }
+}
+
+// Sum over the 8 lanes of a group of v[u] (u = atom of the group); lane m
+// returns the sum of atom m mod kAgAtoms.
+template <class T>
+__device__ __forceinline__ T ag_reduce_scatter(T (&v)[kAgAtoms],
+                                               uint32_t lane) {
+  if constexpr (kAgAtoms == 8) {
+    ag_halve(v, 8, lane, 4);
+  } else {
+#pragma unroll
+    for (uint32_t j = 0; j < kAgAtoms; ++j)
+      v[j] += __shfl_xor(v[j], 4, kAgLanes);
+  }
+  ag_halve(v, 4, lane, 2);
+  ag_halve(v, 2, lane, 1);
+  return v[0];
+}
+
+template <int NR>
+__device__ __forceinline__ void lr_ag_body(
+    const float4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
+    const uint2 *__restrict__ ag, const LrTasks &tk,
+    const float2 *__restrict__ R, uint32_t Kp, uint32_t natom, uint32_t tile,
+    uint32_t split, uint32_t prow, double4 *__restrict__ partial) {
+  const uint32_t lane = threadIdx.x & (kAgLanes - 1);
+  const uint2 g = ag[tile * kAgPerBlock + threadIdx.x / kAgLanes];
+  const uint32_t cnt = g.y, first = cnt ? g.x : 0u;
+  const uint32_t tb = tk.split[split], te = tk.split[split + 1];
+  // uniform per wave: every group of the wave is padding, or no tasks
+  if (__ballot(cnt != 0) == 0 || tb >= te) return;
+  const float2 *Rc = R + static_cast<size_t>(cell_sorted[first]) * Kp;
+  float ax[kAgAtoms], ay[kAgAtoms], az[kAgAtoms];
+#pragma unroll
+  for (uint32_t u = 0; u < kAgAtoms; ++u) {
+    const float4 a = atoms[first + (u < cnt ? u : 0u)];
+    ax[u] = a.x;
+    ay[u] = a.y;
+    az[u] = a.z;
+  }
+  float kap[NR], ikap[NR];
+  const float *kp = tk.kappa32 + tk.koff[tb];
+#pragma unroll
+  for (int i = 0; i < NR; ++i) {
+    kap[i] = kp[i];
+    // uniform: kept in a scalar register (readfirstlane moves 32-bit words)
+    ikap[i] = __int_as_float(
+        __builtin_amdgcn_readfirstlane(__float_as_int(1.0f / kap[i])));
+  }
+  // Forces in FP32 per lane; the potential's terms cancel far more, so each
+  // pass's energy sum goes into FP64.
+  float fx[kAgAtoms], fy[kAgAtoms], fz[kAgAtoms];
+  double en[kAgAtoms];
+#pragma unroll
+  for (uint32_t u = 0; u < kAgAtoms; ++u) {
+    fx[u] = fy[u] = fz[u] = 0.f;
+    en[u] = 0.0;
+  }
+  // One pass: lane direction t, all atoms of the group.  FULL: every lane
+  // has a direction (the last pass of a split may leave lanes idle).
+  auto pass = [&](uint32_t t0, auto full_tag) {
+    constexpr bool FULL = decltype(full_tag)::value;
+    const uint32_t t = t0 + lane;
+    const bool on = FULL || t < te;
+    const uint32_t tt = on ? t : tb;
+    const float4 d = tk.dir32[tt];
+    const float2 *rp = Rc + tk.node0[tt];
+    float A[NR], V[NR], ph[NR];
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      const float2 r = rp[i];
+      A[i] = on ? r.x : 0.f;  // idle lanes add nothing
+      V[i] = A[i] * ikap[i];  // |V|
+      ph[i] = r.y;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kAgAtoms; ++u) {
+      const float p = d.x * ax[u] + d.y * ay[u] + d.z * az[u];
+      float e = 0.f, b = 0.f;
+#pragma unroll
+      for (int i = 0; i < NR; ++i) {
+        const float th = fmaf(kap[i], p, ph[i]);
+        e = fmaf(V[i], cos_rev(th), e);
+        b = fmaf(A[i], sin_rev(th), b);
+      }
+      en[u] += static_cast<double>(e);
+      fx[u] = fmaf(d.x, b, fx[u]);
+      fy[u] = fmaf(d.y, b, fy[u]);
+      fz[u] = fmaf(d.z, b, fz[u]);
+    }
+  };
+  uint32_t t0 = tb;
+  for (; t0 + kAgLanes <= te; t0 += kAgLanes) pass(t0, std::true_type{});
+  if (t0 < te) pass(t0, std::false_type{});
+  const float FX = ag_reduce_scatter(fx, lane);
+  const float FY = ag_reduce_scatter(fy, lane);
+  const float FZ = ag_reduce_scatter(fz, lane);
+  const double EN = ag_reduce_scatter(en, lane);
+  if (lane < cnt) {
+    const uint32_t i = first + lane;
+    const double q = static_cast<double>(atoms[i].w);
+    const double qf = q * kTwoPi;  // k = 2 pi k_rev
+    partial[static_cast<size_t>(prow) * natom + i] =
+        make_double4(qf * FX, qf * FY, qf * FZ, q * EN);
+  }
+}
+
+template <int NR>
+__global__ __launch_bounds__(kLrBlock) void lr_ag_kernel(
+    const float4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
+    const uint2 *__restrict__ ag, LrTasks tk, const float2 *__restrict__ R,
+    uint32_t Kp, uint32_t natom, uint32_t ntile, uint32_t split0,
+    double4 *__restrict__ partial) {
+  const uint32_t split = split0 + blockIdx.y;
+  lr_ag_body<NR>(atoms, cell_sorted, ag, tk, R, Kp, natom,
+                 xcd_remap(blockIdx.x, ntile), split, split, partial);
+}
+
+typedef void (*LrAgKernel)(const float4 *, const uint32_t *, const uint2 *,
+                           LrTasks, const float2 *, uint32_t, uint32_t,
+                           uint32_t, uint32_t, double4 *);
+
+template <int... N>
+constexpr auto lr_ag_table(std::integer_sequence<int, N...>) {
+  return std::array<LrAgKernel, sizeof...(N)>{{&lr_ag_kernel<N + 1>...}};
+}
+
+// index NR - 1 (NR = 1 .. kLrMaxUnrolled)
+const auto kLrAgKernels =
+    lr_ag_table(std::make_integer_sequence<int, kLrMaxUnrolled>{});
 
 // Mixed structure factors, one instantiation per radial count NR: lane = one
 // direction s of an alpha group, one wave per (cell, 64 directions), atom
@@ -1896,8 +2147,10 @@ void hist_offsets(uint32_t *hist, uint32_t nchunk, uint32_t ncell,
 }
 
 void cell_scan(const uint32_t *count, uint32_t ncell, uint32_t *start,
-               uint32_t *maxc, hipStream_t s) {
-  cell_scan_kernel<<<1, 1024, 0, s>>>(count, ncell, start, maxc);
+               uint32_t *maxc, hipStream_t s, uint32_t *ag_start, uint2 *ag,
+               uint32_t nag_max) {
+  cell_scan_kernel<<<1, 1024, 0, s>>>(count, ncell, start, maxc,
+                                      ag ? ag_start : nullptr, ag, nag_max);
 }
 
 void bin_scatter(bool mixed, const double *rx, const double *ry,
@@ -1906,16 +2159,16 @@ void bin_scatter(bool mixed, const double *rx, const double *ry,
                  const uint32_t *hist, const uint32_t *start,
                  const GridShape &g, uint32_t *sorted_idx,
                  uint32_t *cell_sorted, double4 *atoms64, float4 *atoms32,
-                 hipStream_t s) {
+                 hipStream_t s, const uint32_t *ag_start, uint2 *ag) {
   if (natom == 0) return;
   if (mixed)
     bin_scatter_kernel<true><<<blocks_for(natom, 256), 256, 0, s>>>(
         rx, ry, rz, qc, natom, atom_cell, rank, hist, start, g.ncell, g.ny,
-        g.nz, g.h, sorted_idx, cell_sorted, atoms64, atoms32);
+        g.nz, g.h, sorted_idx, cell_sorted, atoms64, atoms32, ag_start, ag);
   else
     bin_scatter_kernel<false><<<blocks_for(natom, 256), 256, 0, s>>>(
         rx, ry, rz, qc, natom, atom_cell, rank, hist, start, g.ncell, g.ny,
-        g.nz, g.h, sorted_idx, cell_sorted, atoms64, atoms32);
+        g.nz, g.h, sorted_idx, cell_sorted, atoms64, atoms32, ag_start, ag);
 }
 
 void sf(bool mixed, const void *atoms, const uint32_t *start,
@@ -2010,10 +2263,17 @@ void rmt_global(bool mixed, const void *S, double2 *G, const double4 *k64,
   }
 }
 
+uint32_t atom_groups_max(uint32_t natom, uint32_t ncell) {
+  const uint64_t n =
+      (static_cast<uint64_t>(natom) + (kAgAtoms - 1ull) * ncell) / kAgAtoms;
+  // whole workgroups of groups
+  return static_cast<uint32_t>((n + kAgPerBlock) / kAgPerBlock * kAgPerBlock);
+}
+
 void lr(bool mixed, const void *atoms, const uint32_t *cell_sorted,
         const LrTaskArgs &ta, const LrGroup *groups, size_t ngroup,
         const void *R, uint32_t Kp, uint32_t natom, uint32_t nsplit,
-        double4 *partial, hipStream_t s) {
+        double4 *partial, hipStream_t s, const uint2 *ag, uint32_t nag_max) {
   if (natom == 0 || nsplit == 0) return;
   const uint32_t ntile = blocks_for(natom, kLrBlock);
   const dim3 grid(ntile, nsplit);
@@ -2023,6 +2283,14 @@ void lr(bool mixed, const void *atoms, const uint32_t *cell_sorted,
     for (size_t q = 0; q < ngroup; ++q) {
       const LrGroup &gr = groups[q];
       if (gr.nsplit == 0) continue;
+      if (ag && gr.nr >= 1 && gr.nr <= static_cast<uint32_t>(kLrMaxUnrolled)) {
+        const uint32_t ngt = nag_max / kAgPerBlock;
+        kLrAgKernels[gr.nr - 1]<<<dim3(ngt, gr.nsplit), kLrBlock, 0, s>>>(
+            static_cast<const float4 *>(atoms), cell_sorted, ag, tk,
+            static_cast<const float2 *>(R), Kp, natom, ngt, gr.split0,
+            partial);
+        continue;
+      }
       const LrNrKernel k =
           kLrNrKernels[gr.nr <= static_cast<uint32_t>(kLrMaxUnrolled) ? gr.nr : 0];
       k<<<dim3(ntile, gr.nsplit), kLrBlock, 0, s>>>(

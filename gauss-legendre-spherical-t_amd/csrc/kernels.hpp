@@ -37,19 +37,24 @@ void bin_chunks(bool ct_float, const double *rx, const double *ry,
 // hist[chunk][cell] -> exclusive offsets over chunks; count[cell] = total.
 void hist_offsets(uint32_t *hist, uint32_t nchunk, uint32_t ncell,
                   uint32_t *count, hipStream_t s);
-// start[0..ncell] = exclusive scan of count; *maxc = max count.
+// start[0..ncell] = exclusive scan of count; *maxc = max count.  With ag
+// (mixed path): ag_start[c] = first atom group of cell c, and ag entries past
+// the last group up to nag_max = (0, 0).
 void cell_scan(const uint32_t *count, uint32_t ncell, uint32_t *start,
-               uint32_t *maxc, hipStream_t s);
+               uint32_t *maxc, hipStream_t s, uint32_t *ag_start = nullptr,
+               uint2 *ag = nullptr, uint32_t nag_max = 0);
 // Scatter atoms to cell-sorted order; writes sorted_idx, the sorted atoms'
 // cells and the sorted atom records: double4 absolute (x,y,z,q) always, and
-// for the mixed path float4 cell-centre-relative (x,y,z,q).
+// for the mixed path float4 cell-centre-relative (x,y,z,q).  With ag: the
+// (first atom, count) entry of every atom group, from cell_scan's ag_start.
 void bin_scatter(bool mixed, const double *rx, const double *ry,
                  const double *rz, const double *qc, uint32_t natom,
                  const uint32_t *atom_cell, const uint32_t *rank,
                  const uint32_t *hist, const uint32_t *start,
                  const GridShape &g, uint32_t *sorted_idx,
                  uint32_t *cell_sorted, double4 *atoms64, float4 *atoms32,
-                 hipStream_t s);
+                 hipStream_t s, const uint32_t *ag_start = nullptr,
+                 uint2 *ag = nullptr);
 
 // ---- structure factors (calc_sf) -------------------------------------------
 // S(k) is stored in node-pair tiles: the slots of nodes 2m and 2m+1 of all
@@ -167,16 +172,24 @@ struct LrGroup {
 // split's tasks.
 // The mixed path launches once per listed group; the FP64 path covers
 // every split in one launch.
+// ag (mixed path, optional): the atom groups of atom_groups(), nag_max of
+// them; groups with an unrolled radial count then use the atom-group kernel.
 void lr(bool mixed, const void *atoms, const uint32_t *cell_sorted,
         const LrTaskArgs &tasks, const LrGroup *groups, size_t ngroup,
         const void *R, uint32_t Kp, uint32_t natom, uint32_t nsplit,
-        double4 *partial, hipStream_t s);
+        double4 *partial, hipStream_t s, const uint2 *ag = nullptr,
+        uint32_t nag_max = 0);
+// Atom groups of the mixed back-projection: up to 4 consecutive sorted atoms
+// of one cell, written by cell_scan and bin_scatter.  atom_groups_max bounds
+// their number (a multiple of the groups per workgroup).
+uint32_t atom_groups_max(uint32_t natom, uint32_t ncell);
 inline void lr(bool mixed, const void *atoms, const uint32_t *cell_sorted,
                const LrTaskArgs &tasks, const std::vector<LrGroup> &groups,
                const void *R, uint32_t Kp, uint32_t natom, uint32_t nsplit,
-               double4 *partial, hipStream_t s) {
+               double4 *partial, hipStream_t s, const uint2 *ag = nullptr,
+               uint32_t nag_max = 0) {
   lr(mixed, atoms, cell_sorted, tasks, groups.data(), groups.size(), R, Kp,
-     natom, nsplit, partial, s);
+     natom, nsplit, partial, s, ag, nag_max);
 }
 // f[i] += sum over splits (fixed order) + near[i] (when near != nullptr).
 void lr_reduce(const double4 *partial, uint32_t nsplit, uint32_t natom,

@@ -347,6 +347,8 @@ struct SplitPlan {
   uint32_t nsplit = 0;
 };
 
+constexpr uint32_t kSplitAlign = 8;
+
 inline SplitPlan make_splits(const NodeTable &nodes, uint32_t K,
                              uint32_t target) {
   SplitPlan P;
@@ -356,15 +358,20 @@ inline SplitPlan make_splits(const NodeTable &nodes, uint32_t K,
     const uint32_t M = static_cast<uint32_t>(nodes.dirs[g].size());
     const double share =
         static_cast<double>(target) * nodes.count[g] / std::max<uint32_t>(K, 1);
+    // split boundaries fall on multiples of kSplitAlign directions (the
+    // mixed back-projection serves directions in passes of that many lanes)
+    const uint32_t units = (M + kSplitAlign - 1) / kSplitAlign;
     uint32_t ns = static_cast<uint32_t>(share + 0.5);
-    ns = std::max<uint32_t>(1, std::min<uint32_t>(ns, M));
+    ns = std::max<uint32_t>(1, std::min<uint32_t>(ns, units));
     P.lr_groups.push_back({static_cast<uint32_t>(nodes.kappa[g].size()),
                            static_cast<uint32_t>(P.split.size() - 1), ns});
     P.sf_groups.push_back(
         {static_cast<uint32_t>(nodes.kappa[g].size()), base, M});
-    for (uint32_t q = 1; q <= ns; ++q)
+    for (uint32_t q = 1; q <= ns; ++q) {
+      const uint64_t u = (static_cast<uint64_t>(units) * q) / ns;
       P.split.push_back(base + static_cast<uint32_t>(
-                                   (static_cast<uint64_t>(M) * q) / ns));
+                                   std::min<uint64_t>(u * kSplitAlign, M)));
+    }
     base += M;
   }
   if (P.split.size() == 1) P.split.push_back(0);  // no tasks at all
