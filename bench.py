@@ -331,9 +331,10 @@ def main():
         f"fp64 {err_fp64:.3e}")
 
     # Roofline of the dominant kernel, the back-projection (calc_lr_ef:
-    # lr_mixed_nr_kernel<NR>, one launch per alpha group; avg_ms is the sum
-    # of the group launches of one step, HIP events on the handle's stream
-    # around them, from the steps run one stage after another).
+    # lr_ag_kernel<NR> over atom groups, one launch per alpha group; avg_ms
+    # is the sum of the group launches of one step, HIP events on the
+    # handle's stream around them, from the steps run one stage after
+    # another).
     # Algorithmic work: 20 flops (sin and cos counted as one each) per
     # (atom, node) pair over the padded node table, the count of the
     # reference's per-pair formulation.
@@ -343,13 +344,13 @@ def main():
     lr_ms = tm["kernel"].get("lr", float("nan"))
     achieved = 20.0 * pairs / (lr_ms * 1e-3) / 1e12
     traffic = None
-    tf = os.path.join(REPO, "profiles", "lr_mixed_288k_traffic.json")
+    tf = os.path.join(REPO, "profiles", "lr_ag_288k_traffic.json")
     if os.path.exists(tf):
         with open(tf) as fh:
             traffic = json.load(fh).get("bytes_per_step")
     roof = {
-        "kernel": "lr_mixed_nr_kernel<NR> (calc_lr_ef back-projection, "
-                  "all alpha-group launches of one step)",
+        "kernel": "lr_ag_kernel<NR> (calc_lr_ef back-projection over atom "
+                  "groups, all alpha-group launches of one step)",
         "bound": "valu",
         "achieved": achieved,
         "peak": PEAK_FP32_VALU_TFLOPS,

@@ -171,7 +171,8 @@ glst_status_t glst_get_cells(glst_handle_t h, uint32_t *atom_cell,
  * calc_sf (sf_re_/sf_im_, ncell x num_nodes, cell-major). */
 glst_status_t glst_get_sf(glst_handle_t h, double *re, double *im);
 /* Host copy of the remote sums R_c(k) = W_k * sum_{c' in shell} S_c'(k)
- * after sum_rmt_sf (rmt_sum_re_/rmt_sum_im_, ncell x num_nodes). */
+ * after sum_rmt_sf (rmt_sum_re_/rmt_sum_im_, ncell x num_nodes).  CT = float
+ * keeps them on the device in polar form; they are converted back here. */
 glst_status_t glst_get_rmt(glst_handle_t h, double *re, double *im);
 
 /* Tuning options of a handle:
@@ -190,7 +191,11 @@ glst_status_t glst_get_rmt(glst_handle_t h, double *re, double *im);
  *   complete, alongside S(k) of later groups and the back-projection of
  *   earlier ones (0: one stage after another).  Bitwise identical.
  *   "lr_blocks" (default 16384): back-projection workgroups aimed for; the
- *   direction ranges of each alpha group are split accordingly.
+ *   direction ranges of each alpha group are split accordingly (on
+ *   multiples of 8 directions).
+ *   "lr_atom_groups" (default 1): CT = float back-projection over atom
+ *   groups (up to 4 same-cell atoms share each remote-sum load, 8 lanes per
+ *   group) (1), or one atom per lane (0).  Results agree to FP32 rounding.
  *   "graph" (default 0): calc_ener_force captures the whole step as a HIP
  *   graph for the input arrays of its first call and replays it while the
  *   same arrays are passed (per-kernel timing is then not recorded).

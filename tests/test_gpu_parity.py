@@ -232,6 +232,44 @@ def test_mixed_options_agree(cuda, water1k):
         f.set_option("no_such_option", 1)
 
 
+@pytest.mark.parametrize("nw,L,rcut,tol", [(1000, 31.04, 9.0, 1e-5),
+                                            (6000, 57.0, 9.0, 1e-4)])
+def test_atom_group_back_projection(cuda, nw, L, rcut, tol):
+    """The mixed back-projection over atom groups (up to four same-cell atoms
+    per group, eight lanes sharing each direction's remote sums) agrees with
+    the one-atom-per-lane kernel and with the FP64 path, for split counts
+    from one split per alpha group to one pass of eight directions per
+    split, with cells whose atom counts are not multiples of four."""
+    import torch
+    import glst_amd as g
+    h = g.make_water_box(nw, L, seed=11)
+    d = _dev(h, torch)
+    f = g.GLSTForce(h[0].size, tol, L, rcut=rcut, ct="double")
+    f.calc_ener_force(*d)
+    f64 = _host(f.get_ef())
+    f.close()
+    out = {}
+    for blocks in (1, 4096, 1 << 22):
+        for agm in (0, 1):
+            f = g.GLSTForce(h[0].size, tol, L, rcut=rcut, ct="float")
+            f.set_option("lr_blocks", blocks)
+            f.set_option("lr_atom_groups", agm)
+            assert f.get_option("lr_atom_groups") == agm
+            f.calc_ener_force(*d)
+            out[blocks, agm] = _host(f.get_ef())
+            f.close()
+    for key, r in out.items():
+        assert rel_err(r[:3], f64[:3]) < 2e-6, key
+        assert rel_err(r[3:], f64[3:]) < 2e-6, key
+        lane = out[key[0], 0]
+        assert rel_err(r[:3], lane[:3]) < 2e-7, key
+        assert rel_err(r[3:], lane[3:]) < 2e-7, key
+    f = g.GLSTForce(h[0].size, tol, L, rcut=rcut, ct="float")
+    with pytest.raises(Exception):
+        f.set_option("lr_atom_groups", 2)
+    f.close()
+
+
 @pytest.mark.parametrize("nw,L,rcut", [(4000, 49.0, 9.0), (12000, 71.0, 9.0),
                                        (20000, 84.0, 10.0)])
 def test_pipelined_far_field_bitwise(cuda, nw, L, rcut):
