@@ -40,6 +40,7 @@ size_t BatchLayout::device_bytes() const {
   b += n * 4 * 4;                         // atom_cell, rank, sorted, cell
   b += n * sizeof(double4) * 2;           // atoms64, near4
   b += mixed() ? n * sizeof(float4) : 0;  // atoms32
+  b += nag * sizeof(uint2) + (nag ? c * 4 : 0);  // atom groups, ag_start
   b += n * 8 * 4;                         // fx..en
   b += hist * 4 + (c + 1) * 4 + c * 4 + sys.size() * 4;
   b += table_bytes + t_split.size() * 4;
@@ -230,6 +231,8 @@ BatchLayout pack_batch(int ct,
     d.T = 1;
     d.split0 = split_base;
     d.partial0 = B.partial;
+    d.ag0 = static_cast<uint32_t>(B.nag);
+    d.agtile = 0;
     B.natom += na;
     B.ncell += u.grid.ncell;
     B.hist += static_cast<uint64_t>(d.nchunk) * u.grid.ncell;
@@ -256,6 +259,13 @@ BatchLayout pack_batch(int ct,
       B.field += static_cast<uint64_t>(u.grid.ncell) * u.Kp;
       B.partial += static_cast<uint64_t>(P.nsplit) * na;
       split_base += P.nsplit + 1;
+      if (ct == GLST_CT_FLOAT) {  // back-projection over atom groups
+        const uint32_t nag = dev::atom_groups_max(na, u.grid.ncell);
+        d.agtile = nag / dev::kAgGroupsPerBlock;
+        B.nag += nag;
+        if (B.nag > 0xFFFFFFFFull)
+          throw ArgError("batch exceeds 2^32 atom groups");
+      }
     }
     B.sys.push_back(d);
   }
@@ -284,7 +294,9 @@ BatchLayout pack_batch(int ct,
       pf.add(s, d.ncell * div_up(sg.ntask, 64), task0 + sg.task0, sg.ntask);
       Plan &pl = B.lr_nr[key];
       pl.nr = key;
-      pl.add(s, d.ntile * lg.nsplit, d.split0 + lg.split0, lg.nsplit);
+      // atom-group tiles for the unrolled radial counts (see lr_batch)
+      const uint32_t tiles = key != 0 && d.agtile ? d.agtile : d.ntile;
+      pl.add(s, tiles * lg.nsplit, d.split0 + lg.split0, lg.nsplit);
     }
   }
   return B;
@@ -356,6 +368,8 @@ void glst_batch::configure(
   cell_sorted.grow(N * 4, hr);
   atoms64.grow(N * sizeof(double4), hr);
   if (mixed()) atoms32.grow(N * sizeof(float4), hr);
+  atom_grp.grow(B.nag * sizeof(uint2), hr);
+  ag_start.grow(B.nag ? C * 4 : 0, hr);
   near4.grow(N * sizeof(double4), hr);
   for (DeviceBuffer *b : {&fx, &fy, &fz, &en}) b->grow(N * 8, hr);
   hist.grow(std::max<uint64_t>(B.hist, 1) * 4, hr);
@@ -425,7 +439,7 @@ void glst_batch::release_arenas() {
   for (DeviceBuffer *b :
        {&d_sys, &d_seg, &atom_cell, &rank, &hist, &cell_count, &cell_start,
         &maxc, &sorted_idx, &cell_sorted, &atoms64, &atoms32, &near4, &fx, &fy,
-        &fz, &en, &t_split, &S, &R, &partial})
+        &fz, &en, &t_split, &S, &R, &partial, &atom_grp, &ag_start})
     b->release();
 }
 
@@ -455,13 +469,16 @@ void glst_batch::step_async(const double *rx, const double *ry,
                         hist.as<uint32_t>(), stream);
   dev::hist_offsets_batch(hist.as<uint32_t>(), sys, L_hist,
                           cell_count.as<uint32_t>(), stream);
+  uint2 *ag = B.nag ? atom_grp.as<uint2>() : nullptr;
   dev::cell_scan_batch(cell_count.as<uint32_t>(), sys, nsys(),
-                       cell_start.as<uint32_t>(), maxc.as<uint32_t>(), stream);
+                       cell_start.as<uint32_t>(), maxc.as<uint32_t>(), stream,
+                       ag_start.as<uint32_t>(), ag);
   dev::bin_scatter_batch(mixed(), rx, ry, rz, qc, sys, L_scatter,
                          atom_cell.as<uint32_t>(), rank.as<uint32_t>(),
                          hist.as<uint32_t>(), cell_start.as<uint32_t>(),
                          sorted_idx.as<uint32_t>(), cell_sorted.as<uint32_t>(),
-                         atoms64.as<double4>(), atoms32.as<float4>(), stream);
+                         atoms64.as<double4>(), atoms32.as<float4>(), stream,
+                         ag_start.as<uint32_t>(), ag);
   check_launch("assign_atoms");
   assigned = true;
   if (overlap_near) {
@@ -484,7 +501,7 @@ void glst_batch::step_async(const double *rx, const double *ry,
                  stream);
   check_launch("sum_rmt_sf");
   dev::lr_batch(mixed(), atoms, cell_sorted.as<uint32_t>(), ta, sys, L_lr,
-                R.as<void>(), partial.as<double4>(), stream);
+                R.as<void>(), partial.as<double4>(), stream, ag);
   check_launch("calc_lr_ef");
   if (overlap_near)
     GLST_HIP(hipStreamWaitEvent(stream, ev_near, 0));
@@ -616,7 +633,7 @@ size_t glst_batch::device_bytes() const {
       &d_sys,   &d_seg,   &atom_cell, &rank,    &hist,    &cell_count,
       &cell_start, &maxc, &sorted_idx, &cell_sorted, &atoms64, &atoms32,
       &near4,   &fx,      &fy,        &fz,      &en,      &t_split,
-      &S,       &R,       &partial};
+      &S,       &R,       &partial, &atom_grp, &ag_start};
   size_t b = tables ? tables->device_bytes() : 0;
   for (auto p : bufs) b += p->bytes();
   return b;

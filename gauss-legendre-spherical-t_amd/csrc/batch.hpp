@@ -107,6 +107,7 @@ struct BatchLayout {
   std::vector<std::shared_ptr<const BatchSystem>> cs;
   std::vector<dev::SysDev> sys;
   uint64_t natom = 0, ncell = 0, hist = 0, field = 0, partial = 0;
+  uint64_t nag = 0;  // mixed: atom-group entries of the back-projection
   size_t lds = 0;          // dynamic LDS of the shell-sum launch
   size_t table_bytes = 0;  // shape tables of the batch's distinct shapes
   std::vector<uint32_t> t_split;
@@ -156,6 +157,7 @@ struct glst_batch {
   glst::DeviceBuffer d_sys, d_seg;
   glst::DeviceBuffer atom_cell, rank, hist, cell_count, cell_start, maxc;
   glst::DeviceBuffer sorted_idx, cell_sorted, atoms64, atoms32, near4;
+  glst::DeviceBuffer atom_grp, ag_start;  // mixed back-projection groups
   glst::DeviceBuffer fx, fy, fz, en;
   glst::DeviceBuffer t_split;
   glst::DeviceBuffer S, R, partial;

@@ -521,12 +521,21 @@ __device__ inline void axis_phase(double2 *dst, double kd, double h,
 constexpr uint32_t kRmtBlock = 1024;
 // Values held in registers per thread while a line of the table is scanned,
 // and S slots loaded per thread before the first is stored.
-constexpr uint32_t kRmtBatch = 16;
+#ifndef GLST_RMT_ZBATCH
+#define GLST_RMT_ZBATCH 16
+#endif
+constexpr uint32_t kRmtBatch = GLST_RMT_ZBATCH;
 constexpr uint32_t kRmtLoadBatch = 4;
 // Threads of the persistent single-system workgroups and the S slots per
 // thread they hold across a gather.
 #ifndef GLST_RMT_PBLOCK
 #define GLST_RMT_PBLOCK 512
+#endif
+// waves per SIMD the persistent kernel's register budget must allow (2 is
+// what one 512-thread workgroup per CU needs; more leaves registers for the
+// kernels that run beside it)
+#ifndef GLST_RMT_MIN_WAVES
+#define GLST_RMT_MIN_WAVES 1
 #endif
 #ifndef GLST_RMT_PREFETCH
 #define GLST_RMT_PREFETCH 8
@@ -899,7 +908,7 @@ __device__ __forceinline__ void rmt_lds_body(
 // a workgroup gathers one tile's shells, the next tile's S slots (the first
 // round) are already on their way into registers.
 template <bool MIXED>
-__global__ __launch_bounds__(kRmtPBlock) void rmt_lds_kernel(
+__global__ __launch_bounds__(kRmtPBlock, GLST_RMT_MIN_WAVES) void rmt_lds_kernel(
     const void *__restrict__ Sv, const double4 *__restrict__ k64,
     const uint32_t *__restrict__ group, const uint32_t *__restrict__ r_in,
     const uint32_t *__restrict__ r_out, uint32_t Kp, uint32_t T,
@@ -1285,6 +1294,7 @@ const auto kLrNrKernels =
 // then gives lane m the sums of atom m.
 constexpr uint32_t kAgLanes = 8;
 constexpr uint32_t kAgPerBlock = kLrBlock / kAgLanes;
+static_assert(kAgPerBlock == kAgGroupsPerBlock, "groups per workgroup");
 
 // v[0 .. n) -> v[0 .. n/2): the lane keeps the half of the atoms selected by
 // its `mask` bit and adds the partner lane's values of that half.
@@ -1319,15 +1329,17 @@ __device__ __forceinline__ T ag_reduce_scatter(T (&v)[kAgAtoms],
   return v[0];
 }
 
+// Atom tile `tile` of one system (atoms, cell_sorted at its first atom, whose
+// index in the group table's numbering is atom_base; R at its field).
 template <int NR>
 __device__ __forceinline__ void lr_ag_body(
     const float4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
-    const uint2 *__restrict__ ag, const LrTasks &tk,
+    const uint2 *__restrict__ ag, uint32_t atom_base, const LrTasks &tk,
     const float2 *__restrict__ R, uint32_t Kp, uint32_t natom, uint32_t tile,
     uint32_t split, uint32_t prow, double4 *__restrict__ partial) {
   const uint32_t lane = threadIdx.x & (kAgLanes - 1);
   const uint2 g = ag[tile * kAgPerBlock + threadIdx.x / kAgLanes];
-  const uint32_t cnt = g.y, first = cnt ? g.x : 0u;
+  const uint32_t cnt = g.y, first = cnt ? g.x - atom_base : 0u;
   const uint32_t tb = tk.split[split], te = tk.split[split + 1];
   // uniform per wave: every group of the wave is padding, or no tasks
   if (__ballot(cnt != 0) == 0 || tb >= te) return;
@@ -1407,14 +1419,17 @@ __device__ __forceinline__ void lr_ag_body(
   }
 }
 
+#ifndef GLST_LR_AG_MIN_WAVES
+#define GLST_LR_AG_MIN_WAVES 1
+#endif
 template <int NR>
-__global__ __launch_bounds__(kLrBlock) void lr_ag_kernel(
+__global__ __launch_bounds__(kLrBlock, GLST_LR_AG_MIN_WAVES) void lr_ag_kernel(
     const float4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
     const uint2 *__restrict__ ag, LrTasks tk, const float2 *__restrict__ R,
     uint32_t Kp, uint32_t natom, uint32_t ntile, uint32_t split0,
     double4 *__restrict__ partial) {
   const uint32_t split = split0 + blockIdx.y;
-  lr_ag_body<NR>(atoms, cell_sorted, ag, tk, R, Kp, natom,
+  lr_ag_body<NR>(atoms, cell_sorted, ag, 0u, tk, R, Kp, natom,
                  xcd_remap(blockIdx.x, ntile), split, split, partial);
 }
 
@@ -1907,12 +1922,16 @@ __global__ void hist_offsets_batch_kernel(uint32_t *__restrict__ hist,
 
 __global__ __launch_bounds__(1024) void cell_scan_batch_kernel(
     const uint32_t *__restrict__ count, const SysDev *__restrict__ sys,
-    uint32_t nsys, uint32_t *__restrict__ start, uint32_t *__restrict__ maxc) {
+    uint32_t nsys, uint32_t *__restrict__ start, uint32_t *__restrict__ maxc,
+    uint32_t *__restrict__ ag_start, uint2 *__restrict__ ag) {
   const SysDev &d = sys[blockIdx.x];
+  const bool groups = ag && d.agtile;
   // start[cell0 + ncell] is the next system's first slot; the last system
   // writes the batch's end
   cell_scan_body(count + d.cell0, d.ncell, start + d.cell0, maxc + blockIdx.x,
-                 d.atom0, blockIdx.x + 1 == nsys);
+                 d.atom0, blockIdx.x + 1 == nsys,
+                 groups ? ag_start + d.cell0 : nullptr,
+                 groups ? ag + d.ag0 : nullptr, d.agtile * kAgPerBlock);
 }
 
 template <bool MIXED>
@@ -1923,14 +1942,17 @@ __global__ void bin_scatter_batch_kernel(
     const uint32_t *__restrict__ atom_cell, const uint32_t *__restrict__ rank,
     const uint32_t *__restrict__ hist, const uint32_t *__restrict__ start,
     uint32_t *__restrict__ sorted_idx, uint32_t *__restrict__ cell_sorted,
-    double4 *__restrict__ atoms64, float4 *__restrict__ atoms32) {
+    double4 *__restrict__ atoms64, float4 *__restrict__ atoms32,
+    const uint32_t *__restrict__ ag_start, uint2 *__restrict__ ag) {
   const Seg g = seg[run_of(seg, nseg, blockIdx.x)];
   const SysDev &d = sys[g.sys];
+  const bool groups = ag && d.agtile;
   bin_scatter_body<MIXED>(
       rx + d.atom0, ry + d.atom0, rz + d.atom0, qc + d.atom0, d.natom,
       atom_cell + d.atom0, rank + d.atom0, hist + d.hist0, start + d.cell0,
       d.ncell, d.ny, d.nz, d.h, sorted_idx, cell_sorted, atoms64, atoms32,
-      (blockIdx.x - g.blk0) * blockDim.x + threadIdx.x, d.atom0);
+      (blockIdx.x - g.blk0) * blockDim.x + threadIdx.x, d.atom0,
+      groups ? ag_start + d.cell0 : nullptr, groups ? ag + d.ag0 : nullptr);
 }
 
 __global__ __launch_bounds__(kSfBlock) void sf_f64_batch_kernel(
@@ -2031,6 +2053,41 @@ __global__ __launch_bounds__(kLrBlock) void lr_mixed_batch_kernel(
 typedef void (*LrBatchKernel)(const float4 *, const uint32_t *, LrTasks,
                               const float2 *, const SysDev *, const Seg *,
                               uint32_t, double4 *);
+
+// The batched back-projection over atom groups: a run's workgroups are
+// (split, atom-group tile) of one system and alpha group.
+template <int NR>
+__global__ __launch_bounds__(kLrBlock, GLST_LR_AG_MIN_WAVES) void
+lr_ag_batch_kernel(const float4 *__restrict__ atoms,
+                   const uint32_t *__restrict__ cell_sorted,
+                   const uint2 *__restrict__ ag, LrTasks tk,
+                   const float2 *__restrict__ R, const SysDev *__restrict__ sys,
+                   const Seg *__restrict__ seg, uint32_t nseg,
+                   double4 *__restrict__ partial) {
+  const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
+  const Seg g = seg[run_of(seg, nseg, wg)];
+  const SysDev &d = sys[g.sys];
+  const uint32_t local = wg - g.blk0;
+  const uint32_t split = g.a + local / d.agtile;
+  lr_ag_body<NR>(atoms + d.atom0, cell_sorted + d.atom0, ag + d.ag0, d.atom0,
+                 tk, R + d.field0, d.Kp, d.natom, local % d.agtile, split,
+                 split - d.split0, partial + d.partial0);
+}
+
+typedef void (*LrAgBatchKernel)(const float4 *, const uint32_t *,
+                                const uint2 *, LrTasks, const float2 *,
+                                const SysDev *, const Seg *, uint32_t,
+                                double4 *);
+
+template <int... N>
+constexpr auto lr_ag_batch_table(std::integer_sequence<int, N...>) {
+  return std::array<LrAgBatchKernel, sizeof...(N)>{
+      {&lr_ag_batch_kernel<N + 1>...}};
+}
+
+// index NR - 1 (NR = 1 .. kLrMaxUnrolled)
+const auto kLrAgBatchKernels =
+    lr_ag_batch_table(std::make_integer_sequence<int, kLrMaxUnrolled>{});
 
 template <int... N>
 constexpr auto lr_batch_table(std::integer_sequence<int, N...>) {
@@ -2373,9 +2430,11 @@ void hist_offsets_batch(uint32_t *hist, const SysDev *sys,
 }
 
 void cell_scan_batch(const uint32_t *count, const SysDev *sys, uint32_t nsys,
-                     uint32_t *start, uint32_t *maxc, hipStream_t s) {
+                     uint32_t *start, uint32_t *maxc, hipStream_t s,
+                     uint32_t *ag_start, uint2 *ag) {
   if (nsys == 0) return;
-  cell_scan_batch_kernel<<<nsys, 1024, 0, s>>>(count, sys, nsys, start, maxc);
+  cell_scan_batch_kernel<<<nsys, 1024, 0, s>>>(count, sys, nsys, start, maxc,
+                                               ag ? ag_start : nullptr, ag);
 }
 
 void bin_scatter_batch(bool mixed, const double *rx, const double *ry,
@@ -2384,16 +2443,18 @@ void bin_scatter_batch(bool mixed, const double *rx, const double *ry,
                        const uint32_t *rank, const uint32_t *hist,
                        const uint32_t *start, uint32_t *sorted_idx,
                        uint32_t *cell_sorted, double4 *atoms64,
-                       float4 *atoms32, hipStream_t s) {
+                       float4 *atoms32, hipStream_t s,
+                       const uint32_t *ag_start, uint2 *ag) {
   if (L.nblk == 0) return;
+  if (!ag) ag_start = nullptr;
   if (mixed)
     bin_scatter_batch_kernel<true><<<L.nblk, 256, 0, s>>>(
         rx, ry, rz, qc, sys, L.seg, L.nseg, atom_cell, rank, hist, start,
-        sorted_idx, cell_sorted, atoms64, atoms32);
+        sorted_idx, cell_sorted, atoms64, atoms32, ag_start, ag);
   else
     bin_scatter_batch_kernel<false><<<L.nblk, 256, 0, s>>>(
         rx, ry, rz, qc, sys, L.seg, L.nseg, atom_cell, rank, hist, start,
-        sorted_idx, cell_sorted, atoms64, atoms32);
+        sorted_idx, cell_sorted, atoms64, atoms32, ag_start, ag);
 }
 
 void sf_batch(bool mixed, const void *atoms, const uint32_t *start,
@@ -2434,7 +2495,7 @@ void rmt_batch(bool mixed, const void *S, const double4 *k64,
 void lr_batch(bool mixed, const void *atoms, const uint32_t *cell_sorted,
               const LrTaskArgs &ta, const SysDev *sys,
               const std::vector<BatchLaunch> &launches, const void *R,
-              double4 *partial, hipStream_t s) {
+              double4 *partial, hipStream_t s, const uint2 *ag) {
   const LrTasks tk{ta.node0, ta.nr,      ta.koff,   ta.dir64,
                    ta.dir32, ta.kappa64, ta.kappa32, ta.split};
   for (const BatchLaunch &L : launches) {
@@ -2443,6 +2504,12 @@ void lr_batch(bool mixed, const void *atoms, const uint32_t *cell_sorted,
       lr_f64_batch_kernel<<<L.nblk, kLrBlock, 0, s>>>(
           static_cast<const double4 *>(atoms), cell_sorted, tk,
           static_cast<const double2 *>(R), sys, L.seg, L.nseg, partial);
+      continue;
+    }
+    if (ag && L.nr >= 1 && L.nr <= static_cast<uint32_t>(kLrMaxUnrolled)) {
+      kLrAgBatchKernels[L.nr - 1]<<<L.nblk, kLrBlock, 0, s>>>(
+          static_cast<const float4 *>(atoms), cell_sorted, ag, tk,
+          static_cast<const float2 *>(R), sys, L.seg, L.nseg, partial);
       continue;
     }
     kLrBatchKernels[L.nr <= static_cast<uint32_t>(kLrMaxUnrolled) ? L.nr : 0]

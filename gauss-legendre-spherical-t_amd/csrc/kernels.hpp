@@ -181,7 +181,9 @@ void lr(bool mixed, const void *atoms, const uint32_t *cell_sorted,
         uint32_t nag_max = 0);
 // Atom groups of the mixed back-projection: up to 4 consecutive sorted atoms
 // of one cell, written by cell_scan and bin_scatter.  atom_groups_max bounds
-// their number (a multiple of the groups per workgroup).
+// their number (a multiple of the kAgGroupsPerBlock groups of one
+// workgroup).
+constexpr uint32_t kAgGroupsPerBlock = 32;
 uint32_t atom_groups_max(uint32_t natom, uint32_t ncell);
 inline void lr(bool mixed, const void *atoms, const uint32_t *cell_sorted,
                const LrTaskArgs &tasks, const std::vector<LrGroup> &groups,
@@ -225,6 +227,8 @@ struct SysDev {
   uint32_t ntile;                     // back-projection atom tiles (256)
   uint32_t nsplit, split0;            // splits [split0, split0 + nsplit)
   uint32_t T;                         // shell-sum nodes per workgroup
+  uint32_t ag0, agtile;               // mixed: atom groups [ag0, + agtile
+                                      // * kAgGroupsPerBlock), agtile WGs
   uint32_t pad;
   uint64_t hist0;                     // histogram [nchunk][ncell]
   uint64_t field0;                    // S and R slots [field0, + ncell*Kp)
@@ -253,16 +257,20 @@ void bin_chunks_batch(bool ct_float, const double *rx, const double *ry,
                       uint32_t *rank, uint32_t *hist, hipStream_t s);
 void hist_offsets_batch(uint32_t *hist, const SysDev *sys,
                         const BatchLaunch &L, uint32_t *count, hipStream_t s);
-// one workgroup per system; maxc[s] = largest cell of system s
+// one workgroup per system; maxc[s] = largest cell of system s; with ag:
+// the atom groups of every system with agtile > 0 (ag_start per cell)
 void cell_scan_batch(const uint32_t *count, const SysDev *sys, uint32_t nsys,
-                     uint32_t *start, uint32_t *maxc, hipStream_t s);
+                     uint32_t *start, uint32_t *maxc, hipStream_t s,
+                     uint32_t *ag_start = nullptr, uint2 *ag = nullptr);
 void bin_scatter_batch(bool mixed, const double *rx, const double *ry,
                        const double *rz, const double *qc, const SysDev *sys,
                        const BatchLaunch &L, const uint32_t *atom_cell,
                        const uint32_t *rank, const uint32_t *hist,
                        const uint32_t *start, uint32_t *sorted_idx,
                        uint32_t *cell_sorted, double4 *atoms64,
-                       float4 *atoms32, hipStream_t s);
+                       float4 *atoms32, hipStream_t s,
+                       const uint32_t *ag_start = nullptr,
+                       uint2 *ag = nullptr);
 // mixed: one launch per radial count (L.nr); FP64: one launch (L.nr unused)
 void sf_batch(bool mixed, const void *atoms, const uint32_t *start,
               const double4 *k64, const LrTaskArgs &tasks, const SysDev *sys,
@@ -272,10 +280,12 @@ void rmt_batch(bool mixed, const void *S, const double4 *k64,
                const uint32_t *group, const uint32_t *r_in,
                const uint32_t *r_out, const SysDev *sys, const BatchLaunch &L,
                size_t lds_bytes, void *R, hipStream_t s);
+// mixed with ag: launches with 1 <= nr <= 40 run over atom groups (their
+// plans count agtile workgroups per split, the others ntile)
 void lr_batch(bool mixed, const void *atoms, const uint32_t *cell_sorted,
               const LrTaskArgs &tasks, const SysDev *sys,
               const std::vector<BatchLaunch> &launches, const void *R,
-              double4 *partial, hipStream_t s);
+              double4 *partial, hipStream_t s, const uint2 *ag = nullptr);
 void lr_reduce_batch(const double4 *partial, const SysDev *sys,
                      const BatchLaunch &L, const double4 *near, double *fx,
                      double *fy, double *fz, double *en, hipStream_t s);
