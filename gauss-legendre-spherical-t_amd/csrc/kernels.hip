@@ -191,8 +191,22 @@ __device__ __forceinline__ void hist_offsets_body(uint32_t *__restrict__ hist,
                                                   uint32_t *__restrict__ count,
                                                   uint32_t c) {
   if (c >= ncell) return;
-  uint32_t run = 0;
-  for (uint32_t ch = 0; ch < nchunk; ++ch) {
+  // the counts of kBatch chunks are loaded together, then overwritten with
+  // their offsets: one memory round trip per batch instead of per chunk
+  constexpr uint32_t kBatch = 32;
+  uint32_t run = 0, ch = 0;
+  for (; ch + kBatch <= nchunk; ch += kBatch) {
+    uint32_t v[kBatch];
+#pragma unroll
+    for (uint32_t u = 0; u < kBatch; ++u)
+      v[u] = hist[static_cast<size_t>(ch + u) * ncell + c];
+#pragma unroll
+    for (uint32_t u = 0; u < kBatch; ++u) {
+      hist[static_cast<size_t>(ch + u) * ncell + c] = run;
+      run += v[u];
+    }
+  }
+  for (; ch < nchunk; ++ch) {
     const size_t o = static_cast<size_t>(ch) * ncell + c;
     const uint32_t v = hist[o];
     hist[o] = run;
