@@ -233,13 +233,16 @@ def test_mixed_options_agree(cuda, water1k):
 
 
 @pytest.mark.parametrize("nw,L,rcut,tol", [(1000, 31.04, 9.0, 1e-5),
-                                            (6000, 57.0, 9.0, 1e-4)])
+                                            (6000, 57.0, 9.0, 1e-4),
+                                            (1000, 31.04, 9.0, 1e-7)])
 def test_atom_group_back_projection(cuda, nw, L, rcut, tol):
     """The mixed back-projection over atom groups (up to four same-cell atoms
     per group, eight lanes sharing each direction's remote sums) agrees with
     the one-atom-per-lane kernel and with the FP64 path, for split counts
     from one split per alpha group to one pass of eight directions per
-    split, with cells whose atom counts are not multiples of four."""
+    split, with cells whose atom counts are not multiples of four.  At
+    tol 1e-7 the first alpha group has 42 radial nodes, past the unrolled
+    kernels: it runs one atom per lane while the second (29) uses groups."""
     import torch
     import glst_amd as g
     h = g.make_water_box(nw, L, seed=11)
