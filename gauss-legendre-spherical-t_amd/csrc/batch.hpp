@@ -122,6 +122,11 @@ struct BatchLayout {
 // shapes to `tables` (host side).  lds_limit: the device's LDS per workgroup
 // (a system whose one-node cell table does not fit is rejected); lr_blocks:
 // back-projection workgroups aimed for over the whole batch.
+// back-projection workgroups a batch aims for unless set (option lr_blocks)
+#ifndef GLST_BATCH_LR_BLOCKS
+#define GLST_BATCH_LR_BLOCKS 4096
+#endif
+
 BatchLayout pack_batch(int ct,
                        const std::vector<std::shared_ptr<const BatchSystem>> &cs,
                        ShapeTables &tables, size_t lds_limit,
@@ -138,7 +143,7 @@ struct glst_batch {
   bool own_stream = false;
   hipEvent_t ev_assigned = nullptr, ev_near = nullptr;
   int overlap_near = 1;
-  uint32_t lr_blocks = 4096;
+  uint32_t lr_blocks = GLST_BATCH_LR_BLOCKS;
   bool assigned = false;
   // HIP graph of the step (graph_mode 1: captured on the first step with
   // given inputs; 2: on the second consecutive step with the same inputs,

@@ -2144,9 +2144,12 @@ __global__ __launch_bounds__(kSrBlock) void sr_batch_kernel(
                         en, out4, wg - g.blk0);
 }
 
+// run blockIdx.x, copied by gridDim.y workgroups
 __global__ void copy_runs_kernel(const CopyRun *__restrict__ runs) {
   const CopyRun r = runs[blockIdx.x];
-  for (uint32_t i = threadIdx.x; i < r.n; i += blockDim.x) r.dst[i] = r.src[i];
+  const uint32_t step = gridDim.y * blockDim.x;
+  for (uint32_t i = blockIdx.y * blockDim.x + threadIdx.x; i < r.n; i += step)
+    r.dst[i] = r.src[i];
 }
 
 }  // namespace
@@ -2565,9 +2568,13 @@ void sr_batch(bool mixed, const double4 *atoms, const uint32_t *start,
   }
 }
 
-void copy_runs(const CopyRun *runs, uint32_t nruns, hipStream_t s) {
+void copy_runs(const CopyRun *runs, uint32_t nruns, uint32_t max_n,
+               hipStream_t s) {
   if (nruns == 0) return;
-  copy_runs_kernel<<<nruns, 256, 0, s>>>(runs);
+  // about four elements per thread, at most 64 workgroups per run
+  const uint32_t per = blocks_for(max_n, 256 * 4);
+  const uint32_t wgs = per < 1 ? 1 : (per > 64 ? 64 : per);
+  copy_runs_kernel<<<dim3(nruns, wgs), 256, 0, s>>>(runs);
 }
 
 }  // namespace dev

@@ -300,7 +300,8 @@ struct CopyRun {
   uint32_t n;
 };
 // dst[i] = src[i], i < n, for every run (one workgroup per run)
-void copy_runs(const CopyRun *runs, uint32_t nruns, hipStream_t s);
+void copy_runs(const CopyRun *runs, uint32_t nruns, uint32_t max_n,
+               hipStream_t s);
 
 // ---- direct all-pairs Coulomb (utils/coulomb.cu) -----------------------------
 void coulomb_direct(const double *rx, const double *ry, const double *rz,

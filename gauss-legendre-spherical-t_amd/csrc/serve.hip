@@ -228,8 +228,10 @@ struct glst_server {
     }
     // gather the requests' inputs into the packed arrays, scatter back after
     sl.host_runs.clear();
+    uint32_t max_n = 0;
     for (size_t i = 0; i < take; ++i) {
       const uint32_t off = b.B.sys[i].atom0, n = b.B.sys[i].natom;
+      max_n = std::max(max_n, n);
       for (int k = 0; k < 4; ++k)
         sl.host_runs.push_back({queue[i].in[k], sl.in[k].as<double>() + off, n});
     }
@@ -244,13 +246,13 @@ struct glst_server {
                             sl.host_runs.size() * sizeof(dev::CopyRun),
                             hipMemcpyHostToDevice, s));
     const uint32_t nrun = static_cast<uint32_t>(4 * take);
-    dev::copy_runs(sl.runs.as<dev::CopyRun>(), nrun, s);
+    dev::copy_runs(sl.runs.as<dev::CopyRun>(), nrun, max_n, s);
     check_launch("gather requests");
     b.step_async(sl.in[0].as<double>(), sl.in[1].as<double>(),
                  sl.in[2].as<double>(), sl.in[3].as<double>());
     b.get_ef_async(sl.out[0].as<double>(), sl.out[1].as<double>(),
                    sl.out[2].as<double>(), sl.out[3].as<double>());
-    dev::copy_runs(sl.runs.as<dev::CopyRun>() + nrun, nrun, s);
+    dev::copy_runs(sl.runs.as<dev::CopyRun>() + nrun, nrun, max_n, s);
     check_launch("scatter results");
     GLST_HIP(hipEventRecord(sl.done, s));
     sl.tickets.clear();
