@@ -167,3 +167,29 @@ def test_batch_shared_shapes_match_single_handles(cuda, ct, bar):
         assert rel_err(got[:3], ref[:3]) < bar
         assert rel_err(got[3:], ref[3:]) < bar
     b.close()
+
+
+def test_batch_tight_tolerance_mixes_kernels(cuda):
+    """At tol 1e-7 the larger grids' first alpha group has more radial nodes
+    than the unrolled kernels cover, so one batch runs S(k) and the
+    back-projection of those groups one atom per lane and the others over
+    atom groups; every system still matches its single handle (mixed bar
+    1e-6) and the FP64 path (2e-6)."""
+    import torch
+    import glst_amd as g
+    systems = _systems([300, 1000, 1500], seed0=31)
+    natoms = [h[0].size for _, h in systems]
+    tol = 1e-7
+    b = g.GLSTBatch(natoms, [L for L, _ in systems], RCUT, tol, ct="float")
+    b.calc_ener_force(*_pack(systems, torch))
+    got = [t.cpu().numpy() for t in b.get_ef()]
+    b.close()
+    single = _single(systems, tol, "float", torch)
+    f64 = _single(systems, tol, "double", torch)
+    for s in range(len(systems)):
+        o = sum(natoms[:s])
+        part = [a[o:o + natoms[s]] for a in got]
+        assert rel_err(part[:3], single[s][:3]) < 1e-6, s
+        assert rel_err(part[3:], single[s][3:]) < 1e-6, s
+        assert rel_err(part[:3], f64[s][:3]) < 2e-6, s
+        assert rel_err(part[3:], f64[s][3:]) < 2e-6, s
