@@ -141,7 +141,7 @@ def cpu_baseline(h, natom):
     }
 
 
-def serve_bench(g, torch, np, n_requests=2000, seed=5):
+def serve_bench(g, torch, np, n_requests=2000, seed=5, max_batch=8):
     """Served latency of the continuous-batching server (mixed path).
 
     Requests are systems of the batch size mix: waters log-uniform in
@@ -149,10 +149,13 @@ def serve_bench(g, torch, np, n_requests=2000, seed=5):
     requests are repeat MD steps of 16 resident systems (positions perturbed
     by N(0, 0.01 A) per request); the rest draw from 32 other systems with
     the box jittered per request (their grid setups come from the server's
-    cache).  Arrivals are Poisson at 75 % of the sustained capacity: the
-    throughput of a closed loop that keeps 8 requests in the server, so that
-    batches are as large as a loaded server forms them (a back-to-back burst
-    forms far larger batches; its rate is reported beside).  Latency =
+    cache).  The server computes at most `max_batch` requests per batch
+    (8: with no limit one batch takes every queued request and the second
+    slot idles; at most 8 keeps both slots busy and bounds a batch's time).
+    Arrivals are Poisson at 75 % of the sustained capacity: the throughput
+    of a closed loop that keeps 8 requests in the server, so that batches
+    are as large as a loaded server forms them (a back-to-back burst forms
+    larger batches; its rate is reported beside).  Latency =
     completion time - scheduled arrival time (queueing and setup
     included)."""
     rng = np.random.default_rng(seed)
@@ -178,7 +181,7 @@ def serve_bench(g, torch, np, n_requests=2000, seed=5):
         jitter[0] += 1  # a new shape: its setup is built at submission
         return L * (1.0 + 1e-9 * jitter[0]), d, o
 
-    srv = g.GLSTServer(TOL, 64 << 30, ct="float")
+    srv = g.GLSTServer(TOL, 64 << 30, max_batch=max_batch, ct="float")
     for L, d, o in resident:  # resident systems are set up once
         srv.submit(L, 9.0, *d, *o)
     while srv.pending():
@@ -245,7 +248,8 @@ def serve_bench(g, torch, np, n_requests=2000, seed=5):
     return {
         "workload": "Poisson arrivals of water systems, 100-10000 waters "
                     "(log-uniform), rcut 9, tol 1e-4, mixed; 30 % repeat "
-                    "steps of 16 resident systems",
+                    "steps of 16 resident systems; server max_batch %d"
+                    % max_batch,
         "requests": n_requests,
         "rate_req_s": rate,
         "capacity_req_s": cap,
