@@ -248,8 +248,8 @@ def serve_bench(g, torch, np, n_requests=2000, seed=5, max_batch=8):
     return {
         "workload": "Poisson arrivals of water systems, 100-10000 waters "
                     "(log-uniform), rcut 9, tol 1e-4, mixed; 30 % repeat "
-                    "steps of 16 resident systems; server max_batch %d"
-                    % max_batch,
+                    "steps of 16 resident systems; server max_batch "
+                    + str(max_batch),
         "requests": n_requests,
         "rate_req_s": rate,
         "capacity_req_s": cap,
