@@ -67,7 +67,9 @@ def time_path(g, torch, d, natom, ct, steps, warmup, tune=False,
         for k, v in zip(keys, schedule):
             f.set_option(k, v)
     elif tune:
-        f.autotune(*d, reps=3)
+        # 6 steps per candidate: with 3 the eager/graph decision followed
+        # run-to-run noise of about 1 %
+        f.autotune(*d, reps=6)
     tuned = {k: f.get_option(k) for k in keys}
 
     def run(graph, timing):
