@@ -284,8 +284,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs one system on one GPU; --gpus "
-              f"{args.gpus} is not supported (multi-GPU node sharding is a "
-              f"design only, see DESIGN.md)", file=sys.stderr)
+              f"{args.gpus} is not supported (node sharding across GPUs is "
+              f"designed, not built)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
