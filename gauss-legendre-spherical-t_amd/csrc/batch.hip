@@ -180,6 +180,13 @@ size_t ShapeTables::device_bytes() const {
   return b;
 }
 
+// 4 with 256-thread workgroups: the 512-system mix ran at 32.5 ms against
+// 34.9 ms for one 1024-thread workgroup per CU (2: 36.4, 8: 34.0)
+#ifndef GLST_BATCH_RMT_LDS_DIV
+#define GLST_BATCH_RMT_LDS_DIV 4
+#endif
+constexpr size_t kBatchRmtLdsDiv = GLST_BATCH_RMT_LDS_DIV;
+
 BatchLayout pack_batch(int ct,
                        const std::vector<std::shared_ptr<const BatchSystem>> &cs,
                        ShapeTables &tables, size_t lds_limit,
@@ -239,8 +246,16 @@ BatchLayout pack_batch(int ct,
     if (B.natom > 0xFFFFFFFFull || B.ncell > 0x7FFFFFFFull)
       throw ArgError("batch exceeds 2^32 atoms or 2^31 cells");
     if (lr) {
+      // One launch covers every system with one dynamic LDS size, the
+      // largest tile's: tiles sized for 1/kBatchRmtLdsDiv of the LDS let
+      // that many workgroups share a CU (a grid whose one-node table needs
+      // more takes the whole LDS).
       d.T = dev::rmt_nodes_per_group(u.grid.nx, u.grid.ny, u.grid.nz,
-                                     lds_limit, dev::kRmtMaxNodes);
+                                     lds_limit / kBatchRmtLdsDiv,
+                                     dev::kRmtMaxNodes);
+      if (d.T == 0)
+        d.T = dev::rmt_nodes_per_group(u.grid.nx, u.grid.ny, u.grid.nz,
+                                       lds_limit, dev::kRmtMaxNodes);
       if (d.T == 0)
         throw ArgError("system " + std::to_string(s) + ": its " +
                        std::to_string(u.grid.ncell) +

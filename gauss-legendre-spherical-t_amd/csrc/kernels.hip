@@ -532,7 +532,12 @@ __device__ inline void axis_phase(double2 *dst, double kd, double h,
 
 // ---- LDS-resident tables (grids whose table of one node fits in LDS) ----
 
-constexpr uint32_t kRmtBlock = 1024;
+// threads of a per-tile shell-sum workgroup (the batched engine): 256, with
+// tiles sized for a quarter of the LDS, lets four workgroups share a CU
+#ifndef GLST_RMT_BLOCK
+#define GLST_RMT_BLOCK 256
+#endif
+constexpr uint32_t kRmtBlock = GLST_RMT_BLOCK;
 // Values held in registers per thread while a line of the table is scanned,
 // and S slots loaded per thread before the first is stored.
 // z-pass elements loaded together (8: 175 VGPRs for the persistent kernel
@@ -2291,7 +2296,7 @@ bool rmt_lds(bool mixed, uint32_t tmax, const void *S, const double4 *k64,
   const uint32_t ntile = tile1 - tile0;
   const size_t lds = rmt_lds_bytes(g.nx, g.ny, g.nz, T);
   // persistent workgroups: as many as are resident at once (LDS allows one
-  // or two 1024-thread workgroups per CU)
+  // or two kRmtPBlock-thread workgroups per CU)
   size_t per_cu = g_lds_per_cu / lds;
   const size_t wave_cap = 2048 / kRmtPBlock;  // 32 waves per CU
   per_cu = per_cu < 1 ? 1 : (per_cu > wave_cap ? wave_cap : per_cu);
