@@ -1714,9 +1714,16 @@ struct PairAcc<false> {
   template <bool SELF>
   __device__ void add(const double4 &ai, const double4 &aj, bool self) {
     const double rx = ai.x - aj.x, ry = ai.y - aj.y, rz = ai.z - aj.z;
-    double r2 = rx * rx + ry * ry + rz * rz;
-    if (SELF) r2 = self ? INFINITY : r2;
-    const double ir = 1.0 / sqrt(r2);
+    const double r2 = rx * rx + ry * ry + rz * rz;
+    // 1/|r|: the FP32 hardware estimate (relative error about 1e-7) refined
+    // by two Newton steps in FP64, each of which squares the relative error
+    // (1e-7 -> 1e-14 -> FP64 rounding); cheaper than an FP64 square root and
+    // division.  Near-field separations lie far inside FP32's range.
+    double ir = static_cast<double>(__builtin_amdgcn_rsqf(static_cast<float>(r2)));
+    const double h = 0.5 * r2;
+    ir = ir * fma(-h, ir * ir, 1.5);
+    ir = ir * fma(-h, ir * ir, 1.5);
+    if (SELF) ir = self ? 0.0 : ir;  // no self interaction
     const double e = aj.w * ir;
     const double f = e * ir * ir;
     fx += f * rx;
