@@ -1362,8 +1362,14 @@ __device__ __forceinline__ void lr_ag_body(
   const uint2 g = ag[tile * kAgPerBlock + threadIdx.x / kAgLanes];
   const uint32_t cnt = g.y, first = cnt ? g.x - atom_base : 0u;
   const uint32_t tb = tk.split[split], te = tk.split[split + 1];
-  // uniform per wave: every group of the wave is padding, or no tasks
-  if (__ballot(cnt != 0) == 0 || tb >= te) return;
+  // uniform per wave: every group of the wave is padding (nothing to write)
+  if (__ballot(cnt != 0) == 0) return;
+  if (tb >= te) {  // a split without directions adds zero
+    if (lane < cnt)
+      partial[static_cast<size_t>(prow) * natom + first + lane] =
+          make_double4(0.0, 0.0, 0.0, 0.0);
+    return;
+  }
   const float2 *Rc = R + static_cast<size_t>(cell_sorted[first]) * Kp;
   float ax[kAgAtoms], ay[kAgAtoms], az[kAgAtoms];
 #pragma unroll
