@@ -512,9 +512,10 @@ struct glst_engine {
       }
       GLST_HIP(hipEventRecord(ev_near, side));
     };
-    // Started with S(k) (mode 1) the near field fills the chip's slack
-    // through the whole far field; started after S(k) (mode 2) it delays
-    // the LDS-heavy shell-sum workgroups and ends up slower.
+    // Mode 1 starts the near field with S(k); mode 2 after all of S(k), so
+    // that it runs beside the shell sums and the back-projection (the
+    // autotuner's choice at 288k atoms).  Starting it after the first alpha
+    // group's S(k) instead was 6 % slower.
     if (overlap_near == 1 || !(has_lr && Kp > 0)) launch_near();
     sf_stage();
     if (overlap_near == 2 && has_lr && Kp > 0) launch_near();
