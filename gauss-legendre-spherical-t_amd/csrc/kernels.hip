@@ -1776,14 +1776,18 @@ __device__ __forceinline__ void sr_body(
       if ((ix == 0 && dx < 0) || (ix == nx - 1 && dx > 0)) continue;
       for (int dy = -1; dy <= 1; ++dy) {
         if ((iy == 0 && dy < 0) || (iy == ny - 1 && dy > 0)) continue;
-        for (int dz = -1; dz <= 1; ++dz) {
-          if ((iz == 0 && dz < 0) || (iz == nz - 1 && dz > 0)) continue;
-          const uint32_t nb = ((ix + dx) * ny + (iy + dy)) * nz + (iz + dz);
-          const uint32_t b0 = start[nb], b1 = start[nb + 1];
-          // Mixed: FP32 sums over one neighbour cell, flushed to FP64.
+        {
+          // The neighbour cells iz-1 .. iz+1 of one (x, y) column are
+          // consecutive in the sorted order: one contiguous atom range,
+          // staged in tiles of up to kSrBlock atoms.
+          const uint32_t col = ((ix + dx) * ny + (iy + dy)) * nz;
+          const uint32_t z0 = iz > 0 ? iz - 1 : iz;
+          const uint32_t z1 = iz + 1 < nz ? iz + 1 : iz;
+          const uint32_t b0 = start[col + z0], b1 = start[col + z1 + 1];
+          // Mixed: FP32 sums over one neighbour column, flushed to FP64.
           PairAcc<MIXED> acc;
-          // The self pair exists only in the atom's own cell: the other 26
-          // cells run the pair loop without the test (uniform branch).
+          // The self pair exists only in the atom's own column: the other
+          // columns run the pair loop without the test (uniform branch).
           auto sweep = [&](auto self_tag) {
             constexpr bool SELF = decltype(self_tag)::value;
             for (uint32_t t0 = b0; t0 < b1; t0 += kSrBlock) {
@@ -1806,7 +1810,7 @@ __device__ __forceinline__ void sr_body(
               }
             }
           };
-          if (nb == cell)
+          if (dx == 0 && dy == 0)
             sweep(std::true_type{});
           else
             sweep(std::false_type{});
