@@ -1305,14 +1305,15 @@ const auto kLrNrKernels =
 
 // ---------------------------------------------------------------------------
 // Mixed back-projection over atom groups.  An atom group is up to kAgAtoms
-// consecutive sorted atoms of one cell (a cell of n atoms has ceil(n / 8)
-// groups); ag[g] = (first atom, count), count 0 past the last group.  Eight
-// lanes serve one group: lane m takes directions tb + m, tb + m + 8, ... of
-// the split, loads that direction's NR remote sums (A, phi) of the group's
-// cell once per pass and applies them to all eight atoms, so a pair costs one
-// FMA for the phase, sin, cos and two FMAs, and the vector memory path
-// carries one R value per eight pairs.  A reduce-scatter over the eight lanes
-// then gives lane m the sums of atom m.
+// consecutive sorted atoms of one cell (4 by default: a cell of n atoms has
+// ceil(n / kAgAtoms) groups); ag[g] = (first atom, count), count 0 past the
+// last group.  Eight lanes serve one group: lane m takes directions tb + m,
+// tb + m + 8, ... of the split, loads that direction's NR remote sums
+// (A, phi) of the group's cell once per pass and applies them to all the
+// group's atoms, so a pair costs one FMA for the phase, sin, cos and two
+// FMAs, and the vector memory path carries one R value per kAgAtoms pairs.
+// A reduce-scatter over the eight lanes then gives lane m the sums of atom
+// m mod kAgAtoms (lanes m >= kAgAtoms hold copies and write nothing).
 constexpr uint32_t kAgLanes = 8;
 constexpr uint32_t kAgPerBlock = kLrBlock / kAgLanes;
 static_assert(kAgPerBlock == kAgGroupsPerBlock, "groups per workgroup");
