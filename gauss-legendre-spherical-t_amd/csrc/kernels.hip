@@ -1664,21 +1664,22 @@ __global__ void lr_reduce_kernel(const double4 *__restrict__ partial,
 //
 // One 256-thread workgroup per cell.  A cell of n atoms is covered S =
 // floor(256/n) times: lane (i, s) takes atom i against every S-th atom
-// (phase s) of each neighbour cell, so a cell of 85 atoms keeps 255 lanes
+// (phase s) of each neighbour range, so a cell of 85 atoms keeps 255 lanes
 // busy instead of 128 lanes in two 64-atom passes (denser cells run in
-// 256-atom chunks with S = 1).  Each neighbour cell is staged through an
-// LDS tile of up to 256 atom records, loaded once by the workgroup and read
-// by every lane (the lanes of one phase read the same record); the S partial
-// sums of an atom are then added in LDS in a fixed order, in the same
-// buffer.
+// 256-atom chunks with S = 1).  The neighbour cells z-1 .. z+1 of one
+// (x, y) column are consecutive in the sorted order, so the 27 neighbour
+// cells are 9 atom ranges.  Each range is staged through LDS tiles of up to
+// 256 atom records, loaded once by the workgroup and read by every lane
+// (the lanes of one phase read the same record); the S partial sums of an
+// atom are then added in LDS in a fixed order, in the same buffer.
 constexpr uint32_t kSrBlock = 256;
 constexpr uint32_t kSrMaxPhase = 16;
 
 // (F, e) / q_i of one pair; r = r_i - r_j, e = q_i q_j / |r|,
-// F = e r / |r|^2.  q_i multiplies the sums over a neighbour cell once, when
-// they are flushed.  Mixed: the separation in FP64, the rest in FP32.  SELF:
-// the pair may be the atom with itself (only in its own cell), which then
-// adds nothing.
+// F = e r / |r|^2.  q_i multiplies the sums over a neighbour range once,
+// when they are flushed.  Mixed: the separation in FP64, the rest in FP32.
+// SELF: the pair may be the atom with itself (only in its own column), which
+// then adds nothing.
 template <bool MIXED>
 struct PairAcc;
 
