@@ -383,3 +383,39 @@ def test_rectangular_grids_match_oracle(cuda, box, ct):
         assert rel_err(mine[:3], one[:3]) < bbar
         assert rel_err(mine[3:], one[3:]) < bbar
     b.close()
+
+
+@pytest.mark.parametrize("box", [(31.0, 40.0, 8.5), (40.0, 31.0, 15.0),
+                                 (16.0, 28.0, 40.0)])
+@pytest.mark.parametrize("ct", ["double", "float"])
+def test_thin_grids_near_field_match_oracle(cuda, box, ct):
+    """Grids one or two cells deep along some axis (4x5x1, 5x4x2, 2x4x5 at
+    rcut 9): the near field sweeps the z-neighbour cells of each (x, y)
+    column as one atom range, clamped at the grid edge; the near field alone
+    and the whole step against the FP64 oracle."""
+    import torch
+    import glst_amd as g
+    from oracle import oracle as o
+    rx, ry, rz, qc = _lattice_charges(box, 2.6, 5)
+    tol, rcut = 1e-4, 9.0
+    ict = 1 if ct == "float" else 0
+    bar = 1e-10 if ct == "double" else 2e-6
+    d = _dev([rx, ry, rz, qc], torch)
+    f = g.GLSTForce(rx.size, tol, box, rcut=rcut, ct=ct)
+    assert tuple(f.info().ncell) == o.grid_dims(ict, *box, rcut)
+    assert min(f.info().ncell) <= 2
+    f.assign_atoms(*d)
+    f.calc_sr_ef()
+    f.comm_ef()
+    near = _host(f.get_ef())
+    f.close()
+    ref_near = o.glst(rx, ry, rz, qc, tol, box, rcut, ct=ict, far=False)
+    assert rel_err(near[:3], ref_near[:3]) < bar
+    assert rel_err(near[3:], ref_near[3:]) < bar
+    f = g.GLSTForce(rx.size, tol, box, rcut=rcut, ct=ct)
+    f.calc_ener_force(*d)
+    got = _host(f.get_ef())
+    f.close()
+    ref = o.glst(rx, ry, rz, qc, tol, box, rcut, ct=ict)
+    assert rel_err(got[:3], ref[:3]) < bar
+    assert rel_err(got[3:], ref[3:]) < bar
