@@ -1449,9 +1449,9 @@ __device__ __forceinline__ void lr_ag_body(
 
 // Waves per SIMD the atom-group kernel's register budget allows: 4 (128
 // VGPRs) up to GLST_LR_AG_4WAVE_NR radial nodes, where that costs at most a
-// few bytes of scratch (12 B per lane at 16 and 19 nodes, reloaded once per
-// pass) and made the kernel 3.5 % faster at 288k atoms; else whatever the
-// kernel needs.
+// few bytes of scratch (one 8-byte spill and one reload, 12 B per lane, at
+// 16 and 19 nodes) and made the kernel 3.5 % faster at 288k atoms; else
+// whatever the kernel needs.
 #ifndef GLST_LR_AG_4WAVE_NR
 #define GLST_LR_AG_4WAVE_NR 19
 #endif
