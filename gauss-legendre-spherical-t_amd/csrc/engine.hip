@@ -1216,8 +1216,12 @@ glst_status_t glst_coulomb_direct(const double *d_rx, const double *d_ry,
                       !d_fz || !d_en))
       throw ArgError("glst_coulomb_direct: null pointer");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    dev::coulomb_direct(d_rx, d_ry, d_rz, d_qc, natom, d_fx, d_fy, d_fz, d_en,
-                        s);
+    // partial sums of the source splits in a buffer of this call
+    const dev::CoulombPlan plan = dev::coulomb_plan(natom, SIZE_MAX);
+    DeviceBuffer part;
+    part.allocate(plan.entries * sizeof(double4));
+    dev::coulomb_direct(d_rx, d_ry, d_rz, d_qc, natom, plan,
+                        part.as<double4>(), d_fx, d_fy, d_fz, d_en, s);
     check_launch("coulomb_direct");
     GLST_HIP(hipStreamSynchronize(s));
   });

@@ -23,6 +23,7 @@
 //    kernel and its batched twin, which looks its system up in a run table.
 #include "kernels.hpp"
 
+#include <algorithm>
 #include <cfloat>
 #include <climits>
 #include <array>
@@ -1884,22 +1885,31 @@ __global__ void unpermute_kernel(const uint32_t *__restrict__ sorted_idx,
 
 constexpr uint32_t kCoulBlock = 256;
 
+// Direct all-pairs sum of utils/coulomb.cu:14-81 over the source range
+// [j_begin, j_end) of split blockIdx.y, j_begin = blockIdx.y * jchunk (jchunk
+// a multiple of kCoulBlock).  With one split the sums go straight to the
+// outputs; otherwise split s writes (fx, fy, fz, en) of atom i to
+// part[s * natom + i] and coulomb_reduce_kernel adds the splits in order.
 __global__ __launch_bounds__(kCoulBlock) void coulomb_kernel(
     const double *__restrict__ rx, const double *__restrict__ ry,
     const double *__restrict__ rz, const double *__restrict__ qc,
-    uint32_t natom, double *__restrict__ fx, double *__restrict__ fy,
-    double *__restrict__ fz, double *__restrict__ en) {
+    uint32_t natom, uint32_t jchunk, double *__restrict__ fx,
+    double *__restrict__ fy, double *__restrict__ fz, double *__restrict__ en,
+    double4 *__restrict__ part) {
   __shared__ double4 tile[kCoulBlock];
   const uint32_t i = blockIdx.x * kCoulBlock + threadIdx.x;
   const uint32_t ii = i < natom ? i : 0;
   const double xi = rx[ii], yi = ry[ii], zi = rz[ii], qi = qc[ii];
+  const uint32_t j_begin = blockIdx.y * jchunk;
+  const uint32_t j_end =
+      (natom - j_begin < jchunk) ? natom : j_begin + jchunk;
   double FX = 0.0, FY = 0.0, FZ = 0.0, EN = 0.0;
-  for (uint32_t j0 = 0; j0 < natom; j0 += kCoulBlock) {
+  for (uint32_t j0 = j_begin; j0 < j_end; j0 += kCoulBlock) {
     __syncthreads();
     const uint32_t j = j0 + threadIdx.x;
-    if (j < natom) tile[threadIdx.x] = make_double4(rx[j], ry[j], rz[j], qc[j]);
+    if (j < j_end) tile[threadIdx.x] = make_double4(rx[j], ry[j], rz[j], qc[j]);
     __syncthreads();
-    const uint32_t n = (natom - j0 < kCoulBlock) ? natom - j0 : kCoulBlock;
+    const uint32_t n = (j_end - j0 < kCoulBlock) ? j_end - j0 : kCoulBlock;
     for (uint32_t t = 0; t < n; ++t) {
       const double4 b = tile[t];
       const double dx = xi - b.x, dy = yi - b.y, dz = zi - b.z;
@@ -1914,12 +1924,36 @@ __global__ __launch_bounds__(kCoulBlock) void coulomb_kernel(
       EN += e;
     }
   }
-  if (i < natom) {
+  if (i >= natom) return;
+  if (part) {
+    part[(size_t)blockIdx.y * natom + i] = make_double4(FX, FY, FZ, EN);
+  } else {
     fx[i] = FX;
     fy[i] = FY;
     fz[i] = FZ;
     en[i] = EN;
   }
+}
+
+// outputs of atom i = sum over splits s = 0, 1, ... of part[s * natom + i]
+__global__ __launch_bounds__(256) void coulomb_reduce_kernel(
+    const double4 *__restrict__ part, uint32_t nsplit, uint32_t natom,
+    double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
+    double *__restrict__ en) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= natom) return;
+  double4 a = part[i];
+  for (uint32_t s = 1; s < nsplit; ++s) {
+    const double4 b = part[(size_t)s * natom + i];
+    a.x += b.x;
+    a.y += b.y;
+    a.z += b.z;
+    a.w += b.w;
+  }
+  fx[i] = a.x;
+  fy[i] = a.y;
+  fz[i] = a.z;
+  en[i] = a.w;
 }
 
 inline uint32_t blocks_for(size_t n, uint32_t b) {
@@ -2450,12 +2484,36 @@ void unpermute(const uint32_t *sorted_idx, uint32_t natom, const double *fx,
       sorted_idx, natom, fx, fy, fz, en, ofx, ofy, ofz, oen);
 }
 
+CoulombPlan coulomb_plan(uint32_t natom, size_t max_entries) {
+  CoulombPlan p{1, natom, 0};
+  if (natom == 0) return p;
+  const uint32_t blocks = blocks_for(natom, kCoulBlock);
+  uint32_t nsplit = (kCoulTargetBlocks + blocks - 1) / blocks;
+  nsplit = std::min<uint32_t>(nsplit, kCoulMaxSplits);
+  nsplit = static_cast<uint32_t>(
+      std::min<size_t>(nsplit, std::max<size_t>(1, max_entries / natom)));
+  if (nsplit <= 1) return p;
+  // source ranges of whole tiles; recount so that no split is empty
+  const uint32_t per = (natom + nsplit - 1) / nsplit;
+  p.jchunk = (per + kCoulBlock - 1) / kCoulBlock * kCoulBlock;
+  p.nsplit = (natom + p.jchunk - 1) / p.jchunk;
+  if (p.nsplit <= 1) return CoulombPlan{1, natom, 0};
+  p.entries = static_cast<size_t>(p.nsplit) * natom;
+  return p;
+}
+
 void coulomb_direct(const double *rx, const double *ry, const double *rz,
-                    const double *qc, uint32_t natom, double *fx, double *fy,
-                    double *fz, double *en, hipStream_t s) {
+                    const double *qc, uint32_t natom, const CoulombPlan &p,
+                    double4 *part, double *fx, double *fy, double *fz,
+                    double *en, hipStream_t s) {
   if (natom == 0) return;
-  coulomb_kernel<<<blocks_for(natom, kCoulBlock), kCoulBlock, 0, s>>>(
-      rx, ry, rz, qc, natom, fx, fy, fz, en);
+  const dim3 grid(blocks_for(natom, kCoulBlock), p.nsplit);
+  coulomb_kernel<<<grid, kCoulBlock, 0, s>>>(rx, ry, rz, qc, natom, p.jchunk,
+                                             fx, fy, fz, en,
+                                             p.nsplit > 1 ? part : nullptr);
+  if (p.nsplit > 1)
+    coulomb_reduce_kernel<<<blocks_for(natom, 256), 256, 0, s>>>(
+        part, p.nsplit, natom, fx, fy, fz, en);
 }
 
 // ---- batched launchers -----------------------------------------------------

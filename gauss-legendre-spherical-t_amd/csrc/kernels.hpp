@@ -304,9 +304,23 @@ void copy_runs(const CopyRun *runs, uint32_t nruns, uint32_t max_n,
                hipStream_t s);
 
 // ---- direct all-pairs Coulomb (utils/coulomb.cu) -----------------------------
+// The source atoms are split into `nsplit` ranges of `jchunk` (a multiple of
+// the 256-atom tile) when the atoms alone give too few workgroups; each
+// split's partial sums take one double4 per atom (`entries` in all), added
+// in split order.
+constexpr uint32_t kCoulTargetBlocks = 2048;  // workgroups aimed for
+constexpr uint32_t kCoulMaxSplits = 64;
+struct CoulombPlan {
+  uint32_t nsplit;  // source ranges (1: no partial sums)
+  uint32_t jchunk;  // atoms per source range
+  size_t entries;   // double4 partial sums needed (0 when nsplit == 1)
+};
+// at most `max_entries` partial sums
+CoulombPlan coulomb_plan(uint32_t natom, size_t max_entries);
 void coulomb_direct(const double *rx, const double *ry, const double *rz,
-                    const double *qc, uint32_t natom, double *fx, double *fy,
-                    double *fz, double *en, hipStream_t s);
+                    const double *qc, uint32_t natom, const CoulombPlan &p,
+                    double4 *part, double *fx, double *fy, double *fz,
+                    double *en, hipStream_t s);
 
 }  // namespace dev
 }  // namespace glst

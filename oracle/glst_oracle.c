@@ -514,6 +514,15 @@ static void cube_sum(const double *Pr, const double *Pi, size_t ld, size_t k,
   box_sum(Pr, Pi, ld, k, x0, y0, z0, x1, y1, z1, ny, nz, cr, ci);
 }
 
+/* Wall seconds of the stages of the last orc_glst call, labelled as the
+ * reference driver times them (test/full_example.cu:215-233): assign_atoms,
+ * calc_sf, sum_rmt_sf, calc_lr_ef, calc_sr_ef, get_ef. */
+static double g_stage_s[6];
+
+void orc_last_stage_times(double *out) {
+  for (int i = 0; i < 6; i++) out[i] = g_stage_s[i];
+}
+
 int orc_glst(int ct, const double *rx, const double *ry, const double *rz,
              const double *qc, uint32_t natom, double tol, double lx,
              double ly, double lz, double rcut, uint32_t node_stride,
@@ -532,6 +541,8 @@ int orc_glst(int ct, const double *rx, const double *ry, const double *rz,
   double *X = (double *)malloc(sizeof(double) * 4 * (natom + 1));
   double *F = (double *)calloc(4 * (size_t)(natom + 1), sizeof(double));
   if (!atom_cell || !sidx || !cpnt || !ccnt || !X || !F) return 1;
+  for (int i = 0; i < 6; i++) g_stage_s[i] = 0.0;
+  double t0 = omp_get_wtime(), t1;
   orc_bin(ct, rx, ry, rz, natom, nx, ny, nz, rcut, atom_cell, sidx, cpnt,
           ccnt);
   for (uint32_t s = 0; s < natom; s++) {
@@ -541,6 +552,9 @@ int orc_glst(int ct, const double *rx, const double *ry, const double *rz,
     X[4 * s + 2] = rz[o];
     X[4 * s + 3] = qc[o];
   }
+  t1 = omp_get_wtime();
+  g_stage_s[0] = t1 - t0;
+  t0 = t1;
   int status = 0;
   const int has_lr = nx > 2 && ny > 2 && nz > 2; /* src/glst_force.cu:667 */
   if (far && has_lr) {
@@ -583,8 +597,12 @@ int orc_glst(int ct, const double *rx, const double *ry, const double *rz,
           Si[c * Ks + ks] = im;
         }
       }
+      t1 = omp_get_wtime();
+      g_stage_s[1] = t1 - t0;
+      t0 = t1;
       if (s_re && node_stride == 1) memcpy(s_re, Sr, sizeof(double) * ncell * K);
       if (s_im && node_stride == 1) memcpy(s_im, Si, sizeof(double) * ncell * K);
+      t0 = omp_get_wtime();
       /* prefix sums z, y, x: src/glst_force.cu:702-772 */
 #pragma omp parallel for schedule(static)
       for (long ks = 0; ks < (long)Ks; ks++) {
@@ -638,8 +656,12 @@ int orc_glst(int ct, const double *rx, const double *ry, const double *rz,
           Ri[c * Ks + ks] = (oi - ii) * wk[k];
         }
       }
+      t1 = omp_get_wtime();
+      g_stage_s[2] = t1 - t0;
+      t0 = t1;
       if (r_re && node_stride == 1) memcpy(r_re, Rr, sizeof(double) * ncell * K);
       if (r_im && node_stride == 1) memcpy(r_im, Ri, sizeof(double) * ncell * K);
+      t0 = omp_get_wtime();
       /* calc_lr_ef_kernel: src/glst_force.cu:1060-1135 */
 #pragma omp parallel for schedule(dynamic, 64)
       for (long s = 0; s < (long)natom; s++) {
@@ -664,6 +686,8 @@ int orc_glst(int ct, const double *rx, const double *ry, const double *rz,
         F[4 * s + 2] += f2;
         F[4 * s + 3] += e0;
       }
+      t1 = omp_get_wtime();
+      g_stage_s[3] = t1 - t0;
       free(Sr);
       free(Si);
       free(Rr);
@@ -675,6 +699,7 @@ int orc_glst(int ct, const double *rx, const double *ry, const double *rz,
     free(wk);
     free(grp);
   }
+  t0 = omp_get_wtime();
   if (near && !status) {
     /* calc_sr_ef_intra/inter: src/glst_force.cu:1248-1324, 1436-1544 */
 #pragma omp parallel for schedule(dynamic, 64)
@@ -714,6 +739,9 @@ int orc_glst(int ct, const double *rx, const double *ry, const double *rz,
       F[4 * s + 3] += e0;
     }
   }
+  t1 = omp_get_wtime();
+  g_stage_s[4] = t1 - t0;
+  t0 = t1;
   /* get_ef: back to original order (src/glst_force.cu:142-176) */
   for (uint32_t s = 0; s < natom; s++) {
     const uint32_t o = sidx[s];
@@ -722,6 +750,7 @@ int orc_glst(int ct, const double *rx, const double *ry, const double *rz,
     fz[o] = F[4 * s + 2];
     en[o] = F[4 * s + 3];
   }
+  g_stage_s[5] = omp_get_wtime() - t0;
   free(atom_cell);
   free(sidx);
   free(cpnt);

@@ -80,6 +80,11 @@ int orc_glst(int ct, const double *rx, const double *ry, const double *rz,
              double *en, double *s_re, double *s_im, double *r_re,
              double *r_im);
 
+/* Wall seconds of the six stages of the last orc_glst call, in the order
+ * the reference driver labels them (test/full_example.cu:215-233):
+ * assign_atoms, calc_sf, sum_rmt_sf, calc_lr_ef, calc_sr_ef, get_ef. */
+void orc_last_stage_times(double *out6);
+
 /* Direct FP64 all-pairs Coulomb (utils/coulomb.cu:14-81). */
 int orc_coulomb(const double *rx, const double *ry, const double *rz,
                 const double *qc, uint32_t natom, double *fx, double *fy,

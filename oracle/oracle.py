@@ -45,6 +45,8 @@ def _load():
     lib.orc_set_threads.restype = None
     lib.orc_get_threads.argtypes = []
     lib.orc_get_threads.restype = c_int
+    lib.orc_last_stage_times.argtypes = [P]
+    lib.orc_last_stage_times.restype = None
     return lib
 
 
@@ -182,6 +184,17 @@ def glst(rx, ry, rz, qc, tol, box, rcut, ct=0, node_stride=1, far=True,
                       box[1], box[2], rcut, node_stride, int(far), int(near),
                       *[_p(o) for o in out], *[_p(f) for f in fld]))
     return (tuple(out), tuple(fld)) if fields else tuple(out)
+
+
+STAGES = ("assign_atoms", "calc_sf", "sum_rmt_sf", "calc_lr_ef", "calc_sr_ef",
+          "get_ef")
+
+
+def last_stage_times() -> dict:
+    """Wall seconds of each stage of the last glst() call."""
+    t = np.zeros(6)
+    _lib.orc_last_stage_times(_p(t))
+    return dict(zip(STAGES, t.tolist()))
 
 
 def coulomb(rx, ry, rz, qc):

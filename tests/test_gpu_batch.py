@@ -193,3 +193,117 @@ def test_batch_tight_tolerance_mixes_kernels(cuda):
         assert rel_err(part[3:], single[s][3:]) < 1e-6, s
         assert rel_err(part[:3], f64[s][:3]) < 2e-6, s
         assert rel_err(part[3:], f64[s][3:]) < 2e-6, s
+
+
+# ---- the batched-ensemble configuration at the served tolerance ------------
+# tol 2e-5 at rcut 9 A keeps water boxes of 100-10,000 waters at <= 4e-6
+# relative force error against the direct sum (tol 1e-4 gives 1-2 % there).
+SERVED_TOL = 2e-5
+
+
+@pytest.mark.parametrize("ct,bar", [("double", 1e-10), ("float", 2e-6)])
+def test_batch_matches_oracle_at_served_tol(cuda, ct, bar):
+    """A batch of four grid shapes (2^3 without a far field, 3^3, 4^3, 5^3):
+    every system against the oracle's FP64 path and the direct sum."""
+    import torch
+    import glst_amd as g
+    from oracle import oracle as o
+    systems = _systems([150, 450, 1100, 2100], seed0=61)
+    natoms = [h[0].size for _, h in systems]
+    b = g.GLSTBatch(natoms, [L for L, _ in systems], RCUT, SERVED_TOL, ct=ct)
+    d = _pack(systems, torch)
+    b.calc_ener_force(*d)
+    got = [t.cpu().numpy() for t in b.get_ef()]
+    b.close()
+    for s, (L, h) in enumerate(systems):
+        o0 = sum(natoms[:s])
+        part = [a[o0:o0 + natoms[s]] for a in got]
+        ref = o.glst(*h, SERVED_TOL, L, RCUT)
+        assert rel_err(part[:3], ref[:3]) < bar, s
+        assert rel_err(part[3:], ref[3:]) < bar, s
+        sl = [x[o0:o0 + natoms[s]] for x in d]
+        direct = [t.cpu().numpy() for t in g.coulomb_direct(*sl)]
+        assert rel_err(part[:3], direct[:3]) < 1e-5, s
+
+
+@pytest.fixture(scope="module")
+def ensemble512(cuda):
+    """512 water boxes of round(exp(U(ln 100, ln 10000))) waters (seed 4) at
+    0.0334 waters/A^3, rcut 9 A, tol 2e-5, packed into one mixed batch:
+    (systems, packed device inputs, host outputs, offsets)."""
+    import torch
+    import glst_amd as g
+    rng = np.random.default_rng(4)
+    nw = np.round(np.exp(rng.uniform(np.log(100), np.log(10000), 512)))
+    systems = []
+    for k, w in enumerate(nw.astype(int)):
+        L = _box(int(w))
+        systems.append((L, g.make_water_box(int(w), L, seed=1000 + k)))
+    natoms = [h[0].size for _, h in systems]
+    d = _pack(systems, torch)
+    b = g.GLSTBatch(natoms, [L for L, _ in systems], RCUT, SERVED_TOL,
+                    ct="float")
+    assert b.num_systems() == 512
+    b.calc_ener_force(*d)
+    got = [t.cpu().numpy() for t in b.get_ef()]
+    offsets = [b.offset(s) for s in range(512)]
+    b.close()
+    assert offsets == list(np.cumsum([0] + natoms[:-1]))
+    return systems, d, got, offsets
+
+
+def _part(ens, s):
+    systems, _, got, offsets = ens
+    n = systems[s][1][0].size
+    return [a[offsets[s]:offsets[s] + n] for a in got]
+
+
+def test_ensemble512_every_system_equals_its_handle(ensemble512):
+    """Each of the 512 systems as its own single-system handle (mixed bar
+    1e-6: only the back-projection's split boundaries differ)."""
+    import torch
+    import glst_amd as g
+    systems = ensemble512[0]
+    for s, (L, h) in enumerate(systems):
+        d = [torch.as_tensor(a, dtype=torch.float64, device="cuda") for a in h]
+        f = g.GLSTForce(h[0].size, SERVED_TOL, L, rcut=RCUT, ct="float")
+        f.calc_ener_force(*d)
+        ref = [t.cpu().numpy() for t in f.get_ef()]
+        f.close()
+        part = _part(ensemble512, s)
+        assert rel_err(part[:3], ref[:3]) < 1e-6, s
+        assert rel_err(part[3:], ref[3:]) < 1e-6, s
+
+
+def test_ensemble512_every_system_meets_the_error_bar(ensemble512):
+    """Each of the 512 systems within 1e-5 of the direct FP64 sum."""
+    import glst_amd as g
+    systems, d, _, offsets = ensemble512
+    worst = 0.0
+    for s, (L, h) in enumerate(systems):
+        n = h[0].size
+        sl = [x[offsets[s]:offsets[s] + n] for x in d]
+        ref = [t.cpu().numpy() for t in g.coulomb_direct(*sl)]
+        err = rel_err(_part(ensemble512, s)[:3], ref[:3])
+        worst = max(worst, err)
+        assert err < 1e-5, (s, n, err)
+    assert worst > 0.0
+
+
+def test_ensemble512_sample_matches_oracle(ensemble512):
+    """The smallest and largest systems and 6 drawn at random (of at most
+    3,000 waters) against the oracle's FP64 path (mixed bar 2e-6)."""
+    from oracle import oracle as o
+    systems = ensemble512[0]
+    sizes = [h[0].size for _, h in systems]
+    rng = np.random.default_rng(9)
+    pick = {int(np.argmin(sizes)), int(np.argmax(sizes))}
+    modest = [s for s, n in enumerate(sizes) if n <= 9000 and s not in pick]
+    pick |= set(rng.choice(modest, 6, replace=False).tolist())
+    assert len(pick) == 8
+    for s in sorted(pick):
+        L, h = systems[s]
+        ref = o.glst(*h, SERVED_TOL, L, RCUT)
+        part = _part(ensemble512, s)
+        assert rel_err(part[:3], ref[:3]) < 2e-6, s
+        assert rel_err(part[3:], ref[3:]) < 2e-6, s

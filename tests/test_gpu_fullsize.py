@@ -5,10 +5,11 @@ not need the oracle to finish at that size:
   pairs through the symmetric Chebyshev shells and the odd sin kernel), so
   the net force sum_i F_i vanishes up to rounding;
 * the mixed path agrees with the FP64 path well inside the method error;
-* at 288k charges the FP64 path is within 1e-5 of the direct FP64 all-pairs
-  sum (the north-star bar), and the energies agree likewise;
-* binning at full size reproduces the bit-exact CT arithmetic (checked with
-  the oracle's vectorised binning, which runs at any size).
+* at 288k and at 1.0M charges both paths are within 1e-5 of the direct FP64
+  all-pairs sum (the metric's bar), and the energies agree likewise;
+* binning at full size reproduces the bit-exact CT arithmetic of both
+  instantiations (checked with the oracle's binning, which runs at any
+  size).
 """
 import numpy as np
 import pytest
@@ -62,12 +63,37 @@ def test_288k_fp64_vs_direct_sum_and_mixed(cuda, water288k):
     oac, osi, ocp, occ = o.bin_atoms(0, *h[:3], (15, 15, 15), 10.0)
     np.testing.assert_array_equal(ac, oac)
     np.testing.assert_array_equal(si, osi)
+    np.testing.assert_array_equal(cp, ocp)
     np.testing.assert_array_equal(cc, occ)
+
+
+def test_288k_float_binning_bit_exact(cuda, water288k):
+    """CT = float bins with FP32 arithmetic (cell edge and coordinates
+    rounded to float, src/glst_force.cu:356-386); at 288k charges some atoms
+    fall in another cell than under CT = double, and the GPU must place
+    every one of them as the FP32 restatement does."""
+    import torch
+    import glst_amd as g
+    from oracle import oracle as o
+    h = water288k
+    d = [torch.as_tensor(a, dtype=torch.float64, device="cuda") for a in h]
+    f = g.GLSTForce(h[0].size, 1e-4, 142.0, rcut=10.0, ct="float")
+    f.assign_atoms(*d)
+    ac, si, cp, cc = f.cells()
+    f.close()
+    oac, osi, ocp, occ = o.bin_atoms(1, *h[:3], (15, 15, 15), 10.0)
+    np.testing.assert_array_equal(ac, oac)
+    np.testing.assert_array_equal(si, osi)
+    np.testing.assert_array_equal(cp, ocp)
+    np.testing.assert_array_equal(cc, occ)
+    dac = o.bin_atoms(0, *h[:3], (15, 15, 15), 10.0)[0]
+    assert (dac != oac).sum() >= 1  # the two instantiations do differ here
 
 
 def test_ionic_1m_mixed_vs_fp64(cuda):
     """333,000 waters plus 898 Na+/Cl- pairs in a 215 A box, rcut 12 A: uneven
-    cell occupancy; tighter tol."""
+    cell occupancy; tighter tol.  Both paths against the direct FP64 sum of
+    all 1.0M charges (1e12 pairs on the GPU)."""
     import torch
     import glst_amd as g
     h = g.make_water_box(333000, 215.0, seed=1, nion=898)
@@ -77,6 +103,13 @@ def test_ionic_1m_mixed_vs_fp64(cuda):
     f64, info, _ = _run(h, tol, 215.0, 12.0, "double", torch)
     mix, _, _ = _run(h, tol, 215.0, 12.0, "float", torch)
     assert tuple(info.ncell) == (18, 18, 18)
+    d = [torch.as_tensor(a, dtype=torch.float64, device="cuda") for a in h]
+    ref = [t.cpu().numpy() for t in g.coulomb_direct(*d)]
+    err64, errmix = rel_err(f64[:3], ref[:3]), rel_err(mix[:3], ref[:3])
+    print(f"1.0M charges, tol {tol}: relative force error vs direct sum "
+          f"FP64 {err64:.3e} mixed {errmix:.3e}", flush=True)
+    assert err64 < 1e-5 and errmix < 1e-5
+    assert abs(f64[3].sum() - ref[3].sum()) < 1e-5 * np.abs(ref[3]).sum()
     assert rel_err(mix[:3], f64[:3]) < 2e-6
     assert rel_err(mix[3:], f64[3:]) < 2e-6
     assert _net_force(f64) < 1e-12
