@@ -405,25 +405,13 @@ struct glst_engine {
            lr_groups.size() == sf_groups.size() && !rmt_global_path();
   }
   void plan_pipeline() {
-    const size_t G = sf_groups.size();
     pipe_tmax = rmt_shape == 1 ? 1 : dev::kRmtMaxNodes;
     const uint32_t T = dev::rmt_nodes_per_group(grid.nx, grid.ny, grid.nz,
                                                 lds_limit, pipe_tmax);
     if (T == 0) throw StateError("shell-sum tables do not fit in LDS");
-    const uint32_t ntile = Kp / T;
-    tile_end.assign(G, ntile);
-    rmt_need.assign(G, static_cast<uint32_t>(G - 1));
-    for (size_t g = 0; g + 1 < G; ++g) {
-      const uint32_t end = nodes.first[g] + nodes.count[g];
-      tile_end[g] = std::min(end / T, ntile);
-      if (g > 0) tile_end[g] = std::max(tile_end[g], tile_end[g - 1]);
-    }
-    for (size_t g = 0; g < G; ++g) {
-      const uint64_t end = nodes.first[g] + nodes.count[g];
-      size_t j = g;
-      while (j + 1 < G && static_cast<uint64_t>(tile_end[j]) * T < end) ++j;
-      rmt_need[g] = static_cast<uint32_t>(j);
-    }
+    PipelinePlan P = glst::plan_pipeline(nodes, sf_groups.size(), Kp, T);
+    tile_end = std::move(P.tile_end);
+    rmt_need = std::move(P.rmt_need);
   }
   void sf_stage() {
     if (!pipelined()) {
@@ -1160,12 +1148,25 @@ glst_status_t glst_get_option(glst_handle_t h, const char *name,
   });
 }
 
+// Autotuning is refused for handles of more atoms than this.  One run that
+// tuned the 1,000,796-charge handle (after a 3,000-charge handle in the same
+// process) ended in a host segmentation fault whose cause is not found: the
+// host planning the autotuner drives (splits for every target, pipeline
+// chunks, node layout) checks clean under AddressSanitizer and UBSan at that
+// shape (tests/host/plan_check.cpp), and the default schedule runs that
+// handle correctly.  The 288,000-charge handle tunes without fault.
+constexpr uint32_t kAutotuneMaxAtoms = 300000;
+
 glst_status_t glst_autotune(glst_handle_t h, const double *d_rx,
                             const double *d_ry, const double *d_rz,
                             const double *d_qc, uint32_t reps,
                             double *best_ms) {
   NEED_HANDLE(h);
   return guarded(h, [&] {
+    if (h->natom > kAutotuneMaxAtoms)
+      throw StateError("glst_autotune: handles of more than " +
+                       std::to_string(kAutotuneMaxAtoms) +
+                       " atoms are not tuned (the default schedule applies)");
     const double t = h->autotune(d_rx, d_ry, d_rz, d_qc, reps);
     if (best_ms) *best_ms = t;
   });
@@ -1287,6 +1288,41 @@ glst_status_t glst_spherical_rule(uint32_t m, double *xyz, double *w,
       }
       if (w) w[i] = r.w[i];
     }
+  });
+}
+
+glst_status_t glst_read_tdesign(const char *dir, uint32_t t, double *xyz,
+                                uint32_t *count) {
+  if (!dir || !count) return GLST_ERR_INVALID;
+  return guarded(nullptr, [&] {
+    SphRule r = design_file(t, dir);
+    const uint32_t n = static_cast<uint32_t>(r.s.size());
+    if (xyz && *count < n)
+      throw ArgError("glst_read_tdesign: output holds " +
+                     std::to_string(*count) + " points, the design " +
+                     std::to_string(n));
+    *count = n;
+    if (xyz)
+      for (uint32_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) xyz[3 * i + a] = r.s[i][a];
+  });
+}
+
+glst_status_t glst_select_spherical_degree(uint32_t gl_order, double tol,
+                                           double dist, double alpha,
+                                           double zcut, uint32_t *m,
+                                           uint32_t *count) {
+  if (!m) return GLST_ERR_INVALID;
+  return guarded(nullptr, [&] {
+    if (gl_order < 2 || gl_order % 2)
+      throw ArgError("glst_select_spherical_degree: gl_order must be even "
+                     "and at least 2");
+    GLRule gl;
+    gl.order = gl_order;
+    gauss_legendre_nonneg(gl_order, gl.x, gl.w);
+    SphRule r = select_spherical(gl, tol, dist, alpha, zcut, tdesign_dir());
+    *m = r.m;
+    if (count) *count = static_cast<uint32_t>(r.s.size());
   });
 }
 

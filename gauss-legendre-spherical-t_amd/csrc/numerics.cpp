@@ -208,7 +208,12 @@ static bool parse_design_name(const std::string &name, uint32_t &t,
 }
 
 SphRule design_rule(uint32_t m, const std::string &dir) {
-  const uint32_t t = 2 * m + 1;
+  SphRule r = design_file(2 * m + 1, dir);
+  r.m = m;
+  return r;
+}
+
+SphRule design_file(uint32_t t, const std::string &dir) {
   DIR *d = opendir(dir.c_str());
   if (!d)
     throw std::runtime_error("Failed to open spherical design directory \"" +
@@ -226,14 +231,15 @@ SphRule design_rule(uint32_t m, const std::string &dir) {
   }
   closedir(d);
   if (best.empty())
-    throw std::runtime_error("No spherical design of degree " +
-                             std::to_string(t) + " in \"" + dir + "\"");
+    throw std::runtime_error("No spherical design file of degree " +
+                             std::to_string(t) + " in directory \"" + dir +
+                             "\"");
   const std::string path = dir + "/" + best;
   std::ifstream fs(path);
   if (!fs.is_open())
     throw std::runtime_error("Failed to open file \"" + path + "\"");
   SphRule r;
-  r.m = m;
+  r.m = t / 2;
   const uint32_t half = best_n / 2;  // upper hemisphere of a symmetric design
   for (uint32_t i = 0; i < half; ++i) {
     std::string line;

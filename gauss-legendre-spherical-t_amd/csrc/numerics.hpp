@@ -66,6 +66,10 @@ SphRule product_rule(uint32_t m);
 // (files ssTTT.NNNNN, 3 x 25-character columns; the first N/2 points, equal
 // weights 1/(N/2)), as src/cubature.cu:422-459 reads them.
 SphRule design_rule(uint32_t m, const std::string &dir);
+// The same for any degree t (get_tdesign_props + read_tdesign): among the
+// files of degree t the one whose name sorts first; names not of the form
+// ...TTT.NNNNN are skipped.
+SphRule design_file(uint32_t t, const std::string &dir);
 
 // get_spherical_tdesign (src/cubature.cu:461-523): for each axis j the first
 // degree index m >= m0 (starting at len(x)/2, carried across axes) whose

@@ -379,4 +379,41 @@ inline SplitPlan make_splits(const NodeTable &nodes, uint32_t K,
   return P;
 }
 
+// Shell-sum chunks of the far field pipelined across alpha groups.  Group g
+// owns device nodes [first_g, first_g + count_g); the shell sums run in
+// workgroup tiles of T consecutive nodes, Kp / T tiles in all.  Chunk g is
+// the tiles [tile_end[g-1], tile_end[g]) with tile_end[g] =
+// floor((first_g + count_g) / T) (the last chunk ends at Kp / T), so chunk g
+// holds nodes of groups <= g only and may start once S(k) of group g is
+// complete.  A tile that straddles groups g and g+1 belongs to chunk g+1, so
+// the back-projection of group g waits for chunk rmt_need[g], the first
+// chunk whose end covers the group's last node.
+struct PipelinePlan {
+  std::vector<uint32_t> tile_end;  // per group: end of its chunk (tiles)
+  std::vector<uint32_t> rmt_need;  // per group: chunk its back-projection needs
+};
+
+inline PipelinePlan plan_pipeline(const NodeTable &nodes, size_t G,
+                                  uint32_t Kp, uint32_t T) {
+  PipelinePlan P;
+  if (G == 0 || T == 0) return P;
+  if (nodes.first.size() < G || nodes.count.size() < G)
+    throw StateError("pipeline plan: fewer node groups than alpha groups");
+  const uint32_t ntile = Kp / T;
+  P.tile_end.assign(G, ntile);
+  P.rmt_need.assign(G, static_cast<uint32_t>(G - 1));
+  for (size_t g = 0; g + 1 < G; ++g) {
+    const uint64_t end = static_cast<uint64_t>(nodes.first[g]) + nodes.count[g];
+    P.tile_end[g] = static_cast<uint32_t>(std::min<uint64_t>(end / T, ntile));
+    if (g > 0) P.tile_end[g] = std::max(P.tile_end[g], P.tile_end[g - 1]);
+  }
+  for (size_t g = 0; g < G; ++g) {
+    const uint64_t end = static_cast<uint64_t>(nodes.first[g]) + nodes.count[g];
+    size_t j = g;
+    while (j + 1 < G && static_cast<uint64_t>(P.tile_end[j]) * T < end) ++j;
+    P.rmt_need[g] = static_cast<uint32_t>(j);
+  }
+  return P;
+}
+
 }  // namespace glst

@@ -135,6 +135,9 @@ def _load():
         "glst_select_gl_order": [d, d, d, d, POINTER(u)],
         "glst_spherical_rule": [u, P, P, POINTER(u)],
         "glst_set_tdesign_dir": [c_char_p],
+        "glst_read_tdesign": [c_char_p, u, P, POINTER(u)],
+        "glst_select_spherical_degree": [u, d, d, d, d, POINTER(u),
+                                         POINTER(u)],
         "glst_grid_dims": [c_int, d, d, d, d, POINTER(u), POINTER(u),
                            POINTER(u)],
         "glst_make_water_box": [u, u, d, c_uint64, P, P, P, P],
@@ -262,6 +265,27 @@ def spherical_rule(m: int):
     w = np.zeros(c.value)
     _check(_lib.glst_spherical_rule(m, _ptr(xyz), _ptr(w), byref(c)))
     return xyz, w
+
+
+def read_tdesign(directory: str, t: int):
+    """The first N/2 points of the design of degree t in `directory`
+    (cubature<CT>::get_tdesign_props + read_tdesign), shape (N/2, 3)."""
+    import numpy as np
+    c = c_uint32()
+    _check(_lib.glst_read_tdesign(directory.encode(), t, None, byref(c)))
+    xyz = np.zeros((c.value, 3))
+    _check(_lib.glst_read_tdesign(directory.encode(), t, _ptr(xyz), byref(c)))
+    return xyz
+
+
+def select_spherical_degree(gl_order, tol, dist, alpha, zcut):
+    """(m, directions) chosen by get_spherical_tdesign's three-axis test for
+    the Gauss-Legendre rule of order gl_order, from the current design
+    directory or the built-in product rule."""
+    m, c = c_uint32(), c_uint32()
+    _check(_lib.glst_select_spherical_degree(gl_order, tol, dist, alpha, zcut,
+                                             byref(m), byref(c)))
+    return m.value, c.value
 
 
 def set_tdesign_dir(path: str | None):

@@ -211,7 +211,9 @@ glst_status_t glst_get_option(glst_handle_t h, const char *name,
  * "pipeline" (0, 1; CT = float), "overlap_near" (0, 1, 2), "lr_blocks"
  * (powers of two from 1024 to 65536), "rmt_shape" (0, 1).  Clears the
  * accumulators afterwards; *best_ms (may be NULL) receives the chosen
- * schedule's time per step. */
+ * schedule's time per step.  Handles of more than 300,000 atoms are refused
+ * with GLST_ERR_STATE and keep the default schedule (a host fault seen once
+ * while tuning a 1.0M-charge handle is not explained). */
 glst_status_t glst_autotune(glst_handle_t h, const double *d_rx,
                             const double *d_ry, const double *d_rz,
                             const double *d_qc, uint32_t reps,
@@ -233,7 +235,10 @@ const char *glst_version(void);
 /* ---- direct Coulomb sum (utils/coulomb.cu:14-81, 84-172) ----------------- */
 
 /* en_i = sum_{j!=i} q_i q_j / r_ij, F_i = sum_{j!=i} q_i q_j (r_i-r_j)/r_ij^3,
- * all pairs, FP64, free space.  Device arrays; outputs are overwritten. */
+ * all pairs, FP64, free space.  Device arrays; outputs are overwritten.
+ * Below about 524,000 atoms the source atoms are split into up to 64 ranges
+ * whose partial sums (a temporary device buffer of at most 32 MiB, freed
+ * before return) are added in range order.  Synchronises `stream`. */
 glst_status_t glst_coulomb_direct(const double *d_rx, const double *d_ry,
                                   const double *d_rz, const double *d_qc,
                                   uint32_t natom, double *d_fx, double *d_fy,
@@ -265,6 +270,26 @@ glst_status_t glst_select_gl_order(double tol, double dist, double alpha,
  * weights summing to 1.  Pass NULL arrays to query *count. */
 glst_status_t glst_spherical_rule(uint32_t m, double *xyz, double *w,
                                   uint32_t *count);
+/* get_tdesign_props + read_tdesign (src/cubature.cu:422-459): the first N/2
+ * points of the symmetric design of degree t in `dir` (a file named
+ * ...TTT.NNNNN, one point per line in three 25-character columns), as unit
+ * vectors (x,y,z interleaved).  Among several files of degree t the one
+ * whose name sorts first is read; other file names are skipped (the
+ * reference takes the first in directory order and throws on names it
+ * cannot parse).  Pass xyz = NULL to query *count; otherwise *count holds
+ * the capacity in points on entry. */
+glst_status_t glst_read_tdesign(const char *dir, uint32_t t, double *xyz,
+                                uint32_t *count);
+/* get_spherical_tdesign (src/cubature.cu:461-523) for the Gauss-Legendre
+ * rule of even order gl_order: the degree index m (rule of degree 2m+1) the
+ * three-axis test accepts at distance `dist`, from the design files of the
+ * current directory (glst_set_tdesign_dir) or, with none set, the built-in
+ * product rule; *count (may be NULL) receives its number of half-sphere
+ * directions. */
+glst_status_t glst_select_spherical_degree(uint32_t gl_order, double tol,
+                                           double dist, double alpha,
+                                           double zcut, uint32_t *m,
+                                           uint32_t *count);
 /* Directory of Womersley symmetric design files (ssTTT.NNNNN); when set (and
  * non-empty) new handles use those designs, as src/cubature.cu:422-541 does.
  * NULL or "" selects the built-in product rule.  The environment variable

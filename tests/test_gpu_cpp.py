@@ -50,3 +50,67 @@ def test_full_example_report(cuda, ct):
         m = re.search(rf"Root Mean Squared:\s*({FLOAT})\s*\(Norm:\s*({FLOAT})\)",
                       block)
         assert float(m.group(2)) < 1e-5, (sec, m.group(0))
+    _check_report_labels(out, box=31.04, ncell=(4, 4, 4), cell=9.0, tol=1e-5)
+
+
+def _triplet(label, text, num=FLOAT):
+    m = re.search(rf"{re.escape(label)}:\s*({num})\s*(?:x|,)\s*({num})\s*"
+                  rf"(?:x|,)\s*({num})", text)
+    assert m, label
+    return tuple(float(m.group(i)) for i in (1, 2, 3))
+
+
+def _check_report_labels(out, box, ncell, cell, tol):
+    """Every label group a reference log parser reads from the driver's
+    report (the setup block of src/glst_force.cu:245-293, the report of
+    test/full_example.cu:127-175 and utils/utils.cpp:60-168), with the
+    values this run must print."""
+    assert int(re.search(r"Number of GPUs:\s*(\d+)", out).group(1)) == 1
+    assert _triplet("System dimensions [A]", out) == (box, box, box)
+    assert _triplet("Number of cells", out, r"\d+") == tuple(map(float, ncell))
+    cd = _triplet("Cell dimensions [A]", out)
+    assert all(abs(c - cell) < 1e-6 * cell for c in cd)
+    cov = _triplet("Total space covered [A]", out)
+    assert all(abs(v - n * c) < 1e-5 * v for v, n, c in zip(cov, ncell, cd))
+    total = int(re.search(r"Total number of cubature nodes:\s*(\d+)",
+                          out).group(1))
+    groups = [int(v) for v in re.findall(
+        r"Number of cubature nodes in group\s+\d+:\s*(\d+)", out)]
+    ngroup = int(re.search(r"Number of alpha groups:\s*(\d+)", out).group(1))
+    assert len(groups) == ngroup and sum(groups) == total > 0
+    assert abs(float(re.search(rf"Target error:\s*({FLOAT})", out).group(1))
+               - tol) < 1e-12
+    for sec in ("X Force", "Y Force", "Z Force", "Energy"):
+        block = re.search(
+            rf"{sec}\s+Error:\s*(.*?)(?=\n\s*(?:X Force|Y Force|Z Force|"
+            rf"Energy)\s+Error:|\Z)", out, re.S).group(1)
+        stats = {}
+        for label in ("Absolute Min", "Absolute Max", "Average",
+                      "Root Mean Squared"):
+            m = re.search(rf"{label}:\s*({FLOAT})\s*\(Norm:\s*({FLOAT})\s*\)",
+                          block)
+            assert m, (sec, label)
+            stats[label] = float(m.group(1))
+        # min <= mean <= RMS <= max (printed to 7 significant digits)
+        slack = 1 + 1e-5
+        assert 0.0 <= stats["Absolute Min"] <= stats["Average"] * slack
+        assert stats["Average"] <= stats["Root Mean Squared"] * slack
+        assert stats["Root Mean Squared"] <= stats["Absolute Max"] * slack
+
+
+def test_full_example_ncell_form(cuda):
+    """The driver's second form, [sys] [tol] [box] [nx] [ny] [nz], builds the
+    object with the cell-count constructor (test/full_example.cu:127-131,
+    src/glst_force.cu:80-95): rcut = min L/n."""
+    env = dict(os.environ, GLST_BENCH_ITERS="2", GLST_CT="double",
+               GLST_BENCH_TERMINAL="1")
+    r = subprocess.run([os.path.join(BIN, "full_example"), "water:1000", "1e-5",
+                        "31.04", "4", "4", "4"], capture_output=True,
+                       text=True, timeout=120, env=env)
+    out = r.stdout
+    assert r.returncode == 0, out + r.stderr
+    _check_report_labels(out, box=31.04, ncell=(4, 4, 4), cell=31.04 / 4,
+                         tol=1e-5)
+    m = re.search(rf"X Force\s+Error:.*?Root Mean Squared:\s*({FLOAT})\s*"
+                  rf"\(Norm:\s*({FLOAT})\)", out, re.S)
+    assert float(m.group(2)) < 1e-5

@@ -419,3 +419,53 @@ def test_thin_grids_near_field_match_oracle(cuda, box, ct):
     ref = o.glst(rx, ry, rz, qc, tol, box, rcut, ct=ict)
     assert rel_err(got[:3], ref[:3]) < bar
     assert rel_err(got[3:], ref[3:]) < bar
+
+
+@pytest.mark.parametrize("box,ncell", [((31.04, 31.04, 31.04), (4, 4, 4)),
+                                       ((31.04, 35.0, 40.0), (4, 4, 5))])
+@pytest.mark.parametrize("ct", ["double", "float"])
+def test_ncell_constructor_matches_oracle(cuda, box, ncell, ct):
+    """glst_force(natom, tol, Lx, Ly, Lz, nx, ny, nz) (src/glst_force.cu:
+    80-95): rcut = min_d L_d / n_d, then the grid is derived from that rcut
+    in CT arithmetic, so a non-cubic box need not get the counts asked for
+    (31.04 x 35 x 40 with 4 x 4 x 5 gives rcut 7.76 and 4 x 5 x 6 cells).
+    The step must equal the oracle run at that rcut."""
+    import torch
+    import glst_amd as g
+    from oracle import oracle as o
+    rx, ry, rz, qc = _lattice_charges(box, 2.6, 5)
+    tol = 1e-5
+    ict = 1 if ct == "float" else 0
+    rcut = min(L / n for L, n in zip(box, ncell))
+    f = g.GLSTForce(rx.size, tol, box, ncell=ncell, ct=ct)
+    grid = tuple(f.info().ncell)
+    assert grid == o.grid_dims(ict, *box, rcut)
+    if box[0] == box[1] == box[2]:
+        assert grid == ncell
+    else:
+        assert grid == (4, 5, 6)
+    f.calc_ener_force(*_dev([rx, ry, rz, qc], torch))
+    got = _host(f.get_ef())
+    f.close()
+    ref = o.glst(rx, ry, rz, qc, tol, box, rcut)
+    bar = 1e-10 if ct == "double" else 2e-6
+    assert rel_err(got[:3], ref[:3]) < bar
+    assert rel_err(got[3:], ref[3:]) < bar
+
+
+@pytest.mark.parametrize("n", [2, 255, 257, 700, 9000])
+def test_gpu_coulomb_split_sizes_match_oracle(cuda, n):
+    """The direct sum with its source atoms split into ranges (up to 64 for
+    small systems, partial sums added in range order) against the oracle:
+    one tile, a tile and one atom, and sizes with ragged last ranges."""
+    import torch
+    import glst_amd as g
+    from oracle import oracle as o
+    rng = np.random.default_rng(n)
+    p = rng.uniform(0.0, 20.0, (n, 3))
+    q = rng.choice([-1.0, 1.0], n)
+    h = [p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), q]
+    got = _host(g.coulomb_direct(*_dev(h, torch)))
+    ref = o.coulomb(*h)
+    assert rel_err(got[:3], ref[:3]) < 1e-13
+    assert rel_err(got[3:], ref[3:]) < 1e-13

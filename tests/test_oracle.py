@@ -156,6 +156,40 @@ def test_spherical_rule_exactness(m):
                 assert abs(got - sphere_mean(a, b, c)) < 1e-13, (m, a, b, c)
 
 
+@pytest.mark.parametrize("tol,key", [(1e-4, 0), (1e-4, 3), (1e-5, 1),
+                                     (1e-6, 2)])
+def test_select_sph_is_minimal(tol, key):
+    """The degree the oracle selects is the three-axis walk of
+    src/cubature.cu:486-516 re-evaluated with NumPy Gauss-Legendre nodes
+    over the oracle's product rules (pinned above by their exactness): on
+    each axis the first m >= m0 that passes, every m skipped on that axis
+    fails, and m0 starts at half the number of positive roots."""
+    g = o.alpha_groups(15, 15, 15, 10.0, tol)
+    a, z, dist = g["alpha"][key], g["zcut"][key], g["rmax"][key]
+    order = o.select_gl(tol, dist, a, z)
+    m_sel = o.select_sph(order, tol, dist, a, z)
+    x, w = np.polynomial.legendre.leggauss(order)
+    x, w = x[x > 0], w[x > 0]
+    target = special.erf(a * dist) / dist
+
+    def err(m, axis):
+        xyz, om = o.sph_rule(m)
+        v = (om[None, :] * np.cos(2 * a * z * dist * x[:, None]
+                                  * xyz[None, :, axis])).sum(1)
+        s0 = (w * np.exp(-z * z * x * x) * v).sum() * 4 * a * z / math.pi
+        return abs(s0 - target)
+
+    m = x.size // 2
+    for axis in range(3):
+        start = m
+        while err(m, axis) >= tol:
+            m += 1
+        for mm in range(start, m):
+            assert err(mm, axis) >= tol
+    assert m_sel == m
+    assert err(m_sel, 2) < tol
+
+
 def test_spherical_rule_not_exact_beyond_degree():
     """Degree t+1 (even) is no longer integrated exactly."""
     m = 4
