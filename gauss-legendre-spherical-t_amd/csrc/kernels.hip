@@ -22,6 +22,7 @@
 //  * Every kernel body is a __device__ function shared by the single-system
 //    kernel and its batched twin, which looks its system up in a run table.
 #include "kernels.hpp"
+#include "sincos64.hpp"
 
 #include <algorithm>
 #include <cfloat>
@@ -373,8 +374,8 @@ __device__ __forceinline__ void sf_f64_body(
       const double4 a = atoms[j];
       const double4 b = atoms[j + 1];
       double sa, ca, sb, cb;
-      sincos(k.x * a.x + k.y * a.y + k.z * a.z, &sa, &ca);
-      sincos(k.x * b.x + k.y * b.y + k.z * b.z, &sb, &cb);
+      sincos_cw(k.x * a.x + k.y * a.y + k.z * a.z, &sa, &ca);
+      sincos_cw(k.x * b.x + k.y * b.y + k.z * b.z, &sb, &cb);
       re0 += a.w * ca;
       im0 -= a.w * sa;
       re1 += b.w * cb;
@@ -383,7 +384,7 @@ __device__ __forceinline__ void sf_f64_body(
     if (j < a1) {
       const double4 a = atoms[j];
       double sa, ca;
-      sincos(k.x * a.x + k.y * a.y + k.z * a.z, &sa, &ca);
+      sincos_cw(k.x * a.x + k.y * a.y + k.z * a.z, &sa, &ca);
       re0 += a.w * ca;
       im0 -= a.w * sa;
     }
@@ -1117,7 +1118,7 @@ __device__ __forceinline__ void lr_f64_body(
     for (uint32_t j = 0; j < nr; ++j) {
       const double kj = kap[j];
       double sn, cs;
-      sincos(kj * p, &sn, &cs);
+      sincos_cw(kj * p, &sn, &cs);
       const double2 r = Rc[n0 + j];
       e += cs * r.x - sn * r.y;
       b += kj * (cs * r.y + sn * r.x);
@@ -1496,14 +1497,26 @@ const auto kLrAgKernels =
 constexpr uint32_t kSfDirs = 64;
 constexpr uint32_t kSfChunk = 128;
 
-// Cell `cell` of one system (start at its first cell, S at its field) and
-// direction chunk `chunk` of the group whose tasks are task0 .. +ntask.
+// Cells cell0 .. cell0 + sf_cells(NR) - 1 of one system (start at its first
+// cell, S at its field), wave w taking cell cell0 + w, for direction chunk
+// `chunk` of the group whose tasks are task0 .. +ntask (nodes of task t:
+// node0[t] + i, i < NR, the tasks' nodes consecutive).  Each lane keeps its
+// direction's NR sums in LDS (the first 128-atom chunk of a cell writes
+// them, denser cells add their further chunks in FP32); after one barrier
+// the workgroup stores, for every node pair of the chunk, the slots of its
+// cells as one contiguous run.
 template <int NR>
 __device__ __forceinline__ void sf_mixed_nr_body(
     const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
     const LrTasks &tk, uint32_t task0, uint32_t ntask, uint32_t ncell,
-    float2 *__restrict__ S, uint32_t cell, uint32_t chunk) {
-  const uint32_t t = task0 + chunk * kSfDirs + threadIdx.x;
+    float2 *__restrict__ S, uint32_t cell0, uint32_t chunk) {
+  constexpr uint32_t CELLS = sf_cells(NR);
+  constexpr uint32_t kStride = kSfDirs * NR + 1;  // float2 per staged cell
+  extern __shared__ float2 sf_stage[];             // [CELLS][kStride]
+  const uint32_t w = threadIdx.x / kSfDirs, lane = threadIdx.x % kSfDirs;
+  const uint32_t cell = cell0 + w;
+  const uint32_t tfirst = task0 + chunk * kSfDirs;
+  const uint32_t t = tfirst + lane;
   const bool active = t < task0 + ntask;
   const uint32_t tt = active ? t : task0;
   float kap[NR];
@@ -1511,16 +1524,26 @@ __device__ __forceinline__ void sf_mixed_nr_body(
 #pragma unroll
   for (int i = 0; i < NR; ++i) kap[i] = kp[i];
   const float4 d = tk.dir32[tt];
+  // one cell per workgroup (staging off): the lane stores its own slots
   const uint32_t n0 = tk.node0[tt];
-  {
+  float2 *st = CELLS == 1 ? nullptr : sf_stage + w * kStride + lane * NR;
+  auto put = [&](int i, float2 v) {
+    if constexpr (CELLS == 1)
+      S[sf_slot(n0 + i, cell, ncell)] = v;
+    else
+      st[i] = v;
+  };
+  auto get = [&](int i) {
+    if constexpr (CELLS == 1)
+      return S[sf_slot(n0 + i, cell, ncell)];
+    else
+      return st[i];
+  };
+  if (cell < ncell && (CELLS > 1 || active)) {
     const uint32_t a0 = start[cell], a1 = start[cell + 1];
     if (a0 == a1) {
-      if (active) {
 #pragma unroll
-        for (int i = 0; i < NR; ++i)
-          S[sf_slot(n0 + i, cell, ncell)] = make_float2(0.f, 0.f);
-      }
-      return;
+      for (int i = 0; i < NR; ++i) put(i, make_float2(0.f, 0.f));
     }
     for (uint32_t jb = a0; jb < a1; jb += kSfChunk) {
       const uint32_t je = (jb + kSfChunk < a1) ? jb + kSfChunk : a1;
@@ -1537,20 +1560,35 @@ __device__ __forceinline__ void sf_mixed_nr_body(
           im[i] = fmaf(-a.w, sin_rev(ph), im[i]);
         }
       }
-      if (active) {
-        if (jb == a0) {
+      if (jb == a0) {
 #pragma unroll
-          for (int i = 0; i < NR; ++i)
-            S[sf_slot(n0 + i, cell, ncell)] = make_float2(re[i], im[i]);
-        } else {  // cells denser than one chunk
+        for (int i = 0; i < NR; ++i) put(i, make_float2(re[i], im[i]));
+      } else {  // cells denser than one chunk
 #pragma unroll
-          for (int i = 0; i < NR; ++i) {
-            float2 &o = S[sf_slot(n0 + i, cell, ncell)];
-            o = make_float2(o.x + re[i], o.y + im[i]);
-          }
+        for (int i = 0; i < NR; ++i) {
+          const float2 o = get(i);
+          put(i, make_float2(o.x + re[i], o.y + im[i]));
         }
       }
     }
+  }
+  if constexpr (CELLS == 1) return;
+  __syncthreads();
+  // nodes [nb, ne) of the chunk; pairs q0 .. q1-1 touch them (a pair shared
+  // with the neighbouring chunk keeps the other node's slot untouched)
+  const uint32_t nact = (ntask - chunk * kSfDirs < kSfDirs)
+                            ? ntask - chunk * kSfDirs
+                            : kSfDirs;
+  const uint32_t nb = tk.node0[tfirst], ne = nb + nact * NR;
+  const uint32_t ncl = (ncell - cell0 < CELLS) ? ncell - cell0 : CELLS;
+  const uint32_t q0 = nb >> 1, q1 = (ne + 1) >> 1;
+  const uint32_t total = (q1 - q0) * CELLS * 2;
+  for (uint32_t f = threadIdx.x; f < total; f += CELLS * kSfDirs) {
+    const uint32_t b = f & 1u, r = f >> 1;
+    const uint32_t wc = r % CELLS, n = 2 * (q0 + r / CELLS) + b;
+    if (n < nb || n >= ne || wc >= ncl) continue;
+    const uint32_t ln = n - nb, m = ln / NR, i = ln - m * NR;
+    S[sf_slot(n, cell0 + wc, ncell)] = sf_stage[wc * kStride + m * NR + i];
   }
 }
 
@@ -1593,18 +1631,20 @@ __device__ __forceinline__ void sf_mixed_any_body(
 // (cell, direction chunk) of a workgroup, numbered so that each XCD runs
 // consecutive cells (their node-pair slots share cache lines).
 template <int NR>
-__global__ __launch_bounds__(kSfDirs) void sf_mixed_nr_kernel(
+__global__ __launch_bounds__(kSfDirs * sf_cells(NR)) void sf_mixed_nr_kernel(
     const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
     LrTasks tk, uint32_t task0, uint32_t ntask, uint32_t ncell,
     float2 *__restrict__ S) {
+  // gridDim.x = cell groups of sf_cells(NR) cells
   const uint32_t wg = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x,
                                 gridDim.x * gridDim.y);
+  const uint32_t ncg = gridDim.x;
   if constexpr (NR > 0)
-    sf_mixed_nr_body<NR>(atoms, start, tk, task0, ntask, ncell, S, wg % ncell,
-                         wg / ncell);
+    sf_mixed_nr_body<NR>(atoms, start, tk, task0, ntask, ncell, S,
+                         (wg % ncg) * sf_cells(NR), wg / ncg);
   else
-    sf_mixed_any_body(atoms, start, tk, task0, ntask, ncell, S, wg % ncell,
-                      wg / ncell);
+    sf_mixed_any_body(atoms, start, tk, task0, ntask, ncell, S, wg % ncg,
+                      wg / ncg);
 }
 
 typedef void (*SfNrKernel)(const float4 *, const uint32_t *, LrTasks,
@@ -1616,7 +1656,6 @@ constexpr auto sf_nr_table(std::integer_sequence<int, N...>) {
   return std::array<SfNrKernel, sizeof...(N)>{{&sf_mixed_nr_kernel<N>...}};
 }
 
-constexpr int kSfMaxUnrolled = 40;
 const auto kSfNrKernels =
     sf_nr_table(std::make_integer_sequence<int, kSfMaxUnrolled + 1>{});
 
@@ -2051,7 +2090,7 @@ __global__ __launch_bounds__(kSfBlock) void sf_f64_batch_kernel(
 }
 
 template <int NR>
-__global__ __launch_bounds__(kSfDirs) void sf_mixed_batch_kernel(
+__global__ __launch_bounds__(kSfDirs * sf_cells(NR)) void sf_mixed_batch_kernel(
     const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
     LrTasks tk, const SysDev *__restrict__ sys, const Seg *__restrict__ seg,
     uint32_t nseg, float2 *__restrict__ S) {
@@ -2059,12 +2098,14 @@ __global__ __launch_bounds__(kSfDirs) void sf_mixed_batch_kernel(
   const Seg g = seg[run_of(seg, nseg, wg)];
   const SysDev &d = sys[g.sys];
   const uint32_t local = wg - g.blk0;
+  constexpr uint32_t CELLS = sf_cells(NR);
+  const uint32_t ncg = (d.ncell + CELLS - 1) / CELLS;
   if constexpr (NR > 0)
     sf_mixed_nr_body<NR>(atoms, start + d.cell0, tk, g.a, g.b, d.ncell,
-                         S + d.field0, local % d.ncell, local / d.ncell);
+                         S + d.field0, (local % ncg) * CELLS, local / ncg);
   else
     sf_mixed_any_body(atoms, start + d.cell0, tk, g.a, g.b, d.ncell,
-                      S + d.field0, local % d.ncell, local / d.ncell);
+                      S + d.field0, local % ncg, local / ncg);
 }
 
 typedef void (*SfBatchKernel)(const float4 *, const uint32_t *, LrTasks,
@@ -2254,6 +2295,16 @@ void prepare_kernels(size_t lds_limit) {
       (void)hipGetLastError();
     }
   }
+  // the S(k) kernels' staging areas (up to 82 KB)
+  for (int nr = 1; nr <= kSfMaxUnrolled; ++nr) {
+    const void *sf[] = {reinterpret_cast<const void *>(kSfNrKernels[nr]),
+                        reinterpret_cast<const void *>(kSfBatchKernels[nr])};
+    for (const void *f : sf)
+      if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(sf_lds_bytes(nr))) !=
+          hipSuccess)
+        (void)hipGetLastError();
+  }
 }
 
 void fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s) {
@@ -2329,11 +2380,15 @@ void sf(bool mixed, const void *atoms, const uint32_t *start,
   for (size_t q = 0; q < ngroup; ++q) {
     const SfGroup &gr = groups[q];
     if (gr.ntask == 0) continue;
-    const dim3 grid(g.ncell, (gr.ntask + kSfDirs - 1) / kSfDirs);
-    kSfNrKernels[gr.nr <= static_cast<uint32_t>(kSfMaxUnrolled) ? gr.nr : 0]
-        <<<grid, kSfDirs, 0, s>>>(static_cast<const float4 *>(atoms), start,
-                                  tk, gr.task0, gr.ntask, g.ncell,
-                                  static_cast<float2 *>(S));
+    const int nr = gr.nr <= static_cast<uint32_t>(kSfMaxUnrolled)
+                       ? static_cast<int>(gr.nr)
+                       : 0;
+    const uint32_t cells = sf_cells(nr);
+    const dim3 grid((g.ncell + cells - 1) / cells,
+                    (gr.ntask + kSfDirs - 1) / kSfDirs);
+    kSfNrKernels[nr]<<<grid, kSfDirs * cells, sf_lds_bytes(nr), s>>>(
+        static_cast<const float4 *>(atoms), start, tk, gr.task0, gr.ntask,
+        g.ncell, static_cast<float2 *>(S));
   }
 }
 
@@ -2580,10 +2635,12 @@ void sf_batch(bool mixed, const void *atoms, const uint32_t *start,
           static_cast<double2 *>(S));
       continue;
     }
-    kSfBatchKernels[L.nr <= static_cast<uint32_t>(kSfMaxUnrolled) ? L.nr : 0]
-        <<<L.nblk, kSfDirs, 0, s>>>(static_cast<const float4 *>(atoms), start,
-                                    tk, sys, L.seg, L.nseg,
-                                    static_cast<float2 *>(S));
+    const int nr = L.nr <= static_cast<uint32_t>(kSfMaxUnrolled)
+                       ? static_cast<int>(L.nr)
+                       : 0;
+    kSfBatchKernels[nr]<<<L.nblk, kSfDirs * sf_cells(nr), sf_lds_bytes(nr),
+                          s>>>(static_cast<const float4 *>(atoms), start, tk,
+                               sys, L.seg, L.nseg, static_cast<float2 *>(S));
   }
 }
 

@@ -65,6 +65,31 @@ __host__ __device__ inline size_t sf_slot(uint32_t node, uint32_t cell,
   return ((static_cast<size_t>(node >> 1) * ncell + cell) << 1) + (node & 1);
 }
 
+// Mixed structure factors: one wave per (cell, 64 directions of a group), the
+// NR radial nodes of a direction unrolled for NR <= kSfMaxUnrolled (larger
+// counts run a generic kernel, one cell per workgroup).  An unrolled
+// workgroup holds sf_cells(NR) consecutive cells, stages their sums in LDS
+// and stores whole runs of node-pair slots: sf_cells consecutive cells of
+// one node pair are 16 sf_cells contiguous bytes (8 cells fill a 128-byte
+// line).  Above 19 radial nodes 4 cells per workgroup keep the staging area
+// (64 NR float2 per wave) from lowering occupancy further.
+constexpr int kSfMaxUnrolled = 40;
+#ifndef GLST_SF_STAGE
+#define GLST_SF_STAGE 1
+#endif
+__host__ __device__ constexpr uint32_t sf_cells(int nr) {
+  return (!GLST_SF_STAGE || nr <= 0 || nr > kSfMaxUnrolled)
+             ? 1u
+             : (nr <= 19 ? 8u : 4u);
+}
+// LDS of one unrolled workgroup: per cell 64 NR float2, plus one to
+// stagger the cells over the banks
+__host__ __device__ constexpr uint32_t sf_lds_bytes(int nr) {
+  return sf_cells(nr) == 1u ? 0u
+                            : sf_cells(nr) * (64u * static_cast<uint32_t>(nr) +
+                                              1u) * 8u;
+}
+
 struct LrTaskArgs;
 // Tasks (directions) of one alpha group for the mixed structure factors.
 struct SfGroup {
