@@ -549,8 +549,7 @@ constexpr uint32_t kRmtBlock = GLST_RMT_BLOCK;
 #endif
 constexpr uint32_t kRmtBatch = GLST_RMT_ZBATCH;
 constexpr uint32_t kRmtLoadBatch = 4;
-// Threads of the persistent single-system workgroups and the S slots per
-// thread they hold across a gather.
+// Threads of the persistent single-system workgroups.
 #ifndef GLST_RMT_PBLOCK
 #define GLST_RMT_PBLOCK 512
 #endif
@@ -560,11 +559,7 @@ constexpr uint32_t kRmtLoadBatch = 4;
 #ifndef GLST_RMT_MIN_WAVES
 #define GLST_RMT_MIN_WAVES 1
 #endif
-#ifndef GLST_RMT_PREFETCH
-#define GLST_RMT_PREFETCH 8
-#endif
 constexpr uint32_t kRmtPBlock = GLST_RMT_PBLOCK;
-constexpr uint32_t kRmtPrefetch = GLST_RMT_PREFETCH;
 
 // n / d for n < 2^22 and 0 < d, with rd = 1.0f / d: the float quotient is
 // within one of the exact one, and one correction step fixes it.
@@ -786,24 +781,23 @@ __device__ inline void rmt_border(double2 *P, const RmtGeom &g) {
 }
 
 // Tables and phase factors of nodes node0.. are in LDS (after a barrier).
-// (2) Mixed path: turns the cell-local sums into S_c = exp(-i k.c) local and
-// takes the running sum along z (the FP64 slots hold z-running sums
-// already).  (3) Running sums along y, then x.  `between` runs next (the
+// (2) Mixed path, unless the tables already hold them (ZDONE): turns the
+// cell-local sums into S_c = exp(-i k.c) local and takes the running sum
+// along z (the FP64 slots hold z-running sums already).  (3) Running sums along y, then x.  `between` runs next (the
 // persistent kernel fetches its next tile there).  (4) Shells: lanes on
 // consecutive (node, z), every thread walking y; the mixed path applies
 // w exp(+i k.c) with the node weight folded into the thread's x and z
 // factors.
-template <uint32_t BS, bool MIXED, class F>
+template <uint32_t BS, bool MIXED, bool ZDONE, class F>
 __device__ __forceinline__ void rmt_compute(
-    double2 *P, const RmtGeom &g, const double4 *__restrict__ k64,
-    const uint32_t *__restrict__ group, const uint32_t *__restrict__ r_in,
-    const uint32_t *__restrict__ r_out, uint32_t Kp, uint32_t node0,
-    void *__restrict__ R, F between) {
+    double2 *P, const double2 *phase, const RmtGeom &g,
+    const double4 *__restrict__ k64, const uint32_t *__restrict__ group,
+    const uint32_t *__restrict__ r_in, const uint32_t *__restrict__ r_out,
+    uint32_t Kp, uint32_t node0, void *__restrict__ R, F between) {
   const uint32_t T = g.T, nx = g.nx, ny = g.ny, nz = g.nz, nax = g.nax;
   const uint32_t zp = g.zp, plane = g.plane, ts = g.ts, lgT = g.lgT;
   const uint32_t tid = threadIdx.x;
-  const double2 *phase = P + T * ts;
-  if (MIXED) {
+  if (MIXED && !ZDONE) {
     // (2) z-lines
     for (uint32_t l = tid; l < T * nx * g.ypad; l += BS) {
       const uint32_t t = l & (T - 1), q = l >> lgT;
@@ -919,8 +913,103 @@ __device__ __forceinline__ void rmt_lds_body(
       phase[e] = rmt_phase(k64, g, node0, h, e);
   rmt_border<kRmtBlock>(P, g);
   __syncthreads();
-  rmt_compute<kRmtBlock, MIXED>(P, g, k64, group, r_in, r_out, Kp, node0, R,
-                                [] {});
+  rmt_compute<kRmtBlock, MIXED, false>(P, phase, g, k64, group, r_in, r_out,
+                                       Kp, node0, R, [] {});
+}
+
+// z-lines loaded straight into the tables (the persistent kernel).  Line l
+// of a tile is node t = l & (T-1) and column q = l >> lg T (x = q / ny,
+// y = q - x ny); its slots sf_slot(node0 + t, q nz + z) are 2 apart.  The
+// first kRmtLine<MIXED> of them travel through registers (fetched while the
+// previous tile's shells are gathered), the rest of a longer line is read
+// when the line is stored.  Storing writes the running sums along z into
+// the node's table; the mixed path first turns the cell-local sums into
+// S_c = exp(-i k.c) local in FP64 (the FP64 slots hold z-running sums).
+template <bool MIXED>
+struct RmtLine;
+template <>
+struct RmtLine<true> {
+  static constexpr uint32_t kLen = 24;
+  float2 v[kLen];
+};
+template <>
+struct RmtLine<false> {
+  static constexpr uint32_t kLen = 16;
+  double re[kLen], im[kLen];  // (a double2 array stays in scratch memory)
+};
+
+template <bool MIXED>
+__device__ __forceinline__ double2 rmt_slot(const void *__restrict__ Sv,
+                                            size_t i) {
+  if constexpr (MIXED) {
+    const float2 f = static_cast<const float2 *>(Sv)[i];
+    return make_double2(f.x, f.y);
+  } else {
+    return static_cast<const double2 *>(Sv)[i];
+  }
+}
+
+template <bool MIXED>
+__device__ __forceinline__ void rmt_line_fetch(const void *__restrict__ Sv,
+                                               const RmtGeom &g,
+                                               uint32_t node0, uint32_t l,
+                                               RmtLine<MIXED> &ln) {
+  constexpr uint32_t L = RmtLine<MIXED>::kLen;
+  const uint32_t t = l & (g.T - 1), q = l >> g.lgT;
+  const size_t base = sf_slot(node0 + t, q * g.nz, g.ncell);
+#pragma unroll
+  for (uint32_t z = 0; z < L; ++z) {
+    const uint32_t zz = z < g.nz ? z : g.nz - 1;  // clamped: in the line
+    if constexpr (MIXED) {
+      ln.v[z] = static_cast<const float2 *>(Sv)[base + 2 * zz];
+    } else {
+      const double2 v = static_cast<const double2 *>(Sv)[base + 2 * zz];
+      ln.re[z] = v.x;
+      ln.im[z] = v.y;
+    }
+  }
+}
+
+template <bool MIXED>
+__device__ __forceinline__ void rmt_line_put(double2 *P,
+                                             const double2 *phase,
+                                             const RmtGeom &g,
+                                             const void *__restrict__ Sv,
+                                             uint32_t node0, uint32_t l,
+                                             const RmtLine<MIXED> &ln) {
+  constexpr uint32_t L = RmtLine<MIXED>::kLen;
+  const uint32_t t = l & (g.T - 1), q = l >> g.lgT;
+  const uint32_t x = fast_div(q, g.ny, g.rny), y = q - x * g.ny;
+  double2 *p = P + t * g.ts + (x + 1) * g.plane + (y + 1) * g.zp + 1;
+  const size_t base = sf_slot(node0 + t, q * g.nz, g.ncell);
+  if constexpr (MIXED) {
+    const double2 *ph = phase + t * g.nax;
+    const double2 a = ph[x], b = ph[g.nx + y];
+    const double pr = a.x * b.x - a.y * b.y, pi = a.x * b.y + a.y * b.x;
+    const double2 *pz = ph + g.nx + g.ny;
+    double2 run = make_double2(0.0, 0.0);
+    auto step = [&](uint32_t z, double2 v) {
+      const double2 c = pz[z];
+      const double er = pr * c.x - pi * c.y, ei = pr * c.y + pi * c.x;
+      // v * conj(e), accumulated along z
+      run.x += v.x * er + v.y * ei;
+      run.y += v.y * er - v.x * ei;
+      p[z] = run;
+    };
+#pragma unroll
+    for (uint32_t z = 0; z < L; ++z)
+      if (z < g.nz) step(z, make_double2(ln.v[z].x, ln.v[z].y));
+    for (uint32_t z = L; z < g.nz; ++z)
+      step(z, rmt_slot<true>(Sv, base + 2 * z));
+  } else {
+    // past the line's end the fetch repeated its last slot, so the clamped
+    // store rewrites that slot's value (a loop bounded by nz instead would
+    // keep the line in scratch memory)
+#pragma unroll
+    for (uint32_t z = 0; z < L; ++z)
+      p[z < g.nz ? z : g.nz - 1] = make_double2(ln.re[z], ln.im[z]);
+    for (uint32_t z = L; z < g.nz; ++z) p[z] = rmt_slot<false>(Sv, base + 2 * z);
+  }
 }
 
 // Shell sums of tiles tile0 .. tile0+ntile-1 (T nodes each) by persistent
@@ -928,8 +1017,9 @@ __device__ __forceinline__ void rmt_lds_body(
 // workgroups b with b % 8 == x) owns a contiguous eighth of the tiles and
 // its workgroups take every (count on x)-th tile of it, so the tiles in
 // flight on one XCD are consecutive and their R rows meet in its L2.  While
-// a workgroup gathers one tile's shells, the next tile's S slots (the first
-// round) are already on their way into registers.
+// a workgroup gathers one tile's shells, the next tile's z-lines are already
+// on their way into registers and its phase factors are written to the
+// second of two phase buffers.
 template <bool MIXED>
 __global__ __launch_bounds__(kRmtPBlock, GLST_RMT_MIN_WAVES) void rmt_lds_kernel(
     const void *__restrict__ Sv, const double4 *__restrict__ k64,
@@ -937,9 +1027,9 @@ __global__ __launch_bounds__(kRmtPBlock, GLST_RMT_MIN_WAVES) void rmt_lds_kernel
     const uint32_t *__restrict__ r_out, uint32_t Kp, uint32_t T,
     uint32_t tile0, uint32_t ntile, uint32_t nx, uint32_t ny, uint32_t nz,
     double h, void *__restrict__ R) {
-  extern __shared__ double2 P[];  // [T][ts], then [T][nx+ny+nz] phases
+  // [T][ts] tables, then two buffers of [T][nx+ny+nz] phase factors
+  extern __shared__ double2 P[];
   const RmtGeom g(T, nx, ny, nz);
-  double2 *phase = P + T * g.ts;
   const uint32_t tid = threadIdx.x;
   const uint32_t nwg = gridDim.x;
   uint32_t tile, end, step;
@@ -954,34 +1044,43 @@ __global__ __launch_bounds__(kRmtPBlock, GLST_RMT_MIN_WAVES) void rmt_lds_kernel
     end = ntile;
     step = nwg;
   }
-  const uint32_t total = g.load_count();
   const uint32_t nph = MIXED ? T * g.nax : 0u;
+  const uint32_t nline = T * nx * ny;
+  double2 *phase0 = P + T * g.ts;
   rmt_border<kRmtPBlock>(P, g);
-  // the first kRmtPrefetch rounds of the next tile are held in registers
-  // across the gather; the rest (elements from tid + kRmtPrefetch kRmtPBlock
-  // on) are read when the tile starts
-  RmtRound<MIXED, kRmtPrefetch> rd;
-  if (tile < end) rmt_fetch<kRmtPBlock>(Sv, g, (tile0 + tile) * T, tid, rd);
-  while (tile < end) {  // uniform over the workgroup
+  RmtLine<MIXED> ln;
+  if (tile < end) {
     const uint32_t node0 = (tile0 + tile) * T;
     for (uint32_t e = tid; e < nph; e += kRmtPBlock)
-      phase[e] = rmt_phase(k64, g, node0, h, e);
-    rmt_put<kRmtPBlock>(P, g, tid, rd);
-    for (uint32_t e0 = tid + kRmtPBlock * kRmtPrefetch; e0 < total;
-         e0 += kRmtPBlock * kRmtLoadBatch) {
-      RmtRound<MIXED, kRmtLoadBatch> more;
-      rmt_fetch<kRmtPBlock>(Sv, g, node0, e0, more);
-      rmt_put<kRmtPBlock>(P, g, e0, more);
+      phase0[e] = rmt_phase(k64, g, node0, h, e);
+    if (tid < nline) rmt_line_fetch(Sv, g, node0, tid, ln);
+  }
+  __syncthreads();  // the first tile's phase factors
+  uint32_t buf = 0;
+  while (tile < end) {  // uniform over the workgroup
+    const uint32_t node0 = (tile0 + tile) * T;
+    const double2 *phase = phase0 + buf * nph;
+    if (tid < nline) rmt_line_put(P, phase, g, Sv, node0, tid, ln);
+    for (uint32_t l = tid + kRmtPBlock; l < nline; l += kRmtPBlock) {
+      RmtLine<MIXED> more;
+      rmt_line_fetch(Sv, g, node0, l, more);
+      rmt_line_put(P, phase, g, Sv, node0, l, more);
     }
     __syncthreads();
     const uint32_t next = tile + step;
-    rmt_compute<kRmtPBlock, MIXED>(
-        P, g, k64, group, r_in, r_out, Kp, node0, R, [&] {
-          if (next < end)
-            rmt_fetch<kRmtPBlock>(Sv, g, (tile0 + next) * T, tid, rd);
+    rmt_compute<kRmtPBlock, MIXED, true>(
+        P, phase, g, k64, group, r_in, r_out, Kp, node0, R, [&] {
+          if (next < end) {
+            const uint32_t n1 = (tile0 + next) * T;
+            double2 *ph1 = phase0 + (buf ^ 1u) * nph;
+            for (uint32_t e = tid; e < nph; e += kRmtPBlock)
+              ph1[e] = rmt_phase(k64, g, n1, h, e);
+            if (tid < nline) rmt_line_fetch(Sv, g, n1, tid, ln);
+          }
         });
-    __syncthreads();  // the tables are free for the next tile
+    __syncthreads();  // the tables are free, the next phases written
     tile = next;
+    buf ^= 1u;
   }
 }
 

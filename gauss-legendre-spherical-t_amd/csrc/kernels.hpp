@@ -117,7 +117,8 @@ inline void sf(bool mixed, const void *atoms, const uint32_t *start,
 // ---- remote (shell) sums (sum_rmt_sf) --------------------------------------
 // LDS layout of the shell-sum workgroups.  A workgroup holds the summed-
 // volume tables of T nodes (T = 1 or a power of two), T tables of
-// rmt_table_stride elements (double2) plus T * (nx+ny+nz) phase factors.
+// rmt_table_stride elements (double2) plus two buffers of T * (nx+ny+nz)
+// phase factors (the persistent kernel fills one while it reads the other).
 // A table has a zero plane before the first cell of each axis and z-lines
 // of odd pitch: (nx+1) x (ny+1) x rmt_zpitch(nz) elements.  An LDS cycle
 // serves 16 consecutive lanes of a 16-byte access, conflict-free when their
@@ -146,7 +147,8 @@ __host__ __device__ inline uint32_t rmt_line_group(uint32_t T) {
 inline size_t rmt_lds_bytes(uint32_t nx, uint32_t ny, uint32_t nz,
                             uint32_t T) {
   return static_cast<size_t>(T) *
-         (rmt_table_stride(rmt_table_elems(nx, ny, nz), T) + nx + ny + nz) *
+         (rmt_table_stride(rmt_table_elems(nx, ny, nz), T) +
+          2 * (nx + ny + nz)) *
          16;
 }
 // Largest T <= tmax (a power of two) whose tables fit lds_limit bytes; 0 when

@@ -15,8 +15,18 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstring>
 
 namespace glst {
+
+// x with its sign bit XORed with bit 63 of m
+__host__ __device__ inline double flip_sign(double x, uint64_t m) {
+  uint64_t u;
+  memcpy(&u, &x, sizeof u);
+  u ^= m;
+  memcpy(&x, &u, sizeof x);
+  return x;
+}
 
 __host__ __device__ inline void sincos_cw(double x, double *s, double *c) {
   constexpr double kTwoOverPi = 0.6366197723675814;
@@ -47,12 +57,14 @@ __host__ __device__ inline void sincos_cw(double x, double *s, double *c) {
   pc = fma(pc, r2, 0.041666666666666664);     // 1/4!
   pc = fma(pc, r2, -0.5);                     // -1/2!
   const double cs = fma(r2, pc, 1.0);
-  // x = n pi/2 + r: quadrant q = n mod 4 (two's complement for n < 0)
-  const uint32_t q = static_cast<uint32_t>(static_cast<int64_t>(n)) & 3u;
+  // x = n pi/2 + r: quadrant q = n mod 4 (two's complement for n < 0;
+  // |n| < 2^20 fits a 32-bit conversion)
+  const uint32_t q = static_cast<uint32_t>(static_cast<int32_t>(n));
   const double a = (q & 1u) ? cs : sn;  // sin x up to sign
   const double b = (q & 1u) ? sn : cs;  // cos x up to sign
-  *s = (q & 2u) ? -a : a;
-  *c = ((q + 1u) & 2u) ? -b : b;
+  // signs: bit 1 of q for sin x, bit 1 of q + 1 for cos x, moved to bit 63
+  *s = flip_sign(a, static_cast<uint64_t>(q & 2u) << 62);
+  *c = flip_sign(b, static_cast<uint64_t>((q + 1u) & 2u) << 62);
 }
 
 }  // namespace glst
