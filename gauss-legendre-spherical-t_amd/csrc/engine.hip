@@ -1228,6 +1228,39 @@ glst_status_t glst_coulomb_direct(const double *d_rx, const double *d_ry,
   });
 }
 
+glst_status_t glst_coulomb_direct_noalloc(const double *d_rx,
+                                          const double *d_ry,
+                                          const double *d_rz,
+                                          const double *d_qc, uint32_t natom,
+                                          double *d_fx, double *d_fy,
+                                          double *d_fz, double *d_en,
+                                          void *stream) {
+  return guarded(nullptr, [&] {
+    if (natom > 0 && (!d_rx || !d_ry || !d_rz || !d_qc || !d_fx || !d_fy ||
+                      !d_fz || !d_en))
+      throw ArgError("glst_coulomb_direct_noalloc: null pointer");
+    if (natom == 0) return;
+    // the same split plan as glst_coulomb_direct, capped by the fixed array
+    const dev::CoulombPlan plan =
+        dev::coulomb_plan(natom, dev::kCoulScratchEntries);
+    if (plan.entries > dev::kCoulScratchEntries)
+      throw ArgError("glst_coulomb_direct_noalloc: " +
+                     std::to_string(plan.entries) +
+                     " partial sums exceed the fixed array of " +
+                     std::to_string(dev::kCoulScratchEntries));
+    double4 *part = nullptr;
+    if (plan.entries > 0) {
+      part = dev::coulomb_scratch();
+      if (!part)
+        throw HipError("glst_coulomb_direct_noalloc: device address of the "
+                       "partial-sum array not found");
+    }
+    dev::coulomb_direct(d_rx, d_ry, d_rz, d_qc, natom, plan, part, d_fx, d_fy,
+                        d_fz, d_en, static_cast<hipStream_t>(stream));
+    check_launch("coulomb_direct_noalloc");
+  });
+}
+
 glst_status_t glst_erfc_inv(double y, double tol, double *x) {
   if (!x) return GLST_ERR_INVALID;
   return guarded(nullptr, [&] { *x = erfc_inv(y, tol); });

@@ -30,6 +30,7 @@
 #include <array>
 #include <cmath>
 #include <type_traits>
+#include <mutex>
 #include <utility>
 
 namespace glst {
@@ -2654,6 +2655,31 @@ CoulombPlan coulomb_plan(uint32_t natom, size_t max_entries) {
   if (p.nsplit <= 1) return CoulombPlan{1, natom, 0};
   p.entries = static_cast<size_t>(p.nsplit) * natom;
   return p;
+}
+
+// Partial sums of the allocation-free direct sum: one array per GPU in the
+// library's code object, held until the process ends.  Host code never
+// passes `scratch` itself to a kernel (in host code the name is a stand-in,
+// not a device address); it passes the address hipGetSymbolAddress returns
+// for the current device.
+static __device__ __align__(256) double4 scratch[kCoulScratchEntries];
+
+double4 *coulomb_scratch() {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static double4 *addr[kMaxDevices] = {};
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess || device < 0 ||
+      device >= kMaxDevices)
+    return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess)
+      return nullptr;
+    addr[device] = static_cast<double4 *>(p);
+  }
+  return addr[device];
 }
 
 void coulomb_direct(const double *rx, const double *ry, const double *rz,

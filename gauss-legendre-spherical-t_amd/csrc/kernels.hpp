@@ -344,6 +344,11 @@ struct CoulombPlan {
 };
 // at most `max_entries` partial sums
 CoulombPlan coulomb_plan(uint32_t natom, size_t max_entries);
+// The library's fixed partial-sum array of the allocation-free direct sum
+// (32 MiB per GPU): its device address on the current device, looked up
+// once per device; nullptr if the lookup fails.
+constexpr size_t kCoulScratchEntries = size_t(1) << 20;
+double4 *coulomb_scratch();
 void coulomb_direct(const double *rx, const double *ry, const double *rz,
                     const double *qc, uint32_t natom, const CoulombPlan &p,
                     double4 *part, double *fx, double *fy, double *fz,

@@ -244,6 +244,25 @@ glst_status_t glst_coulomb_direct(const double *d_rx, const double *d_ry,
                                   uint32_t natom, double *d_fx, double *d_fy,
                                   double *d_fz, double *d_en, void *stream);
 
+/* No counterpart in the reference.  glst_coulomb_direct for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc): the same kernels and split plan,
+ * with the partial sums of the source splits in a fixed 32 MiB array of the
+ * library (1,048,576 double4 entries per GPU, held until the process ends)
+ * instead of a temporary buffer.  The plan never needs more than that
+ * (natom * splits <= 1,048,576; larger systems run unsplit), and a plan
+ * beyond it would be refused with GLST_ERR_INVALID.  Calls no hipMalloc or
+ * hipFree and does not synchronise: the outputs are ready when `stream` is.
+ * Every call on one GPU shares the array, so calls that overlap on one GPU
+ * (other streams or host threads) race; serialise them. */
+glst_status_t glst_coulomb_direct_noalloc(const double *d_rx,
+                                          const double *d_ry,
+                                          const double *d_rz,
+                                          const double *d_qc, uint32_t natom,
+                                          double *d_fx, double *d_fy,
+                                          double *d_fz, double *d_en,
+                                          void *stream);
+
 /* ---- host numerics (no GPU needed) -------------------------------------- */
 
 /* erfc_inv(y, tol): src/glst_force.cu:1616-1640. */
