@@ -37,6 +37,7 @@ NWATER, BOX, RCUT, TOL, SEED = 96000, 142.0, 10.0, 1e-4, 1
 SERVE_RCUT, SERVE_TOL = 9.0, 2e-5
 ERR_BAR = 1e-5  # the metric's relative force error bar
 PEAK_FP32_VALU_TFLOPS = 157.3   # MI355X published FP32 vector peak
+PEAK_FP64_VALU_TFLOPS = 78.6    # MI355X published FP64 vector peak
 PEAK_HBM_GBS = 8000.0
 STAGES = ["bin", "sort", "sf", "rmt", "lr", "lr_reduce", "sr"]
 
@@ -511,6 +512,21 @@ def main():
         "pair_rate_per_s": pairs / (lr_ms * 1e-3),
     }
 
+    # the FP64 path's back-projection (lr_f64_kernel, one launch per step)
+    # at the same 20 flops per pair against the FP64 vector peak
+    lr64_ms = t64["kernel"].get("lr", float("nan"))
+    ach64 = 20.0 * pairs / (lr64_ms * 1e-3) / 1e12
+    roof64 = {
+        "kernel": "lr_f64_kernel (calc_lr_ef back-projection, FP64)",
+        "bound": "valu",
+        "achieved": ach64,
+        "peak": PEAK_FP64_VALU_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": ach64 / PEAK_FP64_VALU_TFLOPS,
+        "avg_ms": lr64_ms,
+        "pairs": pairs,
+    }
+
     served = batched = None
     if not args.no_serve:
         served = serve_bench(g, torch, np)
@@ -549,7 +565,9 @@ def main():
         "ref_style_err": ref_style_errors(got_mixed, ref),
         "per_atom_err": error_report(got_mixed, ref),
         "fp64_path": {"value": natom / dt_fp64, "ms_per_step": dt_fp64 * 1e3,
-                      "steps": args.fp64_steps, "schedule": tuned64},
+                      "steps": args.fp64_steps, "schedule": tuned64,
+                      "kernel_ms": t64["kernel"],
+                      "roofline": roof64},
         "speedup_vs_fp64": dt_fp64 / dt_mixed,
         "schedule": dict(tuned, eager_ms_per_step=tm["eager"] * 1e3,
                          graph_ms_per_step=tm["graph"] * 1e3),

@@ -230,6 +230,7 @@ BatchLayout pack_batch(int ct,
     d.hist0 = B.hist;
     d.h = u.grid.h;
     d.inv_cell = u.inv_cell;
+    d.far_d = far_offset(u.nodes);
     d.ntile = div_up(na, 256);
     d.group0 = off.group0;
     d.Kp = lr ? u.Kp : 0;
@@ -512,7 +513,8 @@ void glst_batch::step_async(const double *rx, const double *ry,
   const dev::LrTaskArgs ta = task_args();
   const ShapeTables &tb = *tables;
   dev::sf_batch(mixed(), atoms, cell_start.as<uint32_t>(),
-                tb.d_k64.as<double4>(), ta, sys, L_sf, S.as<void>(), stream);
+                cell_sorted.as<uint32_t>(), tb.d_k64.as<double4>(), ta, sys,
+                L_sf, S.as<void>(), stream);
   check_launch("calc_sf");
   dev::rmt_batch(mixed(), S.as<void>(), tb.d_k64.as<double4>(),
                  tb.d_group.as<uint32_t>(), tb.d_r_in.as<uint32_t>(),

@@ -55,6 +55,7 @@ struct glst_engine {
   uint32_t K = 0, Kp = 0;
   bool has_lr = false;
   int spherical_rule = 0;
+  float far_d = 3.0e38f;  // mixed: far-atom offset (far_offset)
   int device = 0;
   size_t lds_limit = 65536;
 
@@ -167,6 +168,7 @@ struct glst_engine {
     spherical_rule = su.spherical_rule;
     K = su.K;
     Kp = su.Kp;
+    far_d = far_offset(nodes);
 
     GLST_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
     GLST_HIP(hipEventCreateWithFlags(&ev_assigned, hipEventDisableTiming));
@@ -301,8 +303,8 @@ struct glst_engine {
                        atom_cell.as<uint32_t>(), rank.as<uint32_t>(),
                        hist.as<uint32_t>(), cell_start.as<uint32_t>(), grid,
                        sorted_idx.as<uint32_t>(), cell_sorted.as<uint32_t>(),
-                       atoms64.as<double4>(), atoms32.as<float4>(), stream,
-                       ag_start.as<uint32_t>(), ag);
+                       atoms64.as<double4>(), atoms32.as<float4>(), far_d,
+                       stream, ag_start.as<uint32_t>(), ag);
       check_launch("sort");
     }
     assigned = true;
@@ -313,8 +315,9 @@ struct glst_engine {
     if (!has_lr || Kp == 0) return;  // src/glst_force.cu:667-670
     if (!assigned) throw StateError("calc_sf called before assign_atoms");
     Scope sc(timer, "sf", stream);
-    dev::sf(mixed(), lr_atoms(), cell_start.as<uint32_t>(), k64.as<double4>(),
-            task_args(), sf_groups, Kp, grid, S.as<void>(), stream);
+    dev::sf(mixed(), lr_atoms(), cell_start.as<uint32_t>(),
+            cell_sorted.as<uint32_t>(), k64.as<double4>(), task_args(),
+            sf_groups, Kp, grid, S.as<void>(), stream);
     check_launch("sf");
     sf_done = true;
     rmt_done = false;
@@ -423,8 +426,9 @@ struct glst_engine {
       Scope sc(timer, "sf", stream);
       const dev::LrTaskArgs ta = task_args();
       for (size_t g = 0; g < sf_groups.size(); ++g) {
-        dev::sf(true, lr_atoms(), cell_start.as<uint32_t>(), k64.as<double4>(),
-                ta, &sf_groups[g], 1, Kp, grid, S.as<void>(), stream);
+        dev::sf(true, lr_atoms(), cell_start.as<uint32_t>(),
+                cell_sorted.as<uint32_t>(), k64.as<double4>(), ta,
+                &sf_groups[g], 1, Kp, grid, S.as<void>(), stream);
         GLST_HIP(hipEventRecord(ev_sf[g], stream));
       }
       check_launch("sf");

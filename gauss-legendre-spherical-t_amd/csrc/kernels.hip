@@ -94,6 +94,15 @@ __device__ inline float fract_rev(float t) { return t; }
 #else
 __device__ inline float fract_rev(float t) { return __builtin_amdgcn_fractf(t); }
 #endif
+// Phases of atoms marked kFarAtom (offsets far outside their cell) are
+// reduced to [0, 1) revolutions whatever GLST_NO_FRACT says.
+template <bool FAR>
+__device__ inline float reduce_rev(float t) {
+  if constexpr (FAR)
+    return __builtin_amdgcn_fractf(t);
+  else
+    return fract_rev(t);
+}
 
 // ---------------------------------------------------------------------------
 // fills
@@ -302,7 +311,7 @@ __device__ __forceinline__ void bin_scatter_body(
     const uint32_t *__restrict__ start, uint32_t ncell, uint32_t ny,
     uint32_t nz, double h, uint32_t *__restrict__ sorted_idx,
     uint32_t *__restrict__ cell_sorted, double4 *__restrict__ atoms64,
-    float4 *__restrict__ atoms32, uint32_t i, uint32_t atom0,
+    float4 *__restrict__ atoms32, float far_d, uint32_t i, uint32_t atom0,
     const uint32_t *__restrict__ ag_start = nullptr,
     uint2 *__restrict__ ag = nullptr) {
   if (i >= natom) return;
@@ -311,7 +320,6 @@ __device__ __forceinline__ void bin_scatter_body(
   const uint32_t r = hist[static_cast<size_t>(ch) * ncell + c] + rank[i];
   const uint32_t dst = start[c] + r;
   sorted_idx[dst] = atom0 + i;
-  cell_sorted[dst] = c;
   if (ag && r % kAgAtoms == 0) {  // the first atom of its group
     const uint32_t n = start[c + 1] - start[c] - r;
     ag[ag_start[c] + r / kAgAtoms] =
@@ -319,6 +327,7 @@ __device__ __forceinline__ void bin_scatter_body(
   }
   const double x = rx[i], y = ry[i], z = rz[i], q = qc[i];
   atoms64[dst] = make_double4(x, y, z, q);
+  uint32_t far = 0u;
   if (MIXED) {
     // Coordinates relative to the cell centre, rounded once from FP64: the
     // large part of every phase k.r is carried by the FP64 cell centre.
@@ -326,9 +335,15 @@ __device__ __forceinline__ void bin_scatter_body(
     const double cx = (static_cast<double>(ix) + 0.5) * h;
     const double cy = (static_cast<double>(iy) + 0.5) * h;
     const double cz = (static_cast<double>(iz) + 0.5) * h;
-    atoms32[dst] = make_float4(static_cast<float>(x - cx), static_cast<float>(y - cy),
-                               static_cast<float>(z - cz), static_cast<float>(q));
+    const double dx = x - cx, dy = y - cy, dz = z - cz;
+    atoms32[dst] = make_float4(static_cast<float>(dx), static_cast<float>(dy),
+                               static_cast<float>(dz), static_cast<float>(q));
+    // (a NaN offset is not marked: its phases are NaN either way)
+    far = fabs(dx) + fabs(dy) + fabs(dz) > static_cast<double>(far_d)
+              ? kFarAtom
+              : 0u;
   }
+  cell_sorted[dst] = c | far;
 }
 
 template <bool MIXED>
@@ -340,12 +355,13 @@ __global__ void bin_scatter_kernel(
     const uint32_t *__restrict__ start, uint32_t ncell, uint32_t ny,
     uint32_t nz, double h, uint32_t *__restrict__ sorted_idx,
     uint32_t *__restrict__ cell_sorted, double4 *__restrict__ atoms64,
-    float4 *__restrict__ atoms32, const uint32_t *__restrict__ ag_start,
-    uint2 *__restrict__ ag) {
+    float4 *__restrict__ atoms32, float far_d,
+    const uint32_t *__restrict__ ag_start, uint2 *__restrict__ ag) {
   bin_scatter_body<MIXED>(rx, ry, rz, qc, natom, atom_cell, rank, hist, start,
                           ncell, ny, nz, h, sorted_idx, cell_sorted, atoms64,
-                          atoms32, blockIdx.x * blockDim.x + threadIdx.x, 0u,
-                          ag_start, ag);
+                          atoms32, far_d,
+                          blockIdx.x * blockDim.x + threadIdx.x, 0u, ag_start,
+                          ag);
 }
 
 // ---------------------------------------------------------------------------
@@ -1206,7 +1222,7 @@ __device__ __forceinline__ void lr_f64_body(
   const bool valid = i < natom;
   const uint32_t ii = valid ? i : natom - 1;
   const double4 a = atoms[ii];
-  const double2 *Rc = R + static_cast<size_t>(cell_sorted[ii]) * Kp;
+  const double2 *Rc = R + static_cast<size_t>(cell_sorted[ii] & ~kFarAtom) * Kp;
   double fx = 0.0, fy = 0.0, fz = 0.0, en = 0.0;
   const uint32_t t1 = tk.split[split + 1];
   for (uint32_t t = tk.split[split]; t < t1; ++t) {
@@ -1250,7 +1266,7 @@ __global__ __launch_bounds__(kLrBlock) void lr_f64_kernel(
 // t = kappa_i p + phi (one FMA), sin, cos, b += A sin t, e_i += A cos t.
 // The energy sums e_i are kept per radial node and divided by kappa_i when
 // they are flushed (en = sum_i |V_i| cos t_i).
-template <int NR>
+template <int NR, bool FAR>
 __device__ __forceinline__ void lr_mixed_dirs(
     const float4 &a, const float2 *__restrict__ Rc, const LrTasks &tk,
     uint32_t tb, uint32_t te, double &FX, double &FY, double &FZ,
@@ -1288,7 +1304,7 @@ __device__ __forceinline__ void lr_mixed_dirs(
       float b0 = 0.f, b1 = 0.f;
 #pragma unroll
       for (int i = 0; i < NR; ++i) {
-        const float tt = fmaf(kap[i], p, r[i].y);
+        const float tt = reduce_rev<FAR>(fmaf(kap[i], p, r[i].y));
         const float sn = sin_rev(tt), cs = cos_rev(tt);
         e[i] = fmaf(r[i].x, cs, e[i]);
         if (i & 1)
@@ -1314,6 +1330,7 @@ __device__ __forceinline__ void lr_mixed_dirs(
 }
 
 // Any radial count (larger than the unrolled cases).
+template <bool FAR>
 __device__ __forceinline__ void lr_mixed_dirs_any(
     const float4 &a, const float2 *__restrict__ Rc, const LrTasks &tk,
     uint32_t tb, uint32_t te, double &FX, double &FY, double &FZ,
@@ -1329,7 +1346,7 @@ __device__ __forceinline__ void lr_mixed_dirs_any(
     for (uint32_t j = 0; j < nr; ++j) {
       const float kj = kap[j];
       const float2 r = Rc[n0 + j];
-      const float tt = fmaf(kj, p, r.y);
+      const float tt = reduce_rev<FAR>(fmaf(kj, p, r.y));
       const float sn = sin_rev(tt), cs = cos_rev(tt);
       e = fmaf(r.x * __builtin_amdgcn_rcpf(kj), cs, e);
       b = fmaf(r.x, sn, b);
@@ -1365,14 +1382,24 @@ __device__ __forceinline__ void lr_mixed_nr_body(
   const bool valid = i < natom;
   const uint32_t ii = valid ? i : natom - 1;
   const float4 a = atoms[ii];
-  const float2 *Rc = R + static_cast<size_t>(cell_sorted[ii]) * Kp;
+  const uint32_t cs = cell_sorted[ii];
+  const float2 *Rc = R + static_cast<size_t>(cs & ~kFarAtom) * Kp;
   double FX = 0.0, FY = 0.0, FZ = 0.0, EN = 0.0;
   const uint32_t tb = tk.split[split], te = tk.split[split + 1];
+  // uniform: a wave with a far atom reduces all its phases
+  const bool far = __ballot(valid && (cs & kFarAtom)) != 0;
   if (tb < te) {
-    if constexpr (NR > 0)
-      lr_mixed_dirs<NR>(a, Rc, tk, tb, te, FX, FY, FZ, EN);
-    else
-      lr_mixed_dirs_any(a, Rc, tk, tb, te, FX, FY, FZ, EN);
+    if constexpr (NR > 0) {
+      if (far)
+        lr_mixed_dirs<NR, true>(a, Rc, tk, tb, te, FX, FY, FZ, EN);
+      else
+        lr_mixed_dirs<NR, false>(a, Rc, tk, tb, te, FX, FY, FZ, EN);
+    } else {
+      if (far)
+        lr_mixed_dirs_any<true>(a, Rc, tk, tb, te, FX, FY, FZ, EN);
+      else
+        lr_mixed_dirs_any<false>(a, Rc, tk, tb, te, FX, FY, FZ, EN);
+    }
   }
   if (valid) {
     const double qf = static_cast<double>(a.w) * kTwoPi;  // k = 2 pi k_rev
@@ -1473,15 +1500,21 @@ __device__ __forceinline__ void lr_ag_body(
           make_double4(0.0, 0.0, 0.0, 0.0);
     return;
   }
-  const float2 *Rc = R + static_cast<size_t>(cell_sorted[first]) * Kp;
+  const float2 *Rc =
+      R + static_cast<size_t>(cell_sorted[first] & ~kFarAtom) * Kp;
   float ax[kAgAtoms], ay[kAgAtoms], az[kAgAtoms];
+  uint32_t farg = 0u;
 #pragma unroll
   for (uint32_t u = 0; u < kAgAtoms; ++u) {
-    const float4 a = atoms[first + (u < cnt ? u : 0u)];
+    const uint32_t j = first + (u < cnt ? u : 0u);
+    const float4 a = atoms[j];
     ax[u] = a.x;
     ay[u] = a.y;
     az[u] = a.z;
+    farg |= cnt ? cell_sorted[j] & kFarAtom : 0u;
   }
+  // uniform: a wave with a far atom reduces all its phases
+  const bool far = __ballot(farg != 0u) != 0;
   float kap[NR], ikap[NR];
   const float *kp = tk.kappa32 + tk.koff[tb];
 #pragma unroll
@@ -1502,8 +1535,9 @@ __device__ __forceinline__ void lr_ag_body(
   }
   // One pass: lane direction t, all atoms of the group.  FULL: every lane
   // has a direction (the last pass of a split may leave lanes idle).
-  auto pass = [&](uint32_t t0, auto full_tag) {
+  auto pass = [&](uint32_t t0, auto full_tag, auto far_tag) {
     constexpr bool FULL = decltype(full_tag)::value;
+    constexpr bool FAR = decltype(far_tag)::value;
     const uint32_t t = t0 + lane;
     const bool on = FULL || t < te;
     const uint32_t tt = on ? t : tb;
@@ -1523,7 +1557,7 @@ __device__ __forceinline__ void lr_ag_body(
       float e = 0.f, b = 0.f;
 #pragma unroll
       for (int i = 0; i < NR; ++i) {
-        const float th = fmaf(kap[i], p, ph[i]);
+        const float th = reduce_rev<FAR>(fmaf(kap[i], p, ph[i]));
         e = fmaf(V[i], cos_rev(th), e);
         b = fmaf(A[i], sin_rev(th), b);
       }
@@ -1534,8 +1568,15 @@ __device__ __forceinline__ void lr_ag_body(
     }
   };
   uint32_t t0 = tb;
-  for (; t0 + kAgLanes <= te; t0 += kAgLanes) pass(t0, std::true_type{});
-  if (t0 < te) pass(t0, std::false_type{});
+  if (far) {
+    for (; t0 + kAgLanes <= te; t0 += kAgLanes)
+      pass(t0, std::true_type{}, std::true_type{});
+    if (t0 < te) pass(t0, std::false_type{}, std::true_type{});
+  } else {
+    for (; t0 + kAgLanes <= te; t0 += kAgLanes)
+      pass(t0, std::true_type{}, std::false_type{});
+    if (t0 < te) pass(t0, std::false_type{}, std::false_type{});
+  }
   const float FX = ag_reduce_scatter(fx, lane);
   const float FY = ag_reduce_scatter(fy, lane);
   const float FZ = ag_reduce_scatter(fz, lane);
@@ -1608,9 +1649,13 @@ constexpr uint32_t kSfChunk = 128;
 template <int NR>
 __device__ __forceinline__ void sf_mixed_nr_body(
     const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
-    const LrTasks &tk, uint32_t task0, uint32_t ntask, uint32_t ncell,
-    float2 *__restrict__ S, uint32_t cell0, uint32_t chunk) {
+    const uint32_t *__restrict__ cell_sorted, const LrTasks &tk,
+    uint32_t task0, uint32_t ntask, uint32_t ncell, float2 *__restrict__ S,
+    uint32_t cell0, uint32_t chunk) {
   constexpr uint32_t CELLS = sf_cells(NR);
+  // rounds: the cell's sums stay in registers and kSfRoundNodes of them per
+  // lane pass through LDS at a time
+  constexpr bool ROUNDS = GLST_SF_STAGE == 2 && CELLS > 1;
   constexpr uint32_t kStride = kSfDirs * NR + 1;  // float2 per staged cell
   extern __shared__ float2 sf_stage[];             // [CELLS][kStride]
   const uint32_t w = threadIdx.x / kSfDirs, lane = threadIdx.x % kSfDirs;
@@ -1626,15 +1671,21 @@ __device__ __forceinline__ void sf_mixed_nr_body(
   const float4 d = tk.dir32[tt];
   // one cell per workgroup (staging off): the lane stores its own slots
   const uint32_t n0 = tk.node0[tt];
-  float2 *st = CELLS == 1 ? nullptr : sf_stage + w * kStride + lane * NR;
+  float2 *st = (CELLS == 1 || ROUNDS) ? nullptr
+                                      : sf_stage + w * kStride + lane * NR;
+  float2 sum[ROUNDS ? NR : 1];
   auto put = [&](int i, float2 v) {
-    if constexpr (CELLS == 1)
+    if constexpr (ROUNDS)
+      sum[i] = v;
+    else if constexpr (CELLS == 1)
       S[sf_slot(n0 + i, cell, ncell)] = v;
     else
       st[i] = v;
   };
   auto get = [&](int i) {
-    if constexpr (CELLS == 1)
+    if constexpr (ROUNDS)
+      return sum[i];
+    else if constexpr (CELLS == 1)
       return S[sf_slot(n0 + i, cell, ncell)];
     else
       return st[i];
@@ -1653,12 +1704,19 @@ __device__ __forceinline__ void sf_mixed_nr_body(
       for (uint32_t j = jb; j < je; ++j) {
         const float4 a = atoms[j];
         const float p = d.x * a.x + d.y * a.y + d.z * a.z;
+        auto acc = [&](auto far_tag) {
+          constexpr bool FAR = decltype(far_tag)::value;
 #pragma unroll
-        for (int i = 0; i < NR; ++i) {
-          const float ph = fract_rev(kap[i] * p);
-          re[i] = fmaf(a.w, cos_rev(ph), re[i]);
-          im[i] = fmaf(-a.w, sin_rev(ph), im[i]);
-        }
+          for (int i = 0; i < NR; ++i) {
+            const float ph = reduce_rev<FAR>(kap[i] * p);
+            re[i] = fmaf(a.w, cos_rev(ph), re[i]);
+            im[i] = fmaf(-a.w, sin_rev(ph), im[i]);
+          }
+        };
+        if (cell_sorted[j] & kFarAtom)  // uniform: the atom is
+          acc(std::true_type{});
+        else
+          acc(std::false_type{});
       }
       if (jb == a0) {
 #pragma unroll
@@ -1673,14 +1731,40 @@ __device__ __forceinline__ void sf_mixed_nr_body(
     }
   }
   if constexpr (CELLS == 1) return;
-  __syncthreads();
-  // nodes [nb, ne) of the chunk; pairs q0 .. q1-1 touch them (a pair shared
-  // with the neighbouring chunk keeps the other node's slot untouched)
   const uint32_t nact = (ntask - chunk * kSfDirs < kSfDirs)
                             ? ntask - chunk * kSfDirs
                             : kSfDirs;
   const uint32_t nb = tk.node0[tfirst], ne = nb + nact * NR;
   const uint32_t ncl = (ncell - cell0 < CELLS) ? ncell - cell0 : CELLS;
+  if constexpr (ROUNDS) {
+    // round j0: nodes j0 .. j0+R-1 of every lane; thread f of the store
+    // takes (lane m, node j0+il, cell wc) with the cells fastest, so 16
+    // consecutive threads fill the 128-byte line of a node pair
+    constexpr uint32_t R = kSfRoundNodes;
+    constexpr uint32_t kRStride = kSfDirs * R + 1;
+#pragma unroll
+    for (uint32_t j0 = 0; j0 < static_cast<uint32_t>(NR); j0 += R) {
+#pragma unroll
+      for (uint32_t il = 0; il < R; ++il)
+        if (j0 + il < static_cast<uint32_t>(NR))
+          sf_stage[w * kRStride + lane * R + il] = sum[j0 + il];
+      __syncthreads();
+      for (uint32_t f = threadIdx.x; f < kSfDirs * R * CELLS;
+           f += CELLS * kSfDirs) {
+        const uint32_t wc = f % CELLS, r = f / CELLS;
+        const uint32_t il = r % R, m = r / R, i = j0 + il;
+        if (i >= static_cast<uint32_t>(NR) || m >= nact || wc >= ncl)
+          continue;
+        S[sf_slot(nb + m * NR + i, cell0 + wc, ncell)] =
+            sf_stage[wc * kRStride + m * R + il];
+      }
+      __syncthreads();
+    }
+    return;
+  }
+  __syncthreads();
+  // nodes [nb, ne) of the chunk; pairs q0 .. q1-1 touch them (a pair shared
+  // with the neighbouring chunk keeps the other node's slot untouched)
   const uint32_t q0 = nb >> 1, q1 = (ne + 1) >> 1;
   const uint32_t total = (q1 - q0) * CELLS * 2;
   for (uint32_t f = threadIdx.x; f < total; f += CELLS * kSfDirs) {
@@ -1695,8 +1779,9 @@ __device__ __forceinline__ void sf_mixed_nr_body(
 // Any radial count (beyond the unrolled cases).
 __device__ __forceinline__ void sf_mixed_any_body(
     const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
-    const LrTasks &tk, uint32_t task0, uint32_t ntask, uint32_t ncell,
-    float2 *__restrict__ S, uint32_t cell, uint32_t chunk) {
+    const uint32_t *__restrict__ cell_sorted, const LrTasks &tk,
+    uint32_t task0, uint32_t ntask, uint32_t ncell, float2 *__restrict__ S,
+    uint32_t cell, uint32_t chunk) {
   const uint32_t t = task0 + chunk * kSfDirs + threadIdx.x;
   const bool active = t < task0 + ntask;
   const uint32_t tt = active ? t : task0;
@@ -1713,8 +1798,9 @@ __device__ __forceinline__ void sf_mixed_any_body(
         float re = 0.f, im = 0.f;
         for (uint32_t j = jb; j < je; ++j) {
           const float4 a = atoms[j];
-          const float ph =
-              fract_rev(kap[i] * (d.x * a.x + d.y * a.y + d.z * a.z));
+          const float t = kap[i] * (d.x * a.x + d.y * a.y + d.z * a.z);
+          const float ph = (cell_sorted[j] & kFarAtom) ? reduce_rev<true>(t)
+                                                       : reduce_rev<false>(t);
           re = fmaf(a.w, cos_rev(ph), re);
           im = fmaf(-a.w, sin_rev(ph), im);
         }
@@ -1730,25 +1816,34 @@ __device__ __forceinline__ void sf_mixed_any_body(
 
 // (cell, direction chunk) of a workgroup, numbered so that each XCD runs
 // consecutive cells (their node-pair slots share cache lines).
+// The workgroups of several cells get the register budget of four waves per
+// SIMD (what the one-cell kernel compiles to); left to itself the compiler
+// aims at eight and halves the registers of the unrolled radial loop.
+#ifndef GLST_SF_WAVES
+#define GLST_SF_WAVES 4
+#endif
 template <int NR>
-__global__ __launch_bounds__(kSfDirs * sf_cells(NR)) void sf_mixed_nr_kernel(
+__global__ __launch_bounds__(kSfDirs * sf_cells(NR))
+__attribute__((amdgpu_waves_per_eu(1, sf_cells(NR) > 1 ? GLST_SF_WAVES : 10)))
+void sf_mixed_nr_kernel(
     const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
-    LrTasks tk, uint32_t task0, uint32_t ntask, uint32_t ncell,
-    float2 *__restrict__ S) {
+    const uint32_t *__restrict__ cell_sorted, LrTasks tk, uint32_t task0,
+    uint32_t ntask, uint32_t ncell, float2 *__restrict__ S) {
   // gridDim.x = cell groups of sf_cells(NR) cells
   const uint32_t wg = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x,
                                 gridDim.x * gridDim.y);
   const uint32_t ncg = gridDim.x;
   if constexpr (NR > 0)
-    sf_mixed_nr_body<NR>(atoms, start, tk, task0, ntask, ncell, S,
-                         (wg % ncg) * sf_cells(NR), wg / ncg);
+    sf_mixed_nr_body<NR>(atoms, start, cell_sorted, tk, task0, ntask, ncell,
+                         S, (wg % ncg) * sf_cells(NR), wg / ncg);
   else
-    sf_mixed_any_body(atoms, start, tk, task0, ntask, ncell, S, wg % ncg,
-                      wg / ncg);
+    sf_mixed_any_body(atoms, start, cell_sorted, tk, task0, ntask, ncell, S,
+                      wg % ncg, wg / ncg);
 }
 
-typedef void (*SfNrKernel)(const float4 *, const uint32_t *, LrTasks,
-                           uint32_t, uint32_t, uint32_t, float2 *);
+typedef void (*SfNrKernel)(const float4 *, const uint32_t *,
+                           const uint32_t *, LrTasks, uint32_t, uint32_t,
+                           uint32_t, float2 *);
 
 // index NR (1..kSfMaxUnrolled) -> unrolled kernel; 0 -> any radial count
 template <int... N>
@@ -2173,7 +2268,7 @@ __global__ void bin_scatter_batch_kernel(
       rx + d.atom0, ry + d.atom0, rz + d.atom0, qc + d.atom0, d.natom,
       atom_cell + d.atom0, rank + d.atom0, hist + d.hist0, start + d.cell0,
       d.ncell, d.ny, d.nz, d.h, sorted_idx, cell_sorted, atoms64, atoms32,
-      (blockIdx.x - g.blk0) * blockDim.x + threadIdx.x, d.atom0,
+      d.far_d, (blockIdx.x - g.blk0) * blockDim.x + threadIdx.x, d.atom0,
       groups ? ag_start + d.cell0 : nullptr, groups ? ag + d.ag0 : nullptr);
 }
 
@@ -2190,9 +2285,12 @@ __global__ __launch_bounds__(kSfBlock) void sf_f64_batch_kernel(
 }
 
 template <int NR>
-__global__ __launch_bounds__(kSfDirs * sf_cells(NR)) void sf_mixed_batch_kernel(
+__global__ __launch_bounds__(kSfDirs * sf_cells(NR))
+__attribute__((amdgpu_waves_per_eu(1, sf_cells(NR) > 1 ? GLST_SF_WAVES : 10)))
+void sf_mixed_batch_kernel(
     const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
-    LrTasks tk, const SysDev *__restrict__ sys, const Seg *__restrict__ seg,
+    const uint32_t *__restrict__ cell_sorted, LrTasks tk,
+    const SysDev *__restrict__ sys, const Seg *__restrict__ seg,
     uint32_t nseg, float2 *__restrict__ S) {
   const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
   const Seg g = seg[run_of(seg, nseg, wg)];
@@ -2201,15 +2299,17 @@ __global__ __launch_bounds__(kSfDirs * sf_cells(NR)) void sf_mixed_batch_kernel(
   constexpr uint32_t CELLS = sf_cells(NR);
   const uint32_t ncg = (d.ncell + CELLS - 1) / CELLS;
   if constexpr (NR > 0)
-    sf_mixed_nr_body<NR>(atoms, start + d.cell0, tk, g.a, g.b, d.ncell,
-                         S + d.field0, (local % ncg) * CELLS, local / ncg);
+    sf_mixed_nr_body<NR>(atoms, start + d.cell0, cell_sorted, tk, g.a, g.b,
+                         d.ncell, S + d.field0, (local % ncg) * CELLS,
+                         local / ncg);
   else
-    sf_mixed_any_body(atoms, start + d.cell0, tk, g.a, g.b, d.ncell,
-                      S + d.field0, local % ncg, local / ncg);
+    sf_mixed_any_body(atoms, start + d.cell0, cell_sorted, tk, g.a, g.b,
+                      d.ncell, S + d.field0, local % ncg, local / ncg);
 }
 
-typedef void (*SfBatchKernel)(const float4 *, const uint32_t *, LrTasks,
-                              const SysDev *, const Seg *, uint32_t, float2 *);
+typedef void (*SfBatchKernel)(const float4 *, const uint32_t *,
+                              const uint32_t *, LrTasks, const SysDev *,
+                              const Seg *, uint32_t, float2 *);
 
 template <int... N>
 constexpr auto sf_batch_table(std::integer_sequence<int, N...>) {
@@ -2452,22 +2552,25 @@ void bin_scatter(bool mixed, const double *rx, const double *ry,
                  const uint32_t *hist, const uint32_t *start,
                  const GridShape &g, uint32_t *sorted_idx,
                  uint32_t *cell_sorted, double4 *atoms64, float4 *atoms32,
-                 hipStream_t s, const uint32_t *ag_start, uint2 *ag) {
+                 float far_d, hipStream_t s, const uint32_t *ag_start,
+                 uint2 *ag) {
   if (natom == 0) return;
   if (mixed)
     bin_scatter_kernel<true><<<blocks_for(natom, 256), 256, 0, s>>>(
         rx, ry, rz, qc, natom, atom_cell, rank, hist, start, g.ncell, g.ny,
-        g.nz, g.h, sorted_idx, cell_sorted, atoms64, atoms32, ag_start, ag);
+        g.nz, g.h, sorted_idx, cell_sorted, atoms64, atoms32, far_d, ag_start,
+        ag);
   else
     bin_scatter_kernel<false><<<blocks_for(natom, 256), 256, 0, s>>>(
         rx, ry, rz, qc, natom, atom_cell, rank, hist, start, g.ncell, g.ny,
-        g.nz, g.h, sorted_idx, cell_sorted, atoms64, atoms32, ag_start, ag);
+        g.nz, g.h, sorted_idx, cell_sorted, atoms64, atoms32, far_d, ag_start,
+        ag);
 }
 
 void sf(bool mixed, const void *atoms, const uint32_t *start,
-        const double4 *k64, const LrTaskArgs &ta, const SfGroup *groups,
-        size_t ngroup, uint32_t Kp, const GridShape &g, void *S,
-        hipStream_t s) {
+        const uint32_t *cell_sorted, const double4 *k64,
+        const LrTaskArgs &ta, const SfGroup *groups, size_t ngroup,
+        uint32_t Kp, const GridShape &g, void *S, hipStream_t s) {
   if (!mixed) {
     const dim3 grid(g.nx * g.ny, Kp / kSfBlock);
     sf_f64_kernel<<<grid, kSfBlock, 0, s>>>(static_cast<const double4 *>(atoms),
@@ -2487,8 +2590,8 @@ void sf(bool mixed, const void *atoms, const uint32_t *start,
     const dim3 grid((g.ncell + cells - 1) / cells,
                     (gr.ntask + kSfDirs - 1) / kSfDirs);
     kSfNrKernels[nr]<<<grid, kSfDirs * cells, sf_lds_bytes(nr), s>>>(
-        static_cast<const float4 *>(atoms), start, tk, gr.task0, gr.ntask,
-        g.ncell, static_cast<float2 *>(S));
+        static_cast<const float4 *>(atoms), start, cell_sorted, tk, gr.task0,
+        gr.ntask, g.ncell, static_cast<float2 *>(S));
   }
 }
 
@@ -2747,7 +2850,8 @@ void bin_scatter_batch(bool mixed, const double *rx, const double *ry,
 }
 
 void sf_batch(bool mixed, const void *atoms, const uint32_t *start,
-              const double4 *k64, const LrTaskArgs &ta, const SysDev *sys,
+              const uint32_t *cell_sorted, const double4 *k64,
+              const LrTaskArgs &ta, const SysDev *sys,
               const std::vector<BatchLaunch> &launches, void *S,
               hipStream_t s) {
   const LrTasks tk{ta.node0, ta.nr,      ta.koff,   ta.dir64,
@@ -2764,8 +2868,9 @@ void sf_batch(bool mixed, const void *atoms, const uint32_t *start,
                        ? static_cast<int>(L.nr)
                        : 0;
     kSfBatchKernels[nr]<<<L.nblk, kSfDirs * sf_cells(nr), sf_lds_bytes(nr),
-                          s>>>(static_cast<const float4 *>(atoms), start, tk,
-                               sys, L.seg, L.nseg, static_cast<float2 *>(S));
+                          s>>>(static_cast<const float4 *>(atoms), start,
+                               cell_sorted, tk, sys, L.seg, L.nseg,
+                               static_cast<float2 *>(S));
   }
 }
 

@@ -47,14 +47,22 @@ void cell_scan(const uint32_t *count, uint32_t ncell, uint32_t *start,
 // cells and the sorted atom records: double4 absolute (x,y,z,q) always, and
 // for the mixed path float4 cell-centre-relative (x,y,z,q).  With ag: the
 // (first atom, count) entry of every atom group, from cell_scan's ag_start.
+// The mixed path's FP32 offsets from the cell centre carry phases k.d of at
+// most kappa_max |d|_1 revolutions; the hardware sine and cosine take
+// |t| <= 256 (measured accurate to 64).  An atom whose |d|_1 exceeds far_d
+// (63 / kappa_max, set by the caller; only atoms far outside the grid, which
+// the binning clamps into its edge cells) gets kFarAtom set in its
+// cell_sorted entry, and S(k) and the back-projection reduce its phases
+// with V_FRACT first.  Readers of cell_sorted mask the bit.
+constexpr uint32_t kFarAtom = 1u << 31;
 void bin_scatter(bool mixed, const double *rx, const double *ry,
                  const double *rz, const double *qc, uint32_t natom,
                  const uint32_t *atom_cell, const uint32_t *rank,
                  const uint32_t *hist, const uint32_t *start,
                  const GridShape &g, uint32_t *sorted_idx,
                  uint32_t *cell_sorted, double4 *atoms64, float4 *atoms32,
-                 hipStream_t s, const uint32_t *ag_start = nullptr,
-                 uint2 *ag = nullptr);
+                 float far_d, hipStream_t s,
+                 const uint32_t *ag_start = nullptr, uint2 *ag = nullptr);
 
 // ---- structure factors (calc_sf) -------------------------------------------
 // S(k) is stored in node-pair tiles: the slots of nodes 2m and 2m+1 of all
@@ -75,19 +83,26 @@ __host__ __device__ inline size_t sf_slot(uint32_t node, uint32_t cell,
 // (64 NR float2 per wave) from lowering occupancy further.
 constexpr int kSfMaxUnrolled = 40;
 #ifndef GLST_SF_STAGE
-#define GLST_SF_STAGE 1
+#define GLST_SF_STAGE 0
 #endif
+// GLST_SF_STAGE 2 stages kSfRoundNodes radial nodes of every lane at a time
+// (8 cells per workgroup, 16 KB of LDS whatever NR).
+constexpr uint32_t kSfRoundNodes = 4;
 __host__ __device__ constexpr uint32_t sf_cells(int nr) {
   return (!GLST_SF_STAGE || nr <= 0 || nr > kSfMaxUnrolled)
              ? 1u
-             : (nr <= 19 ? 8u : 4u);
+             : ((GLST_SF_STAGE == 2 || nr <= 19) ? 8u : 4u);
 }
-// LDS of one unrolled workgroup: per cell 64 NR float2, plus one to
-// stagger the cells over the banks
+// LDS of one unrolled workgroup: per cell 64 NR float2 (64 kSfRoundNodes
+// in rounds), plus one to stagger the cells over the banks
 __host__ __device__ constexpr uint32_t sf_lds_bytes(int nr) {
-  return sf_cells(nr) == 1u ? 0u
-                            : sf_cells(nr) * (64u * static_cast<uint32_t>(nr) +
-                                              1u) * 8u;
+  return sf_cells(nr) == 1u
+             ? 0u
+             : sf_cells(nr) *
+                   (64u * (GLST_SF_STAGE == 2 ? kSfRoundNodes
+                                              : static_cast<uint32_t>(nr)) +
+                    1u) *
+                   8u;
 }
 
 struct LrTaskArgs;
@@ -103,15 +118,15 @@ struct SfGroup {
 // The mixed path launches once per listed group (groups[0..ngroup)); the
 // FP64 path ignores the list and covers every node in one launch.
 void sf(bool mixed, const void *atoms, const uint32_t *start,
-        const double4 *k64, const LrTaskArgs &tasks, const SfGroup *groups,
-        size_t ngroup, uint32_t Kp, const GridShape &g, void *S,
-        hipStream_t s);
+        const uint32_t *cell_sorted, const double4 *k64,
+        const LrTaskArgs &tasks, const SfGroup *groups, size_t ngroup,
+        uint32_t Kp, const GridShape &g, void *S, hipStream_t s);
 inline void sf(bool mixed, const void *atoms, const uint32_t *start,
-               const double4 *k64, const LrTaskArgs &tasks,
-               const std::vector<SfGroup> &groups, uint32_t Kp,
-               const GridShape &g, void *S, hipStream_t s) {
-  sf(mixed, atoms, start, k64, tasks, groups.data(), groups.size(), Kp, g, S,
-     s);
+               const uint32_t *cell_sorted, const double4 *k64,
+               const LrTaskArgs &tasks, const std::vector<SfGroup> &groups,
+               uint32_t Kp, const GridShape &g, void *S, hipStream_t s) {
+  sf(mixed, atoms, start, cell_sorted, k64, tasks, groups.data(),
+     groups.size(), Kp, g, S, s);
 }
 
 // ---- remote (shell) sums (sum_rmt_sf) --------------------------------------
@@ -256,7 +271,7 @@ struct SysDev {
   uint32_t T;                         // shell-sum nodes per workgroup
   uint32_t ag0, agtile;               // mixed: atom groups [ag0, + agtile
                                       // * kAgGroupsPerBlock), agtile WGs
-  uint32_t pad;
+  float far_d;                        // mixed: far-atom offset (kFarAtom)
   uint64_t hist0;                     // histogram [nchunk][ncell]
   uint64_t field0;                    // S and R slots [field0, + ncell*Kp)
   uint64_t partial0;                  // partial rows [nsplit][natom]
@@ -300,7 +315,8 @@ void bin_scatter_batch(bool mixed, const double *rx, const double *ry,
                        uint2 *ag = nullptr);
 // mixed: one launch per radial count (L.nr); FP64: one launch (L.nr unused)
 void sf_batch(bool mixed, const void *atoms, const uint32_t *start,
-              const double4 *k64, const LrTaskArgs &tasks, const SysDev *sys,
+              const uint32_t *cell_sorted, const double4 *k64,
+              const LrTaskArgs &tasks, const SysDev *sys,
               const std::vector<BatchLaunch> &launches, void *S,
               hipStream_t s);
 void rmt_batch(bool mixed, const void *S, const double4 *k64,

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <map>
 #include <stdexcept>
@@ -281,6 +282,18 @@ inline SystemSetup setup_system(int ct, double bx, double by, double bz,
   S.K = static_cast<uint32_t>(S.nodes.kx.size());
   S.Kp = (S.K + 255u) / 256u * 256u;
   return S;
+}
+
+// Offset from the cell centre (|d|_1, Angstrom) beyond which the mixed
+// path's FP32 phases of a system's nodes could pass 63 revolutions (kFarAtom
+// in kernels.hpp): 63 / max_k |k| / 2 pi.  No nodes: never.
+inline float far_offset(const NodeTable &nodes) {
+  double kmax = 0.0;
+  for (const auto &kap : nodes.kappa)
+    for (double k : kap) kmax = std::max(kmax, std::fabs(k));
+  if (!(kmax > 0.0)) return 3.0e38f;
+  constexpr double kTwoPi = 6.283185307179586476925286766559;
+  return static_cast<float>(63.0 / (kmax / kTwoPi));
 }
 
 // Node and task tables in device order.  Inside each alpha group the nodes

@@ -469,3 +469,42 @@ def test_gpu_coulomb_split_sizes_match_oracle(cuda, n):
     ref = o.coulomb(*h)
     assert rel_err(got[:3], ref[:3]) < 1e-13
     assert rel_err(got[3:], ref[3:]) < 1e-13
+
+
+@pytest.mark.parametrize("ct", ["double", "float"])
+def test_atoms_far_outside_the_box(cuda, water1k, ct):
+    """Atoms hundreds of Angstrom beyond the box are binned into the edge
+    cells (src/glst_force.cu:372-377), so their offsets from the cell centre
+    are that large.  At 1,000 A the mixed path's FP32 phases k.d reach about
+    280 revolutions, past the input range of the hardware sine and cosine
+    (|t| <= 256), and must be reduced first.  Each far atom's own force and
+    energy, and all forces together, must match the oracle (its FP64
+    restatement uses absolute coordinates, where nothing leaves range).  The
+    FP32 offset of an atom 1,000 A out carries 6e-5 A of rounding, about 1e-4
+    rad of phase, so the mixed path's bars are 1e-5 over all forces (the
+    metric's) and 1e-3 per far atom, far below what unreduced phases give."""
+    import torch
+    import glst_amd as g
+    from oracle import oracle as o
+    rx, ry, rz, qc = [a.copy() for a in water1k]
+    far = [5, 1001, 2002]
+    rx[far[0]] += 1000.0
+    ry[far[1]] += 700.0
+    rz[far[2]] += 450.0
+    rx[far[2]] += 450.0
+    f = g.GLSTForce(rx.size, 1e-5, 31.04, rcut=9.0, ct=ct)
+    f.calc_ener_force(*_dev([rx, ry, rz, qc], torch))
+    got = _host(f.get_ef())
+    f.close()
+    ref = o.glst(rx, ry, rz, qc, 1e-5, 31.04, 9.0)
+    bar, atom_bar = (1e-10, 1e-9) if ct == "double" else (1e-5, 1e-3)
+    assert rel_err(got[:3], ref[:3]) < bar
+    assert rel_err(got[3:], ref[3:]) < bar
+    frms = np.sqrt(np.mean(sum(r ** 2 for r in ref[:3])))
+    erms = np.sqrt(np.mean(ref[3] ** 2))
+    for i in far:
+        fi = np.sqrt(sum(r[i] ** 2 for r in ref[:3]))
+        df = np.sqrt(sum((a[i] - b[i]) ** 2 for a, b in zip(got[:3], ref[:3])))
+        assert df < atom_bar * max(fi, frms), (i, df / max(fi, frms))
+        assert abs(got[3][i] - ref[3][i]) < atom_bar * max(abs(ref[3][i]),
+                                                           erms), i
