@@ -557,15 +557,13 @@ __device__ inline void axis_phase(double2 *dst, double kd, double h,
 #define GLST_RMT_BLOCK 256
 #endif
 constexpr uint32_t kRmtBlock = GLST_RMT_BLOCK;
-// Values held in registers per thread while a line of the table is scanned,
-// and S slots loaded per thread before the first is stored.
-// z-pass elements loaded together (8: 175 VGPRs for the persistent kernel
-// against 204 with 16, and 2 % faster alone)
+// Values held in registers per thread while a line of the table is scanned
+// (8: 175 VGPRs for the persistent kernel against 204 with 16, and 2 %
+// faster alone)
 #ifndef GLST_RMT_ZBATCH
 #define GLST_RMT_ZBATCH 8
 #endif
 constexpr uint32_t kRmtBatch = GLST_RMT_ZBATCH;
-constexpr uint32_t kRmtLoadBatch = 4;
 // Threads of the persistent single-system workgroups.
 #ifndef GLST_RMT_PBLOCK
 #define GLST_RMT_PBLOCK 512
@@ -651,13 +649,13 @@ __device__ inline double2 cube_y(const double2 *P, const CubeXZ &c,
 }
 
 // Geometry of a workgroup's bordered tables (rmt_table_elems): T tables of
-// ts elements, then T * nax phase factors.  Lanes of the passes run over the
-// nodes first, then over a line coordinate padded to ypad / zpad (multiples
-// of rmt_line_group(T)), which with rmt_table_stride keeps every LDS cycle
-// free of bank conflicts.
+// ts elements, then the phase factors (T * nax per buffer).  Lanes of the
+// passes run over the nodes first, then over a line coordinate padded to
+// ypad / zpad (multiples of rmt_line_group(T)), which with rmt_table_stride
+// keeps every LDS cycle free of bank conflicts.
 struct RmtGeom {
   uint32_t T, lgT, nx, ny, nz, ncell, nax, zp, plane, ts, ypad, zpad;
-  float rnz, rny, rnc, rypad, rzpad;
+  float rny, rypad, rzpad;
   __device__ RmtGeom(uint32_t T_, uint32_t nx_, uint32_t ny_, uint32_t nz_)
       : T(T_), lgT(31u - __builtin_clz(T_)), nx(nx_), ny(ny_), nz(nz_),
         ncell(nx_ * ny_ * nz_), nax(nx_ + ny_ + nz_), zp(rmt_zpitch(nz_)),
@@ -666,100 +664,11 @@ struct RmtGeom {
     const uint32_t lg = rmt_line_group(T);
     ypad = (ny + lg - 1) / lg * lg;
     zpad = (nz + lg - 1) / lg * lg;
-    rnz = 1.0f / static_cast<float>(nz);
     rny = 1.0f / static_cast<float>(ny);
-    rnc = 1.0f / static_cast<float>(ncell);
     rypad = 1.0f / static_cast<float>(ypad);
     rzpad = 1.0f / static_cast<float>(zpad);
   }
-  // table index of cell c ((x ny + y) nz + z)
-  __device__ uint32_t cell_index(uint32_t c) const {
-    const uint32_t line = fast_div(c, nz, rnz), z = c - line * nz;
-    const uint32_t x = fast_div(line, ny, rny), y = line - x * ny;
-    return (x + 1) * plane + (y + 1) * zp + z + 1;
-  }
-  // S elements one tile reads: slot pairs (T >= 2) or slots (T = 1)
-  __device__ uint32_t load_count() const {
-    return T >= 2 ? (T >> 1) * ncell : ncell;
-  }
 };
-
-// One round of a tile's S slots held in registers, N per thread (the mixed
-// path keeps the FP32 pairs).
-template <bool MIXED, uint32_t N>
-struct RmtRound;
-template <uint32_t N>
-struct RmtRound<true, N> {
-  float4 v[N];
-};
-template <uint32_t N>
-struct RmtRound<false, N> {
-  double2 lo[N], hi[N];
-};
-
-// Reads round e0 (elements e0 + u BS, u < N; BS threads) of the S slots of nodes
-// node0 .. node0+T-1.  The pair (2m, 2m+1) of cell c sits at 2 (m ncell + c),
-// so the T/2 pairs of an even T are one contiguous run, read coalesced; T = 1
-// reads its node's half of each pair.
-template <uint32_t BS, bool MIXED, uint32_t N>
-__device__ __forceinline__ void rmt_fetch(const void *__restrict__ Sv,
-                                          const RmtGeom &g, uint32_t node0,
-                                          uint32_t e0, RmtRound<MIXED, N> &rd) {
-  const size_t run0 = static_cast<size_t>(node0 >> 1) * g.ncell;
-  const uint32_t total = g.load_count();
-#pragma unroll
-  for (uint32_t u = 0; u < N; ++u) {
-    const uint32_t eu = e0 + u * BS;
-    const uint32_t e = eu < total ? eu : total - 1;
-    if (g.T >= 2) {
-      if constexpr (MIXED) {
-        rd.v[u] = static_cast<const float4 *>(Sv)[run0 + e];
-      } else {
-        const double2 *p = static_cast<const double2 *>(Sv) + 2 * (run0 + e);
-        rd.lo[u] = p[0];
-        rd.hi[u] = p[1];
-      }
-    } else {
-      const size_t slot = 2 * (run0 + e) + (node0 & 1u);
-      if constexpr (MIXED) {
-        const float2 f = static_cast<const float2 *>(Sv)[slot];
-        rd.v[u] = make_float4(f.x, f.y, 0.f, 0.f);
-      } else {
-        rd.lo[u] = static_cast<const double2 *>(Sv)[slot];
-      }
-    }
-  }
-}
-
-// Stores a fetched round into the bordered tables, in FP64.
-template <uint32_t BS, bool MIXED, uint32_t N>
-__device__ __forceinline__ void rmt_put(double2 *P, const RmtGeom &g,
-                                        uint32_t e0,
-                                        const RmtRound<MIXED, N> &rd) {
-  const uint32_t total = g.load_count();
-#pragma unroll
-  for (uint32_t u = 0; u < N; ++u) {
-    const uint32_t e = e0 + u * BS;
-    if (e < total) {
-      double2 lo, hi;
-      if constexpr (MIXED) {
-        lo = make_double2(rd.v[u].x, rd.v[u].y);
-        hi = make_double2(rd.v[u].z, rd.v[u].w);
-      } else {
-        lo = rd.lo[u];
-        hi = rd.hi[u];
-      }
-      if (g.T >= 2) {
-        const uint32_t m = fast_div(e, g.ncell, g.rnc), c = e - m * g.ncell;
-        const uint32_t i = g.cell_index(c);
-        P[(2 * m) * g.ts + i] = lo;
-        P[(2 * m + 1) * g.ts + i] = hi;
-      } else {
-        P[g.cell_index(e)] = lo;
-      }
-    }
-  }
-}
 
 // Axis factor e (< T nax) of nodes node0..: exp(+i k_d (i + 1/2) h).
 __device__ inline double2 rmt_phase(const double4 *__restrict__ k64,
@@ -905,55 +814,31 @@ __device__ __forceinline__ void rmt_compute(
   }
 }
 
-// Shell sums of one tile of T consecutive nodes by one workgroup (the
-// batched engine: every workgroup's tile may belong to another grid).
-template <bool MIXED>
-__device__ __forceinline__ void rmt_lds_body(
-    const void *__restrict__ Sv, const double4 *__restrict__ k64,
-    const uint32_t *__restrict__ group, const uint32_t *__restrict__ r_in,
-    const uint32_t *__restrict__ r_out, uint32_t Kp, uint32_t T,
-    uint32_t nx, uint32_t ny, uint32_t nz, double h, void *__restrict__ R,
-    uint32_t tile) {
-  extern __shared__ double2 P[];  // [T][ts], then [T][nx+ny+nz] phases
-  const RmtGeom g(T, nx, ny, nz);
-  double2 *phase = P + T * g.ts;
-  const uint32_t node0 = tile * T;
-  const uint32_t total = g.load_count();
-  for (uint32_t e0 = threadIdx.x; e0 < total;
-       e0 += kRmtBlock * kRmtLoadBatch) {
-    RmtRound<MIXED, kRmtLoadBatch> rd;
-    rmt_fetch<kRmtBlock>(Sv, g, node0, e0, rd);
-    rmt_put<kRmtBlock>(P, g, e0, rd);
-  }
-  if (MIXED)
-    for (uint32_t e = threadIdx.x; e < T * g.nax; e += kRmtBlock)
-      phase[e] = rmt_phase(k64, g, node0, h, e);
-  rmt_border<kRmtBlock>(P, g);
-  __syncthreads();
-  rmt_compute<kRmtBlock, MIXED, false>(P, phase, g, k64, group, r_in, r_out,
-                                       Kp, node0, R, [] {});
-}
-
-// z-lines loaded straight into the tables (the persistent kernel).  Line l
-// of a tile is node t = l & (T-1) and column q = l >> lg T (x = q / ny,
-// y = q - x ny); its slots sf_slot(node0 + t, q nz + z) are 2 apart.  The
-// first kRmtLine<MIXED> of them travel through registers (fetched while the
+// z-lines loaded straight into the tables.  Line l of a tile is node
+// t = l & (T-1) and column q = l >> lg T (x = q / ny, y = q - x ny); its
+// slots sf_slot(node0 + t, q nz + z) are 2 apart.  The first LEN of them
+// travel through registers (the persistent kernel fetches them while the
 // previous tile's shells are gathered), the rest of a longer line is read
 // when the line is stored.  Storing writes the running sums along z into
 // the node's table; the mixed path first turns the cell-local sums into
 // S_c = exp(-i k.c) local in FP64 (the FP64 slots hold z-running sums).
-template <bool MIXED>
+template <bool MIXED, uint32_t LEN>
 struct RmtLine;
-template <>
-struct RmtLine<true> {
-  static constexpr uint32_t kLen = 24;
-  float2 v[kLen];
+template <uint32_t LEN>
+struct RmtLine<true, LEN> {
+  static constexpr uint32_t kLen = LEN;
+  float2 v[LEN];
 };
-template <>
-struct RmtLine<false> {
-  static constexpr uint32_t kLen = 16;
-  double re[kLen], im[kLen];  // (a double2 array stays in scratch memory)
+template <uint32_t LEN>
+struct RmtLine<false, LEN> {
+  static constexpr uint32_t kLen = LEN;
+  double re[LEN], im[LEN];  // (a double2 array stays in scratch memory)
 };
+// line lengths: the persistent kernel's prefetch, the batched tiles' loads
+template <bool MIXED>
+using RmtLineP = RmtLine<MIXED, MIXED ? 24u : 16u>;
+template <bool MIXED>
+using RmtLineB = RmtLine<MIXED, 8u>;
 
 template <bool MIXED>
 __device__ __forceinline__ double2 rmt_slot(const void *__restrict__ Sv,
@@ -966,12 +851,12 @@ __device__ __forceinline__ double2 rmt_slot(const void *__restrict__ Sv,
   }
 }
 
-template <bool MIXED>
+template <bool MIXED, uint32_t LEN>
 __device__ __forceinline__ void rmt_line_fetch(const void *__restrict__ Sv,
                                                const RmtGeom &g,
                                                uint32_t node0, uint32_t l,
-                                               RmtLine<MIXED> &ln) {
-  constexpr uint32_t L = RmtLine<MIXED>::kLen;
+                                               RmtLine<MIXED, LEN> &ln) {
+  constexpr uint32_t L = LEN;
   const uint32_t t = l & (g.T - 1), q = l >> g.lgT;
   const size_t base = sf_slot(node0 + t, q * g.nz, g.ncell);
 #pragma unroll
@@ -987,14 +872,14 @@ __device__ __forceinline__ void rmt_line_fetch(const void *__restrict__ Sv,
   }
 }
 
-template <bool MIXED>
+template <bool MIXED, uint32_t LEN>
 __device__ __forceinline__ void rmt_line_put(double2 *P,
                                              const double2 *phase,
                                              const RmtGeom &g,
                                              const void *__restrict__ Sv,
                                              uint32_t node0, uint32_t l,
-                                             const RmtLine<MIXED> &ln) {
-  constexpr uint32_t L = RmtLine<MIXED>::kLen;
+                                             const RmtLine<MIXED, LEN> &ln) {
+  constexpr uint32_t L = LEN;
   const uint32_t t = l & (g.T - 1), q = l >> g.lgT;
   const uint32_t x = fast_div(q, g.ny, g.rny), y = q - x * g.ny;
   double2 *p = P + t * g.ts + (x + 1) * g.plane + (y + 1) * g.zp + 1;
@@ -1027,6 +912,35 @@ __device__ __forceinline__ void rmt_line_put(double2 *P,
       p[z < g.nz ? z : g.nz - 1] = make_double2(ln.re[z], ln.im[z]);
     for (uint32_t z = L; z < g.nz; ++z) p[z] = rmt_slot<false>(Sv, base + 2 * z);
   }
+}
+
+// Shell sums of one tile of T consecutive nodes by one workgroup (the
+// batched engine: every workgroup's tile may belong to another grid).
+template <bool MIXED>
+__device__ __forceinline__ void rmt_lds_body(
+    const void *__restrict__ Sv, const double4 *__restrict__ k64,
+    const uint32_t *__restrict__ group, const uint32_t *__restrict__ r_in,
+    const uint32_t *__restrict__ r_out, uint32_t Kp, uint32_t T,
+    uint32_t nx, uint32_t ny, uint32_t nz, double h, void *__restrict__ R,
+    uint32_t tile) {
+  extern __shared__ double2 P[];  // [T][ts], then [T][nx+ny+nz] phases
+  const RmtGeom g(T, nx, ny, nz);
+  double2 *phase = P + T * g.ts;
+  const uint32_t node0 = tile * T;
+  if (MIXED)
+    for (uint32_t e = threadIdx.x; e < T * g.nax; e += kRmtBlock)
+      phase[e] = rmt_phase(k64, g, node0, h, e);
+  rmt_border<kRmtBlock>(P, g);
+  __syncthreads();  // the phase factors
+  const uint32_t nline = T * nx * ny;
+  for (uint32_t l = threadIdx.x; l < nline; l += kRmtBlock) {
+    RmtLineB<MIXED> ln;
+    rmt_line_fetch(Sv, g, node0, l, ln);
+    rmt_line_put(P, phase, g, Sv, node0, l, ln);
+  }
+  __syncthreads();
+  rmt_compute<kRmtBlock, MIXED, true>(P, phase, g, k64, group, r_in, r_out,
+                                      Kp, node0, R, [] {});
 }
 
 // Shell sums of tiles tile0 .. tile0+ntile-1 (T nodes each) by persistent
@@ -1065,7 +979,7 @@ __global__ __launch_bounds__(kRmtPBlock, GLST_RMT_MIN_WAVES) void rmt_lds_kernel
   const uint32_t nline = T * nx * ny;
   double2 *phase0 = P + T * g.ts;
   rmt_border<kRmtPBlock>(P, g);
-  RmtLine<MIXED> ln;
+  RmtLineP<MIXED> ln;
   if (tile < end) {
     const uint32_t node0 = (tile0 + tile) * T;
     for (uint32_t e = tid; e < nph; e += kRmtPBlock)
@@ -1079,7 +993,7 @@ __global__ __launch_bounds__(kRmtPBlock, GLST_RMT_MIN_WAVES) void rmt_lds_kernel
     const double2 *phase = phase0 + buf * nph;
     if (tid < nline) rmt_line_put(P, phase, g, Sv, node0, tid, ln);
     for (uint32_t l = tid + kRmtPBlock; l < nline; l += kRmtPBlock) {
-      RmtLine<MIXED> more;
+      RmtLineP<MIXED> more;
       rmt_line_fetch(Sv, g, node0, l, more);
       rmt_line_put(P, phase, g, Sv, node0, l, more);
     }
@@ -1690,6 +1604,15 @@ __device__ __forceinline__ void sf_mixed_nr_body(
     else
       return st[i];
   };
+  // uniform over the wave (its cell): does the cell hold a far atom?  One
+  // vector read of the cell's cell_sorted entries, then a ballot.
+  bool far = false;
+  if (cell < ncell) {
+    uint32_t f = 0u;
+    for (uint32_t j = start[cell] + lane; j < start[cell + 1]; j += kSfDirs)
+      f |= cell_sorted[j] & kFarAtom;
+    far = __ballot(f != 0u) != 0;
+  }
   if (cell < ncell && (CELLS > 1 || active)) {
     const uint32_t a0 = start[cell], a1 = start[cell + 1];
     if (a0 == a1) {
@@ -1701,23 +1624,23 @@ __device__ __forceinline__ void sf_mixed_nr_body(
       float re[NR], im[NR];
 #pragma unroll
       for (int i = 0; i < NR; ++i) re[i] = im[i] = 0.f;
-      for (uint32_t j = jb; j < je; ++j) {
-        const float4 a = atoms[j];
-        const float p = d.x * a.x + d.y * a.y + d.z * a.z;
-        auto acc = [&](auto far_tag) {
-          constexpr bool FAR = decltype(far_tag)::value;
+      auto atoms_loop = [&](auto far_tag) {
+        constexpr bool FAR = decltype(far_tag)::value;
+        for (uint32_t j = jb; j < je; ++j) {
+          const float4 a = atoms[j];
+          const float p = d.x * a.x + d.y * a.y + d.z * a.z;
 #pragma unroll
           for (int i = 0; i < NR; ++i) {
             const float ph = reduce_rev<FAR>(kap[i] * p);
             re[i] = fmaf(a.w, cos_rev(ph), re[i]);
             im[i] = fmaf(-a.w, sin_rev(ph), im[i]);
           }
-        };
-        if (cell_sorted[j] & kFarAtom)  // uniform: the atom is
-          acc(std::true_type{});
-        else
-          acc(std::false_type{});
-      }
+        }
+      };
+      if (far)
+        atoms_loop(std::true_type{});
+      else
+        atoms_loop(std::false_type{});
       if (jb == a0) {
 #pragma unroll
         for (int i = 0; i < NR; ++i) put(i, make_float2(re[i], im[i]));
@@ -1791,6 +1714,11 @@ __device__ __forceinline__ void sf_mixed_any_body(
   const uint32_t n0 = tk.node0[tt];
   {
     const uint32_t a0 = start[cell], a1 = start[cell + 1];
+    // uniform: does the cell hold a far atom (one vector read, a ballot)?
+    uint32_t f = 0u;
+    for (uint32_t j = a0 + threadIdx.x; j < a1; j += kSfDirs)
+      f |= cell_sorted[j] & kFarAtom;
+    const bool far = __ballot(f != 0u) != 0;
     for (uint32_t i = 0; i < nr; ++i) {
       double RE = 0.0, IM = 0.0;
       for (uint32_t jb = a0; jb < a1; jb += kSfChunk) {
@@ -1799,8 +1727,7 @@ __device__ __forceinline__ void sf_mixed_any_body(
         for (uint32_t j = jb; j < je; ++j) {
           const float4 a = atoms[j];
           const float t = kap[i] * (d.x * a.x + d.y * a.y + d.z * a.z);
-          const float ph = (cell_sorted[j] & kFarAtom) ? reduce_rev<true>(t)
-                                                       : reduce_rev<false>(t);
+          const float ph = far ? reduce_rev<true>(t) : reduce_rev<false>(t);
           re = fmaf(a.w, cos_rev(ph), re);
           im = fmaf(-a.w, sin_rev(ph), im);
         }
