@@ -630,22 +630,31 @@ __device__ inline CubeXZ cube_xz(uint32_t x, uint32_t z, uint32_t r,
   return c;
 }
 
-// Cube sum at row y from the bordered table P (box_sum's term order).
-__device__ inline double2 cube_y(const double2 *P, const CubeXZ &c,
-                                 uint32_t y, uint32_t r, uint32_t ny,
-                                 uint32_t zp) {
-  const uint32_t o1 = ((y + r >= ny) ? ny : y + r + 1) * zp;
-  const uint32_t om = (y < r ? 0 : y - r) * zp;
-  const double2 r0 = P[c.bx1 + o1 + c.z1];  // (x1, y1, z1)
-  const double2 r1 = P[c.bxm + o1 + c.z1];  // (x0-1, y1, z1)
-  const double2 r2 = P[c.bx1 + om + c.z1];  // (x1, y0-1, z1)
-  const double2 r3 = P[c.bx1 + o1 + c.zm];  // (x1, y1, z0-1)
-  const double2 r4 = P[c.bxm + om + c.z1];  // (x0-1, y0-1, z1)
-  const double2 r5 = P[c.bxm + o1 + c.zm];  // (x0-1, y1, z0-1)
-  const double2 r6 = P[c.bx1 + om + c.zm];  // (x1, y0-1, z0-1)
-  const double2 r7 = P[c.bxm + om + c.zm];  // (x0-1, y0-1, z0-1)
-  return make_double2(r0.x - r1.x - r2.x - r3.x + r4.x + r5.x + r6.x - r7.x,
-                      r0.y - r1.y - r2.y - r3.y + r4.y + r5.y + r6.y - r7.y);
+// The 8 corners of a cube at one row from the bordered table P: rows o1
+// (last row of the cube + 1, times zp) and om (first row, times zp).
+struct CubeCorners {
+  double2 v[8];
+};
+__device__ __forceinline__ CubeCorners cube_corners(const double2 *P,
+                                                    const CubeXZ &c,
+                                                    uint32_t o1, uint32_t om) {
+  CubeCorners k;
+  k.v[0] = P[c.bx1 + o1 + c.z1];  // (x1, y1, z1)
+  k.v[1] = P[c.bxm + o1 + c.z1];  // (x0-1, y1, z1)
+  k.v[2] = P[c.bx1 + om + c.z1];  // (x1, y0-1, z1)
+  k.v[3] = P[c.bx1 + o1 + c.zm];  // (x1, y1, z0-1)
+  k.v[4] = P[c.bxm + om + c.z1];  // (x0-1, y0-1, z1)
+  k.v[5] = P[c.bxm + o1 + c.zm];  // (x0-1, y1, z0-1)
+  k.v[6] = P[c.bx1 + om + c.zm];  // (x1, y0-1, z0-1)
+  k.v[7] = P[c.bxm + om + c.zm];  // (x0-1, y0-1, z0-1)
+  return k;
+}
+// inclusion-exclusion in box_sum's term order
+__device__ __forceinline__ double2 cube_sum(const CubeCorners &k) {
+  const double2 *r = k.v;
+  return make_double2(
+      r[0].x - r[1].x - r[2].x - r[3].x + r[4].x + r[5].x + r[6].x - r[7].x,
+      r[0].y - r[1].y - r[2].y - r[3].y + r[4].y + r[5].y + r[6].y - r[7].y);
 }
 
 // Geometry of a workgroup's bordered tables (rmt_table_elems): T tables of
@@ -798,9 +807,21 @@ __device__ __forceinline__ void rmt_compute(
     }
     const uint32_t yb = seg * ylen;
     const uint32_t ye = (yb + ylen < ny) ? yb + ylen : ny;
+    // row offsets of both cubes, advanced with y: the upper row is
+    // min(y + r + 1, ny), the lower one max(y - r, 0)
+    uint32_t o1o = ((yb + rout >= ny) ? ny : yb + rout + 1) * zp;
+    uint32_t omo = (yb < rout ? 0 : yb - rout) * zp;
+    uint32_t o1i = ((yb + rin >= ny) ? ny : yb + rin + 1) * zp;
+    uint32_t omi = (yb < rin ? 0 : yb - rin) * zp;
     for (uint32_t y = yb; y < ye; ++y) {
-      const double2 o = cube_y(Pt, co, y, rout, ny, zp);
-      const double2 in = cube_y(Pt, ci, y, rin, ny, zp);
+      // all 16 corners are read before any is used
+      const CubeCorners ko = cube_corners(Pt, co, o1o, omo);
+      const CubeCorners ki = cube_corners(Pt, ci, o1i, omi);
+      o1o += (y + rout + 1 < ny) ? zp : 0u;
+      omo += (y >= rout) ? zp : 0u;
+      o1i += (y + rin + 1 < ny) ? zp : 0u;
+      omi += (y >= rin) ? zp : 0u;
+      const double2 o = cube_sum(ko), in = cube_sum(ki);
       const double dr = o.x - in.x, di = o.y - in.y;
       const size_t off = static_cast<size_t>((x * ny + y) * nz + z) * Kp + node;
       if (MIXED) {
@@ -1144,15 +1165,31 @@ __device__ __forceinline__ void lr_f64_body(
     const double *kap = tk.kappa64 + tk.koff[t];
     const double4 d = tk.dir64[t];
     const double p = d.x * a.x + d.y * a.y + d.z * a.z;
-    double e = 0.0, b = 0.0;
-    for (uint32_t j = 0; j < nr; ++j) {
+    // two radial nodes per iteration: independent chains, and the sine and
+    // cosine coefficients stay in registers across the loop
+    double e = 0.0, b = 0.0, e1 = 0.0, b1 = 0.0;
+    uint32_t j = 0;
+    for (; j + 2 <= nr; j += 2) {
+      const double k0 = kap[j], k1 = kap[j + 1];
+      const double2 r0 = Rc[n0 + j], r1 = Rc[n0 + j + 1];
+      double s0, c0, s1, c1;
+      sincos_cw(k0 * p, &s0, &c0);
+      sincos_cw(k1 * p, &s1, &c1);
+      e += c0 * r0.x - s0 * r0.y;
+      b += k0 * (c0 * r0.y + s0 * r0.x);
+      e1 += c1 * r1.x - s1 * r1.y;
+      b1 += k1 * (c1 * r1.y + s1 * r1.x);
+    }
+    if (j < nr) {
       const double kj = kap[j];
+      const double2 r = Rc[n0 + j];
       double sn, cs;
       sincos_cw(kj * p, &sn, &cs);
-      const double2 r = Rc[n0 + j];
       e += cs * r.x - sn * r.y;
       b += kj * (cs * r.y + sn * r.x);
     }
+    e += e1;
+    b += b1;
     en += e;
     fx += d.x * b;
     fy += d.y * b;
