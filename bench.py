@@ -13,10 +13,11 @@ plain FP64 path beside it, the relative force error of both against the
 direct FP64 all-pairs sum, the roofline of the dominant kernel (HIP events on
 the handle's stream), a CPU baseline (the oracle's FP64 restatement of the
 reference path, one full step on the host cores), the served p50/p99 latency
-of the continuous-batching server under Poisson arrivals and the throughput
-of a 512-system batch; every served answer and every batched system is
-checked against the direct FP64 sum of its own atoms, and the run exits with
-status 1 when any error is above the metric's 1e-5 bar.
+of the continuous-batching server under Poisson arrivals, the throughput
+of a 512-system batch, and the 1,000-water and 1.0M-charge configurations on
+both paths; every served answer, every batched system and every
+configuration is checked against the direct FP64 sum of its own atoms, and
+the run exits with status 1 when any error is above the metric's 1e-5 bar.
 """
 import argparse
 import json
@@ -402,6 +403,51 @@ def batch_bench(g, torch, np, nsys=512, steps=3, seed=4):
     }
 
 
+def config_errors(g, torch, steps=5):
+    """The two other single-system configurations, each on both paths with
+    the default schedule: 1,000 waters (L 31.04 A, rcut 9, tol 1e-5) and the
+    ionic solution of 333,000 waters and 898 Na+/Cl- pairs (1,000,796
+    charges, L 215 A, rcut 12, tol 1e-5).  Time per step and relative force
+    error against the direct FP64 sum of the same charges."""
+    cases = [("1,000 waters, L 31.04 A, rcut 9 A, tol 1e-5", 1000, 0, 31.04,
+              9.0, 1e-5),
+             ("333,000 waters + 898 Na+/Cl- pairs, L 215 A, rcut 12 A, "
+              "tol 1e-5", 333000, 898, 215.0, 12.0, 1e-5)]
+    out = []
+    for name, nw, nion, L, rc, tol in cases:
+        h = g.make_water_box(nw, L, seed=1, nion=nion)
+        d = [torch.as_tensor(a, dtype=torch.float64, device="cuda") for a in h]
+        ref = g.coulomb_direct(*d)
+        row = {"workload": name, "natom": int(h[0].size)}
+        for ct in ("float", "double"):
+            f = g.GLSTForce(h[0].size, tol, L, rcut=rc, ct=ct)
+            f.zero_ef()
+            f.calc_ener_force(*d)
+            got = f.get_ef()
+            num = sum(float(((a - b) ** 2).sum()) for a, b in zip(got[:3], ref[:3]))
+            den = sum(float((b ** 2).sum()) for b in ref[:3])
+            n = steps if ct == "float" else 2
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(n):
+                f.zero_ef()
+                f.calc_ener_force(*d)
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / n
+            key = "mixed" if ct == "float" else "fp64"
+            row[key] = {"ms_per_step": dt * 1e3,
+                        "atom_steps_per_s": h[0].size / dt,
+                        "rel_force_err": (num / den) ** 0.5,
+                        "num_nodes": f.info().num_nodes}
+            f.close()
+        row["speedup_vs_fp64"] = (row["fp64"]["ms_per_step"] /
+                                  row["mixed"]["ms_per_step"])
+        out.append(row)
+        del d, ref
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -527,6 +573,9 @@ def main():
         "pairs": pairs,
     }
 
+    configs = config_errors(g, torch)
+    log(f"configs: {configs}")
+
     served = batched = None
     if not args.no_serve:
         served = serve_bench(g, torch, np)
@@ -578,6 +627,7 @@ def main():
         "cpu_baseline": cpu,
         "served": served,
         "batched": batched,
+        "other_configs": configs,
     }
     print(json.dumps(line), flush=True)
     # a figure quoted above the metric's error bar is not a result
@@ -585,6 +635,8 @@ def main():
             if err > ERR_BAR]
     over += [name for name, blk in (("served", served), ("batched", batched))
              if blk is not None and not blk["meets_error_bar"]]
+    over += [f"{c['natom']} charges {k}" for c in configs
+             for k in ("mixed", "fp64") if c[k]["rel_force_err"] > ERR_BAR]
     if over:
         log(f"bench: relative force error above {ERR_BAR} in {over}")
         sys.exit(1)
