@@ -220,7 +220,8 @@ def _max_err_vs_direct(g, torch, cases):
     return worst, len(cases)
 
 
-def serve_bench(g, torch, np, n_requests=2000, seed=5, max_batch=8):
+def serve_bench(g, torch, np, n_requests=2000, seed=5, max_batch=8,
+                fused=0):
     """Served latency of the continuous-batching server (mixed path).
 
     Requests are systems of the batch size mix: waters log-uniform in
@@ -238,7 +239,8 @@ def serve_bench(g, torch, np, n_requests=2000, seed=5, max_batch=8):
     completion time - scheduled arrival time (queueing and setup
     included).  Every request of the Poisson stream has its own output
     arrays; afterwards each answer is compared with the direct FP64 sum of
-    its own inputs."""
+    its own inputs.  `fused`: the batches' launch mode (glst_batch option
+    "fused")."""
     rng = np.random.default_rng(seed)
     pool = []
     for L, h in _water_mix(g, np, rng, 48, 1000):
@@ -262,6 +264,8 @@ def serve_bench(g, torch, np, n_requests=2000, seed=5, max_batch=8):
         return L * (1.0 + 1e-9 * jitter[0]), d, outputs(d[0].numel())
 
     srv = g.GLSTServer(SERVE_TOL, 64 << 30, max_batch=max_batch, ct="float")
+    if fused:
+        srv.set_option("fused", fused)
     for L, d in resident:  # resident systems are set up once
         srv.submit(L, SERVE_RCUT, *d, *outputs(d[0].numel()))
     while srv.pending():
@@ -337,7 +341,8 @@ def serve_bench(g, torch, np, n_requests=2000, seed=5, max_batch=8):
         "workload": "Poisson arrivals of water systems, 100-10000 waters "
                     f"(log-uniform), rcut {SERVE_RCUT}, tol {SERVE_TOL}, "
                     "mixed; 30 % repeat steps of 16 resident systems; server "
-                    "max_batch " + str(max_batch),
+                    "max_batch " + str(max_batch) + ", launch mode " +
+                    str(fused),
         "tol": SERVE_TOL,
         "rcut": SERVE_RCUT,
         "requests": n_requests,

@@ -286,6 +286,12 @@ BatchLayout pack_batch(int ct,
     B.sys.push_back(d);
   }
   if (B.t_split.empty()) B.t_split.push_back(0);
+  // units of the fused step's S(k) and back-projection phases, largest
+  // radial count first
+  struct FusedRun {
+    uint32_t nr, sys, blocks, a, b;
+  };
+  std::vector<FusedRun> fsf, flr;
   // run tables
   for (uint32_t s = 0; s < nsys; ++s) {
     const dev::SysDev &d = B.sys[s];
@@ -317,8 +323,23 @@ BatchLayout pack_batch(int ct,
       // atom-group tiles for the unrolled radial counts (see lr_batch)
       const uint32_t tiles = key != 0 && d.agtile ? d.agtile : d.ntile;
       pl.add(s, tiles * lg.nsplit, d.split0 + lg.split0, lg.nsplit);
+      if (ct == GLST_CT_FLOAT) {
+        fsf.push_back({sg.nr, s,
+                       div_up(d.ncell, dev::kFusedSfCells) *
+                           div_up(sg.ntask, 64),
+                       task0 + sg.task0, sg.ntask});
+        const bool unrolled =
+            sg.nr <= static_cast<uint32_t>(dev::kFusedMaxNr) && d.agtile;
+        flr.push_back({sg.nr, s, (unrolled ? d.agtile : d.ntile) * lg.nsplit,
+                       d.split0 + lg.split0, lg.nsplit});
+      }
     }
   }
+  auto by_nr = [](const FusedRun &x, const FusedRun &y) { return x.nr > y.nr; };
+  std::stable_sort(fsf.begin(), fsf.end(), by_nr);
+  std::stable_sort(flr.begin(), flr.end(), by_nr);
+  for (const FusedRun &r : fsf) B.fsf.add(r.sys, r.blocks, r.a, r.b);
+  for (const FusedRun &r : flr) B.flr.add(r.sys, r.blocks, r.a, r.b);
   return B;
 }
 
@@ -326,6 +347,7 @@ BatchLayout pack_batch(int ct,
 
 glst_batch::~glst_batch() {
   drop_graph();
+  if (fail_host) (void)hipHostFree(fail_host);
   if (side) (void)hipStreamDestroy(side);
   if (ev_assigned) (void)hipEventDestroy(ev_assigned);
   if (ev_near) (void)hipEventDestroy(ev_near);
@@ -426,6 +448,8 @@ void glst_batch::configure(
   stage(B.sr, L_sr);
   stage(B.reduce, L_reduce);
   stage(B.rmt, L_rmt);
+  stage(B.fsf, L_fsf);
+  stage(B.flr, L_flr);
   if (mixed()) {
     size_t i = 0;
     for (auto &kv : B.sf_nr) stage(kv.second, L_sf[i++]);
@@ -443,7 +467,90 @@ void glst_batch::configure(
                   stream);
   zero_ef();
   assigned = false;
+  plan_fused();
   check_launch("batch configure");
+}
+
+void glst_batch::plan_fused() {
+  fused_nwg = 0;
+  if (!mixed() || (fused != 1 && fused != 2)) return;
+  fused_lds = dev::fused_lds_bytes(B.lds);
+  const uint32_t cap = dev::fused_workgroups(fused_lds, fused);
+  if (cap == 0) throw StateError("the fused step does not fit on this device");
+  if (!fail_host) {
+    GLST_HIP(hipHostMalloc(reinterpret_cast<void **>(&fail_host), 4,
+                           hipHostMallocMapped));
+    *fail_host = 0u;
+    GLST_HIP(hipHostGetDevicePointer(reinterpret_cast<void **>(&fail_dev),
+                                     fail_host, 0));
+    // a barrier gives up after 10 s (a legitimate phase takes milliseconds)
+    fused_timeout = 10 * dev::wall_clock_rate();
+  }
+  fsync.grow(2 * sizeof(uint32_t), 0);
+  // no more workgroups than the largest phase has units
+  uint64_t units = std::max<uint64_t>(
+      {B.bin.nblk, B.hist_p.nblk, B.sys.size(), B.scatter.nblk,
+       static_cast<uint64_t>(B.fsf.nblk) + B.sr.nblk, B.rmt.nblk, B.flr.nblk,
+       B.reduce.nblk, 1});
+  const uint32_t share = std::max(1u, cap / std::max(1u, fused_share));
+  fused_nwg = static_cast<uint32_t>(std::min<uint64_t>(units, share));
+}
+
+void glst_batch::check_failed() const {
+  if (fail_host && *static_cast<volatile uint32_t *>(fail_host) != 0u)
+    throw StateError(
+        "a fused batched step failed: a grid barrier timed out because not "
+        "all of its workgroups were resident; its outputs are not valid");
+}
+
+void glst_batch::step_fused(const double *rx, const double *ry,
+                            const double *rz, const double *qc) {
+  dev::FusedStep f{};
+  f.rx = rx;
+  f.ry = ry;
+  f.rz = rz;
+  f.qc = qc;
+  f.sys = d_sys.as<dev::SysDev>();
+  f.nsys = nsys();
+  f.bin = L_bin;
+  f.hist_p = L_hist;
+  f.scatter = L_scatter;
+  f.sf = L_fsf;
+  f.sr = L_sr;
+  f.rmt = L_rmt;
+  f.lr = L_flr;
+  f.reduce = L_reduce;
+  f.atom_cell = atom_cell.as<uint32_t>();
+  f.rank = rank.as<uint32_t>();
+  f.hist = hist.as<uint32_t>();
+  f.count = cell_count.as<uint32_t>();
+  f.start = cell_start.as<uint32_t>();
+  f.maxc = maxc.as<uint32_t>();
+  f.sorted_idx = sorted_idx.as<uint32_t>();
+  f.cell_sorted = cell_sorted.as<uint32_t>();
+  f.atoms64 = atoms64.as<double4>();
+  f.atoms32 = atoms32.as<float4>();
+  f.ag_start = B.nag ? ag_start.as<uint32_t>() : nullptr;
+  f.ag = B.nag ? atom_grp.as<uint2>() : nullptr;
+  f.S = S.as<float2>();
+  f.R = R.as<float2>();
+  f.partial = partial.as<double4>();
+  f.near4 = near4.as<double4>();
+  f.fx = fx.as<double>();
+  f.fy = fy.as<double>();
+  f.fz = fz.as<double>();
+  f.en = en.as<double>();
+  const ShapeTables &tb = *tables;
+  f.k64 = tb.d_k64.as<double4>();
+  f.group = tb.d_group.as<uint32_t>();
+  f.r_in = tb.d_r_in.as<uint32_t>();
+  f.r_out = tb.d_r_out.as<uint32_t>();
+  f.tk = task_args();
+  f.sync = fsync.as<uint32_t>();
+  f.fail = fail_dev;
+  f.timeout = fused_timeout;
+  dev::step_fused(f, fused_nwg, fused_lds, stream, fused);
+  check_launch("fused step");
 }
 
 dev::LrTaskArgs glst_batch::task_args() const {
@@ -480,6 +587,11 @@ void glst_batch::step_async(const double *rx, const double *ry,
                             const double *rz, const double *qc) {
   if (!rx || !ry || !rz || !qc)
     throw ArgError("calc_ener_force: null coordinate or charge pointer");
+  if (use_fused()) {
+    step_fused(rx, ry, rz, qc);
+    assigned = true;
+    return;
+  }
   const dev::SysDev *sys = d_sys.as<dev::SysDev>();
   // assign_atoms of every system
   dev::fill_u32(hist.as<uint32_t>(), 0u, std::max<uint64_t>(B.hist, 1),
@@ -512,17 +624,26 @@ void glst_batch::step_async(const double *rx, const double *ry,
                               : static_cast<const void *>(atoms64.as<void>());
   const dev::LrTaskArgs ta = task_args();
   const ShapeTables &tb = *tables;
-  dev::sf_batch(mixed(), atoms, cell_start.as<uint32_t>(),
-                cell_sorted.as<uint32_t>(), tb.d_k64.as<double4>(), ta, sys,
-                L_sf, S.as<void>(), stream);
+  if (fused == 3 && mixed())
+    dev::sf_all_batch(atoms, cell_start.as<uint32_t>(),
+                      cell_sorted.as<uint32_t>(), ta, sys, L_fsf,
+                      S.as<void>(), stream);
+  else
+    dev::sf_batch(mixed(), atoms, cell_start.as<uint32_t>(),
+                  cell_sorted.as<uint32_t>(), tb.d_k64.as<double4>(), ta, sys,
+                  L_sf, S.as<void>(), stream);
   check_launch("calc_sf");
   dev::rmt_batch(mixed(), S.as<void>(), tb.d_k64.as<double4>(),
                  tb.d_group.as<uint32_t>(), tb.d_r_in.as<uint32_t>(),
                  tb.d_r_out.as<uint32_t>(), sys, L_rmt, B.lds, R.as<void>(),
                  stream);
   check_launch("sum_rmt_sf");
-  dev::lr_batch(mixed(), atoms, cell_sorted.as<uint32_t>(), ta, sys, L_lr,
-                R.as<void>(), partial.as<double4>(), stream, ag);
+  if (fused == 3 && mixed())
+    dev::lr_all_batch(atoms, cell_sorted.as<uint32_t>(), ta, sys, L_flr,
+                      R.as<void>(), partial.as<double4>(), stream, ag);
+  else
+    dev::lr_batch(mixed(), atoms, cell_sorted.as<uint32_t>(), ta, sys, L_lr,
+                  R.as<void>(), partial.as<double4>(), stream, ag);
   check_launch("calc_lr_ef");
   if (overlap_near)
     GLST_HIP(hipStreamWaitEvent(stream, ev_near, 0));
@@ -589,6 +710,7 @@ double glst_batch::autotune(const double *rx, const double *ry,
     for (uint32_t r = 0; r < reps; ++r) step(rx, ry, rz, qc);
     GLST_HIP(hipEventRecord(b, stream));
     GLST_HIP(hipEventSynchronize(b));
+    check_failed();
     float ms = 0.f;
     GLST_HIP(hipEventElapsedTime(&ms, a, b));
     return static_cast<double>(ms) / reps;
@@ -596,6 +718,24 @@ double glst_batch::autotune(const double *rx, const double *ry,
   double best;
   try {
     best = time();
+    if (mixed()) {  // one persistent launch or a launch per stage
+      const int keep = fused;
+      int pick = keep;
+      for (int v : {0, 1, 2, 3}) {
+        if (v == keep) continue;
+        fused = v;
+        plan_fused();
+        drop_graph();
+        const double t = time();
+        if (t < best) {
+          best = t;
+          pick = v;
+        }
+      }
+      fused = pick;
+      plan_fused();
+      drop_graph();
+    }
     const int keep_overlap = overlap_near;
     int pick_overlap = keep_overlap;
     for (int v : {0, 1}) {
@@ -803,6 +943,7 @@ glst_status_t glst_batch_calc_ener_force(glst_batch_t b, const double *d_rx,
   return run_guarded(&b->last_error, [&] {
     b->step(d_rx, d_ry, d_rz, d_qc);
     GLST_HIP(hipStreamSynchronize(b->stream));
+    b->check_failed();
   });
 }
 
@@ -812,6 +953,7 @@ glst_status_t glst_batch_get_ef(glst_batch_t b, double *d_fx, double *d_fy,
   return run_guarded(&b->last_error, [&] {
     b->get_ef_async(d_fx, d_fy, d_fz, d_en);
     GLST_HIP(hipStreamSynchronize(b->stream));
+    b->check_failed();
   });
 }
 
@@ -839,6 +981,11 @@ glst_status_t glst_batch_set_option(glst_batch_t b, const char *name,
     } else if (n == "graph") {
       if (value < 0 || value > 2) throw ArgError("graph must be 0, 1 or 2");
       b->graph_mode = static_cast<int>(value);
+    } else if (n == "fused") {
+      if (value < 0 || value > 3) throw ArgError("fused must be 0 .. 3");
+      GLST_HIP(hipStreamSynchronize(b->stream));
+      b->fused = static_cast<int>(value);
+      b->plan_fused();
     } else {
       throw ArgError("unknown option \"" + n + "\"");
     }

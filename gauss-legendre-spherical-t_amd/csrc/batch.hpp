@@ -113,6 +113,9 @@ struct BatchLayout {
   std::vector<uint32_t> t_split;
   Plan bin, hist_p, scatter, sr, reduce, rmt, sf_f64, lr_f64;
   std::map<uint32_t, Plan> sf_nr, lr_nr;  // radial count (0: any) -> plan
+  // mixed: the S(k) and back-projection units of the fused step (all radial
+  // counts, largest first; see dev::FusedStep)
+  Plan fsf, flr;
 
   bool mixed() const { return ct == GLST_CT_FLOAT; }
   size_t device_bytes() const;  // arena and table bytes this layout needs
@@ -152,6 +155,18 @@ struct glst_batch {
   hipGraphExec_t graph_exec = nullptr;
   const double *graph_in[4] = {nullptr, nullptr, nullptr, nullptr};
   const double *last_in[4] = {nullptr, nullptr, nullptr, nullptr};
+  // mixed path: 0 runs every stage as its own launch(es), S(k) and the
+  // back-projection one per radial count; 3 one launch per stage; 1 and 2
+  // the whole step as one persistent launch (dev::step_fused variants).
+  // fused_share: the launch takes 1/fused_share of the device's resident
+  // workgroups (a server's two slots may run theirs at once).
+  int fused = 0;
+  unsigned fused_share = 1;
+  uint32_t fused_nwg = 0;   // workgroups of the configured batch's launch
+  size_t fused_lds = 0;     // its dynamic LDS
+  long long fused_timeout = 0;
+  glst::DeviceBuffer fsync;  // barrier counter and failure word
+  uint32_t *fail_host = nullptr, *fail_dev = nullptr;  // mapped host flag
   // per cent added to an arena when it grows (0: exact sizes)
   unsigned arena_headroom = 25;
   // shape tables (shared with the other batches of a server) and the
@@ -169,7 +184,7 @@ struct glst_batch {
   std::vector<glst::dev::Seg> host_seg;  // run tables (kept for the copy)
   // launches (run tables inside d_seg)
   glst::dev::BatchLaunch L_bin{}, L_hist{}, L_scatter{}, L_sr{}, L_reduce{},
-      L_rmt{};
+      L_rmt{}, L_fsf{}, L_flr{};
   std::vector<glst::dev::BatchLaunch> L_sf, L_lr;
 
   bool mixed() const { return B.mixed(); }
@@ -199,9 +214,21 @@ struct glst_batch {
                   const double *qc, uint32_t reps);
   // unpermuted outputs in packed original order (stream-ordered, no sync)
   void get_ef_async(double *ofx, double *ofy, double *ofz, double *oen);
+  // after the stream's work is complete: throws if a fused step failed
+  // (a barrier timed out; its outputs are not valid)
+  void check_failed() const;
+  // the next step runs as one persistent launch
+  bool use_fused() const {
+    return (fused == 1 || fused == 2) && mixed() && fused_nwg > 0;
+  }
+  // workgroups and LDS of the fused launch for the configured batch (after
+  // configure and whenever fused or fused_share change)
+  void plan_fused();
   size_t device_bytes() const;
 
  private:
   glst::dev::LrTaskArgs task_args() const;
   void near_field(hipStream_t s);
+  void step_fused(const double *rx, const double *ry, const double *rz,
+                  const double *qc);
 };

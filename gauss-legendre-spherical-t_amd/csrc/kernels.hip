@@ -132,58 +132,76 @@ __device__ inline uint32_t cell_coord(double r, CT inv, uint32_t n) {
   return static_cast<uint32_t>(c) < n ? static_cast<uint32_t>(c) : n - 1u;
 }
 
-// One chunk of 1024 atoms of one system (pointers at the system's first
-// atom and histogram).
-template <typename CT>
+// One chunk of kBinChunk atoms of one system (pointers at the system's first
+// atom and histogram), sorted by NT threads (kBinChunk / NT keys each) in
+// `keys` (kBinChunk entries of LDS).  ZERO: the chunk first clears its own
+// histogram row (a launch that has not cleared the histogram before).
+template <typename CT, uint32_t NT = kBinChunk, bool ZERO = false>
 __device__ __forceinline__ void bin_chunk_body(
     const double *__restrict__ rx, const double *__restrict__ ry,
     const double *__restrict__ rz, uint32_t natom, CT inv, uint32_t nx,
     uint32_t ny, uint32_t nz, uint32_t ncell, uint32_t *__restrict__ atom_cell,
-    uint32_t *__restrict__ rank, uint32_t *__restrict__ hist, uint32_t chunk) {
-  __shared__ unsigned long long keys[kBinChunk];
-  const uint32_t tid = threadIdx.x;
-  const uint32_t i = chunk * kBinChunk + tid;
-  uint32_t cell = 0xFFFFFFFFu;  // sorts after every real cell
-  if (i < natom) {
-    const uint32_t cx = cell_coord<CT>(rx[i], inv, nx);
-    const uint32_t cy = cell_coord<CT>(ry[i], inv, ny);
-    const uint32_t cz = cell_coord<CT>(rz[i], inv, nz);
-    cell = (cx * ny + cy) * nz + cz;
-    atom_cell[i] = cell;
+    uint32_t *__restrict__ rank, uint32_t *__restrict__ hist, uint32_t chunk,
+    unsigned long long *__restrict__ keys) {
+  static_assert(kBinChunk % NT == 0, "threads must divide the chunk");
+  constexpr uint32_t E = kBinChunk / NT;
+  if (ZERO)
+    for (uint32_t c = threadIdx.x; c < ncell; c += NT)
+      hist[static_cast<size_t>(chunk) * ncell + c] = 0u;
+#pragma unroll
+  for (uint32_t e = 0; e < E; ++e) {
+    const uint32_t t = threadIdx.x + e * NT;
+    const uint32_t i = chunk * kBinChunk + t;
+    uint32_t cell = 0xFFFFFFFFu;  // sorts after every real cell
+    if (i < natom) {
+      const uint32_t cx = cell_coord<CT>(rx[i], inv, nx);
+      const uint32_t cy = cell_coord<CT>(ry[i], inv, ny);
+      const uint32_t cz = cell_coord<CT>(rz[i], inv, nz);
+      cell = (cx * ny + cy) * nz + cz;
+      atom_cell[i] = cell;
+    }
+    keys[t] = (static_cast<unsigned long long>(cell) << 32) | t;
   }
-  keys[tid] = (static_cast<unsigned long long>(cell) << 32) | tid;
   __syncthreads();
   for (uint32_t k = 2; k <= kBinChunk; k <<= 1) {
     for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      const uint32_t p = tid ^ j;
-      if (p > tid) {
-        const unsigned long long a = keys[tid], b = keys[p];
-        const bool ascending = (tid & k) == 0;
-        if ((a > b) == ascending) {
-          keys[tid] = b;
-          keys[p] = a;
+#pragma unroll
+      for (uint32_t e = 0; e < E; ++e) {
+        const uint32_t t = threadIdx.x + e * NT;
+        const uint32_t p = t ^ j;
+        if (p > t) {
+          const unsigned long long a = keys[t], b = keys[p];
+          const bool ascending = (t & k) == 0;
+          if ((a > b) == ascending) {
+            keys[t] = b;
+            keys[p] = a;
+          }
         }
       }
       __syncthreads();
     }
   }
-  const unsigned long long key = keys[tid];
-  const uint32_t c = static_cast<uint32_t>(key >> 32);
-  if (c != 0xFFFFFFFFu) {
-    // first sorted position holding this cell
-    uint32_t lo = 0, hi = tid;
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (static_cast<uint32_t>(keys[mid] >> 32) < c)
-        lo = mid + 1;
-      else
-        hi = mid;
+#pragma unroll
+  for (uint32_t e = 0; e < E; ++e) {
+    const uint32_t t = threadIdx.x + e * NT;
+    const unsigned long long key = keys[t];
+    const uint32_t c = static_cast<uint32_t>(key >> 32);
+    if (c != 0xFFFFFFFFu) {
+      // first sorted position holding this cell
+      uint32_t lo = 0, hi = t;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (static_cast<uint32_t>(keys[mid] >> 32) < c)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      const uint32_t local = static_cast<uint32_t>(key & 0xFFFFFFFFull);
+      rank[chunk * kBinChunk + local] = t - lo;
+      const bool last = (t == kBinChunk - 1) ||
+                        (static_cast<uint32_t>(keys[t + 1] >> 32) != c);
+      if (last) hist[static_cast<size_t>(chunk) * ncell + c] = t - lo + 1;
     }
-    const uint32_t local = static_cast<uint32_t>(key & 0xFFFFFFFFull);
-    rank[chunk * kBinChunk + local] = tid - lo;
-    const bool last = (tid == kBinChunk - 1) ||
-                      (static_cast<uint32_t>(keys[tid + 1] >> 32) != c);
-    if (last) hist[static_cast<size_t>(chunk) * ncell + c] = tid - lo + 1;
   }
 }
 
@@ -193,8 +211,9 @@ __global__ __launch_bounds__(1024) void bin_chunk_kernel(
     const double *__restrict__ rz, uint32_t natom, CT inv, uint32_t nx,
     uint32_t ny, uint32_t nz, uint32_t ncell, uint32_t *__restrict__ atom_cell,
     uint32_t *__restrict__ rank, uint32_t *__restrict__ hist) {
+  __shared__ unsigned long long keys[kBinChunk];
   bin_chunk_body<CT>(rx, ry, rz, natom, inv, nx, ny, nz, ncell, atom_cell,
-                     rank, hist, blockIdx.x);
+                     rank, hist, blockIdx.x, keys);
 }
 
 __device__ __forceinline__ void hist_offsets_body(uint32_t *__restrict__ hist,
@@ -238,17 +257,17 @@ __global__ void hist_offsets_kernel(uint32_t *__restrict__ hist,
 // write_end); *maxc = max count.  With ag_start: ag_start[c] = exclusive scan
 // of the cells' atom-group counts ceil(count / kAgAtoms), and the entries of
 // ag from the total up to nag_max are set to (0, 0) (bin_scatter fills the
-// others).
+// others).  NT threads, `lds` 3 * NT words.
+template <uint32_t NT = 1024>
 __device__ __forceinline__ void cell_scan_body(
     const uint32_t *__restrict__ count, uint32_t ncell,
     uint32_t *__restrict__ start, uint32_t *__restrict__ maxc, uint32_t base,
-    bool write_end, uint32_t *__restrict__ ag_start = nullptr,
-    uint2 *__restrict__ ag = nullptr, uint32_t nag_max = 0) {
-  __shared__ uint32_t sums[1024];
-  __shared__ uint32_t maxs[1024];
-  __shared__ uint32_t gsum[1024];
+    bool write_end, uint32_t *__restrict__ lds,
+    uint32_t *__restrict__ ag_start = nullptr, uint2 *__restrict__ ag = nullptr,
+    uint32_t nag_max = 0) {
+  uint32_t *const sums = lds, *const maxs = lds + NT, *const gsum = lds + 2 * NT;
   const uint32_t tid = threadIdx.x;
-  const uint32_t seg = (ncell + 1023u) / 1024u;
+  const uint32_t seg = (ncell + NT - 1) / NT;
   const uint32_t b = tid * seg;
   const uint32_t e = (b + seg < ncell) ? b + seg : ncell;
   uint32_t s = 0, m = 0, gs = 0;
@@ -262,7 +281,7 @@ __device__ __forceinline__ void cell_scan_body(
   maxs[tid] = m;
   gsum[tid] = gs;
   __syncthreads();
-  for (uint32_t off = 1; off < 1024; off <<= 1) {
+  for (uint32_t off = 1; off < NT; off <<= 1) {
     const uint32_t add = tid >= off ? sums[tid - off] : 0u;
     const uint32_t mx = tid >= off ? maxs[tid - off] : 0u;
     const uint32_t gadd = tid >= off ? gsum[tid - off] : 0u;
@@ -282,12 +301,12 @@ __device__ __forceinline__ void cell_scan_body(
       grun += (count[c] + kAgAtoms - 1) / kAgAtoms;
     }
   }
-  if (tid == 1023) {
-    if (write_end) start[ncell] = base + sums[1023];
-    *maxc = maxs[1023];
+  if (tid == NT - 1) {
+    if (write_end) start[ncell] = base + sums[NT - 1];
+    *maxc = maxs[NT - 1];
   }
   if (ag)
-    for (uint32_t k = gsum[1023] + tid; k < nag_max; k += 1024)
+    for (uint32_t k = gsum[NT - 1] + tid; k < nag_max; k += NT)
       ag[k] = make_uint2(0u, 0u);
 }
 
@@ -296,7 +315,9 @@ __global__ __launch_bounds__(1024) void cell_scan_kernel(
     uint32_t *__restrict__ start, uint32_t *__restrict__ maxc,
     uint32_t *__restrict__ ag_start, uint2 *__restrict__ ag,
     uint32_t nag_max) {
-  cell_scan_body(count, ncell, start, maxc, 0u, true, ag_start, ag, nag_max);
+  __shared__ uint32_t lds[3 * 1024];
+  cell_scan_body(count, ncell, start, maxc, 0u, true, lds, ag_start, ag,
+                 nag_max);
 }
 
 // Atom i of one system (inputs, atom_cell, rank, hist and start at the
@@ -1741,19 +1762,22 @@ __device__ __forceinline__ void sf_mixed_any_body(
     const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
     const uint32_t *__restrict__ cell_sorted, const LrTasks &tk,
     uint32_t task0, uint32_t ntask, uint32_t ncell, float2 *__restrict__ S,
-    uint32_t cell, uint32_t chunk) {
-  const uint32_t t = task0 + chunk * kSfDirs + threadIdx.x;
+    uint32_t cell0, uint32_t chunk) {
+  // wave w of the workgroup takes cell cell0 + w
+  const uint32_t lane = threadIdx.x % kSfDirs;
+  const uint32_t cell = cell0 + threadIdx.x / kSfDirs;
+  const uint32_t t = task0 + chunk * kSfDirs + lane;
   const bool active = t < task0 + ntask;
   const uint32_t tt = active ? t : task0;
   const float *kap = tk.kappa32 + tk.koff[task0];
   const uint32_t nr = tk.nr[task0];
   const float4 d = tk.dir32[tt];
   const uint32_t n0 = tk.node0[tt];
-  {
+  if (cell < ncell) {  // uniform over the wave
     const uint32_t a0 = start[cell], a1 = start[cell + 1];
     // uniform: does the cell hold a far atom (one vector read, a ballot)?
     uint32_t f = 0u;
-    for (uint32_t j = a0 + threadIdx.x; j < a1; j += kSfDirs)
+    for (uint32_t j = a0 + lane; j < a1; j += kSfDirs)
       f |= cell_sorted[j] & kFarAtom;
     const bool far = __ballot(f != 0u) != 0;
     for (uint32_t i = 0; i < nr; ++i) {
@@ -1950,10 +1974,11 @@ __device__ __forceinline__ void sr_body(
     const double4 *__restrict__ atoms, const uint32_t *__restrict__ start,
     uint32_t nx, uint32_t ny, uint32_t nz, double *__restrict__ fx,
     double *__restrict__ fy, double *__restrict__ fz,
-    double *__restrict__ en, double4 *__restrict__ out4, uint32_t cell) {
-  // one buffer: the neighbour tiles, then the per-lane partial sums (8 KB,
-  // so that two workgroups fit beside the shell sums' tables)
-  __shared__ double4 tile[kSrBlock];
+    double *__restrict__ en, double4 *__restrict__ out4, uint32_t cell,
+    double4 *__restrict__ tile) {
+  // `tile` (kSrBlock records of LDS): the neighbour tiles, then the per-lane
+  // partial sums (8 KB, so that two workgroups fit beside the shell sums'
+  // tables)
   double4 *const part = tile;
   const uint32_t ix = cell / (ny * nz), iy = (cell / nz) % ny, iz = cell % nz;
   const uint32_t a0 = start[cell], a1 = start[cell + 1];
@@ -2057,8 +2082,9 @@ __global__ __launch_bounds__(kSrBlock) void sr_kernel(
     uint32_t nx, uint32_t ny, uint32_t nz, double *__restrict__ fx,
     double *__restrict__ fy, double *__restrict__ fz,
     double *__restrict__ en, double4 *__restrict__ out4) {
+  __shared__ double4 tile[kSrBlock];
   sr_body<MIXED, STORE>(atoms, start, nx, ny, nz, fx, fy, fz, en, out4,
-                        xcd_remap(blockIdx.x, nx * ny * nz));
+                        xcd_remap(blockIdx.x, nx * ny * nz), tile);
 }
 
 // ---------------------------------------------------------------------------
@@ -2162,6 +2188,8 @@ inline uint32_t blocks_for(size_t n, uint32_t b) {
 // batched systems: every workgroup finds its run (system, and alpha group
 // where the launch is per radial count) by binary search over the runs'
 // first workgroups, then runs the single-system body on its system's slice.
+// The *_unit functions take the workgroup's number within its launch (blk);
+// the per-stage kernels and the fused step share them.
 
 __device__ __forceinline__ uint32_t run_of(const Seg *__restrict__ seg,
                                            uint32_t nseg, uint32_t blk) {
@@ -2176,47 +2204,181 @@ __device__ __forceinline__ uint32_t run_of(const Seg *__restrict__ seg,
   return lo;
 }
 
+// threads of the batched histogram, scatter and reduce workgroups
+constexpr uint32_t kBatchBlock = 256;
+
+template <typename CT, uint32_t NT, bool ZERO>
+__device__ __forceinline__ void bin_chunk_unit(
+    const double *__restrict__ rx, const double *__restrict__ ry,
+    const double *__restrict__ rz, const SysDev *__restrict__ sys,
+    const Seg *__restrict__ seg, uint32_t nseg, uint32_t *__restrict__ atom_cell,
+    uint32_t *__restrict__ rank, uint32_t *__restrict__ hist, uint32_t blk,
+    unsigned long long *__restrict__ keys) {
+  const Seg g = seg[run_of(seg, nseg, blk)];
+  const SysDev &d = sys[g.sys];
+  bin_chunk_body<CT, NT, ZERO>(rx + d.atom0, ry + d.atom0, rz + d.atom0,
+                               d.natom, static_cast<CT>(d.inv_cell), d.nx,
+                               d.ny, d.nz, d.ncell, atom_cell + d.atom0,
+                               rank + d.atom0, hist + d.hist0, blk - g.blk0,
+                               keys);
+}
+
+__device__ __forceinline__ void hist_offsets_unit(
+    uint32_t *__restrict__ hist, const SysDev *__restrict__ sys,
+    const Seg *__restrict__ seg, uint32_t nseg, uint32_t *__restrict__ count,
+    uint32_t blk) {
+  const Seg g = seg[run_of(seg, nseg, blk)];
+  const SysDev &d = sys[g.sys];
+  hist_offsets_body(hist + d.hist0, d.nchunk, d.ncell, count + d.cell0,
+                    (blk - g.blk0) * kBatchBlock + threadIdx.x);
+}
+
+// system s; start[cell0 + ncell] is the next system's first slot, the last
+// system writes the batch's end
+template <uint32_t NT>
+__device__ __forceinline__ void cell_scan_unit(
+    const uint32_t *__restrict__ count, const SysDev *__restrict__ sys,
+    uint32_t nsys, uint32_t *__restrict__ start, uint32_t *__restrict__ maxc,
+    uint32_t *__restrict__ ag_start, uint2 *__restrict__ ag, uint32_t s,
+    uint32_t *__restrict__ lds) {
+  const SysDev &d = sys[s];
+  const bool groups = ag && d.agtile;
+  cell_scan_body<NT>(count + d.cell0, d.ncell, start + d.cell0, maxc + s,
+                     d.atom0, s + 1 == nsys, lds,
+                     groups ? ag_start + d.cell0 : nullptr,
+                     groups ? ag + d.ag0 : nullptr, d.agtile * kAgPerBlock);
+}
+
+template <bool MIXED>
+__device__ __forceinline__ void bin_scatter_unit(
+    const double *__restrict__ rx, const double *__restrict__ ry,
+    const double *__restrict__ rz, const double *__restrict__ qc,
+    const SysDev *__restrict__ sys, const Seg *__restrict__ seg, uint32_t nseg,
+    const uint32_t *__restrict__ atom_cell, const uint32_t *__restrict__ rank,
+    const uint32_t *__restrict__ hist, const uint32_t *__restrict__ start,
+    uint32_t *__restrict__ sorted_idx, uint32_t *__restrict__ cell_sorted,
+    double4 *__restrict__ atoms64, float4 *__restrict__ atoms32,
+    const uint32_t *__restrict__ ag_start, uint2 *__restrict__ ag,
+    uint32_t blk) {
+  const Seg g = seg[run_of(seg, nseg, blk)];
+  const SysDev &d = sys[g.sys];
+  const bool groups = ag && d.agtile;
+  bin_scatter_body<MIXED>(
+      rx + d.atom0, ry + d.atom0, rz + d.atom0, qc + d.atom0, d.natom,
+      atom_cell + d.atom0, rank + d.atom0, hist + d.hist0, start + d.cell0,
+      d.ncell, d.ny, d.nz, d.h, sorted_idx, cell_sorted, atoms64, atoms32,
+      d.far_d, (blk - g.blk0) * kBatchBlock + threadIdx.x, d.atom0,
+      groups ? ag_start + d.cell0 : nullptr, groups ? ag + d.ag0 : nullptr);
+}
+
+template <bool MIXED>
+__device__ __forceinline__ void rmt_unit(
+    const void *__restrict__ Sv, const double4 *__restrict__ k64,
+    const uint32_t *__restrict__ group, const uint32_t *__restrict__ r_in,
+    const uint32_t *__restrict__ r_out, const SysDev *__restrict__ sys,
+    const Seg *__restrict__ seg, uint32_t nseg, void *__restrict__ R,
+    uint32_t blk) {
+  const Seg g = seg[run_of(seg, nseg, blk)];
+  const SysDev &d = sys[g.sys];
+  const void *Ss;
+  void *Rs;
+  if (MIXED) {
+    Ss = static_cast<const float2 *>(Sv) + d.field0;
+    Rs = static_cast<float2 *>(R) + d.field0;
+  } else {
+    Ss = static_cast<const double2 *>(Sv) + d.field0;
+    Rs = static_cast<double2 *>(R) + d.field0;
+  }
+  rmt_lds_body<MIXED>(Ss, k64 + d.node0, group + d.node0, r_in + d.group0,
+                      r_out + d.group0, d.Kp, d.T, d.nx, d.ny, d.nz, d.h, Rs,
+                      blk - g.blk0);
+}
+
+// a run's workgroups are (split, atom tile) of one system and alpha group
+template <int NR>
+__device__ __forceinline__ void lr_mixed_unit(
+    const float4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
+    const LrTasks &tk, const float2 *__restrict__ R,
+    const SysDev *__restrict__ sys, const Seg &g, uint32_t blk,
+    double4 *__restrict__ partial) {
+  const SysDev &d = sys[g.sys];
+  const uint32_t local = blk - g.blk0;
+  const uint32_t split = g.a + local / d.ntile;
+  lr_mixed_nr_body<NR>(atoms + d.atom0, cell_sorted + d.atom0, tk,
+                       R + d.field0, d.Kp, d.natom, local % d.ntile, split,
+                       split - d.split0, partial + d.partial0);
+}
+
+// a run's workgroups are (split, atom-group tile) of one system and group
+template <int NR>
+__device__ __forceinline__ void lr_ag_unit(
+    const float4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
+    const uint2 *__restrict__ ag, const LrTasks &tk,
+    const float2 *__restrict__ R, const SysDev *__restrict__ sys, const Seg &g,
+    uint32_t blk, double4 *__restrict__ partial) {
+  const SysDev &d = sys[g.sys];
+  const uint32_t local = blk - g.blk0;
+  const uint32_t split = g.a + local / d.agtile;
+  lr_ag_body<NR>(atoms + d.atom0, cell_sorted + d.atom0, ag + d.ag0, d.atom0,
+                 tk, R + d.field0, d.Kp, d.natom, local % d.agtile, split,
+                 split - d.split0, partial + d.partial0);
+}
+
+__device__ __forceinline__ void lr_reduce_unit(
+    const double4 *__restrict__ partial, const SysDev *__restrict__ sys,
+    const Seg *__restrict__ seg, uint32_t nseg,
+    const double4 *__restrict__ near, double *__restrict__ fx,
+    double *__restrict__ fy, double *__restrict__ fz, double *__restrict__ en,
+    uint32_t blk) {
+  const Seg g = seg[run_of(seg, nseg, blk)];
+  const SysDev &d = sys[g.sys];
+  lr_reduce_body(partial + d.partial0, d.nsplit, d.natom,
+                 near ? near + d.atom0 : nullptr, fx + d.atom0, fy + d.atom0,
+                 fz + d.atom0, en + d.atom0,
+                 (blk - g.blk0) * kBatchBlock + threadIdx.x);
+}
+
+template <bool MIXED, bool STORE>
+__device__ __forceinline__ void sr_unit(
+    const double4 *__restrict__ atoms, const uint32_t *__restrict__ start,
+    const SysDev *__restrict__ sys, const Seg *__restrict__ seg, uint32_t nseg,
+    double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
+    double *__restrict__ en, double4 *__restrict__ out4, uint32_t blk,
+    double4 *__restrict__ tile) {
+  const Seg g = seg[run_of(seg, nseg, blk)];
+  const SysDev &d = sys[g.sys];
+  sr_body<MIXED, STORE>(atoms, start + d.cell0, d.nx, d.ny, d.nz, fx, fy, fz,
+                        en, out4, blk - g.blk0, tile);
+}
+
 template <typename CT>
-__global__ __launch_bounds__(1024) void bin_chunk_batch_kernel(
+__global__ __launch_bounds__(kBinChunk) void bin_chunk_batch_kernel(
     const double *__restrict__ rx, const double *__restrict__ ry,
     const double *__restrict__ rz, const SysDev *__restrict__ sys,
     const Seg *__restrict__ seg, uint32_t nseg, uint32_t *__restrict__ atom_cell,
     uint32_t *__restrict__ rank, uint32_t *__restrict__ hist) {
-  const Seg g = seg[run_of(seg, nseg, blockIdx.x)];
-  const SysDev &d = sys[g.sys];
-  bin_chunk_body<CT>(rx + d.atom0, ry + d.atom0, rz + d.atom0, d.natom,
-                     static_cast<CT>(d.inv_cell), d.nx, d.ny, d.nz, d.ncell,
-                     atom_cell + d.atom0, rank + d.atom0, hist + d.hist0,
-                     blockIdx.x - g.blk0);
+  __shared__ unsigned long long keys[kBinChunk];
+  bin_chunk_unit<CT, kBinChunk, false>(rx, ry, rz, sys, seg, nseg, atom_cell,
+                                       rank, hist, blockIdx.x, keys);
 }
 
-__global__ void hist_offsets_batch_kernel(uint32_t *__restrict__ hist,
-                                          const SysDev *__restrict__ sys,
-                                          const Seg *__restrict__ seg,
-                                          uint32_t nseg,
-                                          uint32_t *__restrict__ count) {
-  const Seg g = seg[run_of(seg, nseg, blockIdx.x)];
-  const SysDev &d = sys[g.sys];
-  hist_offsets_body(hist + d.hist0, d.nchunk, d.ncell, count + d.cell0,
-                    (blockIdx.x - g.blk0) * blockDim.x + threadIdx.x);
+__global__ __launch_bounds__(kBatchBlock) void hist_offsets_batch_kernel(
+    uint32_t *__restrict__ hist, const SysDev *__restrict__ sys,
+    const Seg *__restrict__ seg, uint32_t nseg, uint32_t *__restrict__ count) {
+  hist_offsets_unit(hist, sys, seg, nseg, count, blockIdx.x);
 }
 
 __global__ __launch_bounds__(1024) void cell_scan_batch_kernel(
     const uint32_t *__restrict__ count, const SysDev *__restrict__ sys,
     uint32_t nsys, uint32_t *__restrict__ start, uint32_t *__restrict__ maxc,
     uint32_t *__restrict__ ag_start, uint2 *__restrict__ ag) {
-  const SysDev &d = sys[blockIdx.x];
-  const bool groups = ag && d.agtile;
-  // start[cell0 + ncell] is the next system's first slot; the last system
-  // writes the batch's end
-  cell_scan_body(count + d.cell0, d.ncell, start + d.cell0, maxc + blockIdx.x,
-                 d.atom0, blockIdx.x + 1 == nsys,
-                 groups ? ag_start + d.cell0 : nullptr,
-                 groups ? ag + d.ag0 : nullptr, d.agtile * kAgPerBlock);
+  __shared__ uint32_t lds[3 * 1024];
+  cell_scan_unit<1024>(count, sys, nsys, start, maxc, ag_start, ag,
+                       blockIdx.x, lds);
 }
 
 template <bool MIXED>
-__global__ void bin_scatter_batch_kernel(
+__global__ __launch_bounds__(kBatchBlock) void bin_scatter_batch_kernel(
     const double *__restrict__ rx, const double *__restrict__ ry,
     const double *__restrict__ rz, const double *__restrict__ qc,
     const SysDev *__restrict__ sys, const Seg *__restrict__ seg, uint32_t nseg,
@@ -2225,15 +2387,9 @@ __global__ void bin_scatter_batch_kernel(
     uint32_t *__restrict__ sorted_idx, uint32_t *__restrict__ cell_sorted,
     double4 *__restrict__ atoms64, float4 *__restrict__ atoms32,
     const uint32_t *__restrict__ ag_start, uint2 *__restrict__ ag) {
-  const Seg g = seg[run_of(seg, nseg, blockIdx.x)];
-  const SysDev &d = sys[g.sys];
-  const bool groups = ag && d.agtile;
-  bin_scatter_body<MIXED>(
-      rx + d.atom0, ry + d.atom0, rz + d.atom0, qc + d.atom0, d.natom,
-      atom_cell + d.atom0, rank + d.atom0, hist + d.hist0, start + d.cell0,
-      d.ncell, d.ny, d.nz, d.h, sorted_idx, cell_sorted, atoms64, atoms32,
-      d.far_d, (blockIdx.x - g.blk0) * blockDim.x + threadIdx.x, d.atom0,
-      groups ? ag_start + d.cell0 : nullptr, groups ? ag + d.ag0 : nullptr);
+  bin_scatter_unit<MIXED>(rx, ry, rz, qc, sys, seg, nseg, atom_cell, rank,
+                          hist, start, sorted_idx, cell_sorted, atoms64,
+                          atoms32, ag_start, ag, blockIdx.x);
 }
 
 __global__ __launch_bounds__(kSfBlock) void sf_f64_batch_kernel(
@@ -2290,21 +2446,8 @@ __global__ __launch_bounds__(kRmtBlock) void rmt_lds_batch_kernel(
     const uint32_t *__restrict__ group, const uint32_t *__restrict__ r_in,
     const uint32_t *__restrict__ r_out, const SysDev *__restrict__ sys,
     const Seg *__restrict__ seg, uint32_t nseg, void *__restrict__ R) {
-  const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
-  const Seg g = seg[run_of(seg, nseg, wg)];
-  const SysDev &d = sys[g.sys];
-  const void *Ss;
-  void *Rs;
-  if (MIXED) {
-    Ss = static_cast<const float2 *>(Sv) + d.field0;
-    Rs = static_cast<float2 *>(R) + d.field0;
-  } else {
-    Ss = static_cast<const double2 *>(Sv) + d.field0;
-    Rs = static_cast<double2 *>(R) + d.field0;
-  }
-  rmt_lds_body<MIXED>(Ss, k64 + d.node0, group + d.node0, r_in + d.group0,
-                      r_out + d.group0, d.Kp, d.T, d.nx, d.ny, d.nz, d.h, Rs,
-                      wg - g.blk0);
+  rmt_unit<MIXED>(Sv, k64, group, r_in, r_out, sys, seg, nseg, R,
+                  xcd_remap(blockIdx.x, gridDim.x));
 }
 
 __global__ __launch_bounds__(kLrBlock) void lr_f64_batch_kernel(
@@ -2329,22 +2472,16 @@ __global__ __launch_bounds__(kLrBlock) void lr_mixed_batch_kernel(
     const Seg *__restrict__ seg, uint32_t nseg,
     double4 *__restrict__ partial) {
   const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
-  const Seg g = seg[run_of(seg, nseg, wg)];
-  const SysDev &d = sys[g.sys];
-  const uint32_t local = wg - g.blk0;
-  const uint32_t split = g.a + local / d.ntile;
-  lr_mixed_nr_body<NR>(atoms + d.atom0, cell_sorted + d.atom0, tk,
-                       R + d.field0, d.Kp, d.natom, local % d.ntile, split,
-                       split - d.split0, partial + d.partial0);
+  lr_mixed_unit<NR>(atoms, cell_sorted, tk, R, sys, seg[run_of(seg, nseg, wg)],
+                    wg, partial);
 }
 
 typedef void (*LrBatchKernel)(const float4 *, const uint32_t *, LrTasks,
                               const float2 *, const SysDev *, const Seg *,
                               uint32_t, double4 *);
 
-// The batched back-projection over atom groups: a run's workgroups are
-// (split, atom-group tile) of one system and alpha group.  No register cap:
-// on the batch's mix of small systems it cost 2-3 %.
+// The batched back-projection over atom groups.  No register cap: on the
+// batch's mix of small systems it cost 2-3 %.
 template <int NR>
 __global__ __launch_bounds__(kLrBlock) void lr_ag_batch_kernel(
     const float4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
@@ -2352,13 +2489,8 @@ __global__ __launch_bounds__(kLrBlock) void lr_ag_batch_kernel(
     const SysDev *__restrict__ sys, const Seg *__restrict__ seg, uint32_t nseg,
     double4 *__restrict__ partial) {
   const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
-  const Seg g = seg[run_of(seg, nseg, wg)];
-  const SysDev &d = sys[g.sys];
-  const uint32_t local = wg - g.blk0;
-  const uint32_t split = g.a + local / d.agtile;
-  lr_ag_body<NR>(atoms + d.atom0, cell_sorted + d.atom0, ag + d.ag0, d.atom0,
-                 tk, R + d.field0, d.Kp, d.natom, local % d.agtile, split,
-                 split - d.split0, partial + d.partial0);
+  lr_ag_unit<NR>(atoms, cell_sorted, ag, tk, R, sys,
+                 seg[run_of(seg, nseg, wg)], wg, partial);
 }
 
 typedef void (*LrAgBatchKernel)(const float4 *, const uint32_t *,
@@ -2385,21 +2517,13 @@ constexpr auto lr_batch_table(std::integer_sequence<int, N...>) {
 const auto kLrBatchKernels =
     lr_batch_table(std::make_integer_sequence<int, kLrMaxUnrolled + 1>{});
 
-__global__ void lr_reduce_batch_kernel(const double4 *__restrict__ partial,
-                                       const SysDev *__restrict__ sys,
-                                       const Seg *__restrict__ seg,
-                                       uint32_t nseg,
-                                       const double4 *__restrict__ near,
-                                       double *__restrict__ fx,
-                                       double *__restrict__ fy,
-                                       double *__restrict__ fz,
-                                       double *__restrict__ en) {
-  const Seg g = seg[run_of(seg, nseg, blockIdx.x)];
-  const SysDev &d = sys[g.sys];
-  lr_reduce_body(partial + d.partial0, d.nsplit, d.natom,
-                 near ? near + d.atom0 : nullptr, fx + d.atom0, fy + d.atom0,
-                 fz + d.atom0, en + d.atom0,
-                 (blockIdx.x - g.blk0) * blockDim.x + threadIdx.x);
+__global__ __launch_bounds__(kBatchBlock) void lr_reduce_batch_kernel(
+    const double4 *__restrict__ partial, const SysDev *__restrict__ sys,
+    const Seg *__restrict__ seg, uint32_t nseg,
+    const double4 *__restrict__ near, double *__restrict__ fx,
+    double *__restrict__ fy, double *__restrict__ fz,
+    double *__restrict__ en) {
+  lr_reduce_unit(partial, sys, seg, nseg, near, fx, fy, fz, en, blockIdx.x);
 }
 
 template <bool MIXED, bool STORE>
@@ -2408,11 +2532,230 @@ __global__ __launch_bounds__(kSrBlock) void sr_batch_kernel(
     const SysDev *__restrict__ sys, const Seg *__restrict__ seg, uint32_t nseg,
     double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz,
     double *__restrict__ en, double4 *__restrict__ out4) {
-  const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
-  const Seg g = seg[run_of(seg, nseg, wg)];
+  __shared__ double4 tile[kSrBlock];
+  sr_unit<MIXED, STORE>(atoms, start, sys, seg, nseg, fx, fy, fz, en, out4,
+                        xcd_remap(blockIdx.x, gridDim.x), tile);
+}
+
+// ---------------------------------------------------------------------------
+// The mixed batched step as one persistent launch (FusedStep): every
+// workgroup walks the units of each phase (binning chunks; histogram
+// offsets; cell scans; scatter; S(k) and near field; shell sums;
+// back-projection; reduction) with stride gridDim.x, and a grid-wide barrier
+// separates the phases.  All gridDim.x workgroups must be resident at once
+// (fused_workgroups); a barrier that waits longer than f.timeout wall-clock
+// ticks marks the step failed (f.sync[1] and the host flag f.fail) and every
+// workgroup leaves, so the launch always ends.
+
+static_assert(kFusedBlock == kSrBlock && kFusedBlock == kRmtBlock &&
+                  kFusedBlock == kLrBlock && kFusedBlock == kBatchBlock,
+              "the fused step runs every body with one workgroup size");
+static_assert(kFusedBlock % kSfDirs == 0 &&
+                  kFusedSfCells == kFusedBlock / kSfDirs,
+              "one S(k) wave per cell");
+
+// Arrive at barrier n (1, 2, ...).  Every wave waits until its stores have
+// reached the L2; then one thread makes the workgroup's writes visible at
+// agent scope (a release fence: the XCD's L2 writes its dirty lines back),
+// counts the arrival, waits for n * gridDim.x of them, and drops stale lines
+// of the CU's L1 and the XCD's L2 (an acquire fence) before the workgroup
+// goes on.  False: the step has failed.
+__device__ __forceinline__ bool fused_barrier(uint32_t *__restrict__ sync,
+                                              uint32_t *__restrict__ fail,
+                                              long long timeout, uint32_t n,
+                                              uint32_t *__restrict__ ok) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    atomicAdd(sync, 1u);
+    const uint32_t target = n * gridDim.x;
+    const long long t0 = wall_clock64();
+    uint32_t good = 1u;
+    while (__hip_atomic_load(sync, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(sync + 1, __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+          wall_clock64() - t0 > timeout) {
+        __hip_atomic_store(sync + 1, 1u, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(fail, 1u, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+        good = 0u;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    *ok = good;
+  }
+  __syncthreads();
+  return *ok != 0u;
+}
+
+// S(k) of unit (cell group of kFusedSfCells cells, direction chunk): the
+// unrolled body for radial counts up to kFusedMaxNr, else the any-count body
+template <int N>
+__device__ __forceinline__ void sf_fused_nr(
+    uint32_t nr, const float4 *__restrict__ atoms,
+    const uint32_t *__restrict__ start, const uint32_t *__restrict__ cell_sorted,
+    const LrTasks &tk, uint32_t task0, uint32_t ntask, uint32_t ncell,
+    float2 *__restrict__ S, uint32_t cell0, uint32_t chunk) {
+  if constexpr (N == 0) {
+    sf_mixed_any_body(atoms, start, cell_sorted, tk, task0, ntask, ncell, S,
+                      cell0, chunk);
+  } else {
+    static_assert(sf_cells(N) == 1, "the fused step stores S(k) per wave");
+    if (nr == static_cast<uint32_t>(N))
+      sf_mixed_nr_body<N>(atoms, start, cell_sorted, tk, task0, ntask, ncell,
+                          S, cell0, chunk);
+    else
+      sf_fused_nr<N - 1>(nr, atoms, start, cell_sorted, tk, task0, ntask,
+                         ncell, S, cell0, chunk);
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void lr_fused_nr(
+    uint32_t nr, const float4 *__restrict__ atoms,
+    const uint32_t *__restrict__ cell_sorted, const uint2 *__restrict__ ag,
+    const LrTasks &tk, const float2 *__restrict__ R,
+    const SysDev *__restrict__ sys, const Seg &g, uint32_t blk,
+    double4 *__restrict__ partial) {
+  if constexpr (N == 0) {
+    lr_mixed_unit<0>(atoms, cell_sorted, tk, R, sys, g, blk, partial);
+  } else {
+    if (nr == static_cast<uint32_t>(N))
+      lr_ag_unit<N>(atoms, cell_sorted, ag, tk, R, sys, g, blk, partial);
+    else
+      lr_fused_nr<N - 1>(nr, atoms, cell_sorted, ag, tk, R, sys, g, blk,
+                         partial);
+  }
+}
+
+// S(k) unit u of the fused plan: kFusedSfCells cells of one direction
+// chunk of one system's alpha group
+__device__ __forceinline__ void sf_fused_unit(
+    const float4 *__restrict__ atoms32, const uint32_t *__restrict__ start,
+    const uint32_t *__restrict__ cell_sorted, const LrTasks &tk,
+    const SysDev *__restrict__ sys, const Seg *__restrict__ seg,
+    uint32_t nseg, float2 *__restrict__ S, uint32_t u) {
+  const Seg g = seg[run_of(seg, nseg, u)];
   const SysDev &d = sys[g.sys];
-  sr_body<MIXED, STORE>(atoms, start + d.cell0, d.nx, d.ny, d.nz, fx, fy, fz,
-                        en, out4, wg - g.blk0);
+  const uint32_t local = u - g.blk0;
+  const uint32_t ncg = (d.ncell + kFusedSfCells - 1) / kFusedSfCells;
+  const uint32_t nr = tk.nr[g.a];
+  sf_fused_nr<kFusedMaxNr>(nr <= kFusedMaxNr ? nr : 0u, atoms32,
+                           start + d.cell0, cell_sorted, tk, g.a, g.b, d.ncell,
+                           S + d.field0, (local % ncg) * kFusedSfCells,
+                           local / ncg);
+}
+
+// back-projection unit u of the fused plan
+__device__ __forceinline__ void lr_fused_unit(
+    const float4 *__restrict__ atoms32,
+    const uint32_t *__restrict__ cell_sorted, const uint2 *__restrict__ ag,
+    const LrTasks &tk, const float2 *__restrict__ R,
+    const SysDev *__restrict__ sys, const Seg *__restrict__ seg,
+    uint32_t nseg, double4 *__restrict__ partial, uint32_t u) {
+  const Seg g = seg[run_of(seg, nseg, u)];
+  const uint32_t nr = tk.nr[tk.split[g.a]];
+  // the plan gives atom-group tiles exactly to these runs
+  const bool unrolled = nr <= kFusedMaxNr && sys[g.sys].agtile != 0u;
+  lr_fused_nr<kFusedMaxNr>(unrolled ? nr : 0u, atoms32, cell_sorted, ag, tk, R,
+                           sys, g, u, partial);
+}
+
+// One launch per stage for every radial count (option "fused" 3): the fused
+// plan's S(k) and back-projection units as ordinary launches.
+__global__ __launch_bounds__(kFusedBlock) void sf_all_batch_kernel(
+    const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
+    const uint32_t *__restrict__ cell_sorted, LrTasks tk,
+    const SysDev *__restrict__ sys, const Seg *__restrict__ seg,
+    uint32_t nseg, float2 *__restrict__ S) {
+  sf_fused_unit(atoms, start, cell_sorted, tk, sys, seg, nseg, S,
+                xcd_remap(blockIdx.x, gridDim.x));
+}
+
+__global__ __launch_bounds__(kFusedBlock)
+__attribute__((amdgpu_waves_per_eu(3)))
+void lr_all_batch_kernel(
+    const float4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
+    const uint2 *__restrict__ ag, LrTasks tk, const float2 *__restrict__ R,
+    const SysDev *__restrict__ sys, const Seg *__restrict__ seg, uint32_t nseg,
+    double4 *__restrict__ partial) {
+  lr_fused_unit(atoms, cell_sorted, ag, tk, R, sys, seg, nseg, partial,
+                xcd_remap(blockIdx.x, gridDim.x));
+}
+
+// WAVES: waves per SIMD the registers are held to (2: no spills at
+// kFusedMaxNr 28; 3: spills in the largest unrolled bodies)
+template <int WAVES>
+__global__ __launch_bounds__(kFusedBlock)
+__attribute__((amdgpu_waves_per_eu(WAVES)))
+void step_fused_kernel(FusedStep f) {
+  extern __shared__ double2 fused_lds[];
+  __shared__ uint32_t ok;
+  const uint32_t W = gridDim.x;
+  // the workgroups of one XCD take consecutive units of every phase
+  const uint32_t v = xcd_remap(blockIdx.x, W);
+  const LrTasks tk{f.tk.node0, f.tk.nr,      f.tk.koff,    f.tk.dir64,
+                   f.tk.dir32, f.tk.kappa64, f.tk.kappa32, f.tk.split};
+  uint32_t n = 0;
+  auto barrier = [&] {
+    return fused_barrier(f.sync, f.fail, f.timeout, ++n, &ok);
+  };
+  // binning chunks (each clears its own histogram row first)
+  for (uint32_t u = v; u < f.bin.nblk; u += W) {
+    __syncthreads();
+    bin_chunk_unit<float, kFusedBlock, true>(
+        f.rx, f.ry, f.rz, f.sys, f.bin.seg, f.bin.nseg, f.atom_cell, f.rank,
+        f.hist, u, reinterpret_cast<unsigned long long *>(fused_lds));
+  }
+  if (!barrier()) return;
+  for (uint32_t u = v; u < f.hist_p.nblk; u += W)
+    hist_offsets_unit(f.hist, f.sys, f.hist_p.seg, f.hist_p.nseg, f.count, u);
+  if (!barrier()) return;
+  for (uint32_t u = v; u < f.nsys; u += W) {
+    __syncthreads();
+    cell_scan_unit<kFusedBlock>(f.count, f.sys, f.nsys, f.start, f.maxc,
+                                f.ag_start, f.ag, u,
+                                reinterpret_cast<uint32_t *>(fused_lds));
+  }
+  if (!barrier()) return;
+  for (uint32_t u = v; u < f.scatter.nblk; u += W)
+    bin_scatter_unit<true>(f.rx, f.ry, f.rz, f.qc, f.sys, f.scatter.seg,
+                           f.scatter.nseg, f.atom_cell, f.rank, f.hist,
+                           f.start, f.sorted_idx, f.cell_sorted, f.atoms64,
+                           f.atoms32, f.ag_start, f.ag, u);
+  if (!barrier()) return;
+  // S(k) units first (the longest), then the near field's cells
+  for (uint32_t u = v; u < f.sf.nblk + f.sr.nblk; u += W) {
+    if (u < f.sf.nblk) {
+      sf_fused_unit(f.atoms32, f.start, f.cell_sorted, tk, f.sys, f.sf.seg,
+                    f.sf.nseg, f.S, u);
+    } else {
+      __syncthreads();
+      sr_unit<true, true>(f.atoms64, f.start, f.sys, f.sr.seg, f.sr.nseg,
+                          nullptr, nullptr, nullptr, nullptr, f.near4,
+                          u - f.sf.nblk,
+                          reinterpret_cast<double4 *>(fused_lds));
+    }
+  }
+  if (!barrier()) return;
+  for (uint32_t u = v; u < f.rmt.nblk; u += W) {
+    __syncthreads();
+    rmt_unit<true>(f.S, f.k64, f.group, f.r_in, f.r_out, f.sys, f.rmt.seg,
+                   f.rmt.nseg, f.R, u);
+  }
+  if (!barrier()) return;
+  for (uint32_t u = v; u < f.lr.nblk; u += W)
+    lr_fused_unit(f.atoms32, f.cell_sorted, f.ag, tk, f.R, f.sys, f.lr.seg,
+                  f.lr.nseg, f.partial, u);
+  if (!barrier()) return;
+  for (uint32_t u = v; u < f.reduce.nblk; u += W)
+    lr_reduce_unit(f.partial, f.sys, f.reduce.seg, f.reduce.nseg, f.near4,
+                   f.fx, f.fy, f.fz, f.en, u);
 }
 
 // run blockIdx.x, copied by gridDim.y workgroups
@@ -2451,7 +2794,9 @@ void prepare_kernels(size_t lds_limit) {
   const void *fns[] = {reinterpret_cast<const void *>(&rmt_lds_kernel<true>),
                        reinterpret_cast<const void *>(&rmt_lds_kernel<false>),
                        reinterpret_cast<const void *>(&rmt_lds_batch_kernel<true>),
-                       reinterpret_cast<const void *>(&rmt_lds_batch_kernel<false>)};
+                       reinterpret_cast<const void *>(&rmt_lds_batch_kernel<false>),
+                       reinterpret_cast<const void *>(&step_fused_kernel<2>),
+                       reinterpret_cast<const void *>(&step_fused_kernel<3>)};
   for (const void *f : fns) {
     const hipError_t e = hipFuncSetAttribute(
         f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
@@ -2907,6 +3252,76 @@ void sr_batch(bool mixed, const double4 *atoms, const uint32_t *start,
       sr_batch_kernel<false, false><<<L.nblk, kSrBlock, 0, s>>>(
           atoms, start, sys, L.seg, L.nseg, fx, fy, fz, en, out4);
   }
+}
+
+size_t fused_lds_bytes(size_t rmt_lds) {
+  // binning keys, cell-scan words and near-field tiles share the space
+  const size_t other = std::max<size_t>(
+      {kBinChunk * sizeof(unsigned long long), 3 * kFusedBlock * 4,
+       kSrBlock * sizeof(double4)});
+  return std::max(rmt_lds, other);
+}
+
+static const void *fused_kernel(int variant) {
+  return variant == 2 ? reinterpret_cast<const void *>(&step_fused_kernel<3>)
+                      : reinterpret_cast<const void *>(&step_fused_kernel<2>);
+}
+
+uint32_t fused_workgroups(size_t lds, int variant) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, fused_kernel(variant), static_cast<int>(kFusedBlock),
+          lds) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return per_cu > 0 ? static_cast<uint32_t>(per_cu) * g_cu_count : 0u;
+}
+
+long long wall_clock_rate() {
+  int dev = 0, khz = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) !=
+          hipSuccess ||
+      khz <= 0) {
+    (void)hipGetLastError();
+    return 100000000LL;  // 100 MHz
+  }
+  return static_cast<long long>(khz) * 1000;
+}
+
+void sf_all_batch(const void *atoms, const uint32_t *start,
+                  const uint32_t *cell_sorted, const LrTaskArgs &ta,
+                  const SysDev *sys, const BatchLaunch &L, void *S,
+                  hipStream_t s) {
+  if (L.nblk == 0) return;
+  const LrTasks tk{ta.node0, ta.nr,      ta.koff,   ta.dir64,
+                   ta.dir32, ta.kappa64, ta.kappa32, ta.split};
+  sf_all_batch_kernel<<<L.nblk, kFusedBlock, 0, s>>>(
+      static_cast<const float4 *>(atoms), start, cell_sorted, tk, sys, L.seg,
+      L.nseg, static_cast<float2 *>(S));
+}
+
+void lr_all_batch(const void *atoms, const uint32_t *cell_sorted,
+                  const LrTaskArgs &ta, const SysDev *sys,
+                  const BatchLaunch &L, const void *R, double4 *partial,
+                  hipStream_t s, const uint2 *ag) {
+  if (L.nblk == 0) return;
+  const LrTasks tk{ta.node0, ta.nr,      ta.koff,   ta.dir64,
+                   ta.dir32, ta.kappa64, ta.kappa32, ta.split};
+  lr_all_batch_kernel<<<L.nblk, kFusedBlock, 0, s>>>(
+      static_cast<const float4 *>(atoms), cell_sorted, ag, tk,
+      static_cast<const float2 *>(R), sys, L.seg, L.nseg, partial);
+}
+
+void step_fused(const FusedStep &f, uint32_t nwg, size_t lds, hipStream_t s,
+                int variant) {
+  // a fill kernel, not a memset: the step may be captured in a graph
+  fill_u32(f.sync, 0u, 2, s);
+  if (variant == 2)
+    step_fused_kernel<3><<<nwg, kFusedBlock, lds, s>>>(f);
+  else
+    step_fused_kernel<2><<<nwg, kFusedBlock, lds, s>>>(f);
 }
 
 void copy_runs(const CopyRun *runs, uint32_t nruns, uint32_t max_n,

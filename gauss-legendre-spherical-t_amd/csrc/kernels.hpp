@@ -336,6 +336,69 @@ void sr_batch(bool mixed, const double4 *atoms, const uint32_t *start,
               const SysDev *sys, const BatchLaunch &L, double *fx, double *fy,
               double *fz, double *en, double4 *out4, hipStream_t s);
 
+// ---- the mixed batched step as one persistent launch ---------------------------
+// Every phase of the step (binning, histogram offsets, cell scans, scatter,
+// S(k) with the near field, shell sums, back-projection, reduction) runs in
+// one kernel of gridDim.x resident workgroups, each walking the phase's
+// units with stride gridDim.x; grid-wide barriers separate the phases.
+// Units are the workgroups of the per-stage launches except for S(k), whose
+// units hold kFusedSfCells cells (one wave each) of one direction chunk.
+// S(k) and the back-projection of radial counts up to kFusedMaxNr run the
+// unrolled bodies (over atom groups), others the any-count bodies; the plan
+// of `lr` counts agtile workgroups per split exactly for the unrolled runs.
+constexpr uint32_t kFusedBlock = 256;
+constexpr uint32_t kFusedSfCells = 4;
+#ifndef GLST_FUSED_MAX_NR
+#define GLST_FUSED_MAX_NR 28
+#endif
+constexpr int kFusedMaxNr = GLST_FUSED_MAX_NR;
+struct FusedStep {
+  const double *rx, *ry, *rz, *qc;
+  const SysDev *sys;
+  uint32_t nsys;
+  BatchLaunch bin, hist_p, scatter, sf, sr, rmt, lr, reduce;
+  uint32_t *atom_cell, *rank, *hist, *count, *start, *maxc;
+  uint32_t *sorted_idx, *cell_sorted;
+  double4 *atoms64;
+  float4 *atoms32;
+  uint32_t *ag_start;
+  uint2 *ag;
+  float2 *S, *R;
+  double4 *partial, *near4;
+  double *fx, *fy, *fz, *en;
+  const double4 *k64;
+  const uint32_t *group, *r_in, *r_out;
+  LrTaskArgs tk;
+  uint32_t *sync;     // [0] barrier arrivals, [1] failure (cleared at launch)
+  uint32_t *fail;     // host-mapped failure flag (set on a barrier time-out)
+  long long timeout;  // wall-clock ticks a barrier may wait
+};
+// The fused plan's S(k) and back-projection units (FusedStep::sf, lr) as
+// one ordinary launch per stage for every radial count (mixed path).
+void sf_all_batch(const void *atoms, const uint32_t *start,
+                  const uint32_t *cell_sorted, const LrTaskArgs &tasks,
+                  const SysDev *sys, const BatchLaunch &L, void *S,
+                  hipStream_t s);
+void lr_all_batch(const void *atoms, const uint32_t *cell_sorted,
+                  const LrTaskArgs &tasks, const SysDev *sys,
+                  const BatchLaunch &L, const void *R, double4 *partial,
+                  hipStream_t s, const uint2 *ag);
+// Dynamic LDS of the fused step for a batch whose shell sums need
+// rmt_lds bytes.
+size_t fused_lds_bytes(size_t rmt_lds);
+// Variant 1 holds the kernel to 2 waves per SIMD (252 VGPRs, no spills),
+// variant 2 to 3 waves (168 VGPRs; the largest unrolled bodies spill).
+// Workgroups of the fused step that are resident at once on the current
+// device with this dynamic LDS (0 if the query fails).
+uint32_t fused_workgroups(size_t lds, int variant);
+// Wall-clock ticks per second of the current device.
+long long wall_clock_rate();
+// Clears f.sync and launches the step with nwg workgroups (at most
+// fused_workgroups(lds, variant), less what other persistent launches that
+// may run beside it hold).
+void step_fused(const FusedStep &f, uint32_t nwg, size_t lds, hipStream_t s,
+                int variant);
+
 // ---- copies of many arrays in one launch (serving loop) -----------------------
 struct CopyRun {
   const double *src;

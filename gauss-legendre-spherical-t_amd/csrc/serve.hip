@@ -125,6 +125,9 @@ struct glst_server {
       // a composition run twice in a row (resident systems) is captured as
       // a HIP graph and replayed from then on
       sl->batch.graph_mode = 2;
+      // with the fused step, both slots' persistent launches must be
+      // resident at once: each takes half of the device
+      sl->batch.fused_share = kSlots;
       GLST_HIP(hipEventCreateWithFlags(&sl->done, hipEventDisableTiming));
       slots.push_back(std::move(sl));
     }
@@ -264,6 +267,7 @@ struct glst_server {
   }
 
   uint32_t retire(Slot &sl) {
+    sl.batch.check_failed();
     for (uint64_t t : sl.tickets) done.push_back(t);
     const uint32_t n = static_cast<uint32_t>(sl.tickets.size());
     stats.completed += n;
@@ -418,6 +422,31 @@ glst_status_t glst_server_get_stats(glst_server_t srv,
   if (!stats) return GLST_ERR_INVALID;
   *stats = srv->stats;
   return GLST_OK;
+}
+
+glst_status_t glst_server_set_option(glst_server_t srv, const char *name,
+                                     int64_t value) {
+  NEED_SERVER(srv);
+  return run_guarded(&srv->last_error, [&] {
+    if (!name) throw ArgError("null option name");
+    const std::string n = name;
+    if (n == "fused") {
+      if (value < 0 || value > 3) throw ArgError("fused must be 0 .. 3");
+      for (auto &sl : srv->slots) {
+        GLST_HIP(hipStreamSynchronize(sl->batch.stream));
+        sl->batch.drop_graph();
+        sl->batch.fused = static_cast<int>(value);
+        // a slot is planned when it is next configured
+        if (!sl->composition.empty()) sl->batch.plan_fused();
+      }
+    } else if (n == "max_batch") {
+      if (value < 0 || value > (int64_t(1) << 31))
+        throw ArgError("max_batch must be in 0 .. 2^31");
+      srv->max_batch = static_cast<uint32_t>(value);
+    } else {
+      throw ArgError("unknown option \"" + n + "\"");
+    }
+  });
 }
 
 const char *glst_server_last_error(glst_server_t srv) {

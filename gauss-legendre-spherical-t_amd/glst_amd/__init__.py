@@ -165,6 +165,7 @@ def _load():
         "glst_server_completed": [P, P, u, POINTER(u)],
         "glst_server_pending": [P, POINTER(u)],
         "glst_server_get_stats": [P, P],
+        "glst_server_set_option": [P, c_char_p, ctypes.c_int64],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -686,6 +687,10 @@ class GLSTServer:
         n = c_uint32()
         self._c(_lib.glst_server_completed(self._h, _ptr(out), cap, byref(n)))
         return [int(v) for v in out[: n.value]]
+
+    def set_option(self, name: str, value: int):
+        """Server options ("fused", "max_batch"; glst_server_set_option)."""
+        self._c(_lib.glst_server_set_option(self._h, name.encode(), int(value)))
 
     def pending(self) -> int:
         """Requests not yet answered (queued or in flight)."""

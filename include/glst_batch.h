@@ -6,8 +6,9 @@
  * each with exactly the semantics of one glst_force<CT> (its own box, rcut,
  * cell grid, alpha groups and cubature; open boundaries; outputs accumulate),
  * and runs every stage of calc_ener_force for all of them in one launch per
- * stage (one per radial node count for S(k) and the back-projection), so
- * that small systems fill the GPU together.
+ * stage (one per radial node count for S(k) and the back-projection) or, on
+ * the mixed path, the whole step in one persistent launch (option "fused"),
+ * so that small systems fill the GPU together.
  *
  * Atoms are packed: system s owns atoms [offset_s, offset_s + natom_s) of
  * every input and output array, offset_s = natom_0 + ... + natom_{s-1}
@@ -76,11 +77,20 @@ glst_status_t glst_batch_zero_ef(glst_batch_t b);
  * whole batch (re-packs the batch); "graph" (default 0): 1 captures the
  * whole step as a HIP graph for the input arrays of the first step and
  * replays it while they are passed again, 2 captures only on the second
- * consecutive step with the same inputs. */
+ * consecutive step with the same inputs; "fused" (default 0; mixed path):
+ * 0 launches S(k) and the back-projection once per radial node count, 3
+ * once per stage for all counts, and 1 or 2 runs the whole step of all
+ * systems as one persistent launch whose phases (binning, S(k) with the near
+ * field, shell sums, back-projection, reduction) are separated by grid-wide
+ * barriers, 1 with the kernel's registers held to 2 waves per SIMD and 2 to
+ * 3 waves.  Its workgroups are as many as are resident at once; should a
+ * barrier nevertheless wait 10 s, the step ends and the next synchronising
+ * call returns GLST_ERR_STATE. */
 glst_status_t glst_batch_set_option(glst_batch_t b, const char *name,
                                     int64_t value);
-/* Autotuner: times the step on these inputs under each "overlap_near" and
- * "lr_blocks" candidate (`reps` steps each, one knob at a time), keeps the
+/* Autotuner: times the step on these inputs under each "fused" (mixed
+ * path), "overlap_near" and "lr_blocks" candidate (`reps` steps each, one
+ * knob at a time), keeps the
  * fastest and clears the accumulators; *best_ms (may be NULL) receives its
  * time per step. */
 glst_status_t glst_batch_autotune(glst_batch_t b, const double *d_rx,
