@@ -313,10 +313,7 @@ BatchLayout pack_batch(int ct,
       const uint32_t key = sg.nr <= 40 ? sg.nr : 0;
       Plan &pf = B.sf_nr[key];
       pf.nr = key;
-      const int nr = sg.nr <= static_cast<uint32_t>(dev::kSfMaxUnrolled)
-                         ? static_cast<int>(sg.nr)
-                         : 0;
-      pf.add(s, div_up(d.ncell, dev::sf_cells(nr)) * div_up(sg.ntask, 64),
+      pf.add(s, d.ncell * div_up(sg.ntask, 64),
              task0 + sg.task0, sg.ntask);
       Plan &pl = B.lr_nr[key];
       pl.nr = key;

@@ -714,7 +714,7 @@ struct glst_engine {
                             hipMemcpyDeviceToHost, stream));
   }
 
-  // Device field (R: c*Kp + k; S: node-pair tiles) to host arrays of
+  // Device field (R: c*Kp + k; S: node runs, sf_slot) to host arrays of
   // ncell x K in reference node order.  FP32 fields hold `fstride` floats
   // per element, the first two being (re, im).
   void field_to_host(const DeviceBuffer &buf, bool single, uint32_t fstride,
@@ -737,8 +737,11 @@ struct glst_engine {
     }
     for (uint32_t c = 0; c < grid.ncell; ++c)
       for (uint32_t k = 0; k < K; ++k) {
-        const size_t s = tiled ? dev::sf_slot(dev_of_ref[k], c, grid.ncell)
-                               : static_cast<size_t>(c) * Kp + dev_of_ref[k];
+        const uint32_t node = dev_of_ref[k];
+        const size_t s =
+            !tiled ? static_cast<size_t>(c) * Kp + node
+            : single ? dev::sf_slot<true>(node, c, grid.ncell)
+                     : dev::sf_slot<false>(node, c, grid.ncell);
         const size_t d = static_cast<size_t>(c) * K + k;
         if (re) re[d] = tmp[2 * s];
         if (im) im[d] = tmp[2 * s + 1];

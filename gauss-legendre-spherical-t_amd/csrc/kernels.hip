@@ -9,10 +9,11 @@
 //  * calc_sf (mixed): one wave per (cell, 64 directions of an alpha group),
 //    lane = direction, the group's radial magnitudes unrolled as registers;
 //    atom records are wave-uniform scalar loads, so the inner loop is VALU
-//    and transcendental work only.  S is stored in node-pair tiles.
-//  * sum_rmt_sf: a workgroup holds the whole cell grid of one or two nodes in
-//    LDS (read as one contiguous run of the node-pair tiles) and does the
-//    3-D summed-volume table and the shell gather there.
+//    and transcendental work only.  S is stored in node runs (sf_slot), so
+//    each wave writes whole runs of its cell.
+//  * sum_rmt_sf: a workgroup holds the whole cell grid of one or more nodes
+//    in LDS (read as z-lines of the node runs) and does the 3-D summed-volume
+//    table and the shell gather there.
 //  * calc_lr_ef: lane = atom, 64 consecutive cell-sorted atoms per wave (no
 //    lanes idle at cell boundaries), one launch per alpha group with its
 //    radial magnitudes unrolled; remote sums are per-lane loads that
@@ -393,7 +394,7 @@ constexpr uint32_t kSfBlock = 256;
 // The FP64 structure-factor kernel walks one z-column of cells per lane and
 // stores the running sum along z, i.e. the first pass of the summed-volume
 // table of src/glst_force.cu:702-724, so the shell-sum kernel starts at the y
-// pass.  S is stored in node-pair tiles (sf_slot).
+// pass.  S is stored in node runs (sf_slot).
 
 // FP64: S_c(k) = sum_j q_j (cos(k.r_j) - i sin(k.r_j)), absolute coordinates
 // (src/glst_force.cu:601-664).  blockIdx.x = z-column, blockIdx.y = node tile.
@@ -428,7 +429,7 @@ __device__ __forceinline__ void sf_f64_body(
     }
     pre += re0 + re1;
     pim += im0 + im1;
-    S[sf_slot(node, cell, ncell)] = make_double2(pre, pim);
+    S[sf_slot<false>(node, cell, ncell)] = make_double2(pre, pim);
   }
 }
 
@@ -858,7 +859,7 @@ __device__ __forceinline__ void rmt_compute(
 
 // z-lines loaded straight into the tables.  Line l of a tile is node
 // t = l & (T-1) and column q = l >> lg T (x = q / ny, y = q - x ny); its
-// slots sf_slot(node0 + t, q nz + z) are 2 apart.  The first LEN of them
+// slots sf_slot(node0 + t, q nz + z) are sf_run apart.  The first LEN of them
 // travel through registers (the persistent kernel fetches them while the
 // previous tile's shells are gathered), the rest of a longer line is read
 // when the line is stored.  Storing writes the running sums along z into
@@ -898,16 +899,16 @@ __device__ __forceinline__ void rmt_line_fetch(const void *__restrict__ Sv,
                                                const RmtGeom &g,
                                                uint32_t node0, uint32_t l,
                                                RmtLine<MIXED, LEN> &ln) {
-  constexpr uint32_t L = LEN;
+  constexpr uint32_t L = LEN, G = sf_run<MIXED>();
   const uint32_t t = l & (g.T - 1), q = l >> g.lgT;
-  const size_t base = sf_slot(node0 + t, q * g.nz, g.ncell);
+  const size_t base = sf_slot<MIXED>(node0 + t, q * g.nz, g.ncell);
 #pragma unroll
   for (uint32_t z = 0; z < L; ++z) {
     const uint32_t zz = z < g.nz ? z : g.nz - 1;  // clamped: in the line
     if constexpr (MIXED) {
-      ln.v[z] = static_cast<const float2 *>(Sv)[base + 2 * zz];
+      ln.v[z] = static_cast<const float2 *>(Sv)[base + G * zz];
     } else {
-      const double2 v = static_cast<const double2 *>(Sv)[base + 2 * zz];
+      const double2 v = static_cast<const double2 *>(Sv)[base + G * zz];
       ln.re[z] = v.x;
       ln.im[z] = v.y;
     }
@@ -921,11 +922,11 @@ __device__ __forceinline__ void rmt_line_put(double2 *P,
                                              const void *__restrict__ Sv,
                                              uint32_t node0, uint32_t l,
                                              const RmtLine<MIXED, LEN> &ln) {
-  constexpr uint32_t L = LEN;
+  constexpr uint32_t L = LEN, G = sf_run<MIXED>();
   const uint32_t t = l & (g.T - 1), q = l >> g.lgT;
   const uint32_t x = fast_div(q, g.ny, g.rny), y = q - x * g.ny;
   double2 *p = P + t * g.ts + (x + 1) * g.plane + (y + 1) * g.zp + 1;
-  const size_t base = sf_slot(node0 + t, q * g.nz, g.ncell);
+  const size_t base = sf_slot<MIXED>(node0 + t, q * g.nz, g.ncell);
   if constexpr (MIXED) {
     const double2 *ph = phase + t * g.nax;
     const double2 a = ph[x], b = ph[g.nx + y];
@@ -944,7 +945,7 @@ __device__ __forceinline__ void rmt_line_put(double2 *P,
     for (uint32_t z = 0; z < L; ++z)
       if (z < g.nz) step(z, make_double2(ln.v[z].x, ln.v[z].y));
     for (uint32_t z = L; z < g.nz; ++z)
-      step(z, rmt_slot<true>(Sv, base + 2 * z));
+      step(z, rmt_slot<true>(Sv, base + G * z));
   } else {
     // past the line's end the fetch repeated its last slot, so the clamped
     // store rewrites that slot's value (a loop bounded by nz instead would
@@ -952,7 +953,7 @@ __device__ __forceinline__ void rmt_line_put(double2 *P,
 #pragma unroll
     for (uint32_t z = 0; z < L; ++z)
       p[z < g.nz ? z : g.nz - 1] = make_double2(ln.re[z], ln.im[z]);
-    for (uint32_t z = L; z < g.nz; ++z) p[z] = rmt_slot<false>(Sv, base + 2 * z);
+    for (uint32_t z = L; z < g.nz; ++z) p[z] = rmt_slot<false>(Sv, base + G * z);
   }
 }
 
@@ -1059,7 +1060,7 @@ __global__ __launch_bounds__(kRmtPBlock, GLST_RMT_MIN_WAVES) void rmt_lds_kernel
 
 // ---- global-memory tables (grids too large for LDS) ----
 
-// G[c*Kp + k] = the node-pair-tiled S converted to FP64 cell-major order;
+// G[c*Kp + k] = S (node runs) converted to FP64 cell-major order;
 // the mixed path applies exp(-i k.c) to the cell-local sums and takes the
 // running sum along z (the FP64 slots hold z-running sums already).  One
 // thread per (node, z-column).
@@ -1077,7 +1078,7 @@ __global__ void rmt_global_load_kernel(const void *__restrict__ Sv,
     for (uint32_t z = 0; z < nz; ++z) {
       const uint32_t cell = col * nz + z;
       G[static_cast<size_t>(cell) * Kp + node] =
-          static_cast<const double2 *>(Sv)[sf_slot(node, cell, ncell)];
+          static_cast<const double2 *>(Sv)[sf_slot<false>(node, cell, ncell)];
     }
     return;
   }
@@ -1093,7 +1094,7 @@ __global__ void rmt_global_load_kernel(const void *__restrict__ Sv,
     double2 c;
     axis_phase(&c, k.z, h, z);
     const double er = pr * c.x - pi * c.y, ei = pr * c.y + pi * c.x;
-    const float2 v = static_cast<const float2 *>(Sv)[sf_slot(node, cell, ncell)];
+    const float2 v = static_cast<const float2 *>(Sv)[sf_slot<true>(node, cell, ncell)];
     run.x += v.x * er + v.y * ei;
     run.y += v.y * er - v.x * ei;
     G[static_cast<size_t>(cell) * Kp + node] = run;
@@ -1604,36 +1605,28 @@ const auto kLrAgKernels =
 // the NR radial nodes then costs one multiply for its phase kappa_i p, fract,
 // sin and cos and two FMAs.  The kappa_i are wave-uniform registers.  The
 // lane stores the cell-local sums sum_j q_j exp(-i k.d_j) as FP32 pairs in
-// the node-pair tiles of sf_slot (FP32 running sums over at most 128 atoms,
+// the node runs of sf_slot (FP32 running sums over at most 128 atoms,
 // denser cells add their further chunks to the stored value); the
 // shell-sum kernel applies the FP64 cell-centre phase exp(-i k.c).
 constexpr uint32_t kSfDirs = 64;
 constexpr uint32_t kSfChunk = 128;
 
-// Cells cell0 .. cell0 + sf_cells(NR) - 1 of one system (start at its first
-// cell, S at its field), wave w taking cell cell0 + w, for direction chunk
-// `chunk` of the group whose tasks are task0 .. +ntask (nodes of task t:
-// node0[t] + i, i < NR, the tasks' nodes consecutive).  Each lane keeps its
-// direction's NR sums in LDS (the first 128-atom chunk of a cell writes
-// them, denser cells add their further chunks in FP32); after one barrier
-// the workgroup stores, for every node pair of the chunk, the slots of its
-// cells as one contiguous run.
+// Cells cell0 + w of one system (start at its first cell, S at its field),
+// wave w of the workgroup taking one cell, for direction chunk `chunk` of
+// the group whose tasks are task0 .. +ntask (nodes of task t: node0[t] + i,
+// i < NR, the tasks' nodes consecutive).  The lanes of a wave hold 64 NR
+// consecutive nodes, so their stores fill whole node runs of the cell (all
+// but the chunk's first and last run, which it shares with the neighbouring
+// chunks).
 template <int NR>
 __device__ __forceinline__ void sf_mixed_nr_body(
     const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
     const uint32_t *__restrict__ cell_sorted, const LrTasks &tk,
     uint32_t task0, uint32_t ntask, uint32_t ncell, float2 *__restrict__ S,
     uint32_t cell0, uint32_t chunk) {
-  constexpr uint32_t CELLS = sf_cells(NR);
-  // rounds: the cell's sums stay in registers and kSfRoundNodes of them per
-  // lane pass through LDS at a time
-  constexpr bool ROUNDS = GLST_SF_STAGE == 2 && CELLS > 1;
-  constexpr uint32_t kStride = kSfDirs * NR + 1;  // float2 per staged cell
-  extern __shared__ float2 sf_stage[];             // [CELLS][kStride]
   const uint32_t w = threadIdx.x / kSfDirs, lane = threadIdx.x % kSfDirs;
   const uint32_t cell = cell0 + w;
-  const uint32_t tfirst = task0 + chunk * kSfDirs;
-  const uint32_t t = tfirst + lane;
+  const uint32_t t = task0 + chunk * kSfDirs + lane;
   const bool active = t < task0 + ntask;
   const uint32_t tt = active ? t : task0;
   float kap[NR];
@@ -1641,26 +1634,9 @@ __device__ __forceinline__ void sf_mixed_nr_body(
 #pragma unroll
   for (int i = 0; i < NR; ++i) kap[i] = kp[i];
   const float4 d = tk.dir32[tt];
-  // one cell per workgroup (staging off): the lane stores its own slots
   const uint32_t n0 = tk.node0[tt];
-  float2 *st = (CELLS == 1 || ROUNDS) ? nullptr
-                                      : sf_stage + w * kStride + lane * NR;
-  float2 sum[ROUNDS ? NR : 1];
-  auto put = [&](int i, float2 v) {
-    if constexpr (ROUNDS)
-      sum[i] = v;
-    else if constexpr (CELLS == 1)
-      S[sf_slot(n0 + i, cell, ncell)] = v;
-    else
-      st[i] = v;
-  };
-  auto get = [&](int i) {
-    if constexpr (ROUNDS)
-      return sum[i];
-    else if constexpr (CELLS == 1)
-      return S[sf_slot(n0 + i, cell, ncell)];
-    else
-      return st[i];
+  auto slot = [&](int i) -> float2 & {
+    return S[sf_slot<true>(n0 + i, cell, ncell)];
   };
   // uniform over the wave (its cell): does the cell hold a far atom?  One
   // vector read of the cell's cell_sorted entries, then a ballot.
@@ -1671,11 +1647,11 @@ __device__ __forceinline__ void sf_mixed_nr_body(
       f |= cell_sorted[j] & kFarAtom;
     far = __ballot(f != 0u) != 0;
   }
-  if (cell < ncell && (CELLS > 1 || active)) {
+  if (cell < ncell && active) {
     const uint32_t a0 = start[cell], a1 = start[cell + 1];
     if (a0 == a1) {
 #pragma unroll
-      for (int i = 0; i < NR; ++i) put(i, make_float2(0.f, 0.f));
+      for (int i = 0; i < NR; ++i) slot(i) = make_float2(0.f, 0.f);
     }
     for (uint32_t jb = a0; jb < a1; jb += kSfChunk) {
       const uint32_t je = (jb + kSfChunk < a1) ? jb + kSfChunk : a1;
@@ -1701,59 +1677,15 @@ __device__ __forceinline__ void sf_mixed_nr_body(
         atoms_loop(std::false_type{});
       if (jb == a0) {
 #pragma unroll
-        for (int i = 0; i < NR; ++i) put(i, make_float2(re[i], im[i]));
+        for (int i = 0; i < NR; ++i) slot(i) = make_float2(re[i], im[i]);
       } else {  // cells denser than one chunk
 #pragma unroll
         for (int i = 0; i < NR; ++i) {
-          const float2 o = get(i);
-          put(i, make_float2(o.x + re[i], o.y + im[i]));
+          const float2 o = slot(i);
+          slot(i) = make_float2(o.x + re[i], o.y + im[i]);
         }
       }
     }
-  }
-  if constexpr (CELLS == 1) return;
-  const uint32_t nact = (ntask - chunk * kSfDirs < kSfDirs)
-                            ? ntask - chunk * kSfDirs
-                            : kSfDirs;
-  const uint32_t nb = tk.node0[tfirst], ne = nb + nact * NR;
-  const uint32_t ncl = (ncell - cell0 < CELLS) ? ncell - cell0 : CELLS;
-  if constexpr (ROUNDS) {
-    // round j0: nodes j0 .. j0+R-1 of every lane; thread f of the store
-    // takes (lane m, node j0+il, cell wc) with the cells fastest, so 16
-    // consecutive threads fill the 128-byte line of a node pair
-    constexpr uint32_t R = kSfRoundNodes;
-    constexpr uint32_t kRStride = kSfDirs * R + 1;
-#pragma unroll
-    for (uint32_t j0 = 0; j0 < static_cast<uint32_t>(NR); j0 += R) {
-#pragma unroll
-      for (uint32_t il = 0; il < R; ++il)
-        if (j0 + il < static_cast<uint32_t>(NR))
-          sf_stage[w * kRStride + lane * R + il] = sum[j0 + il];
-      __syncthreads();
-      for (uint32_t f = threadIdx.x; f < kSfDirs * R * CELLS;
-           f += CELLS * kSfDirs) {
-        const uint32_t wc = f % CELLS, r = f / CELLS;
-        const uint32_t il = r % R, m = r / R, i = j0 + il;
-        if (i >= static_cast<uint32_t>(NR) || m >= nact || wc >= ncl)
-          continue;
-        S[sf_slot(nb + m * NR + i, cell0 + wc, ncell)] =
-            sf_stage[wc * kRStride + m * R + il];
-      }
-      __syncthreads();
-    }
-    return;
-  }
-  __syncthreads();
-  // nodes [nb, ne) of the chunk; pairs q0 .. q1-1 touch them (a pair shared
-  // with the neighbouring chunk keeps the other node's slot untouched)
-  const uint32_t q0 = nb >> 1, q1 = (ne + 1) >> 1;
-  const uint32_t total = (q1 - q0) * CELLS * 2;
-  for (uint32_t f = threadIdx.x; f < total; f += CELLS * kSfDirs) {
-    const uint32_t b = f & 1u, r = f >> 1;
-    const uint32_t wc = r % CELLS, n = 2 * (q0 + r / CELLS) + b;
-    if (n < nb || n >= ne || wc >= ncl) continue;
-    const uint32_t ln = n - nb, m = ln / NR, i = ln - m * NR;
-    S[sf_slot(n, cell0 + wc, ncell)] = sf_stage[wc * kStride + m * NR + i];
   }
 }
 
@@ -1796,34 +1728,29 @@ __device__ __forceinline__ void sf_mixed_any_body(
         IM += im;
       }
       if (active)
-        S[sf_slot(n0 + i, cell, ncell)] =
+        S[sf_slot<true>(n0 + i, cell, ncell)] =
             make_float2(static_cast<float>(RE), static_cast<float>(IM));
     }
   }
 }
 
 // (cell, direction chunk) of a workgroup, numbered so that each XCD runs
-// consecutive cells (their node-pair slots share cache lines).
-// The workgroups of several cells get the register budget of four waves per
-// SIMD (what the one-cell kernel compiles to); left to itself the compiler
-// aims at eight and halves the registers of the unrolled radial loop.
-#ifndef GLST_SF_WAVES
-#define GLST_SF_WAVES 4
-#endif
+// consecutive cells.  amdgpu_waves_per_eu(1, 10) lets the compiler size the
+// registers of the unrolled radial loop for as few as one wave per SIMD.
 template <int NR>
-__global__ __launch_bounds__(kSfDirs * sf_cells(NR))
-__attribute__((amdgpu_waves_per_eu(1, sf_cells(NR) > 1 ? GLST_SF_WAVES : 10)))
+__global__ __launch_bounds__(kSfDirs)
+__attribute__((amdgpu_waves_per_eu(1, 10)))
 void sf_mixed_nr_kernel(
     const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
     const uint32_t *__restrict__ cell_sorted, LrTasks tk, uint32_t task0,
     uint32_t ntask, uint32_t ncell, float2 *__restrict__ S) {
-  // gridDim.x = cell groups of sf_cells(NR) cells
+  // gridDim.x = cells
   const uint32_t wg = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x,
                                 gridDim.x * gridDim.y);
   const uint32_t ncg = gridDim.x;
   if constexpr (NR > 0)
     sf_mixed_nr_body<NR>(atoms, start, cell_sorted, tk, task0, ntask, ncell,
-                         S, (wg % ncg) * sf_cells(NR), wg / ncg);
+                         S, wg % ncg, wg / ncg);
   else
     sf_mixed_any_body(atoms, start, cell_sorted, tk, task0, ntask, ncell, S,
                       wg % ncg, wg / ncg);
@@ -2405,8 +2332,8 @@ __global__ __launch_bounds__(kSfBlock) void sf_f64_batch_kernel(
 }
 
 template <int NR>
-__global__ __launch_bounds__(kSfDirs * sf_cells(NR))
-__attribute__((amdgpu_waves_per_eu(1, sf_cells(NR) > 1 ? GLST_SF_WAVES : 10)))
+__global__ __launch_bounds__(kSfDirs)
+__attribute__((amdgpu_waves_per_eu(1, 10)))
 void sf_mixed_batch_kernel(
     const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
     const uint32_t *__restrict__ cell_sorted, LrTasks tk,
@@ -2416,12 +2343,10 @@ void sf_mixed_batch_kernel(
   const Seg g = seg[run_of(seg, nseg, wg)];
   const SysDev &d = sys[g.sys];
   const uint32_t local = wg - g.blk0;
-  constexpr uint32_t CELLS = sf_cells(NR);
-  const uint32_t ncg = (d.ncell + CELLS - 1) / CELLS;
+  const uint32_t ncg = d.ncell;
   if constexpr (NR > 0)
     sf_mixed_nr_body<NR>(atoms, start + d.cell0, cell_sorted, tk, g.a, g.b,
-                         d.ncell, S + d.field0, (local % ncg) * CELLS,
-                         local / ncg);
+                         d.ncell, S + d.field0, local % ncg, local / ncg);
   else
     sf_mixed_any_body(atoms, start + d.cell0, cell_sorted, tk, g.a, g.b,
                       d.ncell, S + d.field0, local % ncg, local / ncg);
@@ -2605,7 +2530,6 @@ __device__ __forceinline__ void sf_fused_nr(
     sf_mixed_any_body(atoms, start, cell_sorted, tk, task0, ntask, ncell, S,
                       cell0, chunk);
   } else {
-    static_assert(sf_cells(N) == 1, "the fused step stores S(k) per wave");
     if (nr == static_cast<uint32_t>(N))
       sf_mixed_nr_body<N>(atoms, start, cell_sorted, tk, task0, ntask, ncell,
                           S, cell0, chunk);
@@ -2804,16 +2728,6 @@ void prepare_kernels(size_t lds_limit) {
       (void)hipGetLastError();
     }
   }
-  // the S(k) kernels' staging areas (up to 82 KB)
-  for (int nr = 1; nr <= kSfMaxUnrolled; ++nr) {
-    const void *sf[] = {reinterpret_cast<const void *>(kSfNrKernels[nr]),
-                        reinterpret_cast<const void *>(kSfBatchKernels[nr])};
-    for (const void *f : sf)
-      if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              static_cast<int>(sf_lds_bytes(nr))) !=
-          hipSuccess)
-        (void)hipGetLastError();
-  }
 }
 
 void fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s) {
@@ -2895,10 +2809,8 @@ void sf(bool mixed, const void *atoms, const uint32_t *start,
     const int nr = gr.nr <= static_cast<uint32_t>(kSfMaxUnrolled)
                        ? static_cast<int>(gr.nr)
                        : 0;
-    const uint32_t cells = sf_cells(nr);
-    const dim3 grid((g.ncell + cells - 1) / cells,
-                    (gr.ntask + kSfDirs - 1) / kSfDirs);
-    kSfNrKernels[nr]<<<grid, kSfDirs * cells, sf_lds_bytes(nr), s>>>(
+    const dim3 grid(g.ncell, (gr.ntask + kSfDirs - 1) / kSfDirs);
+    kSfNrKernels[nr]<<<grid, kSfDirs, 0, s>>>(
         static_cast<const float4 *>(atoms), start, cell_sorted, tk, gr.task0,
         gr.ntask, g.ncell, static_cast<float2 *>(S));
   }
@@ -3176,8 +3088,7 @@ void sf_batch(bool mixed, const void *atoms, const uint32_t *start,
     const int nr = L.nr <= static_cast<uint32_t>(kSfMaxUnrolled)
                        ? static_cast<int>(L.nr)
                        : 0;
-    kSfBatchKernels[nr]<<<L.nblk, kSfDirs * sf_cells(nr), sf_lds_bytes(nr),
-                          s>>>(static_cast<const float4 *>(atoms), start,
+    kSfBatchKernels[nr]<<<L.nblk, kSfDirs, 0, s>>>(static_cast<const float4 *>(atoms), start,
                                cell_sorted, tk, sys, L.seg, L.nseg,
                                static_cast<float2 *>(S));
   }

@@ -65,45 +65,31 @@ void bin_scatter(bool mixed, const double *rx, const double *ry,
                  const uint32_t *ag_start = nullptr, uint2 *ag = nullptr);
 
 // ---- structure factors (calc_sf) -------------------------------------------
-// S(k) is stored in node-pair tiles: the slots of nodes 2m and 2m+1 of all
-// cells form one contiguous run, cell-major inside it, so the shell-sum
-// workgroup of a node pair reads one contiguous block.
+// S(k) is stored in node runs: the slots of nodes G m .. G m + G - 1 of
+// every cell form one contiguous run, cell-major inside it (mixed: float2
+// slots, G = 8, 64-byte runs; FP64: double2, G = 2, 32 bytes).  The wave
+// that computes a cell's sums writes whole runs, so lines leave L2 complete
+// (mixed S(k) at 288k atoms writes 1.13 times its bytes with G = 8, 1.39
+// with 4, 2.5 with 2).  A shell-sum tile of T < G nodes reads part of each
+// run; its neighbours read the rest while the lines are in L2.
+#ifndef GLST_SF_RUN_MIXED
+#define GLST_SF_RUN_MIXED 8
+#endif
+template <bool MIXED>
+__host__ __device__ constexpr uint32_t sf_run() {
+  return MIXED ? GLST_SF_RUN_MIXED : 2u;
+}
+template <bool MIXED>
 __host__ __device__ inline size_t sf_slot(uint32_t node, uint32_t cell,
                                           uint32_t ncell) {
-  return ((static_cast<size_t>(node >> 1) * ncell + cell) << 1) + (node & 1);
+  constexpr uint32_t G = sf_run<MIXED>();
+  return (static_cast<size_t>(node / G) * ncell + cell) * G + node % G;
 }
 
 // Mixed structure factors: one wave per (cell, 64 directions of a group), the
 // NR radial nodes of a direction unrolled for NR <= kSfMaxUnrolled (larger
-// counts run a generic kernel, one cell per workgroup).  An unrolled
-// workgroup holds sf_cells(NR) consecutive cells, stages their sums in LDS
-// and stores whole runs of node-pair slots: sf_cells consecutive cells of
-// one node pair are 16 sf_cells contiguous bytes (8 cells fill a 128-byte
-// line).  Above 19 radial nodes 4 cells per workgroup keep the staging area
-// (64 NR float2 per wave) from lowering occupancy further.
+// counts run a generic kernel); each lane stores its direction's NR slots.
 constexpr int kSfMaxUnrolled = 40;
-#ifndef GLST_SF_STAGE
-#define GLST_SF_STAGE 0
-#endif
-// GLST_SF_STAGE 2 stages kSfRoundNodes radial nodes of every lane at a time
-// (8 cells per workgroup, 16 KB of LDS whatever NR).
-constexpr uint32_t kSfRoundNodes = 4;
-__host__ __device__ constexpr uint32_t sf_cells(int nr) {
-  return (!GLST_SF_STAGE || nr <= 0 || nr > kSfMaxUnrolled)
-             ? 1u
-             : ((GLST_SF_STAGE == 2 || nr <= 19) ? 8u : 4u);
-}
-// LDS of one unrolled workgroup: per cell 64 NR float2 (64 kSfRoundNodes
-// in rounds), plus one to stagger the cells over the banks
-__host__ __device__ constexpr uint32_t sf_lds_bytes(int nr) {
-  return sf_cells(nr) == 1u
-             ? 0u
-             : sf_cells(nr) *
-                   (64u * (GLST_SF_STAGE == 2 ? kSfRoundNodes
-                                              : static_cast<uint32_t>(nr)) +
-                    1u) *
-                   8u;
-}
 
 struct LrTaskArgs;
 // Tasks (directions) of one alpha group for the mixed structure factors.
