@@ -67,13 +67,14 @@ void bin_scatter(bool mixed, const double *rx, const double *ry,
 // ---- structure factors (calc_sf) -------------------------------------------
 // S(k) is stored in node runs: the slots of nodes G m .. G m + G - 1 of
 // every cell form one contiguous run, cell-major inside it (mixed: float2
-// slots, G = 8, 64-byte runs; FP64: double2, G = 2, 32 bytes).  The wave
-// that computes a cell's sums writes whole runs, so lines leave L2 complete
-// (mixed S(k) at 288k atoms writes 1.13 times its bytes with G = 8, 1.39
-// with 4, 2.5 with 2).  A shell-sum tile of T < G nodes reads part of each
-// run; its neighbours read the rest while the lines are in L2.
+// slots, G = 4; FP64: double2, G = 2; 32-byte runs, one HBM sector).  The
+// wave that computes a cell's sums writes whole runs, so fewer lines leave
+// L2 partly written (mixed S(k) at 288k atoms writes 1.39 times its bytes
+// with G = 4, 2.4 with 2, 1.13 with 8).  A shell-sum tile of T < G nodes
+// reads part of each run and its neighbours the rest while the lines are in
+// L2; G = 8 made that cost 5 % on the 18^3 grid (T = 1), G = 4 nothing.
 #ifndef GLST_SF_RUN_MIXED
-#define GLST_SF_RUN_MIXED 8
+#define GLST_SF_RUN_MIXED 4
 #endif
 template <bool MIXED>
 __host__ __device__ constexpr uint32_t sf_run() {
