@@ -221,7 +221,7 @@ def _max_err_vs_direct(g, torch, cases):
 
 
 def serve_bench(g, torch, np, n_requests=2000, seed=5, max_batch=8,
-                fused=0):
+                fused=0, graph=None):
     """Served latency of the continuous-batching server (mixed path).
 
     Requests are systems of the batch size mix: waters log-uniform in
@@ -240,7 +240,7 @@ def serve_bench(g, torch, np, n_requests=2000, seed=5, max_batch=8,
     included).  Every request of the Poisson stream has its own output
     arrays; afterwards each answer is compared with the direct FP64 sum of
     its own inputs.  `fused`: the batches' launch mode (glst_batch option
-    "fused")."""
+    "fused"); `graph`: the slots' graph mode (None: the server's default)."""
     rng = np.random.default_rng(seed)
     pool = []
     for L, h in _water_mix(g, np, rng, 48, 1000):
@@ -266,6 +266,8 @@ def serve_bench(g, torch, np, n_requests=2000, seed=5, max_batch=8,
     srv = g.GLSTServer(SERVE_TOL, 64 << 30, max_batch=max_batch, ct="float")
     if fused:
         srv.set_option("fused", fused)
+    if graph is not None:
+        srv.set_option("graph", graph)
     for L, d in resident:  # resident systems are set up once
         srv.submit(L, SERVE_RCUT, *d, *outputs(d[0].numel()))
     while srv.pending():

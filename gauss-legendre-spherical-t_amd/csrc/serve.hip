@@ -251,8 +251,10 @@ struct glst_server {
     const uint32_t nrun = static_cast<uint32_t>(4 * take);
     dev::copy_runs(sl.runs.as<dev::CopyRun>(), nrun, max_n, s);
     check_launch("gather requests");
-    b.step_async(sl.in[0].as<double>(), sl.in[1].as<double>(),
-                 sl.in[2].as<double>(), sl.in[3].as<double>());
+    // step() replays the slot's captured graph when its graph mode asks for
+    // one and the composition and packed inputs are unchanged
+    b.step(sl.in[0].as<double>(), sl.in[1].as<double>(),
+           sl.in[2].as<double>(), sl.in[3].as<double>());
     b.get_ef_async(sl.out[0].as<double>(), sl.out[1].as<double>(),
                    sl.out[2].as<double>(), sl.out[3].as<double>());
     dev::copy_runs(sl.runs.as<dev::CopyRun>() + nrun, nrun, max_n, s);
@@ -438,6 +440,13 @@ glst_status_t glst_server_set_option(glst_server_t srv, const char *name,
         sl->batch.fused = static_cast<int>(value);
         // a slot is planned when it is next configured
         if (!sl->composition.empty()) sl->batch.plan_fused();
+      }
+    } else if (n == "graph") {
+      if (value < 0 || value > 2) throw ArgError("graph must be 0, 1 or 2");
+      for (auto &sl : srv->slots) {
+        GLST_HIP(hipStreamSynchronize(sl->batch.stream));
+        sl->batch.drop_graph();
+        sl->batch.graph_mode = static_cast<int>(value);
       }
     } else if (n == "max_batch") {
       if (value < 0 || value > (int64_t(1) << 31))

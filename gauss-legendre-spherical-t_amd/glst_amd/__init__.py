@@ -689,7 +689,7 @@ class GLSTServer:
         return [int(v) for v in out[: n.value]]
 
     def set_option(self, name: str, value: int):
-        """Server options ("fused", "max_batch"; glst_server_set_option)."""
+        """Server options ("fused", "graph", "max_batch"; glst_server_set_option)."""
         self._c(_lib.glst_server_set_option(self._h, name.encode(), int(value)))
 
     def pending(self) -> int:

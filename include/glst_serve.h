@@ -78,10 +78,11 @@ glst_status_t glst_server_completed(glst_server_t srv, uint64_t *tickets,
 glst_status_t glst_server_pending(glst_server_t srv, uint32_t *n);
 glst_status_t glst_server_get_stats(glst_server_t srv,
                                     glst_server_stats_t *stats);
-/* Options: "fused" (0 .. 3, default 0) as for glst_batch_set_option, for
- * both batch slots (each slot's persistent launch takes half of the device,
- * so that the two may run at once); "max_batch" (requests per batch, 0 = no
- * limit). */
+/* Options: "fused" (0 .. 3, default 0) and "graph" (0 .. 2, default 2) as
+ * for glst_batch_set_option, for both batch slots (each slot's persistent
+ * launch takes half of the device, so that the two may run at once; graph
+ * mode 2 replays a slot's step while its composition repeats);
+ * "max_batch" (requests per batch, 0 = no limit). */
 glst_status_t glst_server_set_option(glst_server_t srv, const char *name,
                                      int64_t value);
 const char *glst_server_last_error(glst_server_t srv);
