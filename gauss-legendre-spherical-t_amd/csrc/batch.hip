@@ -721,7 +721,13 @@ double glst_batch::autotune(const double *rx, const double *ry,
       for (int v : {0, 1, 2, 3}) {
         if (v == keep) continue;
         fused = v;
-        plan_fused();
+        try {
+          plan_fused();
+        } catch (const StateError &) {  // this device cannot run it
+          fused = keep;
+          plan_fused();
+          continue;
+        }
         drop_graph();
         const double t = time();
         if (t < best) {
