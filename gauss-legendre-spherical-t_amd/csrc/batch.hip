@@ -343,6 +343,8 @@ BatchLayout pack_batch(int ct,
 }  // namespace glst
 
 glst_batch::~glst_batch() {
+  // a step still running may write the failure flag or the arenas
+  if (stream) (void)hipStreamSynchronize(stream);
   drop_graph();
   if (fail_host) (void)hipHostFree(fail_host);
   if (side) (void)hipStreamDestroy(side);
