@@ -466,6 +466,8 @@ void glst_batch::configure(
                   stream);
   zero_ef();
   assigned = false;
+  // a failed fused step left nothing behind that a new configuration uses
+  if (fail_host) *fail_host = 0u;
   plan_fused();
   check_launch("batch configure");
 }

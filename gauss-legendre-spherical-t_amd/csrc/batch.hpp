@@ -215,7 +215,8 @@ struct glst_batch {
   // unpermuted outputs in packed original order (stream-ordered, no sync)
   void get_ef_async(double *ofx, double *ofy, double *ofz, double *oen);
   // after the stream's work is complete: throws if a fused step failed
-  // (a barrier timed out; its outputs are not valid)
+  // (a barrier timed out; its outputs are not valid); the flag stays set
+  // until the batch is configured again
   void check_failed() const;
   // the next step runs as one persistent launch
   bool use_fused() const {
