@@ -990,6 +990,8 @@ glst_status_t glst_batch_set_option(glst_batch_t b, const char *name,
       b->graph_mode = static_cast<int>(value);
     } else if (n == "fused") {
       if (value < 0 || value > 3) throw ArgError("fused must be 0 .. 3");
+      if (value != 0 && !b->mixed())
+        throw ArgError("fused launch modes need the mixed path (CT = float)");
       GLST_HIP(hipStreamSynchronize(b->stream));
       b->fused = static_cast<int>(value);
       b->plan_fused();

@@ -434,6 +434,8 @@ glst_status_t glst_server_set_option(glst_server_t srv, const char *name,
     const std::string n = name;
     if (n == "fused") {
       if (value < 0 || value > 3) throw ArgError("fused must be 0 .. 3");
+      if (value != 0 && srv->ct != GLST_CT_FLOAT)
+        throw ArgError("fused launch modes need the mixed path (CT = float)");
       for (auto &sl : srv->slots) {
         GLST_HIP(hipStreamSynchronize(sl->batch.stream));
         sl->batch.drop_graph();

@@ -202,3 +202,32 @@ def test_fused_server_stream(cuda):
         assert rel_err(fused[k][3:], plain[k][3:]) < 1e-6, k
         ref = [t.cpu().numpy() for t in g.coulomb_direct(*dev[p])]
         assert rel_err(fused[k][:3], ref[:3]) < 1e-5, k
+
+
+def test_fused_option_checks(cuda):
+    """The launch modes belong to the mixed path; out-of-range values and the
+    FP64 path are refused with an error, leaving the batch usable."""
+    import torch
+    import glst_amd as g
+    systems = _systems([300, 700], seed0=5)
+    d = _pack(systems, torch)
+    b = g.GLSTBatch([h[0].size for _, h in systems], [L for L, _ in systems],
+                    RCUT, TOL, ct="double")
+    with pytest.raises(g.GLSTError):
+        b.set_option("fused", 1)
+    b.calc_ener_force(*d)
+    b.close()
+    b = _batch(systems, TOL)
+    with pytest.raises(g.GLSTError):
+        b.set_option("fused", 4)
+    with pytest.raises(g.GLSTError):
+        b.set_option("fused", -1)
+    b.set_option("fused", 3)
+    b.calc_ener_force(*d)
+    b.close()
+    srv = g.GLSTServer(TOL, 1 << 30, ct="double")
+    with pytest.raises(g.GLSTError):
+        srv.set_option("fused", 2)
+    with pytest.raises(g.GLSTError):
+        srv.set_option("graph", 3)
+    srv.close()
