@@ -1567,13 +1567,17 @@ __device__ __forceinline__ void lr_ag_body(
 // VGPRs) up to GLST_LR_AG_4WAVE_NR radial nodes, where that costs at most a
 // few bytes of scratch (one 8-byte spill and one reload, 12 B per lane, at
 // 16 and 19 nodes) and made the kernel 3.5 % faster at 288k atoms; else
-// whatever the kernel needs.
+// whatever the kernel needs, except at 23 and 24 nodes.  Those compile to
+// 169 and 172 VGPRs uncapped, just past the 168 that three waves allow;
+// capped at three they spill 20-44 B per lane and the batched step of 512
+// small systems (most of whose groups have 13-26 nodes) runs 1.6 % faster.
 #ifndef GLST_LR_AG_4WAVE_NR
 #define GLST_LR_AG_4WAVE_NR 19
 #endif
+constexpr bool lr_ag_3wave(int nr) { return nr == 23 || nr == 24; }
 template <int NR>
 constexpr int lr_ag_min_waves() {
-  return NR <= GLST_LR_AG_4WAVE_NR ? 4 : 1;
+  return NR <= GLST_LR_AG_4WAVE_NR ? 4 : lr_ag_3wave(NR) ? 3 : 1;
 }
 template <int NR>
 __global__ __launch_bounds__(kLrBlock, lr_ag_min_waves<NR>()) void lr_ag_kernel(
@@ -2405,10 +2409,12 @@ typedef void (*LrBatchKernel)(const float4 *, const uint32_t *, LrTasks,
                               const float2 *, const SysDev *, const Seg *,
                               uint32_t, double4 *);
 
-// The batched back-projection over atom groups.  No register cap: on the
-// batch's mix of small systems it cost 2-3 %.
+// The batched back-projection over atom groups.  No four-wave register cap
+// (on the batch's mix of small systems it cost 2-3 %); the three-wave cap at
+// 23 and 24 radial nodes as above.
 template <int NR>
-__global__ __launch_bounds__(kLrBlock) void lr_ag_batch_kernel(
+__global__ __launch_bounds__(kLrBlock, lr_ag_3wave(NR) ? 3 : 1)
+void lr_ag_batch_kernel(
     const float4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
     const uint2 *__restrict__ ag, LrTasks tk, const float2 *__restrict__ R,
     const SysDev *__restrict__ sys, const Seg *__restrict__ seg, uint32_t nseg,
