@@ -69,7 +69,12 @@ struct glst_engine {
   // near field on the side stream: 0 off, 1 alongside S(k) onwards,
   // 2 alongside the shell sums onwards
   int overlap_near = 1;
-  int rmt_shape = 0;  // shell sums: 0 auto, 1 one node per workgroup, 2 global
+  // shell sums: 0 LDS tables of as many nodes as fit, 1 one node per
+  // workgroup, 2 global tables, 3 matrix cores where the grid allows (CT =
+  // float, nx and ny up to 16), else as 0; 4 (default) matrix cores where
+  // they apply and a z slice fills at least half of their 16 x 16 tile,
+  // else as 0
+  int rmt_shape = 4;
   // mixed path, tables in LDS: pipeline S(k) -> shell sums ->
   // back-projection across alpha groups, the shell sums on a third stream
   int pipeline = 1;
@@ -327,7 +332,11 @@ struct glst_engine {
     if (!has_lr || Kp == 0) return;
     if (!sf_done) throw StateError("sum_rmt_sf called before calc_sf");
     Scope sc(timer, "rmt", stream);
-    if (rmt_shape == 2 ||
+    if (rmt_mfma_path()) {
+      dev::rmt_mfma(S.as<void>(), k64.as<double4>(), group.as<uint32_t>(),
+                    r_in.as<uint32_t>(), r_out.as<uint32_t>(), Kp, grid,
+                    lds_limit, R.as<void>(), stream);
+    } else if (rmt_shape == 2 ||
         !dev::rmt_lds(mixed(), rmt_shape == 1 ? 1 : dev::kRmtMaxNodes,
                       S.as<void>(),
                       k64.as<double4>(), group.as<uint32_t>(),
@@ -409,8 +418,11 @@ struct glst_engine {
   }
   void plan_pipeline() {
     pipe_tmax = rmt_shape == 1 ? 1 : dev::kRmtMaxNodes;
-    const uint32_t T = dev::rmt_nodes_per_group(grid.nx, grid.ny, grid.nz,
-                                                lds_limit, pipe_tmax);
+    const uint32_t T =
+        rmt_mfma_path()
+            ? dev::rmt_mfma_nodes(grid, lds_limit)
+            : dev::rmt_nodes_per_group(grid.nx, grid.ny, grid.nz, lds_limit,
+                                       pipe_tmax);
     if (T == 0) throw StateError("shell-sum tables do not fit in LDS");
     PipelinePlan P = glst::plan_pipeline(nodes, sf_groups.size(), Kp, T);
     tile_end = std::move(P.tile_end);
@@ -446,12 +458,19 @@ struct glst_engine {
       Scope sc(timer, "rmt", rmt_stream);
       for (size_t g = 0; g < sf_groups.size(); ++g) {
         if (g > 0) GLST_HIP(hipStreamWaitEvent(rmt_stream, ev_sf[g], 0));
-        if (!dev::rmt_lds(true, pipe_tmax, S.as<void>(), k64.as<double4>(),
-                          group.as<uint32_t>(), r_in.as<uint32_t>(),
-                          r_out.as<uint32_t>(), Kp, grid, lds_limit,
-                          R.as<void>(), rmt_stream, g > 0 ? tile_end[g - 1] : 0,
-                          tile_end[g]))
-          throw StateError("shell-sum tables do not fit in LDS");
+        const uint32_t t0 = g > 0 ? tile_end[g - 1] : 0;
+        const bool ok =
+            rmt_mfma_path()
+                ? dev::rmt_mfma(S.as<void>(), k64.as<double4>(),
+                                group.as<uint32_t>(), r_in.as<uint32_t>(),
+                                r_out.as<uint32_t>(), Kp, grid, lds_limit,
+                                R.as<void>(), rmt_stream, t0, tile_end[g])
+                : dev::rmt_lds(true, pipe_tmax, S.as<void>(),
+                               k64.as<double4>(), group.as<uint32_t>(),
+                               r_in.as<uint32_t>(), r_out.as<uint32_t>(), Kp,
+                               grid, lds_limit, R.as<void>(), rmt_stream, t0,
+                               tile_end[g]);
+        if (!ok) throw StateError("shell-sum tables do not fit in LDS");
         GLST_HIP(hipEventRecord(ev_rmt[g], rmt_stream));
       }
       check_launch("rmt");
@@ -525,9 +544,18 @@ struct glst_engine {
   // shell sums through the global-memory tables (one node's table does not
   // fit in LDS, or forced)
   bool rmt_global_path() const {
+    if (rmt_mfma_path()) return false;
     return rmt_shape == 2 ||
            dev::rmt_nodes_per_group(grid.nx, grid.ny, grid.nz, lds_limit,
                                     1) == 0;
+  }
+  // shell sums on the matrix cores (CT = float, nx and ny up to 16)
+  bool rmt_mfma_path() const {
+    const bool want =
+        rmt_shape == 3 ||
+        (rmt_shape == 4 && grid.nx * grid.ny * 2 >= dev::kRmtMfmaMaxXY *
+                                                        dev::kRmtMfmaMaxXY);
+    return want && mixed() && dev::rmt_mfma_nodes(grid, lds_limit) > 0;
   }
 
   // New back-projection split target: splits, their table and the partials.
@@ -661,13 +689,17 @@ struct glst_engine {
              apply_lr_blocks();
            },
            {1024, 2048, 4096, 8192, 16384, 32768, 65536});
-      if (!rmt_global_path())
+      if (!rmt_global_path()) {
+        std::vector<int64_t> shapes{0, 1};
+        if (mixed() && dev::rmt_mfma_nodes(grid, lds_limit) > 0)
+          shapes.push_back(3);
         knob([&] { return static_cast<int64_t>(rmt_shape); },
              [&](int64_t v) {
                rmt_shape = static_cast<int>(v);
                drop_graph();
              },
-             {0, 1});
+             shapes);
+      }
     }
     graph_knob();
     zero_ef();
@@ -1110,7 +1142,8 @@ glst_status_t glst_set_option(glst_handle_t h, const char *name,
       h->overlap_near = static_cast<int>(value);
     }
     else if (n == "rmt_shape") {
-      if (value < 0 || value > 2) throw ArgError("rmt_shape must be 0, 1 or 2");
+      if (value < 0 || value > 4)
+        throw ArgError("rmt_shape must be 0, 1, 2, 3 or 4");
       h->rmt_shape = static_cast<int>(value);
     } else if (n == "pipeline") {
       if (value < 0 || value > 1) throw ArgError("pipeline must be 0 or 1");

@@ -1058,6 +1058,290 @@ __global__ __launch_bounds__(kRmtPBlock, GLST_RMT_MIN_WAVES) void rmt_lds_kernel
   }
 }
 
+// ---- shell sums on the matrix cores (mixed path, nx and ny up to 16) ----
+//
+// A cube sum is separable: B_r(c) sums Y = exp(-i k.c') S_c' over the cells
+// c' with |x'-x|, |y'-y|, |z'-z| <= r.  Per node, the running sums of Y
+// along z sit in LDS in FP32 with a zero before each line, so the z window
+// of a cell is the difference of two of them.  For every z slice the x and
+// y windows of both cubes are products with 0/1 band matrices on the
+// matrix cores (v_mfma_f32_16x16x4f32: exact products, FP32 sums).  Pass 1
+// forms Q^T = T^T Kx (rows y, columns x) from the z-windowed slice T; its
+// accumulator registers are, one for one, the B operands of pass 2,
+// W^T = Ky Q^T, whose k index runs over y in the order the registers hold
+// it.  Every sum is a direct sum of at most nz + 32 terms, so FP32 suffices
+// where the summed-volume tables need FP64 for their differences of large
+// running sums.
+//
+// Operand layouts of the 16x16x4 FP32 instruction for lane l (c = l % 16,
+// g = l / 16): A[c][g], B[g][c], accumulator register v = D[4g + v][c].
+typedef float rmt_f4 __attribute__((ext_vector_type(4)));
+
+__device__ inline float2 cmulf(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// NPW consecutive S slots of one cell (the nodes n0 .. n0+NPW-1 of one node
+// run, n0 a multiple of NPW)
+template <uint32_t NPW>
+__device__ __forceinline__ void rmt_mfma_load(const float2 *__restrict__ p,
+                                              float2 (&v)[NPW]) {
+  if constexpr (NPW == 1) {
+    v[0] = p[0];
+  } else {
+#pragma unroll
+    for (uint32_t h = 0; h < NPW / 2; ++h) {
+      const float4 a = reinterpret_cast<const float4 *>(p)[h];
+      v[2 * h] = make_float2(a.x, a.y);
+      v[2 * h + 1] = make_float2(a.z, a.w);
+    }
+  }
+}
+
+// One tile of the matrix-core shell sums: nodes n0 .. n0+NPW-1 (see
+// rmt_mfma_kernel).
+template <uint32_t NPW>
+__device__ __forceinline__ void rmt_mfma_tile(
+    const float2 *__restrict__ S, const double4 *__restrict__ k64,
+    const uint32_t *__restrict__ group, const uint32_t *__restrict__ r_in,
+    const uint32_t *__restrict__ r_out, uint32_t Kp, uint32_t n0,
+    uint32_t nx, uint32_t ny, uint32_t nz, double h, float2 *__restrict__ R,
+    float2 *__restrict__ rmt_mfma_lds) {
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nax = nx + ny + nz, zp = rmt_zpitch(nz);
+  const uint32_t nline = nx * ny, ts = nline * zp, ncell = nline * nz;
+  float2 *tab = rmt_mfma_lds;
+  float2 *ph = rmt_mfma_lds + NPW * ts;
+  float2 *stage = ph + NPW * nax;  // line factors
+  // axis factors exp(+i k_d (i + 1/2) h), FP64 arguments, FP32 values
+  for (uint32_t e = tid; e < NPW * nax; e += kRmtMfmaBlock) {
+    const uint32_t t = e / nax, a = e - t * nax;
+    const double4 k = k64[n0 + t];
+    double2 r;
+    if (a < nx)
+      axis_phase(&r, k.x, h, a);
+    else if (a < nx + ny)
+      axis_phase(&r, k.y, h, a - nx);
+    else
+      axis_phase(&r, k.z, h, a - nx - ny);
+    ph[e] = make_float2(static_cast<float>(r.x), static_cast<float>(r.y));
+  }
+  __syncthreads();
+  // line factors exp(+i (k_x c_x + k_y c_y)) and the zero before each line
+  // (nline <= 256: one line per thread)
+  if (tid < nline) {
+    const uint32_t x = tid / ny, y = tid - x * ny;
+#pragma unroll
+    for (uint32_t t = 0; t < NPW; ++t) {
+      stage[t * nline + tid] = cmulf(ph[t * nax + x], ph[t * nax + nx + y]);
+      tab[t * ts + tid * zp] = make_float2(0.f, 0.f);
+    }
+  }
+  __syncthreads();
+  // (1a) Y = S conj(exp(i k.c)): sixteen lanes per line, consecutive lanes
+  // on consecutive z, so a wave reads consecutive cells
+  {
+    // batches of 8 lines per thread, all 8 loads in flight at once
+    constexpr uint32_t kB = 8, kLs = kRmtMfmaBlock / 16;
+    const uint32_t zl = tid & 15u;
+    for (uint32_t z = zl; z < nz; z += 16) {
+      for (uint32_t lb = tid >> 4; lb < nline; lb += kB * kLs) {
+        float2 v[kB][NPW];
+#pragma unroll
+        for (uint32_t u = 0; u < kB; ++u) {
+          const uint32_t l = lb + u * kLs < nline ? lb + u * kLs : lb;
+          rmt_mfma_load<NPW>(S + sf_slot<true>(n0, l * nz + z, ncell), v[u]);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kB; ++u) {
+          const uint32_t l = lb + u * kLs;
+          if (l < nline) {
+#pragma unroll
+            for (uint32_t t = 0; t < NPW; ++t) {
+              const float2 e = cmulf(stage[t * nline + l],
+                                     ph[t * nax + nx + ny + z]);
+              tab[t * ts + l * zp + z + 1] =
+                  make_float2(v[u][t].x * e.x + v[u][t].y * e.y,
+                              v[u][t].y * e.x - v[u][t].x * e.y);
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // (1b) running sums along z, one thread per line and node
+  if (tid < nline) {
+#pragma unroll
+    for (uint32_t t = 0; t < NPW; ++t) {
+      float2 *p = tab + t * ts + tid * zp + 1;
+      float2 run = make_float2(0.f, 0.f);
+      for (uint32_t z0 = 0; z0 < nz; z0 += 8) {
+        float2 a[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u)  // clamped: always in the line
+          a[u] = p[z0 + u < nz ? z0 + u : nz - 1];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u)
+          if (z0 + u < nz) {
+            run.x += a[u].x;
+            run.y += a[u].y;
+            p[z0 + u] = run;
+          }
+      }
+    }
+  }
+  __syncthreads();
+  // (2) each wave takes every fourth z slice, for all NPW nodes
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const uint32_t c = tid & 15u, g4 = (tid >> 4) & 3u;
+  uint32_t rin[NPW], rout[NPW];
+  float wk[NPW], kr[NPW];
+  // band matrices: pass 1's B (row x = 4m + g4, column x' = c) and pass 2's
+  // A (row y' = c, k slot g4 of instruction v: y = 4 g4 + v)
+  float bxo[NPW][4], bxi[NPW][4], ayo[NPW][4], ayi[NPW][4];
+#pragma unroll
+  for (uint32_t t = 0; t < NPW; ++t) {
+    const uint32_t gr = group[n0 + t];
+    rin[t] = r_in[gr];
+    rout[t] = r_out[gr];
+    const double4 kn = k64[n0 + t];
+    wk[t] = static_cast<float>(kn.w);
+    kr[t] = kappa_rev(kn);
+#pragma unroll
+    for (uint32_t m = 0; m < 4; ++m) {
+      const int dx = static_cast<int>(c) - static_cast<int>(4 * m + g4);
+      const uint32_t ax = static_cast<uint32_t>(dx < 0 ? -dx : dx);
+      bxo[t][m] = ax <= rout[t] ? 1.f : 0.f;
+      bxi[t][m] = ax <= rin[t] ? 1.f : 0.f;
+      const int dy = static_cast<int>(c) - static_cast<int>(4 * g4 + m);
+      const uint32_t ay = static_cast<uint32_t>(dy < 0 ? -dy : dy);
+      ayo[t][m] = ay <= rout[t] ? 1.f : 0.f;
+      ayi[t][m] = ay <= rin[t] ? 1.f : 0.f;
+    }
+  }
+  for (uint32_t z = wave; z < nz; z += kRmtMfmaBlock / 64) {
+    float2 val[NPW][4];
+#pragma unroll
+    for (uint32_t t = 0; t < NPW; ++t) {
+      const uint32_t hio = z + rout[t] + 1 < nz ? z + rout[t] + 1 : nz;
+      const uint32_t loo = z > rout[t] ? z - rout[t] : 0u;
+      const uint32_t hii = z + rin[t] + 1 < nz ? z + rin[t] + 1 : nz;
+      const uint32_t loi = z > rin[t] ? z - rin[t] : 0u;
+      // the z windows of this lane's cells (x = 4m + g4, y = c)
+      float aor[4], aoi[4], air[4], aii[4];
+#pragma unroll
+      for (uint32_t m = 0; m < 4; ++m) {
+        const uint32_t x = 4 * m + g4;
+        float2 o = make_float2(0.f, 0.f), in = o;
+        if (x < nx && c < ny) {
+          const float2 *p = tab + t * ts + (x * ny + c) * zp;
+          const float2 a = p[hio], b = p[loo], d = p[hii], e = p[loi];
+          o = make_float2(a.x - b.x, a.y - b.y);
+          in = make_float2(d.x - e.x, d.y - e.y);
+        }
+        aor[m] = o.x;
+        aoi[m] = o.y;
+        air[m] = in.x;
+        aii[m] = in.y;
+      }
+      rmt_f4 qor = {0.f, 0.f, 0.f, 0.f}, qoi = qor, qir = qor, qii = qor;
+#pragma unroll
+      for (uint32_t m = 0; m < 4; ++m) {
+        qor = __builtin_amdgcn_mfma_f32_16x16x4f32(aor[m], bxo[t][m], qor, 0,
+                                                   0, 0);
+        qoi = __builtin_amdgcn_mfma_f32_16x16x4f32(aoi[m], bxo[t][m], qoi, 0,
+                                                   0, 0);
+        qir = __builtin_amdgcn_mfma_f32_16x16x4f32(air[m], bxi[t][m], qir, 0,
+                                                   0, 0);
+        qii = __builtin_amdgcn_mfma_f32_16x16x4f32(aii[m], bxi[t][m], qii, 0,
+                                                   0, 0);
+      }
+      rmt_f4 wor = {0.f, 0.f, 0.f, 0.f}, woi = wor, wir = wor, wii = wor;
+#pragma unroll
+      for (uint32_t v = 0; v < 4; ++v) {
+        wor = __builtin_amdgcn_mfma_f32_16x16x4f32(ayo[t][v], qor[v], wor, 0,
+                                                   0, 0);
+        woi = __builtin_amdgcn_mfma_f32_16x16x4f32(ayo[t][v], qoi[v], woi, 0,
+                                                   0, 0);
+        wir = __builtin_amdgcn_mfma_f32_16x16x4f32(ayi[t][v], qir[v], wir, 0,
+                                                   0, 0);
+        wii = __builtin_amdgcn_mfma_f32_16x16x4f32(ayi[t][v], qii[v], wii, 0,
+                                                   0, 0);
+      }
+      // register v of lane (c, g4): the cell (x' = c, y' = 4 g4 + v, z);
+      // V = w exp(+i k.c) (outer - inner) in polar form
+      const float2 *pt = ph + t * nax;
+      const uint32_t cx = c < nx ? c : 0u;
+      float2 e0 = cmulf(pt[cx], pt[nx + ny + z]);
+      e0.x *= wk[t];
+      e0.y *= wk[t];
+#pragma unroll
+      for (uint32_t v = 0; v < 4; ++v) {
+        const uint32_t yo = 4 * g4 + v < ny ? 4 * g4 + v : 0u;
+        const float dr = wor[v] - wir[v], di = woi[v] - wii[v];
+        const float2 e = cmulf(e0, pt[nx + yo]);
+        const float vr = dr * e.x - di * e.y, vi = dr * e.y + di * e.x;
+        const float mag = __builtin_amdgcn_sqrtf(fmaf(vr, vr, vi * vi));
+        val[t][v] = make_float2(kr[t] * mag, arg_rev(vr, vi));
+      }
+    }
+    // each cell's NPW consecutive nodes with one store
+    if (c < nx) {
+#pragma unroll
+      for (uint32_t v = 0; v < 4; ++v) {
+        const uint32_t yo = 4 * g4 + v;
+        if (yo < ny) {
+          float2 *dst = R + static_cast<size_t>((c * ny + yo) * nz + z) * Kp + n0;
+          if constexpr (NPW == 1) {
+            dst[0] = val[0][v];
+          } else {
+#pragma unroll
+            for (uint32_t hh = 0; hh < NPW / 2; ++hh)
+              reinterpret_cast<float4 *>(dst)[hh] =
+                  make_float4(val[2 * hh][v].x, val[2 * hh][v].y,
+                              val[2 * hh + 1][v].x, val[2 * hh + 1][v].y);
+          }
+        }
+      }
+    }
+  }
+}
+
+// Shell sums of tiles tile0 .. tile0+ntile-1 of NPW nodes, one workgroup of
+// four waves per tile.  (1) Consecutive threads read consecutive cells' S
+// slots of the tile's nodes and write Y into the tables, which then take
+// their running sums along z, one thread per (node, line).  (2) Each wave
+// takes every fourth z slice and sums it for all NPW nodes (independent
+// chains of matrix instructions), so each lane holds a cell's NPW
+// consecutive nodes and stores them at once (R is cell-major: a lane
+// storing one node alone writes 8 bytes into a line of its own).  The tiles
+// of XCD x (workgroups b with b % 8 == x) are a contiguous eighth of the
+// range, so the tiles that share S node runs and R lines run on one XCD at
+// once.  (Persistent workgroups, two per CU taking their XCD's tiles in
+// rounds, ran 0.68 ms alone at 288k atoms against 0.72 ms, but the tuned
+// step, where S(k) and the back-projection share the CUs, took 4.77 ms
+// against 4.72.)
+template <uint32_t NPW>
+__global__ __launch_bounds__(kRmtMfmaBlock) void rmt_mfma_kernel(
+    const float2 *__restrict__ S, const double4 *__restrict__ k64,
+    const uint32_t *__restrict__ group, const uint32_t *__restrict__ r_in,
+    const uint32_t *__restrict__ r_out, uint32_t Kp, uint32_t tile0,
+    uint32_t ntile, uint32_t nx, uint32_t ny, uint32_t nz, double h,
+    float2 *__restrict__ R) {
+  static_assert(NPW == 1 || NPW == 2 || NPW == 4, "nodes per workgroup");
+  // [NPW][nx ny][zp] tables, [NPW][nax] axis factors, [NPW][nx ny] line
+  // factors
+  extern __shared__ float2 rmt_mfma_lds[];
+  uint32_t tile = blockIdx.x;
+  if (gridDim.x >= 8) {
+    const uint32_t x = blockIdx.x & 7u, q = ntile >> 3, r = ntile & 7u;
+    tile = x * q + (x < r ? x : r) + (blockIdx.x >> 3);
+  }
+  rmt_mfma_tile<NPW>(S, k64, group, r_in, r_out, Kp, (tile0 + tile) * NPW, nx,
+                     ny, nz, h, R, rmt_mfma_lds);
+}
+
 // ---- global-memory tables (grids too large for LDS) ----
 
 // G[c*Kp + k] = S (node runs) converted to FP64 cell-major order;
@@ -2726,7 +3010,10 @@ void prepare_kernels(size_t lds_limit) {
                        reinterpret_cast<const void *>(&rmt_lds_batch_kernel<true>),
                        reinterpret_cast<const void *>(&rmt_lds_batch_kernel<false>),
                        reinterpret_cast<const void *>(&step_fused_kernel<2>),
-                       reinterpret_cast<const void *>(&step_fused_kernel<3>)};
+                       reinterpret_cast<const void *>(&step_fused_kernel<3>),
+                       reinterpret_cast<const void *>(&rmt_mfma_kernel<1>),
+                       reinterpret_cast<const void *>(&rmt_mfma_kernel<2>),
+                       reinterpret_cast<const void *>(&rmt_mfma_kernel<4>)};
   for (const void *f : fns) {
     const hipError_t e = hipFuncSetAttribute(
         f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
@@ -2850,6 +3137,46 @@ bool rmt_lds(bool mixed, uint32_t tmax, const void *S, const double4 *k64,
         S, k64, group, r_in, r_out, Kp, T, tile0, ntile, g.nx, g.ny, g.nz,
         g.h, R);
   }
+  return true;
+}
+
+// Nodes per tile of the matrix-core shell sums: two (two workgroups of 63 KB
+// share a CU at 15^3) where they fit, else one.
+#ifndef GLST_RMT_MFMA_NPW
+#define GLST_RMT_MFMA_NPW 2
+#endif
+uint32_t rmt_mfma_nodes(const GridShape &g, size_t lds_limit) {
+  if (g.nx > kRmtMfmaMaxXY || g.ny > kRmtMfmaMaxXY || g.ncell == 0) return 0;
+  for (uint32_t npw = GLST_RMT_MFMA_NPW; npw >= 1; npw >>= 1)
+    if (rmt_mfma_lds_bytes(g.nx, g.ny, g.nz, npw) <= lds_limit) return npw;
+  return 0;
+}
+
+bool rmt_mfma(const void *S, const double4 *k64, const uint32_t *group,
+              const uint32_t *r_in, const uint32_t *r_out, uint32_t Kp,
+              const GridShape &g, size_t lds_limit, void *R, hipStream_t s,
+              uint32_t tile0, uint32_t tile1) {
+  const uint32_t npw = rmt_mfma_nodes(g, lds_limit);
+  if (npw == 0) return false;
+  const uint32_t all = Kp / npw;  // Kp is a multiple of 256
+  if (tile1 > all) tile1 = all;
+  if (tile0 >= tile1) return true;
+  const uint32_t ntile = tile1 - tile0;
+  const size_t lds = rmt_mfma_lds_bytes(g.nx, g.ny, g.nz, npw);
+  const float2 *Sf = static_cast<const float2 *>(S);
+  float2 *Rf = static_cast<float2 *>(R);
+  if (npw == 4)
+    rmt_mfma_kernel<4><<<ntile, kRmtMfmaBlock, lds, s>>>(
+        Sf, k64, group, r_in, r_out, Kp, tile0, ntile, g.nx, g.ny, g.nz, g.h,
+        Rf);
+  else if (npw == 2)
+    rmt_mfma_kernel<2><<<ntile, kRmtMfmaBlock, lds, s>>>(
+        Sf, k64, group, r_in, r_out, Kp, tile0, ntile, g.nx, g.ny, g.nz, g.h,
+        Rf);
+  else
+    rmt_mfma_kernel<1><<<ntile, kRmtMfmaBlock, lds, s>>>(
+        Sf, k64, group, r_in, r_out, Kp, tile0, ntile, g.nx, g.ny, g.nz, g.h,
+        Rf);
   return true;
 }
 

@@ -172,6 +172,26 @@ bool rmt_lds(bool mixed, uint32_t tmax, const void *S, const double4 *k64,
              const uint32_t *r_out, uint32_t Kp, const GridShape &g,
              size_t lds_limit, void *R, hipStream_t s, uint32_t tile0 = 0,
              uint32_t tile1 = 0xffffffffu);
+// Shell sums on the matrix cores (CT = float, nx and ny up to 16):
+// workgroups of kRmtMfmaBlock threads take tiles of NPW consecutive nodes
+// and hold NPW z-running-sum tables of nx * ny * rmt_zpitch(nz) float2, the
+// nodes' nx + ny + nz axis factors and nx ny line factors, in LDS.
+constexpr uint32_t kRmtMfmaBlock = 256;
+constexpr uint32_t kRmtMfmaMaxXY = 16;
+inline size_t rmt_mfma_lds_bytes(uint32_t nx, uint32_t ny, uint32_t nz,
+                                 uint32_t npw) {
+  const size_t nline = static_cast<size_t>(nx) * ny;
+  return npw * (nline * (rmt_zpitch(nz) + 1) + nx + ny + nz) * 8;
+}
+// Nodes per workgroup tile of the matrix-core shell sums on this grid (0:
+// the grid is too wide for them, or one node's table does not fit).
+uint32_t rmt_mfma_nodes(const GridShape &g, size_t lds_limit);
+// Tiles [tile0, tile1) of rmt_mfma_nodes() nodes (tile1 clamped to
+// Kp / nodes).  False: the grid is not supported (nothing launched).
+bool rmt_mfma(const void *S, const double4 *k64, const uint32_t *group,
+              const uint32_t *r_in, const uint32_t *r_out, uint32_t Kp,
+              const GridShape &g, size_t lds_limit, void *R, hipStream_t s,
+              uint32_t tile0 = 0, uint32_t tile1 = 0xffffffffu);
 // Global-memory tables: G (ncell*Kp double2, scratch) receives S in FP64
 // cell-major order, is prefix-summed in place, and the shells are gathered
 // from it.  S is left unchanged.

@@ -180,11 +180,15 @@ glst_status_t glst_get_rmt(glst_handle_t h, double *re, double *im);
  *   second stream of the handle, started with the structure factors (1) or
  *   after them, overlapping the shell sums and the back-projection (2); 0
  *   runs the stages one after another.  The results are bitwise identical.
- *   "rmt_shape" (default 0): shell sums with LDS-resident tables of as
- *   many nodes per workgroup as fit (a power of two up to 64) (0); always
- *   one node per workgroup (1); or the global-memory tables otherwise used
- *   when one node's table does not fit in LDS (2).  Results agree to FP64
- *   rounding.
+ *   "rmt_shape" (default 4): shell sums with LDS-resident summed-volume
+ *   tables of as many nodes per workgroup as fit (a power of two up to 64)
+ *   (0); always one node per workgroup (1); the global-memory tables
+ *   otherwise used when one node's table does not fit in LDS (2); for
+ *   CT = float on grids at most 16 cells wide in x and y, separable window
+ *   sums on the matrix cores (FP32 direct sums; elsewhere as 0) (3); or 3
+ *   where nx * ny >= 128 (a z slice fills at least half of the matrix
+ *   cores' 16 x 16 tile), else 0 (4).  Shapes 0-2 agree to FP64 rounding,
+ *   shape 3 with them to FP32 rounding of the remote sums.
  *   "pipeline" (default 1): CT = float with LDS-resident tables only;
  *   calc_ener_force overlaps the stages of successive alpha groups: the
  *   shell sums of group g run on a third stream once S(k) of group g is
@@ -209,7 +213,8 @@ glst_status_t glst_get_option(glst_handle_t h, const char *name,
  * these inputs under candidate schedules, `reps` steps each, and keeps the
  * fastest, one knob at a time: "graph" (0, 1; tuned first and again last),
  * "pipeline" (0, 1; CT = float), "overlap_near" (0, 1, 2), "lr_blocks"
- * (powers of two from 1024 to 65536), "rmt_shape" (0, 1).  Clears the
+ * (powers of two from 1024 to 65536), "rmt_shape" (0, 1 and, where it
+ * applies, 3).  Clears the
  * accumulators afterwards; *best_ms (may be NULL) receives the chosen
  * schedule's time per step.  Handles of more than 300,000 atoms are refused
  * with GLST_ERR_STATE and keep the default schedule (a host fault seen once
