@@ -81,6 +81,9 @@ struct glst_engine {
   // mixed back-projection: 1 over atom groups (eight same-cell atoms share
   // each remote-sum load), 0 one atom per lane
   int lr_atom_groups = 1;
+  // mixed structure factors: 1 the Chebyshev form for the alpha groups where
+  // it saves work (plan_chebyshev), 0 the direct sum everywhere
+  int sf_cheb = 1;
   hipStream_t rmt_stream = nullptr;
   std::vector<hipEvent_t> ev_sf, ev_rmt;  // per alpha group
   // back-projection workgroups aimed for (many short workgroups balance the
@@ -105,6 +108,8 @@ struct glst_engine {
   // back-projection tasks (one per group and direction) and their splits
   DeviceBuffer t_node0, t_nr, t_koff, t_dir64, t_dir32, kap64, kap32, t_split;
   uint32_t ntask = 0;
+  // Chebyshev matrices of the mixed structure factors (plan_chebyshev)
+  DeviceBuffer cheb;
   // fields
   DeviceBuffer S, R, partial;
   DeviceBuffer rmt_work;  // FP64 tables of the global shell-sum path
@@ -132,7 +137,8 @@ struct glst_engine {
                                   &r_in,      &r_out, &S,          &R,
                                   &partial,   &atoms32, &t_node0,   &t_nr,
                                   &t_koff,    &t_dir64, &t_dir32,   &kap64,
-                                  &kap32,     &t_split, &near4, &atom_grp, &ag_start};
+                                  &kap32,     &t_split, &near4, &atom_grp, &ag_start,
+                                  &cheb};
     size_t s = 0;
     for (auto b : bufs) s += b->bytes();
     return s;
@@ -242,6 +248,19 @@ struct glst_engine {
     lr_groups = std::move(P.lr_groups);
     sf_groups = std::move(P.sf_groups);
     nsplit = P.nsplit;
+    if (mixed()) {
+      const std::vector<float> tab =
+          glst::plan_chebyshev(nodes, grid.h, sf_groups);
+      cheb.resize(tab.size() * sizeof(float));
+      if (!tab.empty())
+        GLST_HIP(hipMemcpy(cheb.as<void>(), tab.data(),
+                           tab.size() * sizeof(float),
+                           hipMemcpyHostToDevice));
+    }
+  }
+  // the Chebyshev table when the option allows it (nullptr: direct sums)
+  const float *cheb_table() const {
+    return sf_cheb && mixed() && cheb.bytes() ? cheb.as<float>() : nullptr;
   }
 
   void upload_nodes() {
@@ -322,7 +341,7 @@ struct glst_engine {
     Scope sc(timer, "sf", stream);
     dev::sf(mixed(), lr_atoms(), cell_start.as<uint32_t>(),
             cell_sorted.as<uint32_t>(), k64.as<double4>(), task_args(),
-            sf_groups, Kp, grid, S.as<void>(), stream);
+            sf_groups, Kp, grid, S.as<void>(), stream, cheb_table());
     check_launch("sf");
     sf_done = true;
     rmt_done = false;
@@ -440,7 +459,8 @@ struct glst_engine {
       for (size_t g = 0; g < sf_groups.size(); ++g) {
         dev::sf(true, lr_atoms(), cell_start.as<uint32_t>(),
                 cell_sorted.as<uint32_t>(), k64.as<double4>(), ta,
-                &sf_groups[g], 1, Kp, grid, S.as<void>(), stream);
+                &sf_groups[g], 1, Kp, grid, S.as<void>(), stream,
+                cheb_table());
         GLST_HIP(hipEventRecord(ev_sf[g], stream));
       }
       check_launch("sf");
@@ -1160,6 +1180,9 @@ glst_status_t glst_set_option(glst_handle_t h, const char *name,
       if (value != 0 && value != 1)
         throw ArgError("lr_atom_groups must be 0 or 1");
       h->lr_atom_groups = static_cast<int>(value);
+    } else if (n == "sf_cheb") {
+      if (value != 0 && value != 1) throw ArgError("sf_cheb must be 0 or 1");
+      h->sf_cheb = static_cast<int>(value);
     } else
       throw ArgError("unknown option \"" + n + "\"");
   });
@@ -1183,6 +1206,8 @@ glst_status_t glst_get_option(glst_handle_t h, const char *name,
       *value = h->use_graph ? 1 : 0;
     else if (n == "lr_atom_groups")
       *value = h->lr_atom_groups;
+    else if (n == "sf_cheb")
+      *value = h->sf_cheb;
     else
       throw ArgError("unknown option \"" + n + "\"");
   });

@@ -2022,6 +2022,102 @@ __device__ __forceinline__ void sf_mixed_any_body(
   }
 }
 
+// Chebyshev form (plan_chebyshev): per atom the lane projects x = s.d / rho
+// and adds q T_m(x), m < M, to its M moments (the recurrence T_{m+1} =
+// 2 x T_m - T_{m-1} carried on q T_m: two FMAs per term); then S_i =
+// sum_m C[i M + m] moment_m over the group's matrix C (wave-uniform, scalar
+// loads), even m into the real part, odd m into the imaginary part.  A cell
+// with a far atom, or with an atom farther than dlim from its centre in some
+// axis (binned into an edge cell from outside the grid), takes the direct
+// sum instead (uniform over the wave: one vector read of the cell's atoms
+// and a ballot).
+template <int M>
+__device__ __forceinline__ void sf_cheb_body(
+    const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
+    const uint32_t *__restrict__ cell_sorted, const LrTasks &tk,
+    uint32_t task0, uint32_t ntask, uint32_t ncell, float2 *__restrict__ S,
+    uint32_t cell0, uint32_t chunk, const float *__restrict__ C,
+    float inv_rho, float dlim) {
+  const uint32_t lane = threadIdx.x % kSfDirs;
+  const uint32_t cell = cell0 + threadIdx.x / kSfDirs;
+  if (cell >= ncell) return;  // uniform over the wave
+  const uint32_t a0 = start[cell], a1 = start[cell + 1];
+  uint32_t bad = 0u;
+  for (uint32_t j = a0 + lane; j < a1; j += kSfDirs) {
+    const float4 a = atoms[j];
+    const float m = fmaxf(fabsf(a.x), fmaxf(fabsf(a.y), fabsf(a.z)));
+    bad |= (cell_sorted[j] & kFarAtom) | (m > dlim ? 1u : 0u);
+  }
+  if (__ballot(bad != 0u) != 0) {
+    sf_mixed_any_body(atoms, start, cell_sorted, tk, task0, ntask, ncell, S,
+                      cell0, chunk);
+    return;
+  }
+  const uint32_t t = task0 + chunk * kSfDirs + lane;
+  if (t >= task0 + ntask) return;
+  const float4 d = tk.dir32[t];
+  const float sx = d.x * inv_rho, sy = d.y * inv_rho, sz = d.z * inv_rho;
+  float mom[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) mom[m] = 0.f;
+  for (uint32_t j = a0; j < a1; ++j) {
+    const float4 a = atoms[j];
+    const float x = fmaf(sx, a.x, fmaf(sy, a.y, sz * a.z));
+    const float q = a.w;
+    mom[0] += q;
+    if constexpr (M > 1) {
+      float u0 = q, u1 = q * x;
+      mom[1] += u1;
+      const float x2 = x + x;
+#pragma unroll
+      for (int m = 2; m < M; ++m) {
+        const float u = fmaf(x2, u1, -u0);
+        mom[m] += u;
+        u0 = u1;
+        u1 = u;
+      }
+    }
+  }
+  const uint32_t nr = tk.nr[task0];
+  const uint32_t n0 = tk.node0[t];
+  for (uint32_t i = 0; i < nr; ++i) {
+    const float *c = C + i * M;
+    float re = 0.f, im = 0.f;
+#pragma unroll
+    for (int m = 0; m < M; m += 2) re = fmaf(c[m], mom[m], re);
+#pragma unroll
+    for (int m = 1; m < M; m += 2) im = fmaf(c[m], mom[m], im);
+    S[sf_slot<true>(n0 + i, cell, ncell)] = make_float2(re, im);
+  }
+}
+
+template <int M>
+__global__ __launch_bounds__(kSfDirs) void sf_cheb_kernel(
+    const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
+    const uint32_t *__restrict__ cell_sorted, LrTasks tk, uint32_t task0,
+    uint32_t ntask, uint32_t ncell, float2 *__restrict__ S,
+    const float *__restrict__ C, float inv_rho, float dlim) {
+  // gridDim.x = cells
+  const uint32_t wg = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x,
+                                gridDim.x * gridDim.y);
+  const uint32_t ncg = gridDim.x;
+  sf_cheb_body<M>(atoms, start, cell_sorted, tk, task0, ntask, ncell, S,
+                  wg % ncg, wg / ncg, C, inv_rho, dlim);
+}
+
+typedef void (*SfChebKernel)(const float4 *, const uint32_t *,
+                             const uint32_t *, LrTasks, uint32_t, uint32_t,
+                             uint32_t, float2 *, const float *, float, float);
+
+// index M - 1 (M = 1 .. kChebMaxTerms)
+template <int... N>
+constexpr auto sf_cheb_table(std::integer_sequence<int, N...>) {
+  return std::array<SfChebKernel, sizeof...(N)>{{&sf_cheb_kernel<N + 1>...}};
+}
+
+const auto kSfChebKernels =
+    sf_cheb_table(std::make_integer_sequence<int, kChebMaxTerms>{});
+
 // (cell, direction chunk) of a workgroup, numbered so that each XCD runs
 // consecutive cells.  amdgpu_waves_per_eu(1, 10) lets the compiler size the
 // registers of the unrolled radial loop for as few as one wave per SIMD.
@@ -3086,7 +3182,8 @@ void bin_scatter(bool mixed, const double *rx, const double *ry,
 void sf(bool mixed, const void *atoms, const uint32_t *start,
         const uint32_t *cell_sorted, const double4 *k64,
         const LrTaskArgs &ta, const SfGroup *groups, size_t ngroup,
-        uint32_t Kp, const GridShape &g, void *S, hipStream_t s) {
+        uint32_t Kp, const GridShape &g, void *S, hipStream_t s,
+        const float *cheb) {
   if (!mixed) {
     const dim3 grid(g.nx * g.ny, Kp / kSfBlock);
     sf_f64_kernel<<<grid, kSfBlock, 0, s>>>(static_cast<const double4 *>(atoms),
@@ -3096,13 +3193,22 @@ void sf(bool mixed, const void *atoms, const uint32_t *start,
   }
   const LrTasks tk{ta.node0, ta.nr,      ta.koff,   ta.dir64,
                    ta.dir32, ta.kappa64, ta.kappa32, ta.split};
+  const float inv_rho = static_cast<float>(1.0 / cheb_rho(g.h));
+  const float dlim = static_cast<float>(cheb_dlim(g.h));
   for (size_t q = 0; q < ngroup; ++q) {
     const SfGroup &gr = groups[q];
     if (gr.ntask == 0) continue;
+    const dim3 grid(g.ncell, (gr.ntask + kSfDirs - 1) / kSfDirs);
+    if (cheb && gr.m >= 1 && gr.m <= kChebMaxTerms) {
+      kSfChebKernels[gr.m - 1]<<<grid, kSfDirs, 0, s>>>(
+          static_cast<const float4 *>(atoms), start, cell_sorted, tk,
+          gr.task0, gr.ntask, g.ncell, static_cast<float2 *>(S),
+          cheb + gr.coff, inv_rho, dlim);
+      continue;
+    }
     const int nr = gr.nr <= static_cast<uint32_t>(kSfMaxUnrolled)
                        ? static_cast<int>(gr.nr)
                        : 0;
-    const dim3 grid(g.ncell, (gr.ntask + kSfDirs - 1) / kSfDirs);
     kSfNrKernels[nr]<<<grid, kSfDirs, 0, s>>>(
         static_cast<const float4 *>(atoms), start, cell_sorted, tk, gr.task0,
         gr.ntask, g.ncell, static_cast<float2 *>(S));

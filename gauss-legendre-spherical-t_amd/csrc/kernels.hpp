@@ -93,9 +93,22 @@ __host__ __device__ inline size_t sf_slot(uint32_t node, uint32_t cell,
 constexpr int kSfMaxUnrolled = 40;
 
 struct LrTaskArgs;
-// Tasks (directions) of one alpha group for the mixed structure factors.
+// Chebyshev form of the mixed structure factors (plan_chebyshev in
+// runtime.hpp): a cell's atoms within dlim = (1 + kChebMargin) h / 2 of its
+// centre in every axis have |s.d| <= rho = sqrt(3) dlim.
+constexpr double kChebMargin = 1e-3;
+constexpr uint32_t kChebMaxTerms = 40;
+inline double cheb_dlim(double h) { return (1.0 + kChebMargin) * 0.5 * h; }
+inline double cheb_rho(double h) {
+  return 1.7320508075688772 * cheb_dlim(h);
+}
+
+// Tasks (directions) of one alpha group for the mixed structure factors; m
+// Chebyshev terms (0: the direct sum) with their matrix at coff of the
+// Chebyshev table.
 struct SfGroup {
   uint32_t nr, task0, ntask;
+  uint32_t m = 0, coff = 0;
 };
 // FP64: S[sf_slot(k, c)] (double2) = running sum along z of
 // sum_{j in cell} q_j exp(-i k.r_j).  Mixed: S[sf_slot(k, c)] (float2) =
@@ -103,17 +116,23 @@ struct SfGroup {
 // centre; one lane per direction of an alpha group, the group's radii
 // unrolled (padded node columns are not written).
 // The mixed path launches once per listed group (groups[0..ngroup)); the
-// FP64 path ignores the list and covers every node in one launch.
+// FP64 path ignores the list and covers every node in one launch.  Mixed
+// groups with m > 0 take the Chebyshev form (cheb: the device table of
+// plan_chebyshev for this grid's cell edge); a cell holding an atom more
+// than half a cell edge (with the margin) from its centre, or a far atom,
+// takes the direct sum.
 void sf(bool mixed, const void *atoms, const uint32_t *start,
         const uint32_t *cell_sorted, const double4 *k64,
         const LrTaskArgs &tasks, const SfGroup *groups, size_t ngroup,
-        uint32_t Kp, const GridShape &g, void *S, hipStream_t s);
+        uint32_t Kp, const GridShape &g, void *S, hipStream_t s,
+        const float *cheb = nullptr);
 inline void sf(bool mixed, const void *atoms, const uint32_t *start,
                const uint32_t *cell_sorted, const double4 *k64,
                const LrTaskArgs &tasks, const std::vector<SfGroup> &groups,
-               uint32_t Kp, const GridShape &g, void *S, hipStream_t s) {
+               uint32_t Kp, const GridShape &g, void *S, hipStream_t s,
+               const float *cheb = nullptr) {
   sf(mixed, atoms, start, cell_sorted, k64, tasks, groups.data(),
-     groups.size(), Kp, g, S, s);
+     groups.size(), Kp, g, S, s, cheb);
 }
 
 // ---- remote (shell) sums (sum_rmt_sf) --------------------------------------

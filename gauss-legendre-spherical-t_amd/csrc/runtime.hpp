@@ -392,6 +392,99 @@ inline SplitPlan make_splits(const NodeTable &nodes, uint32_t K,
   return P;
 }
 
+// ---- Chebyshev form of the mixed structure factors -------------------------
+//
+// The atoms of a cell lie within half a cell edge of its centre in every
+// axis, so p = s.d stays in [-rho, rho] (cheb_rho: sqrt(3) h / 2 with a
+// margin for rounding), and (Jacobi-Anger, x = p / rho, eps_0 = 1, eps_m = 2)
+//   exp(-i kappa p) = sum_m (-i)^m eps_m J_m(kappa rho) T_m(x).
+// S at the NR radial nodes of a direction is then a fixed NR x M matrix
+// applied to the cell's M Chebyshev moments sum_j q_j T_m(x_j), which cost two
+// FMAs per (atom, term) against a multiply, a sine, a cosine and two FMAs
+// per (atom, radial node).  M is the smallest count whose tail bound
+// 2 sum_{m >= M} |J_m(kappa_max rho)| is below kChebTol.
+constexpr double kChebTol = 3e-8;
+using dev::kChebMaxTerms;
+using dev::cheb_rho;
+
+// J_0(z) .. J_{n-1}(z) for z >= 0: Miller's backward recurrence
+// J_{m-1} = (2m / z) J_m - J_{m+1} from well above max(n, z), normalised by
+// J_0 + 2 (J_2 + J_4 + ...) = 1.
+inline void bessel_j(double z, uint32_t n, double *out) {
+  if (n == 0) return;
+  if (!(z > 0.0)) {
+    out[0] = 1.0;
+    for (uint32_t m = 1; m < n; ++m) out[m] = 0.0;
+    return;
+  }
+  const double big = std::max(static_cast<double>(n), z);
+  const uint32_t top =
+      2 * ((static_cast<uint32_t>(big) + 20 +
+            static_cast<uint32_t>(std::sqrt(40.0 * big))) / 2);
+  std::vector<double> b(top + 2, 0.0);
+  b[top] = 1e-30;
+  for (uint32_t m = top; m >= 1; --m) {
+    b[m - 1] = (2.0 * m / z) * b[m] - b[m + 1];
+    if (std::fabs(b[m - 1]) > 1e250)
+      for (uint32_t k = m - 1; k <= top; ++k) b[k] *= 1e-250;
+  }
+  double norm = b[0];
+  for (uint32_t m = 2; m <= top; m += 2) norm += 2.0 * b[m];
+  for (uint32_t m = 0; m < n; ++m) out[m] = m <= top ? b[m] / norm : 0.0;
+}
+
+// Terms needed for phases up to z radians (kChebMaxTerms + 1: more than the
+// kernels hold).
+inline uint32_t cheb_terms(double z) {
+  constexpr uint32_t kTail = 40;
+  std::vector<double> J(kChebMaxTerms + kTail + 1);
+  bessel_j(z, static_cast<uint32_t>(J.size()), J.data());
+  for (uint32_t M = 1; M <= kChebMaxTerms; ++M) {
+    double tail = 0.0;
+    for (uint32_t m = M; m < J.size(); ++m) tail += std::fabs(J[m]);
+    if (2.0 * tail < kChebTol) return M;
+  }
+  return kChebMaxTerms + 1;
+}
+
+// Chooses the Chebyshev form for the groups where it takes fewer issue
+// slots per (atom, direction) than the direct sum (2 M + 4 plus the matrix,
+// NR M over at least 32 atoms, against about 6 NR: the sine and cosine cost
+// about 1.6 FMA slots each) and fills their matrices: for group g with
+// sf_groups[g].m = M > 0, table[coff + i M + m] = eps_m J_m(kappa_i rho) times
+// the real (m even: (-1)^(m/2)) or imaginary (m odd: (-1)^((m+1)/2)) part of
+// (-i)^m.  Groups left at m = 0 keep the direct sum.
+inline std::vector<float> plan_chebyshev(const NodeTable &nodes, double h,
+                                         std::vector<dev::SfGroup> &groups) {
+  std::vector<float> table;
+  const double rho = cheb_rho(h);
+  for (size_t g = 0; g < groups.size() && g < nodes.kappa.size(); ++g) {
+    dev::SfGroup &sg = groups[g];
+    sg.m = 0;
+    sg.coff = 0;
+    const std::vector<double> &kap = nodes.kappa[g];
+    if (kap.empty() || sg.ntask == 0) continue;
+    const double zmax = *std::max_element(kap.begin(), kap.end()) * rho;
+    const uint32_t M = cheb_terms(zmax);
+    const double nr = static_cast<double>(kap.size());
+    if (M > kChebMaxTerms || 2.0 * M + 4.0 + nr * M / 32.0 >= 6.0 * nr)
+      continue;
+    sg.m = M;
+    sg.coff = static_cast<uint32_t>(table.size());
+    std::vector<double> J(M);
+    for (double k : kap) {
+      bessel_j(k * rho, M, J.data());
+      for (uint32_t m = 0; m < M; ++m) {
+        const double eps = m == 0 ? 1.0 : 2.0;
+        const double sign = (m % 2 == 0) ? ((m / 2) % 2 ? -1.0 : 1.0)
+                                         : (((m + 1) / 2) % 2 ? -1.0 : 1.0);
+        table.push_back(static_cast<float>(eps * sign * J[m]));
+      }
+    }
+  }
+  return table;
+}
+
 // Shell-sum chunks of the far field pipelined across alpha groups.  Group g
 // owns device nodes [first_g, first_g + count_g); the shell sums run in
 // workgroup tiles of T consecutive nodes, Kp / T tiles in all.  Chunk g is
