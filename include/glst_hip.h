@@ -200,6 +200,12 @@ glst_status_t glst_get_rmt(glst_handle_t h, double *re, double *im);
  *   "lr_atom_groups" (default 1): CT = float back-projection over atom
  *   groups (up to 4 same-cell atoms share each remote-sum load, 8 lanes per
  *   group) (1), or one atom per lane (0).  Results agree to FP32 rounding.
+ *   "sf_cheb" (default 1): CT = float structure factors as the Chebyshev
+ *   moments of each cell's atoms times a matrix of Bessel values (the
+ *   Jacobi-Anger expansion of exp(-i k.d) over the cell, truncated below
+ *   3e-8), in the alpha groups where that takes fewer operations than the
+ *   direct sum (1), or direct sums everywhere (0).  Results agree to FP32
+ *   rounding.
  *   "graph" (default 0): calc_ener_force captures the whole step as a HIP
  *   graph for the input arrays of its first call and replays it while the
  *   same arrays are passed (per-kernel timing is then not recorded).
