@@ -20,8 +20,18 @@ std::shared_ptr<const ShapeSetup> make_shape_setup(int ct, const double box[3],
                                                    double rcut, double tol) {
   auto p = std::make_shared<ShapeSetup>();
   p->su = setup_system(ct, box[0], box[1], box[2], rcut, tol);
-  if (p->su.has_lr && p->su.K > 0)
+  if (p->su.has_lr && p->su.K > 0) {
     p->nl = layout_nodes(p->su.nodes, p->su.K, p->su.Kp);
+    if (ct == GLST_CT_FLOAT) {
+      std::vector<dev::SfGroup> groups =
+          make_splits(p->su.nodes, p->su.K, 1).sf_groups;
+      p->cheb = plan_chebyshev(p->su.nodes, p->su.grid.h, groups);
+      for (const dev::SfGroup &g : groups) {
+        p->cheb_m.push_back(g.m);
+        p->cheb_coff.push_back(g.coff);
+      }
+    }
+  }
   return p;
 }
 
@@ -59,8 +69,9 @@ size_t BatchLayout::device_bytes() const {
 size_t ShapeTables::shape_bytes(const ShapeSetup &shape) {
   const NodeLayout &L = shape.nl;
   return L.k64.size() * (sizeof(double4) + 4) +
-         L.node0.size() * (12 + sizeof(double4) + sizeof(float4)) +
-         L.kappa64.size() * 12 + shape.su.groups.r_in.size() * 8;
+         L.node0.size() * (16 + sizeof(double4) + sizeof(float4)) +
+         L.kappa64.size() * 12 + shape.su.groups.r_in.size() * 8 +
+         shape.cheb.size() * 4;
 }
 
 const ShapeTables::Offsets &ShapeTables::add(
@@ -73,6 +84,7 @@ const ShapeTables::Offsets &ShapeTables::add(
   o.task0 = static_cast<uint32_t>(t_node0.size());
   o.kap0 = static_cast<uint32_t>(kap64.size());
   o.group0 = static_cast<uint32_t>(r_in.size());
+  o.cheb0 = static_cast<uint32_t>(cheb.size());
   if (k64.size() + L.k64.size() > 0xFFFFFFFFull ||
       t_node0.size() + L.node0.size() > 0xFFFFFFFFull)
     throw ArgError("shape tables exceed 2^32 entries");
@@ -85,6 +97,19 @@ const ShapeTables::Offsets &ShapeTables::add(
     t_dir64.push_back(L.dir64[t]);
     t_dir32.push_back(L.dir32[t]);
   }
+  // the Chebyshev matrix of each task's group (one task per direction,
+  // group-major)
+  {
+    const auto &dirs = shape->su.nodes.dirs;
+    size_t t = 0;
+    for (size_t g = 0; g < dirs.size() && t < L.node0.size(); ++g)
+      for (size_t j = 0; j < dirs[g].size() && t < L.node0.size(); ++j, ++t)
+        t_coff.push_back(o.cheb0 + (g < shape->cheb_coff.size()
+                                        ? shape->cheb_coff[g]
+                                        : 0u));
+    for (; t < L.node0.size(); ++t) t_coff.push_back(o.cheb0);
+  }
+  cheb.insert(cheb.end(), shape->cheb.begin(), shape->cheb.end());
   kap64.insert(kap64.end(), L.kappa64.begin(), L.kappa64.end());
   kap32.insert(kap32.end(), L.kappa32.begin(), L.kappa32.end());
   r_in.insert(r_in.end(), shape->su.groups.r_in.begin(),
@@ -113,16 +138,19 @@ void ShapeTables::upload(hipStream_t s) {
       {&d_dir64, t_dir64.data(), t_dir64.size(), sizeof(double4)},
       {&d_dir32, t_dir32.data(), t_dir32.size(), sizeof(float4)},
       {&d_kap64, kap64.data(), kap64.size(), 8},
-      {&d_kap32, kap32.data(), kap32.size(), 4}};
+      {&d_kap32, kap32.data(), kap32.size(), 4},
+      {&d_coff, t_coff.data(), t_coff.size(), 4},
+      {&d_cheb, cheb.data(), cheb.size(), 4}};
+  constexpr size_t kArr = sizeof(arrs) / sizeof(arrs[0]);
   // entries of the shapes already on the device
-  size_t done[11] = {};
+  size_t done[kArr] = {};
   if (on_dev > 0) {
     const Offsets &o = reg.at(order[on_dev]).second;
     const size_t groups = o.group0;
     const size_t tasks = o.task0, nodes = o.node0, kaps = o.kap0;
-    const size_t v[11] = {nodes, nodes, groups, groups, tasks, tasks,
-                          tasks, tasks, tasks, kaps, kaps};
-    std::copy(v, v + 11, done);
+    const size_t v[kArr] = {nodes, nodes, groups, groups, tasks, tasks, tasks,
+                            tasks, tasks, kaps,   kaps,  tasks, o.cheb0};
+    std::copy(v, v + kArr, done);
   }
   bool grow = false;
   for (const Arr &a : arrs) grow = grow || a.n * a.size > a.d->bytes();
@@ -136,10 +164,10 @@ void ShapeTables::upload(hipStream_t s) {
         a.d->release();
         a.d->allocate(want);
       }
-    std::fill(done, done + 11, size_t{0});
+    std::fill(done, done + kArr, size_t{0});
     ++gen;
   }
-  for (size_t i = 0; i < 11; ++i) {
+  for (size_t i = 0; i < kArr; ++i) {
     const Arr &a = arrs[i];
     if (a.n > done[i])
       GLST_HIP(hipMemcpyAsync(a.d->as<char>() + done[i] * a.size,
@@ -153,7 +181,9 @@ void ShapeTables::upload(hipStream_t s) {
 }
 
 void ShapeTables::clear() {
-  for (auto *v : {&group, &r_in, &r_out, &t_node0, &t_nr, &t_koff}) v->clear();
+  for (auto *v : {&group, &r_in, &r_out, &t_node0, &t_nr, &t_koff, &t_coff})
+    v->clear();
+  cheb.clear();
   k64.clear();
   t_dir64.clear();
   t_dir32.clear();
@@ -167,15 +197,15 @@ void ShapeTables::clear() {
 
 size_t ShapeTables::host_bytes() const {
   return k64.size() * (sizeof(double4) + 4) +
-         t_node0.size() * (12 + sizeof(double4) + sizeof(float4)) +
-         kap64.size() * 12 + r_in.size() * 8;
+         t_node0.size() * (16 + sizeof(double4) + sizeof(float4)) +
+         kap64.size() * 12 + r_in.size() * 8 + cheb.size() * 4;
 }
 
 size_t ShapeTables::device_bytes() const {
   size_t b = 0;
   for (const DeviceBuffer *d : {&d_k64, &d_group, &d_r_in, &d_r_out, &d_node0,
                                 &d_nr, &d_koff, &d_dir64, &d_dir32, &d_kap64,
-                                &d_kap32})
+                                &d_kap32, &d_coff, &d_cheb})
     b += d->bytes();
   return b;
 }
@@ -190,7 +220,7 @@ constexpr size_t kBatchRmtLdsDiv = GLST_BATCH_RMT_LDS_DIV;
 BatchLayout pack_batch(int ct,
                        const std::vector<std::shared_ptr<const BatchSystem>> &cs,
                        ShapeTables &tables, size_t lds_limit,
-                       uint32_t lr_blocks) {
+                       uint32_t lr_blocks, bool cheb) {
   if (cs.empty()) throw ArgError("a batch needs at least one system");
   const uint32_t nsys = static_cast<uint32_t>(cs.size());
   BatchLayout B;
@@ -311,8 +341,13 @@ BatchLayout pack_batch(int ct,
       const dev::LrGroup &lg = P.lr_groups[g];
       if (sg.ntask == 0) continue;
       const uint32_t key = sg.nr <= 40 ? sg.nr : 0;
-      Plan &pf = B.sf_nr[key];
-      pf.nr = key;
+      // S(k) launches: direct sums per radial count, Chebyshev forms per
+      // term count (keys kSfChebKey + M)
+      const std::vector<uint32_t> &cm = cs[s]->shape->cheb_m;
+      const uint32_t m = cheb && g < cm.size() ? cm[g] : 0u;
+      const uint32_t skey = m ? dev::kSfChebKey + m : key;
+      Plan &pf = B.sf_nr[skey];
+      pf.nr = skey;
       pf.add(s, d.ncell * div_up(sg.ntask, 64),
              task0 + sg.task0, sg.ntask);
       Plan &pl = B.lr_nr[key];
@@ -398,7 +433,7 @@ void glst_batch::configure(
     GLST_HIP(hipDeviceSynchronize());
     tables->clear();
   }
-  B = pack_batch(ct, cs, *tables, lds_limit, lr_blocks);
+  B = pack_batch(ct, cs, *tables, lds_limit, lr_blocks, sf_cheb != 0);
   tables->upload(stream);
   tables_gen = tables->gen;
   const size_t N = B.natom, C = B.ncell;
@@ -632,7 +667,8 @@ void glst_batch::step_async(const double *rx, const double *ry,
   else
     dev::sf_batch(mixed(), atoms, cell_start.as<uint32_t>(),
                   cell_sorted.as<uint32_t>(), tb.d_k64.as<double4>(), ta, sys,
-                  L_sf, S.as<void>(), stream);
+                  L_sf, S.as<void>(), stream, tb.d_cheb.as<float>(),
+                  tb.d_coff.as<uint32_t>());
   check_launch("calc_sf");
   dev::rmt_batch(mixed(), S.as<void>(), tb.d_k64.as<double4>(),
                  tb.d_group.as<uint32_t>(), tb.d_r_in.as<uint32_t>(),
@@ -988,6 +1024,10 @@ glst_status_t glst_batch_set_option(glst_batch_t b, const char *name,
     } else if (n == "graph") {
       if (value < 0 || value > 2) throw ArgError("graph must be 0, 1 or 2");
       b->graph_mode = static_cast<int>(value);
+    } else if (n == "sf_cheb") {
+      if (value != 0 && value != 1) throw ArgError("sf_cheb must be 0 or 1");
+      b->sf_cheb = static_cast<int>(value);
+      b->configure(b->B.ct, b->B.cs);
     } else if (n == "fused") {
       if (value < 0 || value > 3) throw ArgError("fused must be 0 .. 3");
       if (value != 0 && !b->mixed())

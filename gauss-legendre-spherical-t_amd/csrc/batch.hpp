@@ -27,6 +27,10 @@ namespace glst {
 struct ShapeSetup {
   SystemSetup su;
   NodeLayout nl;  // device node/task order (empty without a far field)
+  // CT = float: Chebyshev form of S(k) per alpha group (plan_chebyshev; m 0
+  // keeps the direct sum), the matrices at cheb_coff of `cheb`
+  std::vector<uint32_t> cheb_m, cheb_coff;
+  std::vector<float> cheb;
 };
 
 std::shared_ptr<const ShapeSetup> make_shape_setup(int ct, const double box[3],
@@ -53,7 +57,7 @@ std::shared_ptr<const BatchSystem> make_batch_system(
 // be configured again, which also drops its captured graph).
 struct ShapeTables {
   struct Offsets {
-    uint32_t node0 = 0, task0 = 0, kap0 = 0, group0 = 0;
+    uint32_t node0 = 0, task0 = 0, kap0 = 0, group0 = 0, cheb0 = 0;
   };
   std::vector<double4> k64;  // per node (kx, ky, kz, W)
   std::vector<uint32_t> group;
@@ -63,8 +67,12 @@ struct ShapeTables {
   std::vector<float4> t_dir32;
   std::vector<double> kap64;
   std::vector<float> kap32;
+  // Chebyshev matrices of all shapes, and per task the offset of its
+  // group's matrix
+  std::vector<float> cheb;
+  std::vector<uint32_t> t_coff;
   DeviceBuffer d_k64, d_group, d_r_in, d_r_out, d_node0, d_nr, d_koff,
-      d_dir64, d_dir32, d_kap64, d_kap32;
+      d_dir64, d_dir32, d_kap64, d_kap32, d_cheb, d_coff;
   std::map<const ShapeSetup *,
            std::pair<std::shared_ptr<const ShapeSetup>, Offsets>>
       reg;
@@ -124,7 +132,8 @@ struct BatchLayout {
 // Packs systems (all of precision ct) into one batch layout, adding their
 // shapes to `tables` (host side).  lds_limit: the device's LDS per workgroup
 // (a system whose one-node cell table does not fit is rejected); lr_blocks:
-// back-projection workgroups aimed for over the whole batch.
+// back-projection workgroups aimed for over the whole batch; cheb: CT =
+// float S(k) in Chebyshev form where the shapes planned it.
 // back-projection workgroups a batch aims for unless set (option lr_blocks)
 #ifndef GLST_BATCH_LR_BLOCKS
 #define GLST_BATCH_LR_BLOCKS 4096
@@ -133,7 +142,7 @@ struct BatchLayout {
 BatchLayout pack_batch(int ct,
                        const std::vector<std::shared_ptr<const BatchSystem>> &cs,
                        ShapeTables &tables, size_t lds_limit,
-                       uint32_t lr_blocks = 4096);
+                       uint32_t lr_blocks = 4096, bool cheb = true);
 
 }  // namespace glst
 
@@ -147,6 +156,9 @@ struct glst_batch {
   hipEvent_t ev_assigned = nullptr, ev_near = nullptr;
   int overlap_near = 1;
   uint32_t lr_blocks = GLST_BATCH_LR_BLOCKS;
+  // mixed S(k) in Chebyshev form where the shapes planned it (option
+  // sf_cheb; the persistent and one-launch-per-stage modes sum directly)
+  int sf_cheb = 1;
   bool assigned = false;
   // HIP graph of the step (graph_mode 1: captured on the first step with
   // given inputs; 2: on the second consecutive step with the same inputs,

@@ -32,6 +32,7 @@
 #include <cmath>
 #include <type_traits>
 #include <mutex>
+#include <stdexcept>
 #include <utility>
 
 namespace glst {
@@ -2736,6 +2737,41 @@ void sf_mixed_batch_kernel(
                       d.ncell, S + d.field0, local % ncg, local / ncg);
 }
 
+// The Chebyshev form over a batch's runs (system, alpha group): the group's
+// matrix at coff[first task], rho and dlim from the system's cell edge.
+template <int M>
+__global__ __launch_bounds__(kSfDirs) void sf_cheb_batch_kernel(
+    const float4 *__restrict__ atoms, const uint32_t *__restrict__ start,
+    const uint32_t *__restrict__ cell_sorted, LrTasks tk,
+    const SysDev *__restrict__ sys, const Seg *__restrict__ seg,
+    uint32_t nseg, float2 *__restrict__ S, const float *__restrict__ cheb,
+    const uint32_t *__restrict__ coff) {
+  const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
+  const Seg g = seg[run_of(seg, nseg, wg)];
+  const SysDev &d = sys[g.sys];
+  const uint32_t local = wg - g.blk0;
+  const uint32_t ncg = d.ncell;
+  sf_cheb_body<M>(atoms, start + d.cell0, cell_sorted, tk, g.a, g.b, d.ncell,
+                  S + d.field0, local % ncg, local / ncg, cheb + coff[g.a],
+                  static_cast<float>(1.0 / cheb_rho(d.h)),
+                  static_cast<float>(cheb_dlim(d.h)));
+}
+
+typedef void (*SfChebBatchKernel)(const float4 *, const uint32_t *,
+                                  const uint32_t *, LrTasks, const SysDev *,
+                                  const Seg *, uint32_t, float2 *,
+                                  const float *, const uint32_t *);
+
+// index M - 1 (M = 1 .. kChebMaxTerms)
+template <int... N>
+constexpr auto sf_cheb_batch_table(std::integer_sequence<int, N...>) {
+  return std::array<SfChebBatchKernel, sizeof...(N)>{
+      {&sf_cheb_batch_kernel<N + 1>...}};
+}
+
+const auto kSfChebBatchKernels =
+    sf_cheb_batch_table(std::make_integer_sequence<int, kChebMaxTerms>{});
+
 typedef void (*SfBatchKernel)(const float4 *, const uint32_t *,
                               const uint32_t *, LrTasks, const SysDev *,
                               const Seg *, uint32_t, float2 *);
@@ -3513,7 +3549,7 @@ void sf_batch(bool mixed, const void *atoms, const uint32_t *start,
               const uint32_t *cell_sorted, const double4 *k64,
               const LrTaskArgs &ta, const SysDev *sys,
               const std::vector<BatchLaunch> &launches, void *S,
-              hipStream_t s) {
+              hipStream_t s, const float *cheb, const uint32_t *coff) {
   const LrTasks tk{ta.node0, ta.nr,      ta.koff,   ta.dir64,
                    ta.dir32, ta.kappa64, ta.kappa32, ta.split};
   for (const BatchLaunch &L : launches) {
@@ -3522,6 +3558,15 @@ void sf_batch(bool mixed, const void *atoms, const uint32_t *start,
       sf_f64_batch_kernel<<<L.nblk, kSfBlock, 0, s>>>(
           static_cast<const double4 *>(atoms), start, k64, sys, L.seg, L.nseg,
           static_cast<double2 *>(S));
+      continue;
+    }
+    if (L.nr > kSfChebKey) {
+      const uint32_t M = L.nr - kSfChebKey;
+      if (M > kChebMaxTerms || !cheb || !coff)
+        throw std::runtime_error("sf_batch: no Chebyshev kernel for this launch");
+      kSfChebBatchKernels[M - 1]<<<L.nblk, kSfDirs, 0, s>>>(
+          static_cast<const float4 *>(atoms), start, cell_sorted, tk, sys,
+          L.seg, L.nseg, static_cast<float2 *>(S), cheb, coff);
       continue;
     }
     const int nr = L.nr <= static_cast<uint32_t>(kSfMaxUnrolled)

@@ -98,10 +98,15 @@ struct LrTaskArgs;
 // centre in every axis have |s.d| <= rho = sqrt(3) dlim.
 constexpr double kChebMargin = 1e-3;
 constexpr uint32_t kChebMaxTerms = 40;
-inline double cheb_dlim(double h) { return (1.0 + kChebMargin) * 0.5 * h; }
-inline double cheb_rho(double h) {
+__host__ __device__ inline double cheb_dlim(double h) {
+  return (1.0 + kChebMargin) * 0.5 * h;
+}
+__host__ __device__ inline double cheb_rho(double h) {
   return 1.7320508075688772 * cheb_dlim(h);
 }
+// Batched S(k) launches in Chebyshev form carry kSfChebKey + M in
+// BatchLaunch::nr (direct sums: the radial count, at most 40).
+constexpr uint32_t kSfChebKey = 64;
 
 // Tasks (directions) of one alpha group for the mixed structure factors; m
 // Chebyshev terms (0: the direct sum) with their matrix at coff of the
@@ -339,12 +344,15 @@ void bin_scatter_batch(bool mixed, const double *rx, const double *ry,
                        float4 *atoms32, hipStream_t s,
                        const uint32_t *ag_start = nullptr,
                        uint2 *ag = nullptr);
-// mixed: one launch per radial count (L.nr); FP64: one launch (L.nr unused)
+// mixed: one launch per radial count (L.nr), or per Chebyshev term count
+// (L.nr = kSfChebKey + M: cheb holds the shapes' matrices, coff per task the
+// offset of its group's); FP64: one launch (L.nr unused)
 void sf_batch(bool mixed, const void *atoms, const uint32_t *start,
               const uint32_t *cell_sorted, const double4 *k64,
               const LrTaskArgs &tasks, const SysDev *sys,
               const std::vector<BatchLaunch> &launches, void *S,
-              hipStream_t s);
+              hipStream_t s, const float *cheb = nullptr,
+              const uint32_t *coff = nullptr);
 void rmt_batch(bool mixed, const void *S, const double4 *k64,
                const uint32_t *group, const uint32_t *r_in,
                const uint32_t *r_out, const SysDev *sys, const BatchLaunch &L,

@@ -26,6 +26,8 @@ def main():
     ap.add_argument("--overlap", type=int, default=1,
                     help="overlap_near option of the batch (0: serial)")
     ap.add_argument("--tol", type=float, default=2e-5)
+    ap.add_argument("--sf-cheb", type=int, default=1, choices=(0, 1),
+                    help="sf_cheb option of the batch (0: direct S(k) sums)")
     args = ap.parse_args()
     import numpy as np
     import torch
@@ -41,6 +43,7 @@ def main():
             for i in range(4)]
     b = g.GLSTBatch(natoms, boxes, 9.0, args.tol, ct=args.ct)
     b.set_option("overlap_near", args.overlap)
+    b.set_option("sf_cheb", args.sf_cheb)
     b.zero_ef()
     b.calc_ener_force(*cols)
     torch.cuda.synchronize()

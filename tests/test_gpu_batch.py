@@ -307,3 +307,33 @@ def test_ensemble512_sample_matches_oracle(ensemble512):
         part = _part(ensemble512, s)
         assert rel_err(part[:3], ref[:3]) < 2e-6, s
         assert rel_err(part[3:], ref[3:]) < 2e-6, s
+
+
+def test_batch_chebyshev_sf_matches_direct(cuda):
+    """The batch's S(k) in Chebyshev form (the default) against direct sums
+    (option sf_cheb 0) and the direct FP64 Coulomb sum, on boxes of 2^3 to
+    6^3 cells at the served tolerance."""
+    import torch
+    import glst_amd as g
+    systems = _systems([150, 700, 1500, 3000, 5200], seed0=41)
+    natoms = [h[0].size for _, h in systems]
+    d = _pack(systems, torch)
+    out = []
+    for cheb in (1, 0):
+        b = g.GLSTBatch(natoms, [L for L, _ in systems], RCUT, SERVED_TOL,
+                        ct="float")
+        b.set_option("sf_cheb", cheb)
+        b.calc_ener_force(*d)
+        out.append([t.cpu().numpy() for t in b.get_ef()])
+        b.close()
+    o = 0
+    for s, (L, h) in enumerate(systems):
+        part = [a[o:o + natoms[s]] for a in out[0]]
+        ref = [a[o:o + natoms[s]] for a in out[1]]
+        assert rel_err(part[:3], ref[:3]) < 1e-6, s
+        assert rel_err(part[3:], ref[3:]) < 1e-6, s
+        direct = [t.cpu().numpy() for t in g.coulomb_direct(
+            *[torch.as_tensor(a, dtype=torch.float64, device="cuda")
+              for a in h])]
+        assert rel_err(part[:3], direct[:3]) < 1e-5, s
+        o += natoms[s]

@@ -39,9 +39,14 @@ def _pack(systems, torch):
 
 
 def _batch(systems, tol):
+    """A mixed batch whose per-stage S(k) sums directly, as the fused
+    kernels do (the per-stage default, the Chebyshev form, agrees with it to
+    FP32 rounding)."""
     import glst_amd as g
-    return g.GLSTBatch([h[0].size for _, h in systems],
-                       [L for L, _ in systems], RCUT, tol, ct="float")
+    b = g.GLSTBatch([h[0].size for _, h in systems],
+                    [L for L, _ in systems], RCUT, tol, ct="float")
+    b.set_option("sf_cheb", 0)
+    return b
 
 
 def _step(b, d):
