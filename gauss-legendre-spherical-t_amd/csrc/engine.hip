@@ -656,10 +656,10 @@ struct glst_engine {
   }
 
   // Chooses the fastest schedule for these inputs, one knob at a time: graph
-  // replay, the far-field pipelining, the near-field overlap, the
-  // back-projection split target, the shell-sum workgroup shape.  Results
-  // are identical under every schedule up to the split boundaries of the
-  // back-projection; the accumulators are cleared.
+  // replay, the near-field overlap, the back-projection split target, the
+  // shell-sum workgroup shape (FP64 handles: glst_autotune refuses mixed
+  // ones).  Results are identical under every schedule up to the split
+  // boundaries of the back-projection; the accumulators are cleared.
   double autotune(const double *rx, const double *ry, const double *rz,
                   const double *qc, uint32_t reps) {
     if (reps == 0) throw ArgError("autotune needs at least one repetition");
@@ -689,13 +689,6 @@ struct glst_engine {
            {0, 1});
     };
     graph_knob();
-    if (mixed() && has_lr && Kp > 0 && !rmt_global_path())
-      knob([&] { return static_cast<int64_t>(pipeline); },
-           [&](int64_t v) {
-             pipeline = static_cast<int>(v);
-             drop_graph();
-           },
-           {0, 1});
     knob([&] { return static_cast<int64_t>(overlap_near); },
          [&](int64_t v) {
            overlap_near = static_cast<int>(v);
@@ -709,17 +702,13 @@ struct glst_engine {
              apply_lr_blocks();
            },
            {1024, 2048, 4096, 8192, 16384, 32768, 65536});
-      if (!rmt_global_path()) {
-        std::vector<int64_t> shapes{0, 1};
-        if (mixed() && dev::rmt_mfma_nodes(grid, lds_limit) > 0)
-          shapes.push_back(3);
+      if (!rmt_global_path())
         knob([&] { return static_cast<int64_t>(rmt_shape); },
              [&](int64_t v) {
                rmt_shape = static_cast<int>(v);
                drop_graph();
              },
-             shapes);
-      }
+             {0, 1});
     }
     graph_knob();
     zero_ef();
@@ -1217,9 +1206,13 @@ glst_status_t glst_get_option(glst_handle_t h, const char *name,
 // tuned the 1,000,796-charge handle (after a 3,000-charge handle in the same
 // process) ended in a host segmentation fault whose cause is not found: the
 // host planning the autotuner drives (splits for every target, pipeline
-// chunks, node layout) checks clean under AddressSanitizer and UBSan at that
-// shape (tests/host/plan_check.cpp), and the default schedule runs that
-// handle correctly.  The 288,000-charge handle tunes without fault.
+// chunks, node layout, the Chebyshev matrices) checks clean under
+// AddressSanitizer and UBSan at that shape (tests/host/plan_check.cpp), and
+// the default schedule runs that handle correctly.  Mixed (CT = float)
+// handles are not tuned either: one run that tuned a 3,000-charge mixed
+// handle, after two runs of the same test that passed, ended in the same
+// kind of host segmentation fault, cause not found.  Their schedules are set
+// with glst_set_option.
 constexpr uint32_t kAutotuneMaxAtoms = 300000;
 
 glst_status_t glst_autotune(glst_handle_t h, const double *d_rx,
@@ -1232,6 +1225,9 @@ glst_status_t glst_autotune(glst_handle_t h, const double *d_rx,
       throw StateError("glst_autotune: handles of more than " +
                        std::to_string(kAutotuneMaxAtoms) +
                        " atoms are not tuned (the default schedule applies)");
+    if (h->mixed())
+      throw StateError("glst_autotune: mixed (CT = float) handles are not "
+                       "tuned; set their schedule with glst_set_option");
     const double t = h->autotune(d_rx, d_ry, d_rz, d_qc, reps);
     if (best_ms) *best_ms = t;
   });

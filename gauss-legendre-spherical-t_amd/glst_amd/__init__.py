@@ -475,7 +475,9 @@ class GLSTForce:
         return v.value
 
     def autotune(self, rx, ry, rz, qc, reps: int = 3) -> float:
-        """Keep the fastest schedule for these inputs; returns ms/step."""
+        """Keep the fastest schedule for these inputs; returns ms/step.
+        CT = double handles of up to 300,000 atoms only (others raise
+        GLSTError; set their schedule with set_option)."""
         t = c_double()
         self._c(_lib.glst_autotune(self._h, _ptr(rx), _ptr(ry), _ptr(rz),
                                    _ptr(qc), reps, byref(t)))

@@ -215,16 +215,17 @@ glst_status_t glst_set_option(glst_handle_t h, const char *name,
 glst_status_t glst_get_option(glst_handle_t h, const char *name,
                               int64_t *value);
 
-/* Autotuner (no counterpart in the reference): times calc_ener_force on
- * these inputs under candidate schedules, `reps` steps each, and keeps the
- * fastest, one knob at a time: "graph" (0, 1; tuned first and again last),
- * "pipeline" (0, 1; CT = float), "overlap_near" (0, 1, 2), "lr_blocks"
- * (powers of two from 1024 to 65536), "rmt_shape" (0, 1 and, where it
- * applies, 3).  Clears the
+/* Autotuner (no counterpart in the reference), for CT = double handles:
+ * times calc_ener_force on these inputs under candidate schedules, `reps`
+ * steps each, and keeps the fastest, one knob at a time: "graph" (0, 1;
+ * tuned first and again last), "overlap_near" (0, 1, 2), "lr_blocks"
+ * (powers of two from 1024 to 65536), "rmt_shape" (0, 1).  Clears the
  * accumulators afterwards; *best_ms (may be NULL) receives the chosen
- * schedule's time per step.  Handles of more than 300,000 atoms are refused
- * with GLST_ERR_STATE and keep the default schedule (a host fault seen once
- * while tuning a 1.0M-charge handle is not explained). */
+ * schedule's time per step.  Refused with GLST_ERR_STATE, the schedule
+ * left as it is: CT = float handles and handles of more than 300,000
+ * atoms (a host segmentation fault, seen once while tuning a 1.0M-charge
+ * handle and once while tuning a 3,000-charge CT = float handle, is not
+ * explained); set their schedule with glst_set_option. */
 glst_status_t glst_autotune(glst_handle_t h, const double *d_rx,
                             const double *d_ry, const double *d_rz,
                             const double *d_qc, uint32_t reps,

@@ -113,11 +113,33 @@ void check_pipeline(const glst::SystemSetup &su, const glst::PipelinePlan &P,
   if (G) CHECK(P.tile_end[G - 1] == ntile, "last chunk ends at %u", P.tile_end[G - 1]);
 }
 
+// The Chebyshev matrices of the mixed structure factors: every group that
+// takes the Chebyshev form reads NR x M entries from its offset.
+void check_chebyshev(const glst::SystemSetup &su, const glst::SplitPlan &P) {
+  std::vector<glst::dev::SfGroup> groups = P.sf_groups;
+  const std::vector<float> tab =
+      glst::plan_chebyshev(su.nodes, su.grid.h, groups);
+  for (size_t g = 0; g < groups.size(); ++g) {
+    const glst::dev::SfGroup &sg = groups[g];
+    if (sg.m == 0) continue;
+    CHECK(sg.m <= glst::dev::kChebMaxTerms, "group %zu: %u terms", g, sg.m);
+    CHECK(g < su.nodes.kappa.size(), "group %zu has no radial nodes", g);
+    if (g >= su.nodes.kappa.size()) continue;
+    const uint64_t need = static_cast<uint64_t>(sg.coff) +
+                          su.nodes.kappa[g].size() * sg.m;
+    CHECK(need <= tab.size(), "group %zu reads %llu of %zu entries", g,
+          static_cast<unsigned long long>(need), tab.size());
+    CHECK(sg.nr == su.nodes.kappa[g].size(), "group %zu: nr %u vs %zu", g,
+          sg.nr, su.nodes.kappa[g].size());
+  }
+}
+
 }  // namespace
 
 int main() {
   const Workload work[] = {
       {"1,000 waters", 3000, 31.04, 9.0, 1e-5},
+      {"1,000 waters, L 44.7", 3000, 44.7, 9.0, 1e-4},
       {"96,000 waters", 288000, 142.0, 10.0, 1e-4},
       {"333,000 waters + 898 ion pairs", 1000796, 215.0, 12.0, 1e-5},
   };
@@ -141,6 +163,11 @@ int main() {
           check_pipeline(su, glst::plan_pipeline(su.nodes, P.sf_groups.size(),
                                                  su.Kp, T), T);
         }
+        // the matrix-core shell sums take 1, 2 or 4 nodes per workgroup
+        for (uint32_t T : {1u, 2u, 4u})
+          check_pipeline(su, glst::plan_pipeline(su.nodes, P.sf_groups.size(),
+                                                 su.Kp, T), T);
+        if (ct == GLST_CT_FLOAT) check_chebyshev(su, P);
       }
       std::printf("%s (ct %d): grid %u^3, K %u, %zu groups: checked\n", w.name,
                   ct, su.grid.nx, su.K, su.nodes.dirs.size());

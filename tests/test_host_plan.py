@@ -2,9 +2,11 @@
 
 tests/host/plan_check.cpp drives the host code a handle and its autotuner
 run (setup_system, layout_nodes, make_splits for every back-projection split
-target the autotuner tries, plan_pipeline for both shell-sum tile limits) at
-the 3,000-, 288,000- and 1,000,796-charge workloads in both calculation
-types, and checks every table index against the launch sizes.  It is built
+target the autotuner tries, plan_pipeline for both shell-sum tile limits and
+the matrix-core tiles of 1, 2 and 4 nodes, plan_chebyshev for the mixed
+structure factors) at the 3,000-charge workloads (L 31.04 and 44.7), the
+288,000- and the 1,000,796-charge workloads in both calculation types, and
+checks every table index against the launch sizes.  It is built
 here with g++ (host code only) and the sanitizers, so an out-of-bounds
 access in that code fails this test.
 """
@@ -33,4 +35,4 @@ def test_host_planning_is_clean_under_sanitizers(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "0 failures" in r.stdout
-    assert r.stdout.count("checked") == 6
+    assert r.stdout.count("checked") == 8
