@@ -43,7 +43,8 @@ def main():
             for i in range(4)]
     b = g.GLSTBatch(natoms, boxes, 9.0, args.tol, ct=args.ct)
     b.set_option("overlap_near", args.overlap)
-    b.set_option("sf_cheb", args.sf_cheb)
+    if args.sf_cheb != 1:  # 1 is the default; setting it packs the batch again
+        b.set_option("sf_cheb", args.sf_cheb)
     b.zero_ef()
     b.calc_ener_force(*cols)
     torch.cuda.synchronize()
