@@ -59,6 +59,7 @@ size_t BatchLayout::device_bytes() const {
   b += sys.size() * sizeof(dev::SysDev);
   size_t nseg = bin.seg.size() + hist_p.seg.size() + scatter.seg.size() +
                 sr.seg.size() + reduce.seg.size() + rmt.seg.size() +
+                rmt_small.seg.size() + rmt_rest.seg.size() +
                 sf_f64.seg.size() + lr_f64.seg.size();
   for (auto &kv : sf_nr) nseg += kv.second.seg.size();
   for (auto &kv : lr_nr) nseg += kv.second.seg.size();
@@ -220,7 +221,7 @@ constexpr size_t kBatchRmtLdsDiv = GLST_BATCH_RMT_LDS_DIV;
 BatchLayout pack_batch(int ct,
                        const std::vector<std::shared_ptr<const BatchSystem>> &cs,
                        ShapeTables &tables, size_t lds_limit,
-                       uint32_t lr_blocks, bool cheb) {
+                       uint32_t lr_blocks, bool cheb, bool wave) {
   if (cs.empty()) throw ArgError("a batch needs at least one system");
   const uint32_t nsys = static_cast<uint32_t>(cs.size());
   BatchLayout B;
@@ -332,6 +333,11 @@ BatchLayout pack_batch(int ct,
     B.sr.add(s, d.ncell);
     if (d.Kp == 0) continue;
     B.rmt.add(s, d.Kp / d.T);
+    const dev::GridShape gs{d.nx, d.ny, d.nz, d.ncell, d.h};
+    if (wave && ct == GLST_CT_FLOAT && dev::rmt_wave_fits(gs))
+      B.rmt_small.add(s, d.Kp / dev::kRmtWaveNodes);
+    else
+      B.rmt_rest.add(s, d.Kp / d.T);
     B.sf_f64.add(s, d.nx * d.ny * (d.Kp / 256));
     B.lr_f64.add(s, d.ntile * d.nsplit);
     const SplitPlan &P = splits[s];
@@ -433,7 +439,8 @@ void glst_batch::configure(
     GLST_HIP(hipDeviceSynchronize());
     tables->clear();
   }
-  B = pack_batch(ct, cs, *tables, lds_limit, lr_blocks, sf_cheb != 0);
+  B = pack_batch(ct, cs, *tables, lds_limit, lr_blocks, sf_cheb != 0,
+                 rmt_wave != 0);
   tables->upload(stream);
   tables_gen = tables->gen;
   const size_t N = B.natom, C = B.ncell;
@@ -482,6 +489,8 @@ void glst_batch::configure(
   stage(B.sr, L_sr);
   stage(B.reduce, L_reduce);
   stage(B.rmt, L_rmt);
+  stage(B.rmt_small, L_rmt_small);
+  stage(B.rmt_rest, L_rmt_rest);
   stage(B.fsf, L_fsf);
   stage(B.flr, L_flr);
   if (mixed()) {
@@ -672,8 +681,12 @@ void glst_batch::step_async(const double *rx, const double *ry,
   check_launch("calc_sf");
   dev::rmt_batch(mixed(), S.as<void>(), tb.d_k64.as<double4>(),
                  tb.d_group.as<uint32_t>(), tb.d_r_in.as<uint32_t>(),
-                 tb.d_r_out.as<uint32_t>(), sys, L_rmt, B.lds, R.as<void>(),
-                 stream);
+                 tb.d_r_out.as<uint32_t>(), sys, L_rmt_rest, B.lds,
+                 R.as<void>(), stream);
+  dev::rmt_wave_batch(S.as<void>(), tb.d_k64.as<double4>(),
+                      tb.d_group.as<uint32_t>(), tb.d_r_in.as<uint32_t>(),
+                      tb.d_r_out.as<uint32_t>(), sys, L_rmt_small,
+                      R.as<void>(), stream);
   check_launch("sum_rmt_sf");
   if (fused == 3 && mixed())
     dev::lr_all_batch(atoms, cell_sorted.as<uint32_t>(), ta, sys, L_flr,
@@ -1027,6 +1040,10 @@ glst_status_t glst_batch_set_option(glst_batch_t b, const char *name,
     } else if (n == "sf_cheb") {
       if (value != 0 && value != 1) throw ArgError("sf_cheb must be 0 or 1");
       b->sf_cheb = static_cast<int>(value);
+      b->configure(b->B.ct, b->B.cs);
+    } else if (n == "rmt_wave") {
+      if (value != 0 && value != 1) throw ArgError("rmt_wave must be 0 or 1");
+      b->rmt_wave = static_cast<int>(value);
       b->configure(b->B.ct, b->B.cs);
     } else if (n == "fused") {
       if (value < 0 || value > 3) throw ArgError("fused must be 0 .. 3");

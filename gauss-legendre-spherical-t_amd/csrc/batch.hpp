@@ -120,6 +120,10 @@ struct BatchLayout {
   size_t table_bytes = 0;  // shape tables of the batch's distinct shapes
   std::vector<uint32_t> t_split;
   Plan bin, hist_p, scatter, sr, reduce, rmt, sf_f64, lr_f64;
+  // the shell sums split by kernel (per-stage launches): the mixed systems
+  // whose grids fit dev::rmt_wave (rmt_small, when asked for) and the rest
+  // (rmt_rest); rmt holds every system (the fused step's phase)
+  Plan rmt_small, rmt_rest;
   std::map<uint32_t, Plan> sf_nr, lr_nr;  // radial count (0: any) -> plan
   // mixed: the S(k) and back-projection units of the fused step (all radial
   // counts, largest first; see dev::FusedStep)
@@ -133,7 +137,8 @@ struct BatchLayout {
 // shapes to `tables` (host side).  lds_limit: the device's LDS per workgroup
 // (a system whose one-node cell table does not fit is rejected); lr_blocks:
 // back-projection workgroups aimed for over the whole batch; cheb: CT =
-// float S(k) in Chebyshev form where the shapes planned it.
+// float S(k) in Chebyshev form where the shapes planned it; wave: CT = float
+// shell sums of the grids up to 8 cells a side as window sums (rmt_small).
 // back-projection workgroups a batch aims for unless set (option lr_blocks)
 #ifndef GLST_BATCH_LR_BLOCKS
 #define GLST_BATCH_LR_BLOCKS 4096
@@ -142,7 +147,8 @@ struct BatchLayout {
 BatchLayout pack_batch(int ct,
                        const std::vector<std::shared_ptr<const BatchSystem>> &cs,
                        ShapeTables &tables, size_t lds_limit,
-                       uint32_t lr_blocks = 4096, bool cheb = true);
+                       uint32_t lr_blocks = 4096, bool cheb = true,
+                       bool wave = true);
 
 }  // namespace glst
 
@@ -159,6 +165,9 @@ struct glst_batch {
   // mixed S(k) in Chebyshev form where the shapes planned it (option
   // sf_cheb; the persistent and one-launch-per-stage modes sum directly)
   int sf_cheb = 1;
+  // mixed shell sums of grids up to 8 cells a side as window sums (option
+  // rmt_wave; the persistent fused step keeps the LDS tables)
+  int rmt_wave = 1;
   bool assigned = false;
   // HIP graph of the step (graph_mode 1: captured on the first step with
   // given inputs; 2: on the second consecutive step with the same inputs,
@@ -196,7 +205,7 @@ struct glst_batch {
   std::vector<glst::dev::Seg> host_seg;  // run tables (kept for the copy)
   // launches (run tables inside d_seg)
   glst::dev::BatchLaunch L_bin{}, L_hist{}, L_scatter{}, L_sr{}, L_reduce{},
-      L_rmt{}, L_fsf{}, L_flr{};
+      L_rmt{}, L_fsf{}, L_flr{}, L_rmt_small{}, L_rmt_rest{};
   std::vector<glst::dev::BatchLaunch> L_sf, L_lr;
 
   bool mixed() const { return B.mixed(); }

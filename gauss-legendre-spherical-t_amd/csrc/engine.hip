@@ -73,7 +73,9 @@ struct glst_engine {
   // workgroup, 2 global tables, 3 matrix cores where the grid allows (CT =
   // float, nx and ny up to 16), else as 0; 4 (default) matrix cores where
   // they apply and a z slice fills at least half of their 16 x 16 tile,
-  // else as 0
+  // else the small-grid window sums where they apply (CT = float, every
+  // dimension up to 8), else as 0; 5 the small-grid window sums where they
+  // apply, else as 0
   int rmt_shape = 4;
   // mixed path, tables in LDS: pipeline S(k) -> shell sums ->
   // back-projection across alpha groups, the shell sums on a third stream
@@ -355,6 +357,10 @@ struct glst_engine {
       dev::rmt_mfma(S.as<void>(), k64.as<double4>(), group.as<uint32_t>(),
                     r_in.as<uint32_t>(), r_out.as<uint32_t>(), Kp, grid,
                     lds_limit, R.as<void>(), stream);
+    } else if (rmt_wave_path()) {
+      dev::rmt_wave(S.as<void>(), k64.as<double4>(), group.as<uint32_t>(),
+                    r_in.as<uint32_t>(), r_out.as<uint32_t>(), Kp, grid,
+                    R.as<void>(), stream);
     } else if (rmt_shape == 2 ||
         !dev::rmt_lds(mixed(), rmt_shape == 1 ? 1 : dev::kRmtMaxNodes,
                       S.as<void>(),
@@ -440,6 +446,8 @@ struct glst_engine {
     const uint32_t T =
         rmt_mfma_path()
             ? dev::rmt_mfma_nodes(grid, lds_limit)
+        : rmt_wave_path()
+            ? dev::kRmtWaveNodes
             : dev::rmt_nodes_per_group(grid.nx, grid.ny, grid.nz, lds_limit,
                                        pipe_tmax);
     if (T == 0) throw StateError("shell-sum tables do not fit in LDS");
@@ -485,6 +493,11 @@ struct glst_engine {
                                 group.as<uint32_t>(), r_in.as<uint32_t>(),
                                 r_out.as<uint32_t>(), Kp, grid, lds_limit,
                                 R.as<void>(), rmt_stream, t0, tile_end[g])
+            : rmt_wave_path()
+                ? dev::rmt_wave(S.as<void>(), k64.as<double4>(),
+                                group.as<uint32_t>(), r_in.as<uint32_t>(),
+                                r_out.as<uint32_t>(), Kp, grid, R.as<void>(),
+                                rmt_stream, t0, tile_end[g])
                 : dev::rmt_lds(true, pipe_tmax, S.as<void>(),
                                k64.as<double4>(), group.as<uint32_t>(),
                                r_in.as<uint32_t>(), r_out.as<uint32_t>(), Kp,
@@ -564,7 +577,7 @@ struct glst_engine {
   // shell sums through the global-memory tables (one node's table does not
   // fit in LDS, or forced)
   bool rmt_global_path() const {
-    if (rmt_mfma_path()) return false;
+    if (rmt_mfma_path() || rmt_wave_path()) return false;
     return rmt_shape == 2 ||
            dev::rmt_nodes_per_group(grid.nx, grid.ny, grid.nz, lds_limit,
                                     1) == 0;
@@ -576,6 +589,12 @@ struct glst_engine {
         (rmt_shape == 4 && grid.nx * grid.ny * 2 >= dev::kRmtMfmaMaxXY *
                                                         dev::kRmtMfmaMaxXY);
     return want && mixed() && dev::rmt_mfma_nodes(grid, lds_limit) > 0;
+  }
+  // shell sums as window sums of small grids (CT = float, every dimension
+  // up to 8)
+  bool rmt_wave_path() const {
+    const bool want = rmt_shape == 5 || (rmt_shape == 4 && !rmt_mfma_path());
+    return want && mixed() && dev::rmt_wave_fits(grid);
   }
 
   // New back-projection split target: splits, their table and the partials.
@@ -1151,8 +1170,8 @@ glst_status_t glst_set_option(glst_handle_t h, const char *name,
       h->overlap_near = static_cast<int>(value);
     }
     else if (n == "rmt_shape") {
-      if (value < 0 || value > 4)
-        throw ArgError("rmt_shape must be 0, 1, 2, 3 or 4");
+      if (value < 0 || value > 5)
+        throw ArgError("rmt_shape must be 0 .. 5");
       h->rmt_shape = static_cast<int>(value);
     } else if (n == "pipeline") {
       if (value < 0 || value > 1) throw ArgError("pipeline must be 0 or 1");

@@ -1343,6 +1343,212 @@ __global__ __launch_bounds__(kRmtMfmaBlock) void rmt_mfma_kernel(
                      ny, nz, h, R, rmt_mfma_lds);
 }
 
+// ---- shell sums of small grids (mixed path, every dimension up to 8) ----
+//
+// The cube sums are separable, and on these grids a line has at most 8
+// cells, so each axis is one pass over lines held in registers: a lane
+// loads a line, takes its running sums (the cells past the grid hold zero,
+// so the last running sum repeats to the 8th slot) and forms the windows of
+// both cubes, |i' - i| <= r, as differences of two running sums.  The
+// half-widths select compile-time windows in a switch, so every register
+// index is a constant.  z: Y = S exp(-i k.c) from S; x, then y: the windows
+// of the previous pass, the lines read across through LDS.  All sums are
+// FP32 over at most 8 terms.
+template <int W>
+__device__ __forceinline__ void rmt_win8(const float2 (&P)[8],
+                                         float2 (&out)[8]) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int hi = i + W < 7 ? i + W : 7;
+    const int lo = i - W - 1;
+    out[i] = lo >= 0 ? make_float2(P[hi].x - P[lo].x, P[hi].y - P[lo].y)
+                     : P[hi];
+  }
+}
+
+__device__ __forceinline__ void rmt_window8(const float2 (&P)[8],
+                                            float2 (&out)[8], uint32_t r) {
+  switch (r < 7u ? r : 7u) {
+    case 0: rmt_win8<0>(P, out); break;
+    case 1: rmt_win8<1>(P, out); break;
+    case 2: rmt_win8<2>(P, out); break;
+    case 3: rmt_win8<3>(P, out); break;
+    case 4: rmt_win8<4>(P, out); break;
+    case 5: rmt_win8<5>(P, out); break;
+    case 6: rmt_win8<6>(P, out); break;
+    default: rmt_win8<7>(P, out); break;
+  }
+}
+
+__device__ __forceinline__ void rmt_prefix8(float2 (&v)[8]) {
+#pragma unroll
+  for (int i = 1; i < 8; ++i) {
+    v[i].x += v[i - 1].x;
+    v[i].y += v[i - 1].y;
+  }
+}
+
+// One tile: nodes n0 .. n0 + kRmtWaveNodes - 1, M of them at a time
+// (M = 16, 8 or 4 as the longest pass has at most 16, 32 or 64 lines).
+// Thread t serves node n0 + step M + t % M and line t / M: consecutive
+// threads take consecutive nodes of one cell, so that each S load of a node
+// run (four adjacent slots of a cell) and each R store (cell-major, the
+// nodes of a cell adjacent) is one contiguous segment, and the window
+// tables, [cell][M] in LDS, are read and written without bank conflicts.
+__device__ __forceinline__ void rmt_wave_tile(
+    const float2 *__restrict__ S, const double4 *__restrict__ k64,
+    const uint32_t *__restrict__ group, const uint32_t *__restrict__ r_in,
+    const uint32_t *__restrict__ r_out, uint32_t Kp, uint32_t n0,
+    uint32_t nx, uint32_t ny, uint32_t nz, double h,
+    float2 *__restrict__ R) {
+  constexpr uint32_t kAx = 3 * kRmtWaveMaxDim;
+  // M ncell <= 2048: M 16 needs every pair of dimensions at most 16 cells
+  // (ncell <= 64), M 8 at most 32 (ncell <= 181), M 4 ncell <= 512
+  constexpr uint32_t kWin = 2048;
+  static_assert(kRmtWaveBlock == 256 && kRmtWaveNodes == 16,
+                "thread map of the small-grid shell sums");
+  __shared__ float2 win_o[kWin], win_i[kWin];
+  __shared__ float2 axis[kRmtWaveNodes][kAx];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nax = nx + ny + nz;
+  // axis factors exp(+i k_d (i + 1/2) h): FP64 arguments, FP32 values
+  for (uint32_t e = tid; e < kRmtWaveNodes * nax; e += kRmtWaveBlock) {
+    const uint32_t t = e / nax, a = e - t * nax;
+    const double4 k = k64[n0 + t];
+    double2 r;
+    if (a < nx)
+      axis_phase(&r, k.x, h, a);
+    else if (a < nx + ny)
+      axis_phase(&r, k.y, h, a - nx);
+    else
+      axis_phase(&r, k.z, h, a - nx - ny);
+    axis[t][a] = make_float2(static_cast<float>(r.x), static_cast<float>(r.y));
+  }
+  const uint32_t lxy = nx * ny, lyz = ny * nz, lxz = nx * nz;
+  const uint32_t lmax = lxy > lyz ? (lxy > lxz ? lxy : lxz)
+                                  : (lyz > lxz ? lyz : lxz);
+  const uint32_t M = lmax <= 16 ? 16u : (lmax <= 32 ? 8u : 4u);
+  const uint32_t j = tid & (M - 1), q = tid / M;
+  const uint32_t ncell = nx * ny * nz;
+  __syncthreads();  // the axis factors
+  for (uint32_t step = 0; step < kRmtWaveNodes; step += M) {
+    const uint32_t t = step + j;
+    const uint32_t nd = n0 + t;
+    const uint32_t gr = group[nd];
+    const uint32_t ro = r_out[gr], ri = r_in[gr];
+    const float2 *pt = axis[t];
+    // (z) line (x, y): Y = S conj(exp(i k.c)), windows along z
+    if (q < lxy) {
+      const uint32_t x = q / ny, y = q - x * ny;
+      const float2 exy = cmulf(pt[x], pt[nx + y]);
+      float2 v[8];
+#pragma unroll
+      for (uint32_t z = 0; z < 8; ++z) {
+        v[z] = make_float2(0.f, 0.f);
+        if (z < nz) {
+          const float2 sv = S[sf_slot<true>(nd, q * nz + z, ncell)];
+          const float2 e = cmulf(exy, pt[nx + ny + z]);
+          v[z] = make_float2(sv.x * e.x + sv.y * e.y, sv.y * e.x - sv.x * e.y);
+        }
+      }
+      rmt_prefix8(v);
+      float2 wo[8], wi[8];
+      rmt_window8(v, wo, ro);
+      rmt_window8(v, wi, ri);
+#pragma unroll
+      for (uint32_t z = 0; z < 8; ++z)
+        if (z < nz) {
+          win_o[(q * nz + z) * M + j] = wo[z];
+          win_i[(q * nz + z) * M + j] = wi[z];
+        }
+    }
+    __syncthreads();
+    // (x) line (y, z): cells (x ny + y) nz + z
+    if (q < lyz) {
+      const uint32_t y = q / nz, z = q - y * nz;
+      float2 vo[8], vi[8];
+#pragma unroll
+      for (uint32_t x = 0; x < 8; ++x) {
+        vo[x] = vi[x] = make_float2(0.f, 0.f);
+        if (x < nx) {
+          vo[x] = win_o[((x * ny + y) * nz + z) * M + j];
+          vi[x] = win_i[((x * ny + y) * nz + z) * M + j];
+        }
+      }
+      rmt_prefix8(vo);
+      rmt_prefix8(vi);
+      float2 wo[8], wi[8];
+      rmt_window8(vo, wo, ro);
+      rmt_window8(vi, wi, ri);
+#pragma unroll
+      for (uint32_t x = 0; x < 8; ++x)
+        if (x < nx) {
+          win_o[((x * ny + y) * nz + z) * M + j] = wo[x];
+          win_i[((x * ny + y) * nz + z) * M + j] = wi[x];
+        }
+    }
+    __syncthreads();
+    // (y) line (x, z): the two cubes, then V = w exp(+i k.c) (outer -
+    // inner) in polar form
+    if (q < lxz) {
+      const uint32_t x = q / nz, z = q - x * nz;
+      float2 vo[8], vi[8];
+#pragma unroll
+      for (uint32_t y = 0; y < 8; ++y) {
+        vo[y] = vi[y] = make_float2(0.f, 0.f);
+        if (y < ny) {
+          vo[y] = win_o[((x * ny + y) * nz + z) * M + j];
+          vi[y] = win_i[((x * ny + y) * nz + z) * M + j];
+        }
+      }
+      rmt_prefix8(vo);
+      rmt_prefix8(vi);
+      float2 co[8], ci[8];
+      rmt_window8(vo, co, ro);
+      rmt_window8(vi, ci, ri);
+      const double4 kn = k64[nd];
+      const float wk = static_cast<float>(kn.w), kr = kappa_rev(kn);
+      float2 e0 = cmulf(pt[x], pt[nx + ny + z]);
+      e0.x *= wk;
+      e0.y *= wk;
+#pragma unroll
+      for (uint32_t y = 0; y < 8; ++y)
+        if (y < ny) {
+          const float dr = co[y].x - ci[y].x, di = co[y].y - ci[y].y;
+          const float2 e = cmulf(e0, pt[nx + y]);
+          const float vr = dr * e.x - di * e.y, vi2 = dr * e.y + di * e.x;
+          const float mag = __builtin_amdgcn_sqrtf(fmaf(vr, vr, vi2 * vi2));
+          R[static_cast<size_t>((x * ny + y) * nz + z) * Kp + nd] =
+              make_float2(kr * mag, arg_rev(vr, vi2));
+        }
+    }
+    __syncthreads();  // the windows are free for the next nodes
+  }
+}
+
+// Shell sums of tiles tile0 .. tile0 + ntile - 1 on a small grid, one
+// workgroup per tile; the tiles of XCD x (workgroups b with b % 8 == x) are
+// a contiguous eighth of the range.
+// Workgroups per CU the register budget allows (4: 128 VGPRs, 12 bytes of
+// scratch per lane)
+#ifndef GLST_RMT_WAVE_MIN_WG
+#define GLST_RMT_WAVE_MIN_WG 4
+#endif
+__global__ __launch_bounds__(kRmtWaveBlock, GLST_RMT_WAVE_MIN_WG) void rmt_wave_kernel(
+    const float2 *__restrict__ S, const double4 *__restrict__ k64,
+    const uint32_t *__restrict__ group, const uint32_t *__restrict__ r_in,
+    const uint32_t *__restrict__ r_out, uint32_t Kp, uint32_t tile0,
+    uint32_t ntile, uint32_t nx, uint32_t ny, uint32_t nz, double h,
+    float2 *__restrict__ R) {
+  uint32_t tile = blockIdx.x;
+  if (gridDim.x >= 8) {
+    const uint32_t x = blockIdx.x & 7u, q = ntile >> 3, r = ntile & 7u;
+    tile = x * q + (x < r ? x : r) + (blockIdx.x >> 3);
+  }
+  rmt_wave_tile(S, k64, group, r_in, r_out, Kp,
+                (tile0 + tile) * kRmtWaveNodes, nx, ny, nz, h, R);
+}
+
 // ---- global-memory tables (grids too large for LDS) ----
 
 // G[c*Kp + k] = S (node runs) converted to FP64 cell-major order;
@@ -2795,6 +3001,21 @@ __global__ __launch_bounds__(kRmtBlock) void rmt_lds_batch_kernel(
                   xcd_remap(blockIdx.x, gridDim.x));
 }
 
+// a run's workgroups are the kRmtWaveNodes-node tiles of one small system
+__global__ __launch_bounds__(kRmtWaveBlock, GLST_RMT_WAVE_MIN_WG) void rmt_wave_batch_kernel(
+    const float2 *__restrict__ S, const double4 *__restrict__ k64,
+    const uint32_t *__restrict__ group, const uint32_t *__restrict__ r_in,
+    const uint32_t *__restrict__ r_out, const SysDev *__restrict__ sys,
+    const Seg *__restrict__ seg, uint32_t nseg, float2 *__restrict__ R) {
+  const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  const Seg g = seg[run_of(seg, nseg, blk)];
+  const SysDev &d = sys[g.sys];
+  rmt_wave_tile(S + d.field0, k64 + d.node0, group + d.node0,
+                r_in + d.group0, r_out + d.group0, d.Kp,
+                (blk - g.blk0) * kRmtWaveNodes, d.nx, d.ny, d.nz, d.h,
+                R + d.field0);
+}
+
 __global__ __launch_bounds__(kLrBlock) void lr_f64_batch_kernel(
     const double4 *__restrict__ atoms, const uint32_t *__restrict__ cell_sorted,
     LrTasks tk, const double2 *__restrict__ R, const SysDev *__restrict__ sys,
@@ -3322,6 +3543,21 @@ bool rmt_mfma(const void *S, const double4 *k64, const uint32_t *group,
   return true;
 }
 
+bool rmt_wave(const void *S, const double4 *k64, const uint32_t *group,
+              const uint32_t *r_in, const uint32_t *r_out, uint32_t Kp,
+              const GridShape &g, void *R, hipStream_t s, uint32_t tile0,
+              uint32_t tile1) {
+  if (!rmt_wave_fits(g)) return false;
+  const uint32_t all = Kp / kRmtWaveNodes;  // Kp is a multiple of 256
+  if (tile1 > all) tile1 = all;
+  if (tile0 >= tile1) return true;
+  const uint32_t ntile = tile1 - tile0;
+  rmt_wave_kernel<<<ntile, kRmtWaveBlock, 0, s>>>(
+      static_cast<const float2 *>(S), k64, group, r_in, r_out, Kp, tile0,
+      ntile, g.nx, g.ny, g.nz, g.h, static_cast<float2 *>(R));
+  return true;
+}
+
 void rmt_global(bool mixed, const void *S, double2 *G, const double4 *k64,
                 const uint32_t *group, const uint32_t *r_in,
                 const uint32_t *r_out, uint32_t Kp, const GridShape &g,
@@ -3590,6 +3826,16 @@ void rmt_batch(bool mixed, const void *S, const double4 *k64,
     rmt_lds_batch_kernel<false><<<L.nblk, kRmtBlock, lds_bytes, s>>>(
         S, k64, group, r_in, r_out, sys, L.seg, L.nseg, R);
   }
+}
+
+void rmt_wave_batch(const void *S, const double4 *k64, const uint32_t *group,
+                    const uint32_t *r_in, const uint32_t *r_out,
+                    const SysDev *sys, const BatchLaunch &L, void *R,
+                    hipStream_t s) {
+  if (L.nblk == 0) return;
+  rmt_wave_batch_kernel<<<L.nblk, kRmtWaveBlock, 0, s>>>(
+      static_cast<const float2 *>(S), k64, group, r_in, r_out, sys, L.seg,
+      L.nseg, static_cast<float2 *>(R));
 }
 
 void lr_batch(bool mixed, const void *atoms, const uint32_t *cell_sorted,

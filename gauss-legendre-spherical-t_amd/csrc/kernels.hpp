@@ -216,6 +216,24 @@ bool rmt_mfma(const void *S, const double4 *k64, const uint32_t *group,
               const uint32_t *r_in, const uint32_t *r_out, uint32_t Kp,
               const GridShape &g, size_t lds_limit, void *R, hipStream_t s,
               uint32_t tile0 = 0, uint32_t tile1 = 0xffffffffu);
+// Shell sums of small grids (CT = float, every dimension up to
+// kRmtWaveMaxDim): workgroups of kRmtWaveBlock threads take tiles of
+// kRmtWaveNodes consecutive nodes, 16, 8 or 4 at a time, and sum them as
+// three passes of FP32 window sums along lines of at most 8 cells, with the
+// lines exchanged through LDS (static, 35 KB per workgroup).
+constexpr uint32_t kRmtWaveMaxDim = 8;
+constexpr uint32_t kRmtWaveNodes = 16;
+constexpr uint32_t kRmtWaveBlock = 256;
+inline bool rmt_wave_fits(const GridShape &g) {
+  return g.ncell > 0 && g.nx <= kRmtWaveMaxDim && g.ny <= kRmtWaveMaxDim &&
+         g.nz <= kRmtWaveMaxDim;
+}
+// Tiles [tile0, tile1) of kRmtWaveNodes nodes (tile1 clamped to
+// Kp / kRmtWaveNodes).  False: the grid does not fit (nothing launched).
+bool rmt_wave(const void *S, const double4 *k64, const uint32_t *group,
+              const uint32_t *r_in, const uint32_t *r_out, uint32_t Kp,
+              const GridShape &g, void *R, hipStream_t s, uint32_t tile0 = 0,
+              uint32_t tile1 = 0xffffffffu);
 // Global-memory tables: G (ncell*Kp double2, scratch) receives S in FP64
 // cell-major order, is prefix-summed in place, and the shells are gathered
 // from it.  S is left unchanged.
@@ -357,6 +375,11 @@ void rmt_batch(bool mixed, const void *S, const double4 *k64,
                const uint32_t *group, const uint32_t *r_in,
                const uint32_t *r_out, const SysDev *sys, const BatchLaunch &L,
                size_t lds_bytes, void *R, hipStream_t s);
+// mixed systems whose grids fit rmt_wave: kRmtWaveNodes-node tiles
+void rmt_wave_batch(const void *S, const double4 *k64, const uint32_t *group,
+                    const uint32_t *r_in, const uint32_t *r_out,
+                    const SysDev *sys, const BatchLaunch &L, void *R,
+                    hipStream_t s);
 // mixed with ag: launches with 1 <= nr <= 40 run over atom groups (their
 // plans count agtile workgroups per split, the others ntile)
 void lr_batch(bool mixed, const void *atoms, const uint32_t *cell_sorted,

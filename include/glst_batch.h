@@ -85,7 +85,11 @@ glst_status_t glst_batch_zero_ef(glst_batch_t b);
  * barriers, 1 with the kernel's registers held to 2 waves per SIMD and 2 to
  * 3 waves.  Its workgroups are as many as are resident at once; should a
  * barrier nevertheless wait 10 s, the step ends and the next synchronising
- * call returns GLST_ERR_STATE. */
+ * call returns GLST_ERR_STATE.  Mixed path, each re-packs the batch:
+ * "sf_cheb" (default 1) S(k) in Chebyshev form where the grid shapes
+ * planned it, 0 direct sums; "rmt_wave" (default 1) the shell sums of grids
+ * up to 8 cells a side as FP32 window sums, 0 the LDS summed-volume tables.
+ * The persistent launch (fused 1, 2) sums directly and on the tables. */
 glst_status_t glst_batch_set_option(glst_batch_t b, const char *name,
                                     int64_t value);
 /* Autotuner: times the step on these inputs under each "fused" (mixed

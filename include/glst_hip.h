@@ -185,10 +185,13 @@ glst_status_t glst_get_rmt(glst_handle_t h, double *re, double *im);
  *   (0); always one node per workgroup (1); the global-memory tables
  *   otherwise used when one node's table does not fit in LDS (2); for
  *   CT = float on grids at most 16 cells wide in x and y, separable window
- *   sums on the matrix cores (FP32 direct sums; elsewhere as 0) (3); or 3
+ *   sums on the matrix cores (FP32 direct sums; elsewhere as 0) (3); 3
  *   where nx * ny >= 128 (a z slice fills at least half of the matrix
- *   cores' 16 x 16 tile), else 0 (4).  Shapes 0-2 agree to FP64 rounding,
- *   shape 3 with them to FP32 rounding of the remote sums.
+ *   cores' 16 x 16 tile), else 5 where it applies, else 0 (4); for
+ *   CT = float on grids at most 8 cells a side, FP32 window sums along
+ *   lines held in registers, one wave per four nodes (elsewhere as 0) (5).
+ *   Shapes 0-2 agree to FP64 rounding, shapes 3 and 5 with them to FP32
+ *   rounding of the remote sums.
  *   "pipeline" (default 1): CT = float with LDS-resident tables only;
  *   calc_ener_force overlaps the stages of successive alpha groups: the
  *   shell sums of group g run on a third stream once S(k) of group g is

@@ -39,13 +39,14 @@ def _pack(systems, torch):
 
 
 def _batch(systems, tol):
-    """A mixed batch whose per-stage S(k) sums directly, as the fused
-    kernels do (the per-stage default, the Chebyshev form, agrees with it to
-    FP32 rounding)."""
+    """A mixed batch whose per-stage S(k) sums directly and whose shell sums
+    use the LDS tables, as the fused kernels do (the per-stage defaults, the
+    Chebyshev form and the window sums, agree with them to FP32 rounding)."""
     import glst_amd as g
     b = g.GLSTBatch([h[0].size for _, h in systems],
                     [L for L, _ in systems], RCUT, tol, ct="float")
     b.set_option("sf_cheb", 0)
+    b.set_option("rmt_wave", 0)
     return b
 
 

@@ -168,7 +168,7 @@ def test_accumulate_and_stages(cuda, water1k, ct):
         np.testing.assert_array_equal(a, b)
 
 
-@pytest.mark.parametrize("rmt_shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("rmt_shape", [0, 1, 2, 3, 5])
 def test_fields_mixed_match_oracle(cuda, water1k, rmt_shape):
     """Mixed-precision S(k) and shell sums R(k) against the FP64 oracle's
     fields, for every shell-sum shape (bar: 1e-5 of the field's largest
@@ -199,13 +199,13 @@ def test_mixed_options_agree(cuda, water1k):
     """Every combination of the tuning options gives the same forces: the
     near field's overlap changes no arithmetic at all, the summed-volume
     shapes agree to FP64 rounding before the FP32 store, and the matrix-core
-    shape to FP32 rounding of the remote sums."""
+    and small-grid window shapes to FP32 rounding of the remote sums."""
     import torch
     import glst_amd as g
     rx, ry, rz, qc = water1k
     d = _dev([rx, ry, rz, qc], torch)
     out = {}
-    for rs in (0, 1, 2, 3):
+    for rs in (0, 1, 2, 3, 5):
         for ov in (0, 1, 2):
             for pl in (0, 1):
                 f = g.GLSTForce(rx.size, 1e-5, 31.04, rcut=9.0, ct="float")
@@ -216,17 +216,18 @@ def test_mixed_options_agree(cuda, water1k):
                 f.calc_ener_force(*d)
                 out[rs, ov, pl] = _host(f.get_ef())
                 f.close()
-    for rs in (0, 1, 2, 3):
+    for rs in (0, 1, 2, 3, 5):
         for ov in (0, 1, 2):
             for pl in (0, 1):
                 for a, b in zip(out[rs, 0, 0], out[rs, ov, pl]):
                     np.testing.assert_array_equal(a, b)
-        # the matrix-core shell sums round in FP32 (the tables in FP64)
+        # the matrix-core and window shell sums round in FP32 (the tables
+        # in FP64)
         assert rel_err(out[rs, 0, 0][:3], out[0, 0, 0][:3]) < (
-            1e-6 if rs == 3 else 1e-7)
+            1e-6 if rs in (3, 5) else 1e-7)
     f = g.GLSTForce(rx.size, 1e-5, 31.04, rcut=9.0, ct="float")
     with pytest.raises(Exception):
-        f.set_option("rmt_shape", 5)
+        f.set_option("rmt_shape", 6)
     with pytest.raises(Exception):
         f.set_option("overlap_near", 3)
     with pytest.raises(Exception):
@@ -343,7 +344,7 @@ def test_rectangular_grids_match_oracle(cuda, box, ct):
     sbar, rbar, fbar = ((1e-12, 1e-11, 1e-10) if ct == "double"
                         else (1e-5, 1e-5, 2e-6))
     d = _dev([rx, ry, rz, qc], torch)
-    for rs in (0, 1, 2, 3):
+    for rs in (0, 1, 2, 3, 5):
         f = g.GLSTForce(rx.size, tol, box, rcut=rcut, ct=ct)
         f.set_option("rmt_shape", rs)
         assert tuple(f.info().ncell) == o.grid_dims(ict, *box, rcut)

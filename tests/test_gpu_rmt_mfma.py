@@ -148,14 +148,16 @@ def test_matrix_core_shape_falls_back(cuda, ct, box):
 
 
 @pytest.mark.parametrize("box,rcut,same_as", [((142.0,) * 3, 10.0, 3),
-                                               ((31.04,) * 3, 9.0, 0)])
+                                               ((31.04,) * 3, 9.0, 5),
+                                               ((90.0,) * 3, 9.0, 0)])
 def test_default_shape(cuda, box, rcut, same_as):
     """The default (rmt_shape 4) takes the matrix cores on the 15^3 grid,
-    whose z slices fill most of their 16 x 16 tile, and the summed-volume
-    tables on the 4^3 grid, whose slices fill a sixteenth of it."""
+    whose z slices fill most of their 16 x 16 tile, the small-grid window
+    sums on the 4^3 grid, whose slices fill a sixteenth of it, and the
+    summed-volume tables on the 10^3 grid, which neither fits."""
     import torch
     import glst_amd as g
-    h = _lattice(box, 7.0 if box[0] > 100 else 2.6, 4)
+    h = _lattice(box, 7.0 if box[0] > 50 else 2.6, 4)
     d = _dev(h, torch)
     out = []
     for rs in (None, same_as):
