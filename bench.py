@@ -44,9 +44,11 @@ STAGES = ["bin", "sort", "sf", "rmt", "lr", "lr_reduce", "sr"]
 # Schedules (overlap_near, lr_blocks, rmt_shape, pipeline, graph) the
 # engine's autotuner chose for this workload on MI355X, pinned here: the
 # mixed path's autotuner is refused by the engine (glst_autotune), and with
-# the schedule pinned the result does not depend on a tuning pass.  Eager
-# steps and graph replay are both timed below.
-MIXED_SCHEDULE = (2, 8192, 4, 1, 1)
+# the schedule pinned the result does not depend on a tuning pass.  The
+# autotuner picked graph replay for the mixed path, within 1 % of eager
+# steps; eager steps measured 0.2-1.2 % faster in three later runs, so the
+# result is timed eagerly.  Eager steps and graph replay are both reported.
+MIXED_SCHEDULE = (2, 8192, 4, 1, 0)
 FP64_SCHEDULE = (1, 16384, 0, 1, 0)
 
 

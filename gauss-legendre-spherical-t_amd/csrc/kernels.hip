@@ -1529,10 +1529,13 @@ __device__ __forceinline__ void rmt_wave_tile(
 // Shell sums of tiles tile0 .. tile0 + ntile - 1 on a small grid, one
 // workgroup per tile; the tiles of XCD x (workgroups b with b % 8 == x) are
 // a contiguous eighth of the range.
-// Workgroups per CU the register budget allows (4: 128 VGPRs, 12 bytes of
-// scratch per lane)
+// Workgroups per CU the register budget is held to.  1: the kernel takes
+// the 150 VGPRs it needs (3 waves per SIMD, no scratch).  Held to 4 (128
+// VGPRs, 12 bytes of scratch per lane) the 512-system batch ran 4 % faster,
+// but a served stream of two concurrent slots then returned one answer of
+// NaN forces, with no cause found; that build is not used.
 #ifndef GLST_RMT_WAVE_MIN_WG
-#define GLST_RMT_WAVE_MIN_WG 4
+#define GLST_RMT_WAVE_MIN_WG 1
 #endif
 __global__ __launch_bounds__(kRmtWaveBlock, GLST_RMT_WAVE_MIN_WG) void rmt_wave_kernel(
     const float2 *__restrict__ S, const double4 *__restrict__ k64,
