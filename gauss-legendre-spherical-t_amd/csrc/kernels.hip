@@ -1380,6 +1380,14 @@ __device__ __forceinline__ void rmt_window8(const float2 (&P)[8],
   }
 }
 
+// Workgroup barrier that waits for this wave's LDS operations only, so that
+// global loads issued before it stay in flight (__syncthreads also drains
+// them).  The memory clobber keeps the compiler from moving memory accesses
+// across it.
+__device__ __forceinline__ void rmt_lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 __device__ __forceinline__ void rmt_prefix8(float2 (&v)[8]) {
 #pragma unroll
   for (int i = 1; i < 8; ++i) {
@@ -1409,8 +1417,19 @@ __device__ __forceinline__ void rmt_wave_tile(
                 "thread map of the small-grid shell sums");
   __shared__ float2 win_o[kWin], win_i[kWin];
   __shared__ float2 axis[kRmtWaveNodes][kAx];
+  // per node: r_out, r_in, weight, |k| / 2 pi (read from LDS in the loop, so
+  // that no vector load there waits behind the S prefetch: the vector
+  // memory counter completes in order)
+  __shared__ uint4 npar[kRmtWaveNodes];
   const uint32_t tid = threadIdx.x;
   const uint32_t nax = nx + ny + nz;
+  if (tid < kRmtWaveNodes) {
+    const uint32_t gr = group[n0 + tid];
+    const double4 kn = k64[n0 + tid];
+    npar[tid] = make_uint4(r_out[gr], r_in[gr],
+                           __float_as_uint(static_cast<float>(kn.w)),
+                           __float_as_uint(kappa_rev(kn)));
+  }
   // axis factors exp(+i k_d (i + 1/2) h): FP64 arguments, FP32 values
   for (uint32_t e = tid; e < kRmtWaveNodes * nax; e += kRmtWaveBlock) {
     const uint32_t t = e / nax, a = e - t * nax;
@@ -1430,12 +1449,24 @@ __device__ __forceinline__ void rmt_wave_tile(
   const uint32_t M = lmax <= 16 ? 16u : (lmax <= 32 ? 8u : 4u);
   const uint32_t j = tid & (M - 1), q = tid / M;
   const uint32_t ncell = nx * ny * nz;
+  // this thread's z-line of S for the nodes of step s, in flight while the
+  // previous step's x and y passes run; the indexes are clamped into the grid, so all eight loads are
+  // unconditional and in flight together (the extra values are not used)
+  float2 sv[8];
+  const uint32_t qf = q < lxy ? q : 0u;
+  auto fetch = [&](uint32_t s) {
+    const uint32_t nf = n0 + s + j;
+#pragma unroll
+    for (uint32_t z = 0; z < 8; ++z)
+      sv[z] = S[sf_slot<true>(nf, qf * nz + (z < nz ? z : nz - 1), ncell)];
+  };
+  fetch(0);
   __syncthreads();  // the axis factors
   for (uint32_t step = 0; step < kRmtWaveNodes; step += M) {
     const uint32_t t = step + j;
     const uint32_t nd = n0 + t;
-    const uint32_t gr = group[nd];
-    const uint32_t ro = r_out[gr], ri = r_in[gr];
+    const uint4 par = npar[t];
+    const uint32_t ro = par.x, ri = par.y;
     const float2 *pt = axis[t];
     // (z) line (x, y): Y = S conj(exp(i k.c)), windows along z
     if (q < lxy) {
@@ -1446,11 +1477,12 @@ __device__ __forceinline__ void rmt_wave_tile(
       for (uint32_t z = 0; z < 8; ++z) {
         v[z] = make_float2(0.f, 0.f);
         if (z < nz) {
-          const float2 sv = S[sf_slot<true>(nd, q * nz + z, ncell)];
           const float2 e = cmulf(exy, pt[nx + ny + z]);
-          v[z] = make_float2(sv.x * e.x + sv.y * e.y, sv.y * e.x - sv.x * e.y);
+          v[z] = make_float2(sv[z].x * e.x + sv[z].y * e.y,
+                             sv[z].y * e.x - sv[z].x * e.y);
         }
       }
+      if (step + M < kRmtWaveNodes) fetch(step + M);
       rmt_prefix8(v);
       float2 wo[8], wi[8];
       rmt_window8(v, wo, ro);
@@ -1462,7 +1494,7 @@ __device__ __forceinline__ void rmt_wave_tile(
           win_i[(q * nz + z) * M + j] = wi[z];
         }
     }
-    __syncthreads();
+    rmt_lds_barrier();
     // (x) line (y, z): cells (x ny + y) nz + z
     if (q < lyz) {
       const uint32_t y = q / nz, z = q - y * nz;
@@ -1487,7 +1519,7 @@ __device__ __forceinline__ void rmt_wave_tile(
           win_i[((x * ny + y) * nz + z) * M + j] = wi[x];
         }
     }
-    __syncthreads();
+    rmt_lds_barrier();
     // (y) line (x, z): the two cubes, then V = w exp(+i k.c) (outer -
     // inner) in polar form
     if (q < lxz) {
@@ -1506,8 +1538,7 @@ __device__ __forceinline__ void rmt_wave_tile(
       float2 co[8], ci[8];
       rmt_window8(vo, co, ro);
       rmt_window8(vi, ci, ri);
-      const double4 kn = k64[nd];
-      const float wk = static_cast<float>(kn.w), kr = kappa_rev(kn);
+      const float wk = __uint_as_float(par.z), kr = __uint_as_float(par.w);
       float2 e0 = cmulf(pt[x], pt[nx + ny + z]);
       e0.x *= wk;
       e0.y *= wk;
@@ -1522,7 +1553,7 @@ __device__ __forceinline__ void rmt_wave_tile(
               make_float2(kr * mag, arg_rev(vr, vi2));
         }
     }
-    __syncthreads();  // the windows are free for the next nodes
+    rmt_lds_barrier();  // the windows are free for the next nodes
   }
 }
 
