@@ -44,10 +44,10 @@ STAGES = ["bin", "sort", "sf", "rmt", "lr", "lr_reduce", "sr"]
 # Schedules (overlap_near, lr_blocks, rmt_shape, pipeline, graph) the
 # engine's autotuner chose for this workload on MI355X, pinned here: the
 # mixed path's autotuner is refused by the engine (glst_autotune), and with
-# the schedule pinned the result does not depend on a tuning pass.  The
-# autotuner picked graph replay for the mixed path, within 1 % of eager
-# steps; eager steps measured 0.2-1.2 % faster in three later runs, so the
-# result is timed eagerly.  Eager steps and graph replay are both reported.
+# the schedule pinned the result does not depend on a tuning pass.  For the
+# mixed path eager steps and graph replay are within about 2 % and trade
+# places from box to box, so both are timed and the faster is timed again
+# for the result (time_path's pick_graph); both are reported.
 MIXED_SCHEDULE = (2, 8192, 4, 1, 0)
 FP64_SCHEDULE = (1, 16384, 0, 1, 0)
 
@@ -103,7 +103,7 @@ def error_report(got, ref):
 
 
 def time_path(g, torch, d, natom, ct, steps, warmup, schedule,
-              profile=False):
+              profile=False, pick_graph=False):
     """Sets `schedule` (overlap_near, lr_blocks, rmt_shape[, pipeline[,
     graph]]), then times `steps` steps after `warmup`: as scheduled (the
     result), eagerly and as a replayed graph (both reported), eagerly with
@@ -111,7 +111,9 @@ def time_path(g, torch, d, natom, ct, steps, warmup, schedule,
     per-kernel events), and eagerly with the near field and the far-field
     stages one after another on one stream ("kernel_ms": each stage's events
     then bracket only its own kernels).  `profile`: only the scheduled and
-    the stage runs."""
+    the stage runs.  `pick_graph`: after both modes are timed, the faster
+    one is timed once more and that run is the result (the two are within
+    about 2 % and trade places from box to box)."""
     f = g.GLSTForce(natom, TOL, BOX, rcut=RCUT, ct=ct)
     keys = ("overlap_near", "lr_blocks", "rmt_shape", "pipeline", "graph")
     for k, v in zip(keys, schedule):
@@ -147,6 +149,9 @@ def time_path(g, torch, d, natom, ct, steps, warmup, schedule,
     modes = {sched["graph"]: dt}
     if not profile:
         modes[1 - sched["graph"]] = run(1 - sched["graph"], False)
+        if pick_graph:
+            sched["graph"] = min(modes, key=modes.get)
+            dt = run(sched["graph"], False)
     run(0, True)
     stages = stage_times()
     kernel = stages
@@ -495,7 +500,8 @@ def main():
     pinned = ([int(v) for v in args.schedule.split(",")]
               if args.schedule else MIXED_SCHEDULE)
     tm = time_path(g, torch, d, natom, "float", args.steps, args.warmup,
-                   pinned, profile=args.profile)
+                   pinned, profile=args.profile,
+                   pick_graph=args.schedule is None)
     # the step as run for the result: the pinned schedule, eager or
     # replayed as a HIP graph as it says (per-stage times come from eager
     # steps with events)
