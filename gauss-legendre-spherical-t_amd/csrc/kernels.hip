@@ -1396,9 +1396,11 @@ __device__ __forceinline__ void rmt_prefix8(float2 (&v)[8]) {
   }
 }
 
-// One tile: nodes n0 .. n0 + kRmtWaveNodes - 1, M of them at a time
-// (M = 16, 8 or 4 as the longest pass has at most 16, 32 or 64 lines).
-// Thread t serves node n0 + step M + t % M and line t / M: consecutive
+// One tile: nodes n0 .. n0 + kRmtWaveNodes - 1, M of them at a time, M <=
+// min(16, 256 / L) for passes of at most L lines (6^3: 7 nodes of 36 lines,
+// the last of the three groups two nodes; the threads of node slots past
+// the tile compute on the tile's last node and store nothing).
+// Thread t serves node n0 + step + t % M and line t / M: consecutive
 // threads take consecutive nodes of one cell, so that each S load of a node
 // run (four adjacent slots of a cell) and each R store (cell-major, the
 // nodes of a cell adjacent) is one contiguous segment, and the window
@@ -1410,8 +1412,8 @@ __device__ __forceinline__ void rmt_wave_tile(
     uint32_t nx, uint32_t ny, uint32_t nz, double h,
     float2 *__restrict__ R) {
   constexpr uint32_t kAx = 3 * kRmtWaveMaxDim;
-  // M ncell <= 2048: M 16 needs every pair of dimensions at most 16 cells
-  // (ncell <= 64), M 8 at most 32 (ncell <= 181), M 4 ncell <= 512
+  // M ncell <= 2048: ncell <= L^(3/2) and M <= 256 / L, so M ncell <=
+  // 256 sqrt(L) <= 256 * 8
   constexpr uint32_t kWin = 2048;
   static_assert(kRmtWaveBlock == 256 && kRmtWaveNodes == 16,
                 "thread map of the small-grid shell sums");
@@ -1446,8 +1448,18 @@ __device__ __forceinline__ void rmt_wave_tile(
   const uint32_t lxy = nx * ny, lyz = ny * nz, lxz = nx * nz;
   const uint32_t lmax = lxy > lyz ? (lxy > lxz ? lxy : lxz)
                                   : (lyz > lxz ? lyz : lxz);
-  const uint32_t M = lmax <= 16 ? 16u : (lmax <= 32 ? 8u : 4u);
-  const uint32_t j = tid & (M - 1), q = tid / M;
+  // the largest power of two that fits, or the largest count if that
+  // takes fewer groups (6^3: 7, three groups against four): a power of two
+  // keeps a group's S loads on whole node runs
+  const uint32_t mmax = 256u / lmax < kRmtWaveNodes ? 256u / lmax
+                                                    : kRmtWaveNodes;
+  uint32_t mp = kRmtWaveNodes;
+  while (mp > mmax) mp >>= 1;
+  const uint32_t M = (kRmtWaveNodes + mmax - 1) / mmax <
+                             (kRmtWaveNodes + mp - 1) / mp
+                         ? mmax
+                         : mp;
+  const uint32_t q = tid / M, j = tid - q * M;
   const uint32_t ncell = nx * ny * nz;
   // this thread's z-line of S for the nodes of step s, in flight while the
   // previous step's x and y passes run; the indexes are clamped into the grid, so all eight loads are
@@ -1455,7 +1467,8 @@ __device__ __forceinline__ void rmt_wave_tile(
   float2 sv[8];
   const uint32_t qf = q < lxy ? q : 0u;
   auto fetch = [&](uint32_t s) {
-    const uint32_t nf = n0 + s + j;
+    const uint32_t nf =
+        n0 + (s + j < kRmtWaveNodes ? s + j : kRmtWaveNodes - 1);
 #pragma unroll
     for (uint32_t z = 0; z < 8; ++z)
       sv[z] = S[sf_slot<true>(nf, qf * nz + (z < nz ? z : nz - 1), ncell)];
@@ -1463,7 +1476,8 @@ __device__ __forceinline__ void rmt_wave_tile(
   fetch(0);
   __syncthreads();  // the axis factors
   for (uint32_t step = 0; step < kRmtWaveNodes; step += M) {
-    const uint32_t t = step + j;
+    const bool live = step + j < kRmtWaveNodes;
+    const uint32_t t = live ? step + j : kRmtWaveNodes - 1;
     const uint32_t nd = n0 + t;
     const uint4 par = npar[t];
     const uint32_t ro = par.x, ri = par.y;
@@ -1522,7 +1536,7 @@ __device__ __forceinline__ void rmt_wave_tile(
     rmt_lds_barrier();
     // (y) line (x, z): the two cubes, then V = w exp(+i k.c) (outer -
     // inner) in polar form
-    if (q < lxz) {
+    if (q < lxz && live) {
       const uint32_t x = q / nz, z = q - x * nz;
       float2 vo[8], vi[8];
 #pragma unroll

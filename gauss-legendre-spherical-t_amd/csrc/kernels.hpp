@@ -218,7 +218,7 @@ bool rmt_mfma(const void *S, const double4 *k64, const uint32_t *group,
               uint32_t tile0 = 0, uint32_t tile1 = 0xffffffffu);
 // Shell sums of small grids (CT = float, every dimension up to
 // kRmtWaveMaxDim): workgroups of kRmtWaveBlock threads take tiles of
-// kRmtWaveNodes consecutive nodes, 16, 8 or 4 at a time, and sum them as
+// kRmtWaveNodes consecutive nodes, up to 16 at a time, and sum them as
 // three passes of FP32 window sums along lines of at most 8 cells, with the
 // lines exchanged through LDS (static, 35 KB per workgroup).
 constexpr uint32_t kRmtWaveMaxDim = 8;

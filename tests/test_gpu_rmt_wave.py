@@ -1,7 +1,7 @@
 """Shell sums of small grids as FP32 window sums (option "rmt_shape" 5, CT =
 float, every grid dimension up to 8 cells; the default picks them there).
 
-A workgroup takes 16 nodes, 16, 8 or 4 at a time, and sums the two cubes of
+A workgroup takes 16 nodes, up to 16 at a time, and sums the two cubes of
 every cell as three passes of windows along lines of at most 8 cells, each
 window the difference of two FP32 running sums.  Its remote sums R(k) meet the bar of
 the other shapes against the oracle (1e-5 of the field's largest entry) and
@@ -55,10 +55,11 @@ def _fields_and_forces(h, box, rcut, tol, rmt_shape, torch, g):
     return ncell, (rre, rim), ef
 
 
-# (box, rcut, tol, lattice spacing): 3^3, 4^3, 4 x 5 x 6, 6 x 4 x 8,
-# 6 x 8 x 7 and 8^3 grids (16, 16, 8, 4, 4 and 4 nodes at a time)
+# (box, rcut, tol, lattice spacing): 3^3, 4^3, 6^3, 4 x 5 x 6, 6 x 4 x 8,
+# 6 x 8 x 7 and 8^3 grids (16, 16, 7, 8, 4, 4 and 4 nodes at a time)
 CASES = [((26.0, 26.0, 26.0), 9.0, 1e-4, 2.6),
          ((31.04, 31.04, 31.04), 9.0, 1e-5, 2.6),
+         ((50.0, 50.0, 50.0), 9.0, 1e-4, 3.0),
          ((31.0, 40.0, 50.0), 9.0, 1e-4, 2.6),
          ((52.0, 28.0, 70.0), 9.0, 1e-4, 2.6),
          ((54.0, 72.0, 63.0), 9.0, 1e-4, 3.4),
