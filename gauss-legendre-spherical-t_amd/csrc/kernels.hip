@@ -1462,8 +1462,9 @@ __device__ __forceinline__ void rmt_wave_tile(
   const uint32_t q = tid / M, j = tid - q * M;
   const uint32_t ncell = nx * ny * nz;
   // this thread's z-line of S for the nodes of step s, in flight while the
-  // previous step's x and y passes run; the indexes are clamped into the grid, so all eight loads are
-  // unconditional and in flight together (the extra values are not used)
+  // previous step's x and y passes run; the indexes are clamped into the
+  // grid, so all eight loads are unconditional and in flight together (the
+  // extra values are not used)
   float2 sv[8];
   const uint32_t qf = q < lxy ? q : 0u;
   auto fetch = [&](uint32_t s) {
@@ -1474,7 +1475,7 @@ __device__ __forceinline__ void rmt_wave_tile(
       sv[z] = S[sf_slot<true>(nf, qf * nz + (z < nz ? z : nz - 1), ncell)];
   };
   fetch(0);
-  __syncthreads();  // the axis factors
+  __syncthreads();  // the axis factors and node parameters
   for (uint32_t step = 0; step < kRmtWaveNodes; step += M) {
     const bool live = step + j < kRmtWaveNodes;
     const uint32_t t = live ? step + j : kRmtWaveNodes - 1;
